@@ -1,0 +1,341 @@
+// Hierarchical DDA over the School-B tree: one lane per ray, hit-for-hit identical to
+// the f32 iterative mirror (voxel-core/src/mirror.rs:254-326).
+//
+// Every floating-point operation is the mirror's, in its order, in binary32, with no
+// FMA contraction (-ffp-contract=off) and IEEE division. Two exact rewrites:
+//  * (entry - origin) / cell_size with cell_size = 2^e is computed as a multiply by
+//    2^-e: both are the correctly rounded value of the same real number;
+//  * cell_size / d is computed as cell_size * (1/d) when |d| < 2^100: scaling by a
+//    power of two commutes with rounding unless 1/d is subnormal, which that bound
+//    excludes. Otherwise the division is done as written.
+//
+// Frame storage: the active frame lives in registers. Each ancestor frame is kept in
+// a register slot named by its level (the stack index of a frame is root_level - level,
+// so "the frame at level L" is unique at any time); saving and restoring go through a
+// switch on the level, so no frame is ever addressed dynamically.
+//
+// The hit's ray parameter t is the entry parameter of the f64 slab of the hit voxel for
+// the ray widened to f64: max(0, max_a min((v_a - o_a)/d_a, (v_a + 1 - o_a)/d_a)) over
+// axes with d_a != 0, which is the t_enter the f64 reference traversal reports for that
+// voxel (voxel-core/src/layout.rs:176-183).
+#include "vxl_internal.hpp"
+
+namespace vxl {
+namespace {
+
+constexpr uint32_t kFull = (7u << 9) | (7u << 12) | (7u << 15);
+constexpr uint32_t kMaxIter = 1u << 22;
+
+struct Saved {
+  uint32_t node, cell;
+  float tm0, tm1, tm2, td0, td1, td2, te;
+};
+
+__device__ __forceinline__ uint32_t floor_to_cell(float v, uint32_t dim) {
+  // mirror.rs:31-33: v.floor().clamp(0, dim-1) as u32 (NaN passes the clamp -> 0)
+  float x = floorf(v);
+  const float hi = float(dim - 1);
+  if (x < 0.0f) x = 0.0f;
+  if (x > hi) x = hi;
+  return (x == x) ? uint32_t(x) : 0u;
+}
+
+struct Ray {
+  float o[3], d[3], rinv[3];
+  bool big[3];
+};
+
+// Walker::enter (mirror.rs:48-85) for one axis.
+__device__ __forceinline__ void axis_enter(const Ray &r, int a, float org, uint32_t dim, float cs,
+                                           float inv_cs, float t_enter, uint32_t &cell,
+                                           float &tmax, float &tdel) {
+  const float o = r.o[a], d = r.d[a];
+  const float entry = o + t_enter * d;
+  const float local = (entry - org) * inv_cs;
+  const uint32_t idx = floor_to_cell(local, dim);
+  cell = idx;
+  tmax = INFINITY;
+  tdel = INFINITY;
+  if (d > 0.0f) {
+    const float next = org + (float(idx) + 1.0f) * cs;
+    tmax = t_enter + (next - entry) / d;
+    tdel = r.big[a] ? cs / d : cs * r.rinv[a];
+  } else if (d < 0.0f) {
+    const float next = org + float(idx) * cs;
+    tmax = t_enter + (next - entry) / d;
+    tdel = -(r.big[a] ? cs / d : cs * r.rinv[a]);
+  }
+}
+
+__device__ __forceinline__ uint32_t frame_cell_size(uint32_t level) {
+  return level == 1 ? 1u : (1u << (2 * level - 1));
+}
+
+// leaf_reaches_f32 (mirror.rs:202-249), with 1/d precomputed (same value per call).
+__device__ __forceinline__ bool leaf_reaches(uint32_t packed, const Ray &r, uint32_t ox,
+                                             uint32_t oy, uint32_t oz, float t_enter) {
+  if (packed == kFull) return true;
+  const uint32_t mn[3] = {packed & 7u, (packed >> 3) & 7u, (packed >> 6) & 7u};
+  const uint32_t mx[3] = {(packed >> 9) & 7u, (packed >> 12) & 7u, (packed >> 15) & 7u};
+  const uint32_t org[3] = {ox, oy, oz};
+  float t_near = -1e30f, t_far = 1e30f;
+  bool miss = false;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const float lo = float(org[a] + mn[a]) - 1.0f;
+    const float hi = float(org[a] + mx[a] + 1u) + 1.0f;
+    if (r.d[a] == 0.0f) {
+      if (r.o[a] < lo || r.o[a] > hi) miss = true;
+    } else {
+      float t1 = (lo - r.o[a]) * r.rinv[a];
+      float t2 = (hi - r.o[a]) * r.rinv[a];
+      if (t1 > t2) {
+        const float s = t1;
+        t1 = t2;
+        t2 = s;
+      }
+      t_near = fmaxf(t_near, t1);
+      t_far = fminf(t_far, t2);
+    }
+  }
+  if (miss || t_near > t_far) return false;
+  return t_far >= t_enter;
+}
+
+__global__ __launch_bounds__(256) void k_traverse(
+    const float *__restrict__ rays, uint64_t R, const uint32_t *__restrict__ nodes,
+    const uint32_t *__restrict__ leafw, const uint32_t *__restrict__ bounds, uint32_t n,
+    uint32_t k, uint64_t leaf_count, uint4 *__restrict__ hits, double *__restrict__ tout) {
+  const uint64_t ri = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (ri >= R) return;
+  Ray r;
+  {
+    const float2 a = *reinterpret_cast<const float2 *>(rays + 6 * ri);
+    const float2 b = *reinterpret_cast<const float2 *>(rays + 6 * ri + 2);
+    const float2 c = *reinterpret_cast<const float2 *>(rays + 6 * ri + 4);
+    r.o[0] = a.x; r.o[1] = a.y; r.o[2] = b.x;
+    r.d[0] = b.y; r.d[1] = c.x; r.d[2] = c.y;
+  }
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    r.rinv[a] = 1.0f / r.d[a];
+    r.big[a] = !(fabsf(r.d[a]) < 0x1p100f);
+  }
+  uint4 result = make_uint4(0u, 0u, 0u, 0u);
+  bool hit = false;
+
+  // grid clip, ray_aabb_f32 (mirror.rs:169-192) + t1 < 0 check (:273-277)
+  const float nf = float(n);
+  float t0 = -INFINITY, t1 = INFINITY;
+  bool missed = leaf_count == 0;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    if (missed) break;
+    if (r.d[a] == 0.0f) {
+      if (r.o[a] < 0.0f || r.o[a] > nf) missed = true;
+    } else {
+      float ta = (0.0f - r.o[a]) * r.rinv[a];
+      float tb = (nf - r.o[a]) * r.rinv[a];
+      if (ta > tb) {
+        const float s = ta;
+        ta = tb;
+        tb = s;
+      }
+      t0 = fmaxf(t0, ta);
+      t1 = fminf(t1, tb);
+      if (t0 > t1) missed = true;
+    }
+  }
+  if (!missed && t1 < 0.0f) missed = true;
+
+  if (!missed) {
+    const uint32_t root_level = k == 0 ? 1u : k + 1;
+    // active frame
+    uint32_t level = root_level, node = 0;
+    uint32_t ox = 0, oy = 0, oz = 0;
+    uint32_t cx, cy, cz;
+    float tm0, tm1, tm2, td0, td1, td2;
+    float te = fmaxf(t0, 0.0f);
+    uint32_t mlo = 0, mhi = 0, cbase = 0;  // cached node record of a node frame
+    Saved s2, s3, s4, s5, s6;
+
+#define VXL_MAKE_FRAME(LEVEL, ORGX, ORGY, ORGZ, TENTER)                                    \
+  {                                                                                        \
+    const uint32_t dim_ = (LEVEL) == 1 ? 8u : 4u;                                          \
+    const uint32_t csu_ = frame_cell_size(LEVEL);                                          \
+    const float cs_ = float(csu_), ics_ = 1.0f / float(csu_);                              \
+    axis_enter(r, 0, float(ORGX), dim_, cs_, ics_, TENTER, cx, tm0, td0);                  \
+    axis_enter(r, 1, float(ORGY), dim_, cs_, ics_, TENTER, cy, tm1, td1);                  \
+    axis_enter(r, 2, float(ORGZ), dim_, cs_, ics_, TENTER, cz, tm2, td2);                  \
+    te = (TENTER);                                                                         \
+  }
+
+    // Walker::step (mirror.rs:87-116) on the active frame; returns false when exhausted.
+#define VXL_STEP(OK)                                                                       \
+  {                                                                                        \
+    const uint32_t dim_ = level == 1 ? 8u : 4u;                                            \
+    int a_ = 0;                                                                            \
+    float best_ = tm0;                                                                     \
+    if (tm1 < best_) { a_ = 1; best_ = tm1; }                                              \
+    if (tm2 < best_) { a_ = 2; best_ = tm2; }                                              \
+    OK = false;                                                                            \
+    if (isfinite(best_)) {                                                                 \
+      if (a_ == 0) {                                                                       \
+        if (r.d[0] > 0.0f) { if (cx + 1 < dim_) { cx += 1; OK = true; } }                  \
+        else if (r.d[0] < 0.0f) { if (cx != 0) { cx -= 1; OK = true; } }                   \
+        if (OK) tm0 = tm0 + td0;                                                           \
+      } else if (a_ == 1) {                                                                \
+        if (r.d[1] > 0.0f) { if (cy + 1 < dim_) { cy += 1; OK = true; } }                  \
+        else if (r.d[1] < 0.0f) { if (cy != 0) { cy -= 1; OK = true; } }                   \
+        if (OK) tm1 = tm1 + td1;                                                           \
+      } else {                                                                             \
+        if (r.d[2] > 0.0f) { if (cz + 1 < dim_) { cz += 1; OK = true; } }                  \
+        else if (r.d[2] < 0.0f) { if (cz != 0) { cz -= 1; OK = true; } }                   \
+        if (OK) tm2 = tm2 + td2;                                                           \
+      }                                                                                    \
+      if (OK) te = best_;                                                                  \
+    }                                                                                      \
+  }
+
+#define VXL_SAVE_CASE(L)                                                                   \
+  case L:                                                                                  \
+    s##L.node = node; s##L.cell = cx | (cy << 8) | (cz << 16);                             \
+    s##L.tm0 = tm0; s##L.tm1 = tm1; s##L.tm2 = tm2;                                        \
+    s##L.td0 = td0; s##L.td1 = td1; s##L.td2 = td2; s##L.te = te;                          \
+    break;
+#define VXL_LOAD_CASE(L)                                                                   \
+  case L:                                                                                  \
+    node = s##L.node; cx = s##L.cell & 0xffu; cy = (s##L.cell >> 8) & 0xffu;               \
+    cz = s##L.cell >> 16;                                                                  \
+    tm0 = s##L.tm0; tm1 = s##L.tm1; tm2 = s##L.tm2;                                        \
+    td0 = s##L.td0; td1 = s##L.td1; td2 = s##L.td2; te = s##L.te;                          \
+    break;
+
+    VXL_MAKE_FRAME(root_level, 0u, 0u, 0u, te);
+    if (level >= 2) {
+      mlo = nodes[0];
+      mhi = nodes[1];
+      cbase = nodes[2];
+    }
+    bool done = false;
+    uint32_t iter = 0;
+    while (!done) {
+      if (++iter > kMaxIter) {  // bounded: a legitimate walk never comes close
+        result = make_uint4(0u, 0u, 0u, 0xFFFFFFFFu);
+        break;
+      }
+      bool stepped;
+      if (level == 1) {
+        const uint32_t m = dev::morton9(cx, cy, cz);
+        const uint32_t word = leafw[uint64_t(node) * 16 + (m >> 5)];
+        if ((word >> (m & 31)) & 1u) {
+          result = make_uint4(ox + cx, oy + cy, oz + cz, 1u);
+          hit = true;
+          break;
+        }
+        VXL_STEP(stepped);
+      } else {
+        const uint32_t bit = dev::child_bit(cx, cy, cz);
+        const bool has = bit < 32 ? ((mlo >> bit) & 1u) : ((mhi >> (bit - 32)) & 1u);
+        const uint32_t size = frame_cell_size(level);
+        const uint32_t cox = ox + cx * size, coy = oy + cy * size, coz = oz + cz * size;
+        bool descend = has;
+        uint32_t slot = 0;
+        if (has) {
+          const uint32_t below = bit < 32 ? __popc(mlo & ((1u << bit) - 1u))
+                                          : __popc(mlo) + __popc(mhi & ((1u << (bit - 32)) - 1u));
+          slot = cbase + below;
+          if (level == 2) descend = leaf_reaches(bounds[slot], r, cox, coy, coz, te);
+        }
+        if (descend) {
+          switch (level) {
+            VXL_SAVE_CASE(2)
+            VXL_SAVE_CASE(3)
+            VXL_SAVE_CASE(4)
+            VXL_SAVE_CASE(5)
+            VXL_SAVE_CASE(6)
+            default: break;
+          }
+          const float t_in = te;
+          level -= 1;
+          node = slot;
+          ox = cox;
+          oy = coy;
+          oz = coz;
+          VXL_MAKE_FRAME(level, ox, oy, oz, t_in);
+          if (level >= 2) {
+            mlo = nodes[3ull * node];
+            mhi = nodes[3ull * node + 1];
+            cbase = nodes[3ull * node + 2];
+          }
+          continue;
+        }
+        VXL_STEP(stepped);
+      }
+      // ascend (mirror.rs:155-165): pop parents until one steps to a new cell
+      while (!stepped) {
+        if (level == root_level) {
+          done = true;
+          break;
+        }
+        level += 1;
+        switch (level) {
+          VXL_LOAD_CASE(2)
+          VXL_LOAD_CASE(3)
+          VXL_LOAD_CASE(4)
+          VXL_LOAD_CASE(5)
+          VXL_LOAD_CASE(6)
+          default: break;
+        }
+        const uint32_t size = frame_cell_size(level);
+        ox -= cx * size;
+        oy -= cy * size;
+        oz -= cz * size;
+        mlo = nodes[3ull * node];
+        mhi = nodes[3ull * node + 1];
+        cbase = nodes[3ull * node + 2];
+        VXL_STEP(stepped);
+      }
+    }
+#undef VXL_MAKE_FRAME
+#undef VXL_STEP
+#undef VXL_SAVE_CASE
+#undef VXL_LOAD_CASE
+  }
+  hits[ri] = result;
+  if (tout) {
+    double t = 0.0;
+    if (hit) {
+      const double v[3] = {double(result.x), double(result.y), double(result.z)};
+      double tn = 0.0;
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        if (r.d[a] != 0.0f) {
+          const double o = double(r.o[a]);
+          const double inv = 1.0 / double(r.d[a]);
+          const double ta = (v[a] - o) * inv, tb = (v[a] + 1.0 - o) * inv;
+          tn = fmax(tn, fmin(ta, tb));
+        }
+      }
+      t = tn;
+    }
+    tout[ri] = t;
+  }
+}
+
+}  // namespace
+
+void launch_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, uint64_t R,
+                     uint32_t *d_hits, double *d_t, hipStream_t s) {
+  if (R == 0) return;
+  const int tb = 256;
+  const uint64_t blocks = (R + tb - 1) / tb;
+  if (blocks > 0xffffffffull)
+    throw Error(VXL_WORKGROUPS_EXCEEDED, "traverse: ray batch exceeds the grid limit");
+  hipLaunchKernelGGL(k_traverse, dim3(uint32_t(blocks)), dim3(tb), 0, s, d_rays, R, tree->nodes,
+                     reinterpret_cast<const uint32_t *>(tree->leaves), tree->bounds, tree->n,
+                     tree->k, tree->leaf_count, reinterpret_cast<uint4 *>(d_hits), d_t);
+  VXL_HIP(hipGetLastError());
+}
+
+}  // namespace vxl

@@ -1,0 +1,145 @@
+/*
+ * vxl.h -- C ABI of the MI355X voxelize + sparse-tree + DDA hot path (libvxl.so).
+ *
+ * Drop-in boundary for the reference's GPU voxelizer and GPU traverser. Each entry
+ * point names the reference interface it replaces (paths relative to the reference
+ * repository root). Plain pointers and sizes only; no C++ or torch types.
+ *
+ * Conventions (mirroring the reference boundary, SURVEY-independent summary):
+ *  - Inputs are borrowed; outputs go to caller-owned buffers or library-owned
+ *    opaque handles released with an explicit *_free / *_destroy.
+ *  - Entry points without a `_dev` suffix take HOST pointers and are synchronous,
+ *    like the reference's blocking `voxelize_surface` / `traverse` calls.
+ *    `_dev` entry points take DEVICE pointers and enqueue on the given HIP stream
+ *    (NULL = the context's stream); outputs are valid once that stream is synced.
+ *  - Validation happens before any device work, in the reference's order:
+ *    grid -> mesh -> opts -> storage (voxelizer/src/gpu/dense.rs:30-40).
+ *  - An empty mesh short-circuits to the all-empty output (gpu/dense.rs:45-47).
+ *  - A context serialises its calls (internal mutex); use one context per stream
+ *    of work for concurrency.
+ *  - vxl_last_error() returns the calling thread's last message.
+ */
+#ifndef VXL_H
+#define VXL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes mirror voxelizer/src/error.rs:121-177 (VoxelizeGpuError) and
+ * voxel-gpu/src/error.rs:9-50 (GpuError). */
+typedef enum vxl_status {
+  VXL_OK = 0,
+  VXL_NO_DEVICE = 1,            /* NoAdapter */
+  VXL_INVALID_ARG = 2,          /* PipelineValidation (grid / mesh / opts checks) */
+  VXL_STORAGE_EXCEEDED = 3,     /* StorageExceeded / BufferTooLarge */
+  VXL_WORKGROUPS_EXCEEDED = 4,  /* WorkgroupsExceeded */
+  VXL_OOM = 5,                  /* device allocation failed */
+  VXL_BUSY = 6,                 /* serving queue full / admission refused */
+  VXL_DEVICE_ERROR = 7,         /* DeviceRequest / Poll / BufferMap */
+  VXL_NOT_REPRESENTABLE = 8     /* Resolution::new rejected n (resolution.rs:76-118) */
+} vxl_status;
+
+typedef struct vxl_ctx vxl_ctx;
+typedef struct vxl_tree vxl_tree;
+
+/* A cubic grid: n = 8*4^k voxels per axis, placed by origin + uniform voxel size
+ * (VoxelGrid, voxelizer/src/core.rs:25-35). */
+typedef struct vxl_grid {
+  uint32_t n;
+  float origin[3];
+  float voxel_size;
+} vxl_grid;
+
+/* School-B node, 12 bytes: voxel-core/src/node.rs:24-31 (GpuNode). */
+typedef struct vxl_node {
+  uint32_t mask_lo;
+  uint32_t mask_hi;
+  uint32_t child_base;
+} vxl_node;
+
+typedef struct vxl_tree_info {
+  uint32_t n;
+  uint32_t k;
+  uint64_t node_count;
+  uint64_t leaf_count;
+  uint64_t level_nodes[16]; /* School-A node count per traversal level L (2..k+1) */
+  uint64_t device_bytes;
+} vxl_tree_info;
+
+/* ---- context (replaces GpuContext::try_new, voxel-gpu/src/context.rs:24, and
+ *      GpuVoxelizer::from_device / new_standalone, voxelizer/src/gpu/mod.rs:172,222) */
+vxl_status vxl_ctx_create(int device, vxl_ctx **out);
+void vxl_ctx_destroy(vxl_ctx *ctx);
+/* The context's HIP stream (hipStream_t), for callers that order their own work. */
+void *vxl_ctx_stream(vxl_ctx *ctx);
+const char *vxl_last_error(void);
+const char *vxl_status_str(vxl_status s);
+
+/* ---- voxelize, dense output: replaces GpuVoxelizer::voxelize_surface
+ *      (voxelizer/src/gpu/dense.rs:23-58) and matches voxelize_surface_cpu
+ *      (voxelizer/src/reference_cpu.rs:110-200) bit for bit.
+ * tris: T x 9 f32 world-space vertices. occ_words: ceil(n^3/32) u32, x-major,
+ * bit = x + n*(y + n*z), LSB first (VoxelOccupancy, core.rs:135-253).
+ * pairs_out (nullable): number of (triangle, candidate voxel) pairs tested. */
+vxl_status vxl_voxelize_dense(vxl_ctx *ctx, const float *tris, uint64_t T, const vxl_grid *grid,
+                              float epsilon, uint32_t *occ_words, uint64_t *pairs_out);
+vxl_status vxl_voxelize_dense_dev(vxl_ctx *ctx, const float *d_tris, uint64_t T,
+                                  const vxl_grid *grid, float epsilon, uint32_t *d_occ_words,
+                                  uint64_t *pairs_out, void *stream);
+
+/* ---- voxelize straight into a device-resident School-B tree: the composition
+ *      voxelize_surface -> VoxelOccupancy::to_school_b (core.rs:224-233) =
+ *      SparseTree::build (sparse.rs:241-305) + SchoolBBuffer::from_sparse
+ *      (school_b.rs:113-151), without the n^3 host scan. */
+vxl_status vxl_voxelize_tree(vxl_ctx *ctx, const float *tris, uint64_t T, const vxl_grid *grid,
+                             float epsilon, vxl_tree **out, uint64_t *pairs_out);
+vxl_status vxl_voxelize_tree_dev(vxl_ctx *ctx, const float *d_tris, uint64_t T,
+                                 const vxl_grid *grid, float epsilon, vxl_tree **out,
+                                 uint64_t *pairs_out, void *stream);
+
+/* ---- occupancy field -> tree: VoxelOccupancy::to_school_b (core.rs:231-233).
+ * words: ceil(n^3/32) x-major u32 (host pointer). */
+vxl_status vxl_tree_from_words(vxl_ctx *ctx, const uint32_t *words, uint32_t n, vxl_tree **out);
+
+/* ---- tree handles (replace SchoolBBuffer accessors nodes/leaves/leaf_bounds_words,
+ *      school_b.rs:266-281, and upload_structure, voxel-gpu/src/buffers.rs:21-77) */
+vxl_status vxl_tree_info_get(const vxl_tree *tree, vxl_tree_info *info);
+/* Copies the School-B arrays to host buffers sized from vxl_tree_info. Any output
+ * pointer may be NULL. leaf_words: 8 u64 per leaf; codes: brick Morton code per leaf. */
+vxl_status vxl_tree_download(const vxl_tree *tree, vxl_node *nodes, uint64_t *leaf_words,
+                             uint32_t *bounds, uint64_t *codes);
+/* Uploads a host School-B structure (GpuTraverser::new's upload, traverse.rs:70-150). */
+vxl_status vxl_tree_upload(vxl_ctx *ctx, uint32_t n, const vxl_node *nodes, uint64_t node_count,
+                           const uint64_t *leaf_words, const uint32_t *bounds,
+                           uint64_t leaf_count, vxl_tree **out);
+void vxl_tree_free(vxl_tree *tree);
+
+/* ---- traversal: replaces GpuTraverser::traverse (voxel-gpu/src/traverse.rs:154-278),
+ *      hit-for-hit identical to mirror_traverse (voxel-core/src/mirror.rs:254-326).
+ * rays: R x 6 f32 {origin.xyz, dir.xyz} in voxel space. hits: R x 4 u32 {x,y,z,hit}
+ * (the kernel's vec4<u32>, hdda.wgsl:29-33). t (nullable): R f64 ray parameter at
+ * which the hit voxel is entered (units of |dir|; 0 for a miss) -- the t_enter of
+ * layout::traverse (voxel-core/src/layout.rs:176-183). */
+vxl_status vxl_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *rays, uint64_t R,
+                        uint32_t *hits, double *t);
+vxl_status vxl_traverse_dev(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, uint64_t R,
+                            uint32_t *d_hits, double *d_t, void *stream);
+
+/* ---- timing: when enabled, the last voxelize/traverse call records per-stage
+ * device times (ms) measured with HIP events on the stream the kernels ran on.
+ * stage 0 = setup+scan, 1 = SAT, 2 = tree build, 3 = traverse. */
+void vxl_set_timing(vxl_ctx *ctx, int enable);
+void vxl_last_timing(vxl_ctx *ctx, float out_ms[8]);
+
+/* ---- tuning knobs (autotuner surface): block size and pairs per thread for the SAT
+ * kernel, block size for the DDA kernel. 0 keeps the current value. */
+vxl_status vxl_set_tuning(vxl_ctx *ctx, int sat_block, int sat_pairs_per_thread, int dda_block);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VXL_H */

@@ -1,0 +1,85 @@
+/*
+ * vxl_oracle.h -- CPU restatement of the reference voxelize -> sparse tree -> DDA path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the parity checker and the CPU baseline.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ * The product (friendly-journey_amd/libvxl.so) never links, loads or calls it.
+ *
+ * Every function cites the reference source it restates (paths relative to the
+ * reference repository root). Arithmetic is IEEE binary32/binary64 with no FMA
+ * contraction (built -ffp-contract=off, no fast-math), in the reference's order.
+ */
+#ifndef VXL_ORACLE_H
+#define VXL_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- morton.rs / node.rs / leaf.rs / resolution.rs ---- */
+uint64_t vo_morton_encode(uint32_t x, uint32_t y, uint32_t z);
+void vo_morton_decode(uint64_t code, uint32_t out_xyz[3]);
+uint32_t vo_encode_brick(uint32_t x, uint32_t y, uint32_t z);
+uint32_t vo_child_bit(uint32_t cx, uint32_t cy, uint32_t cz);
+uint32_t vo_child_slot(uint32_t mask_lo, uint32_t mask_hi, uint32_t child_base, uint32_t bit);
+/* 0 ok (k set), 1 too small, 2 too large, 3 not representable (below/above set) */
+int vo_resolution(uint32_t n, uint32_t *k, uint32_t *below, uint32_t *above);
+uint32_t vo_bounds_pack(const uint32_t mn[3], const uint32_t mx[3]);
+void vo_bounds_unpack(uint32_t p, uint32_t mn[3], uint32_t mx[3]);
+uint32_t vo_leaf_occupied_bounds(const uint64_t bits[8]);
+
+/* ---- reference_cpu.rs ---- */
+int vo_triangle_box_overlap(const float center[3], const float half[3], const float v0[3],
+                            const float v1[3], const float v2[3]);
+/* world->grid point transform of core.rs:101-107 (glam Mat4 product, see .c) */
+void vo_world_to_grid(const float p[3], const float origin[3], float voxel_size, float out[3]);
+/* voxelize_surface_cpu: tris T x 9 f32 world space; writes ceil(n^3/32) x-major words
+ * (zeroed here) and, when owner != NULL, n^3 lowest-triangle owners. Returns the number
+ * of (triangle, candidate voxel) pairs the epsilon-padded AABB scan enumerates. */
+uint64_t vo_voxelize_surface(const float *tris, uint64_t T, uint32_t n, const float origin[3],
+                             float voxel_size, float eps, uint32_t *words, uint32_t *owner);
+/* Same candidate count without doing the SAT (for roofline accounting). */
+uint64_t vo_count_candidates(const float *tris, uint64_t T, uint32_t n, const float origin[3],
+                             float voxel_size, float eps);
+
+/* ---- sparse.rs + school_b.rs ---- */
+typedef struct vo_tree vo_tree;
+/* SparseTree::build over an x-major word field, then SchoolBBuffer::from_sparse. */
+vo_tree *vo_tree_build(const uint32_t *words, uint32_t n, int threads);
+/* Wrap School-B arrays produced elsewhere (e.g. by the GPU) for traversal. */
+vo_tree *vo_tree_from_school_b(uint32_t n, const uint32_t *nodes, uint64_t node_count,
+                               const uint64_t *leaves, uint64_t leaf_count,
+                               const uint32_t *bounds);
+void vo_tree_free(vo_tree *t);
+/* counts[0]=leaves, counts[1]=School-B nodes, counts[2+L]=School-A nodes at level L
+ * (L in 0..15; 0 and 1 unused). */
+void vo_tree_counts(const vo_tree *t, uint64_t counts[18]);
+void vo_tree_export(const vo_tree *t, uint32_t *nodes_b /*3*N*/, uint64_t *leaves /*8*L*/,
+                    uint32_t *bounds /*L*/, uint64_t *codes /*L, may be NULL*/);
+/* School-A level L nodes (3 u32 each). */
+void vo_tree_export_level(const vo_tree *t, uint32_t level, uint32_t *nodes_a);
+int vo_tree_is_occupied(const vo_tree *t, uint32_t x, uint32_t y, uint32_t z);
+
+/* ---- mirror.rs (f32) ---- rays R x 6 f32; hits R x 4 u32 (x,y,z,flag).
+ * stats (nullable) R x 3 u32: leaf descents, node descents, cell steps. */
+void vo_mirror_traverse(const vo_tree *t, const float *rays, uint64_t R, uint32_t *hits,
+                        int enable_skip, uint32_t *stats, int threads);
+/* ---- layout.rs (f64) ---- rays R x 6 f64; school 0 = School-A arrays, 1 = School-B. */
+void vo_layout_traverse(const vo_tree *t, const double *rays, uint64_t R, uint32_t *hits,
+                        double *t_enter, int school, int threads);
+/* ---- oracle.rs (dense f64 tier-A) ---- */
+void vo_first_hit_dense(const uint32_t *words, uint32_t n, const double *rays, uint64_t R,
+                        uint32_t *hits, double *t_enter);
+
+/* ---- fixtures.rs procedural fields as x-major words ----
+ * kind: 0 Empty, 1 Solid, 2 SingleVoxel(p0,p1,p2), 3 Checkerboard, 4 OctantFractal(keep=p0),
+ *       5 WireLattice(period=p0, thickness=p1), 6 Dust(density=p0) */
+void vo_fixture_words(int kind, uint32_t n, uint32_t p0, uint32_t p1, uint32_t p2,
+                      uint32_t *words);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
