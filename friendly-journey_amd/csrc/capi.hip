@@ -1,0 +1,512 @@
+// extern "C" entry points of libvxl.so (declared in include/vxl.h).
+#include <cmath>
+#include <cstdio>
+
+#include "vxl_internal.hpp"
+
+namespace vxl {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+void DevBuf::reserve(size_t want) {
+  if (want <= bytes && ptr) return;
+  release();
+  size_t b = want < 256 ? 256 : want;
+  hipError_t e = hipMalloc(&ptr, b);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    ptr = nullptr;
+    bytes = 0;
+    throw Error(VXL_OOM, std::string("device allocation of ") + std::to_string(b) +
+                             " bytes failed: " + hipGetErrorString(e));
+  }
+  bytes = b;
+}
+
+void DevBuf::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+bool resolution_k(uint32_t n, uint32_t *k) {
+  // voxel-core/src/resolution.rs:76-99: n = 8 * 4^k
+  if (n < 8 || n % 8) return false;
+  uint32_t m = n / 8;
+  if (m & (m - 1)) return false;
+  uint32_t tz = __builtin_ctz(m);
+  if (tz % 2) return false;
+  *k = tz / 2;
+  return true;
+}
+
+}  // namespace vxl
+
+using vxl::Error;
+
+namespace {
+
+constexpr uint32_t kMaxK = 5;  // the traversal's frame stack holds 6 levels (mirror.rs:22)
+
+template <class F>
+vxl_status guarded(vxl_ctx *ctx, F &&f) {
+  if (!ctx) {
+    vxl::set_error("null context");
+    return VXL_INVALID_ARG;
+  }
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if (hipSetDevice(ctx->device) != hipSuccess) {
+    (void)hipGetLastError();
+    vxl::set_error("hipSetDevice failed");
+    return VXL_DEVICE_ERROR;
+  }
+  try {
+    f();
+    return VXL_OK;
+  } catch (const Error &e) {
+    vxl::set_error(e.msg);
+    ctx->leafgrid_dirty = true;  // a failed call may leave scratch non-zero
+    return e.code;
+  } catch (const std::exception &e) {
+    vxl::set_error(e.what());
+    ctx->leafgrid_dirty = true;
+    return VXL_DEVICE_ERROR;
+  }
+}
+
+// VoxelGrid::validate (voxelizer/src/core.rs:116-132) + Resolution::new (resolution.rs:76).
+void validate_grid(const vxl_grid *g, uint32_t *k) {
+  if (!g) throw Error(VXL_INVALID_ARG, "grid is null");
+  if (!vxl::resolution_k(g->n, k)) {
+    throw Error(VXL_NOT_REPRESENTABLE,
+                "resolution " + std::to_string(g->n) + " is not representable (must be 8*4^k)");
+  }
+  if (*k > kMaxK)
+    throw Error(VXL_STORAGE_EXCEEDED,
+                "resolution " + std::to_string(g->n) + " exceeds the supported 8*4^5 = 8192");
+  if (!std::isfinite(g->voxel_size) || g->voxel_size <= 0.0f)
+    throw Error(VXL_INVALID_ARG, "voxel_size must be finite and > 0");
+  if (!std::isfinite(1.0f / g->voxel_size))
+    throw Error(VXL_INVALID_ARG, "voxel_size is too small: its reciprocal is not finite");
+  for (int a = 0; a < 3; a++)
+    if (!std::isfinite(g->origin[a])) throw Error(VXL_INVALID_ARG, "origin_world must be finite");
+}
+
+// MeshInput::validate (core.rs:427-459): every vertex finite.
+void validate_mesh_host(const float *tris, uint64_t T) {
+  if (T && !tris) throw Error(VXL_INVALID_ARG, "triangles pointer is null");
+  for (uint64_t i = 0; i < T * 9; i++)
+    if (!std::isfinite(tris[i])) throw Error(VXL_INVALID_ARG, "triangle contains a non-finite vertex");
+}
+
+// VoxelizeOpts::validate (core.rs:492-500)
+void validate_eps(float eps) {
+  if (!std::isfinite(eps) || eps < 0.0f)
+    throw Error(VXL_INVALID_ARG, "voxelize epsilon must be finite and >= 0");
+}
+
+vxl::VoxelizeParams make_params(const vxl_grid *g, float eps) {
+  vxl::VoxelizeParams p;
+  p.n = g->n;
+  // world_to_grid_matrix (core.rs:101-107): scale(1/vs) * translate(-origin);
+  // per component g = (inv * p) + (inv * -o), evaluated in binary32.
+  volatile float inv = 1.0f / g->voxel_size;
+  p.inv = inv;
+  for (int a = 0; a < 3; a++) {
+    volatile float t = p.inv * (-g->origin[a]);
+    p.tx[a] = t;
+  }
+  p.eps = eps;
+  return p;
+}
+
+
+vxl_tree *empty_tree(vxl_ctx *ctx, uint32_t n, uint32_t k) {
+  vxl_tree *t = new vxl_tree();
+  t->ctx = ctx;
+  t->n = n;
+  t->k = k;
+  // one padded entry each so every pointer is valid (buffers.rs:26-44)
+  VXL_HIP(hipMalloc(&t->nodes, 16));
+  VXL_HIP(hipMalloc(&t->leaves, 64));
+  VXL_HIP(hipMalloc(&t->bounds, 16));
+  VXL_HIP(hipMalloc(&t->codes, 16));
+  return t;
+}
+
+// The device part of voxelize -> tree, on ctx->stream.
+vxl_tree *voxelize_tree_impl(vxl_ctx *ctx, const float *d_tris, uint64_t T, const vxl_grid *g,
+                             uint32_t k, float eps, uint64_t *pairs_out) {
+  if (T == 0) {
+    if (pairs_out) *pairs_out = 0;
+    return empty_tree(ctx, g->n, k);
+  }
+  vxl::mark(ctx, 0);
+  vxl::prepare_grid(ctx, g->n);
+  const uint64_t P = vxl::run_sat(ctx, d_tris, T, make_params(g, eps));
+  if (pairs_out) *pairs_out = P;
+  vxl_tree *t = vxl::build_tree_from_grid(ctx, g->n, k);
+  vxl::mark(ctx, 3);
+  if (ctx->timing && P) {
+    VXL_HIP(hipEventSynchronize(ctx->ev[3]));
+    VXL_HIP(hipEventElapsedTime(&ctx->last_ms[0], ctx->ev[0], ctx->ev[1]));
+    VXL_HIP(hipEventElapsedTime(&ctx->last_ms[1], ctx->ev[1], ctx->ev[2]));
+    VXL_HIP(hipEventElapsedTime(&ctx->last_ms[2], ctx->ev[2], ctx->ev[3]));
+  }
+  return t;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *vxl_last_error(void) { return vxl::g_last_error.c_str(); }
+
+const char *vxl_status_str(vxl_status s) {
+  switch (s) {
+    case VXL_OK: return "ok";
+    case VXL_NO_DEVICE: return "no device";
+    case VXL_INVALID_ARG: return "invalid argument";
+    case VXL_STORAGE_EXCEEDED: return "storage exceeded";
+    case VXL_WORKGROUPS_EXCEEDED: return "workgroups exceeded";
+    case VXL_OOM: return "out of device memory";
+    case VXL_BUSY: return "busy";
+    case VXL_DEVICE_ERROR: return "device error";
+    case VXL_NOT_REPRESENTABLE: return "resolution not representable";
+  }
+  return "unknown";
+}
+
+vxl_status vxl_ctx_create(int device, vxl_ctx **out) {
+  if (!out) {
+    vxl::set_error("out is null");
+    return VXL_INVALID_ARG;
+  }
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+    (void)hipGetLastError();
+    vxl::set_error("no HIP device present");
+    return VXL_NO_DEVICE;
+  }
+  if (device < 0 || device >= count) {
+    vxl::set_error("device index out of range");
+    return VXL_NO_DEVICE;
+  }
+  vxl_ctx *ctx = new vxl_ctx();
+  ctx->device = device;
+  try {
+    VXL_HIP(hipSetDevice(device));
+    VXL_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    VXL_HIP(hipHostMalloc(&ctx->host_pinned, 4096, hipHostMallocDefault));
+    hipDeviceProp_t prop;
+    VXL_HIP(hipGetDeviceProperties(&prop, device));
+    ctx->cu_count = prop.multiProcessorCount;
+    for (auto &e : ctx->ev) VXL_HIP(hipEventCreate(&e));
+  } catch (const Error &e) {
+    vxl::set_error(e.msg);
+    delete ctx;
+    return e.code;
+  }
+  *out = ctx;
+  return VXL_OK;
+}
+
+void vxl_ctx_destroy(vxl_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  vxl::DevBuf *bufs[] = {&ctx->tris_in, &ctx->tri_count, &ctx->tri_flag, &ctx->tri_cidx,
+                         &ctx->rec,     &ctx->ccount,    &ctx->coff,     &ctx->bstart,
+                         &ctx->leafgrid, &ctx->bitmap,   &ctx->levelscratch, &ctx->scan_tmp,
+                         &ctx->words_io, &ctx->rays_io,  &ctx->hits_io,  &ctx->t_io};
+  for (auto *b : bufs) b->release();
+  for (auto &e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+void *vxl_ctx_stream(vxl_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+void vxl_set_timing(vxl_ctx *ctx, int enable) {
+  if (ctx) ctx->timing = enable != 0;
+}
+
+void vxl_last_timing(vxl_ctx *ctx, float out_ms[8]) {
+  if (!ctx || !out_ms) return;
+  for (int i = 0; i < 8; i++) out_ms[i] = ctx->last_ms[i];
+}
+
+vxl_status vxl_set_tuning(vxl_ctx *ctx, int sat_block, int sat_ppt, int dda_block) {
+  return guarded(ctx, [&] {
+    if (sat_block) {
+      if (sat_block % 64 || sat_block < 64 || sat_block > 1024)
+        throw Error(VXL_INVALID_ARG, "sat_block must be a multiple of 64 in [64, 1024]");
+      ctx->sat_block = sat_block;
+    }
+    if (sat_ppt) {
+      if (sat_ppt < 1 || sat_ppt > 64) throw Error(VXL_INVALID_ARG, "sat_pairs_per_thread in [1,64]");
+      ctx->sat_pairs_per_thread = sat_ppt;
+    }
+    if (dda_block) {
+      if (dda_block % 64 || dda_block < 64 || dda_block > 256)
+        throw Error(VXL_INVALID_ARG, "dda_block must be a multiple of 64 in [64, 256]");
+      ctx->dda_block = dda_block;
+    }
+  });
+}
+
+vxl_status vxl_voxelize_tree_dev(vxl_ctx *ctx, const float *d_tris, uint64_t T,
+                                 const vxl_grid *grid, float epsilon, vxl_tree **out,
+                                 uint64_t *pairs_out, void *stream) {
+  return guarded(ctx, [&] {
+    if (!out) throw Error(VXL_INVALID_ARG, "out is null");
+    *out = nullptr;
+    uint32_t k = 0;
+    validate_grid(grid, &k);
+    if (T && !d_tris) throw Error(VXL_INVALID_ARG, "triangles pointer is null");
+    validate_eps(epsilon);
+    hipStream_t user = static_cast<hipStream_t>(stream);
+    if (user && user != ctx->stream) {
+      hipEvent_t ev;
+      VXL_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      VXL_HIP(hipEventRecord(ev, user));
+      VXL_HIP(hipStreamWaitEvent(ctx->stream, ev, 0));
+      VXL_HIP(hipEventDestroy(ev));
+    }
+    *out = voxelize_tree_impl(ctx, d_tris, T, grid, k, epsilon, pairs_out);
+    if (user && user != ctx->stream) {
+      hipEvent_t ev;
+      VXL_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      VXL_HIP(hipEventRecord(ev, ctx->stream));
+      VXL_HIP(hipStreamWaitEvent(user, ev, 0));
+      VXL_HIP(hipEventDestroy(ev));
+    }
+  });
+}
+
+vxl_status vxl_voxelize_tree(vxl_ctx *ctx, const float *tris, uint64_t T, const vxl_grid *grid,
+                             float epsilon, vxl_tree **out, uint64_t *pairs_out) {
+  return guarded(ctx, [&] {
+    if (!out) throw Error(VXL_INVALID_ARG, "out is null");
+    *out = nullptr;
+    uint32_t k = 0;
+    validate_grid(grid, &k);
+    validate_mesh_host(tris, T);
+    validate_eps(epsilon);
+    const float *d = nullptr;
+    if (T) {
+      ctx->tris_in.reserve(T * 36);
+      VXL_HIP(hipMemcpyAsync(ctx->tris_in.ptr, tris, T * 36, hipMemcpyHostToDevice, ctx->stream));
+      d = ctx->tris_in.as<float>();
+    }
+    *out = voxelize_tree_impl(ctx, d, T, grid, k, epsilon, pairs_out);
+    VXL_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+vxl_status vxl_voxelize_dense_dev(vxl_ctx *ctx, const float *d_tris, uint64_t T,
+                                  const vxl_grid *grid, float epsilon, uint32_t *d_occ_words,
+                                  uint64_t *pairs_out, void *stream) {
+  return guarded(ctx, [&] {
+    uint32_t k = 0;
+    validate_grid(grid, &k);
+    if (T && !d_tris) throw Error(VXL_INVALID_ARG, "triangles pointer is null");
+    validate_eps(epsilon);
+    if (!d_occ_words) throw Error(VXL_INVALID_ARG, "occupancy output is null");
+    const uint64_t nn = grid->n;
+    const uint64_t nwords = (nn * nn * nn + 31) / 32;
+    hipStream_t user = static_cast<hipStream_t>(stream);
+    if (user && user != ctx->stream) VXL_HIP(hipStreamSynchronize(user));
+    if (T == 0) {
+      VXL_HIP(hipMemsetAsync(d_occ_words, 0, nwords * 4, ctx->stream));
+      if (pairs_out) *pairs_out = 0;
+    } else {
+      vxl::prepare_grid(ctx, grid->n);
+      const uint64_t P = vxl::run_sat(ctx, d_tris, T, make_params(grid, epsilon));
+      if (pairs_out) *pairs_out = P;
+      vxl::grid_to_words(ctx, grid->n, d_occ_words);
+    }
+    if (user && user != ctx->stream) VXL_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+vxl_status vxl_voxelize_dense(vxl_ctx *ctx, const float *tris, uint64_t T, const vxl_grid *grid,
+                              float epsilon, uint32_t *occ_words, uint64_t *pairs_out) {
+  return guarded(ctx, [&] {
+    uint32_t k = 0;
+    validate_grid(grid, &k);
+    validate_mesh_host(tris, T);
+    validate_eps(epsilon);
+    if (!occ_words) throw Error(VXL_INVALID_ARG, "occupancy output is null");
+    const uint64_t nn = grid->n;
+    const uint64_t nwords = (nn * nn * nn + 31) / 32;
+    if (T == 0) {
+      std::memset(occ_words, 0, nwords * 4);
+      if (pairs_out) *pairs_out = 0;
+      return;
+    }
+    ctx->tris_in.reserve(T * 36);
+    ctx->words_io.reserve(nwords * 4);
+    VXL_HIP(hipMemcpyAsync(ctx->tris_in.ptr, tris, T * 36, hipMemcpyHostToDevice, ctx->stream));
+    vxl::prepare_grid(ctx, grid->n);
+    const uint64_t P = vxl::run_sat(ctx, ctx->tris_in.as<float>(), T, make_params(grid, epsilon));
+    if (pairs_out) *pairs_out = P;
+    vxl::grid_to_words(ctx, grid->n, ctx->words_io.as<uint32_t>());
+    VXL_HIP(hipMemcpyAsync(occ_words, ctx->words_io.ptr, nwords * 4, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    VXL_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+vxl_status vxl_tree_from_words(vxl_ctx *ctx, const uint32_t *words, uint32_t n, vxl_tree **out) {
+  return guarded(ctx, [&] {
+    if (!out) throw Error(VXL_INVALID_ARG, "out is null");
+    *out = nullptr;
+    vxl_grid g{n, {0, 0, 0}, 1.0f};
+    uint32_t k = 0;
+    validate_grid(&g, &k);
+    if (!words) throw Error(VXL_INVALID_ARG, "words is null");
+    const uint64_t nn = n;
+    const uint64_t nwords = (nn * nn * nn + 31) / 32;
+    ctx->words_io.reserve(nwords * 4);
+    VXL_HIP(hipMemcpyAsync(ctx->words_io.ptr, words, nwords * 4, hipMemcpyHostToDevice,
+                           ctx->stream));
+    vxl::prepare_grid(ctx, n);
+    vxl::words_to_grid(ctx, n, ctx->words_io.as<uint32_t>());
+    *out = vxl::build_tree_from_grid(ctx, n, k);
+    VXL_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+vxl_status vxl_tree_info_get(const vxl_tree *tree, vxl_tree_info *info) {
+  if (!tree || !info) {
+    vxl::set_error("null tree or info");
+    return VXL_INVALID_ARG;
+  }
+  info->n = tree->n;
+  info->k = tree->k;
+  info->node_count = tree->node_count;
+  info->leaf_count = tree->leaf_count;
+  for (int i = 0; i < 16; i++) info->level_nodes[i] = tree->level_nodes[i];
+  info->device_bytes = tree->bytes;
+  return VXL_OK;
+}
+
+vxl_status vxl_tree_download(const vxl_tree *tree, vxl_node *nodes, uint64_t *leaf_words,
+                             uint32_t *bounds, uint64_t *codes) {
+  if (!tree) {
+    vxl::set_error("null tree");
+    return VXL_INVALID_ARG;
+  }
+  return guarded(tree->ctx, [&] {
+    hipStream_t s = tree->ctx->stream;
+    if (nodes && tree->node_count)
+      VXL_HIP(hipMemcpyAsync(nodes, tree->nodes, tree->node_count * 12, hipMemcpyDeviceToHost, s));
+    if (leaf_words && tree->leaf_count)
+      VXL_HIP(hipMemcpyAsync(leaf_words, tree->leaves, tree->leaf_count * 64,
+                             hipMemcpyDeviceToHost, s));
+    if (bounds && tree->leaf_count)
+      VXL_HIP(hipMemcpyAsync(bounds, tree->bounds, tree->leaf_count * 4, hipMemcpyDeviceToHost, s));
+    if (codes && tree->leaf_count)
+      VXL_HIP(hipMemcpyAsync(codes, tree->codes, tree->leaf_count * 8, hipMemcpyDeviceToHost, s));
+    VXL_HIP(hipStreamSynchronize(s));
+  });
+}
+
+vxl_status vxl_tree_upload(vxl_ctx *ctx, uint32_t n, const vxl_node *nodes, uint64_t node_count,
+                           const uint64_t *leaf_words, const uint32_t *bounds,
+                           uint64_t leaf_count, vxl_tree **out) {
+  return guarded(ctx, [&] {
+    if (!out) throw Error(VXL_INVALID_ARG, "out is null");
+    *out = nullptr;
+    vxl_grid g{n, {0, 0, 0}, 1.0f};
+    uint32_t k = 0;
+    validate_grid(&g, &k);
+    if ((node_count && !nodes) || (leaf_count && (!leaf_words || !bounds)))
+      throw Error(VXL_INVALID_ARG, "null structure array");
+    if (leaf_count && k >= 1 && node_count == 0)
+      throw Error(VXL_INVALID_ARG, "a non-empty tree with k >= 1 needs nodes");
+    vxl_tree *t = new vxl_tree();
+    t->ctx = ctx;
+    t->n = n;
+    t->k = k;
+    t->node_count = node_count;
+    t->leaf_count = leaf_count;
+    try {
+      VXL_HIP(hipMalloc(&t->nodes, node_count ? node_count * 12 : 16));
+      VXL_HIP(hipMalloc(&t->leaves, leaf_count ? leaf_count * 64 : 64));
+      VXL_HIP(hipMalloc(&t->bounds, leaf_count ? leaf_count * 4 : 16));
+      VXL_HIP(hipMalloc(&t->codes, leaf_count ? leaf_count * 8 : 16));
+      hipStream_t s = ctx->stream;
+      if (node_count)
+        VXL_HIP(hipMemcpyAsync(t->nodes, nodes, node_count * 12, hipMemcpyHostToDevice, s));
+      if (leaf_count) {
+        VXL_HIP(hipMemcpyAsync(t->leaves, leaf_words, leaf_count * 64, hipMemcpyHostToDevice, s));
+        VXL_HIP(hipMemcpyAsync(t->bounds, bounds, leaf_count * 4, hipMemcpyHostToDevice, s));
+        VXL_HIP(hipMemsetAsync(t->codes, 0, leaf_count * 8, s));
+      }
+      VXL_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+      vxl::tree_free_device(t);
+      delete t;
+      throw;
+    }
+    t->bytes = node_count * 12 + leaf_count * 76;
+    *out = t;
+  });
+}
+
+void vxl_tree_free(vxl_tree *tree) {
+  if (!tree) return;
+  if (tree->ctx) {
+    std::lock_guard<std::mutex> lock(tree->ctx->mu);
+    (void)hipSetDevice(tree->ctx->device);
+    (void)hipStreamSynchronize(tree->ctx->stream);
+    vxl::tree_free_device(tree);
+  } else {
+    vxl::tree_free_device(tree);
+  }
+  delete tree;
+}
+
+vxl_status vxl_traverse_dev(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, uint64_t R,
+                            uint32_t *d_hits, double *d_t, void *stream) {
+  return guarded(ctx, [&] {
+    if (!tree) throw Error(VXL_INVALID_ARG, "tree is null");
+    if (R && (!d_rays || !d_hits)) throw Error(VXL_INVALID_ARG, "null ray or hit buffer");
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    if (ctx->timing) VXL_HIP(hipEventRecord(ctx->ev[4], s));
+    vxl::launch_traverse(ctx, tree, d_rays, R, d_hits, d_t, s);
+    if (ctx->timing) {
+      VXL_HIP(hipEventRecord(ctx->ev[5], s));
+      VXL_HIP(hipEventSynchronize(ctx->ev[5]));
+      VXL_HIP(hipEventElapsedTime(&ctx->last_ms[3], ctx->ev[4], ctx->ev[5]));
+    }
+  });
+}
+
+vxl_status vxl_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *rays, uint64_t R,
+                        uint32_t *hits, double *t) {
+  return guarded(ctx, [&] {
+    if (!tree) throw Error(VXL_INVALID_ARG, "tree is null");
+    if (R == 0) return;
+    if (!rays || !hits) throw Error(VXL_INVALID_ARG, "null ray or hit buffer");
+    hipStream_t s = ctx->stream;
+    ctx->rays_io.reserve(R * 24);
+    ctx->hits_io.reserve(R * 16);
+    if (t) ctx->t_io.reserve(R * 8);
+    VXL_HIP(hipMemcpyAsync(ctx->rays_io.ptr, rays, R * 24, hipMemcpyHostToDevice, s));
+    vxl::launch_traverse(ctx, tree, ctx->rays_io.as<float>(), R, ctx->hits_io.as<uint32_t>(),
+                         t ? ctx->t_io.as<double>() : nullptr, s);
+    VXL_HIP(hipMemcpyAsync(hits, ctx->hits_io.ptr, R * 16, hipMemcpyDeviceToHost, s));
+    if (t) VXL_HIP(hipMemcpyAsync(t, ctx->t_io.ptr, R * 8, hipMemcpyDeviceToHost, s));
+    VXL_HIP(hipStreamSynchronize(s));
+  });
+}
+
+}  // extern "C"

@@ -163,7 +163,8 @@ __global__ __launch_bounds__(256) void k_traverse(
   {                                                                                        \
     const uint32_t dim_ = (LEVEL) == 1 ? 8u : 4u;                                          \
     const uint32_t csu_ = frame_cell_size(LEVEL);                                          \
-    const float cs_ = float(csu_), ics_ = 1.0f / float(csu_);                              \
+    const float cs_ = float(csu_);                                                         \
+    const float ics_ = __uint_as_float((127u - (31u - __clz(csu_))) << 23); /* 2^-e */       \
     axis_enter(r, 0, float(ORGX), dim_, cs_, ics_, TENTER, cx, tm0, td0);                  \
     axis_enter(r, 1, float(ORGY), dim_, cs_, ics_, TENTER, cy, tm1, td1);                  \
     axis_enter(r, 2, float(ORGZ), dim_, cs_, ics_, TENTER, cz, tm2, td2);                  \

@@ -318,9 +318,11 @@ uint64_t run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizePa
   hipLaunchKernelGGL(k_bstart, dim3((Tc + tb - 1) / tb), dim3(tb), 0, s, coff, ccount, Tc, chunk,
                      nchunks, bstart);
   const size_t lds = (chunk + 1) * sizeof(uint32_t);
+  mark(ctx, 1);
   hipLaunchKernelGGL(k_sat, dim3(uint32_t(nchunks)), dim3(block), lds, s, rec, coff, bstart, Tc,
                      P, ppt, ctx->leafgrid.as<unsigned long long>(),
                      ctx->bitmap.as<unsigned long long>());
+  mark(ctx, 2);
   VXL_HIP(hipGetLastError());
   return P;
 }

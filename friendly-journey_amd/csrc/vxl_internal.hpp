@@ -91,6 +91,7 @@ struct vxl_ctx {
   int dda_block = 256;
   // last-call timing (ms), filled when timing is enabled
   bool timing = false;
+  hipEvent_t ev[8] = {};  // ev[0] call start, ev[1]/ev[2] around the SAT kernel, ev[3] end
   float last_ms[8] = {0};
 };
 
@@ -107,6 +108,11 @@ struct vxl_tree {
 };
 
 namespace vxl {
+
+// Records timing event i on the context stream when timing is enabled.
+inline void mark(vxl_ctx *ctx, int i) {
+  if (ctx->timing) (void)hipEventRecord(ctx->ev[i], ctx->stream);
+}
 
 // ---------------------------------------------------------------------------
 // host-side helpers

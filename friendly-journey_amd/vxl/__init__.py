@@ -1,0 +1,329 @@
+"""Python binding of libvxl.so, the MI355X voxelize + sparse-tree + DDA path.
+
+This is a thin ctypes layer over the C ABI in include/vxl.h. It mirrors the
+reference's GPU boundary:
+
+* ``GpuVoxelizer.voxelize_surface`` (voxelizer/src/gpu/dense.rs:23-58) ->
+  :meth:`GpuVoxelizer.voxelize_surface` (dense x-major occupancy words) and
+  :meth:`GpuVoxelizer.voxelize_tree` (device-resident School-B tree);
+* ``GpuTraverser::new`` / ``traverse`` (voxel-gpu/src/traverse.rs:70,154) ->
+  :class:`GpuTraverser`.
+
+There is no CPU fallback: if libvxl.so is missing or no GPU is present the calls
+raise, they never compute on the host.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libvxl.so")
+
+STATUS = {
+    0: "VXL_OK", 1: "VXL_NO_DEVICE", 2: "VXL_INVALID_ARG", 3: "VXL_STORAGE_EXCEEDED",
+    4: "VXL_WORKGROUPS_EXCEEDED", 5: "VXL_OOM", 6: "VXL_BUSY", 7: "VXL_DEVICE_ERROR",
+    8: "VXL_NOT_REPRESENTABLE",
+}
+
+# Every symbol include/vxl.h declares.
+EXPORTS = [
+    "vxl_ctx_create", "vxl_ctx_destroy", "vxl_ctx_stream", "vxl_last_error", "vxl_status_str",
+    "vxl_voxelize_dense", "vxl_voxelize_dense_dev", "vxl_voxelize_tree", "vxl_voxelize_tree_dev",
+    "vxl_tree_from_words", "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload",
+    "vxl_tree_free", "vxl_traverse", "vxl_traverse_dev", "vxl_set_timing", "vxl_last_timing",
+    "vxl_set_tuning",
+]
+
+
+class VxlError(RuntimeError):
+    """A non-OK vxl_status, with the library's message (the Rust error enums' Display)."""
+
+    def __init__(self, status: int, message: str):
+        self.status = status
+        self.name = STATUS.get(status, str(status))
+        super().__init__(f"{self.name}: {message}")
+
+
+class Grid(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("origin", C.c_float * 3), ("voxel_size", C.c_float)]
+
+
+class TreeInfo(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("k", C.c_uint32), ("node_count", C.c_uint64),
+                ("leaf_count", C.c_uint64), ("level_nodes", C.c_uint64 * 16),
+                ("device_bytes", C.c_uint64)]
+
+
+_lib = None
+_vp = C.c_void_p
+_u64 = C.c_uint64
+_pu64 = C.POINTER(C.c_uint64)
+
+
+def load(path: str = LIB_PATH):
+    """Loads libvxl.so; raises FileNotFoundError if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is missing: run __graft_entry__.build()")
+    L = C.CDLL(path)
+    L.vxl_ctx_create.argtypes = [C.c_int, C.POINTER(_vp)]
+    L.vxl_ctx_destroy.argtypes = [_vp]
+    L.vxl_ctx_destroy.restype = None
+    L.vxl_ctx_stream.argtypes = [_vp]
+    L.vxl_ctx_stream.restype = _vp
+    L.vxl_last_error.restype = C.c_char_p
+    L.vxl_status_str.restype = C.c_char_p
+    L.vxl_status_str.argtypes = [C.c_int]
+    L.vxl_voxelize_dense.argtypes = [_vp, _vp, _u64, C.POINTER(Grid), C.c_float, _vp, _pu64]
+    L.vxl_voxelize_dense_dev.argtypes = [_vp, _vp, _u64, C.POINTER(Grid), C.c_float, _vp, _pu64,
+                                         _vp]
+    L.vxl_voxelize_tree.argtypes = [_vp, _vp, _u64, C.POINTER(Grid), C.c_float, C.POINTER(_vp),
+                                    _pu64]
+    L.vxl_voxelize_tree_dev.argtypes = [_vp, _vp, _u64, C.POINTER(Grid), C.c_float,
+                                        C.POINTER(_vp), _pu64, _vp]
+    L.vxl_tree_from_words.argtypes = [_vp, _vp, C.c_uint32, C.POINTER(_vp)]
+    L.vxl_tree_info_get.argtypes = [_vp, C.POINTER(TreeInfo)]
+    L.vxl_tree_download.argtypes = [_vp, _vp, _vp, _vp, _vp]
+    L.vxl_tree_upload.argtypes = [_vp, C.c_uint32, _vp, _u64, _vp, _vp, _u64, C.POINTER(_vp)]
+    L.vxl_tree_free.argtypes = [_vp]
+    L.vxl_tree_free.restype = None
+    L.vxl_traverse.argtypes = [_vp, _vp, _vp, _u64, _vp, _vp]
+    L.vxl_traverse_dev.argtypes = [_vp, _vp, _vp, _u64, _vp, _vp, _vp]
+    L.vxl_set_timing.argtypes = [_vp, C.c_int]
+    L.vxl_set_timing.restype = None
+    L.vxl_last_timing.argtypes = [_vp, C.POINTER(C.c_float)]
+    L.vxl_last_timing.restype = None
+    L.vxl_set_tuning.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
+    for name in ("vxl_ctx_create", "vxl_voxelize_dense", "vxl_voxelize_dense_dev",
+                 "vxl_voxelize_tree", "vxl_voxelize_tree_dev", "vxl_tree_from_words",
+                 "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload", "vxl_traverse",
+                 "vxl_traverse_dev", "vxl_set_tuning"):
+        getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise VxlError(rc, (load().vxl_last_error() or b"").decode())
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_vp)
+
+
+def make_grid(n: int, origin=(0.0, 0.0, 0.0), voxel_size: float = 1.0) -> Grid:
+    g = Grid()
+    g.n = n
+    for i in range(3):
+        g.origin[i] = origin[i]
+    g.voxel_size = voxel_size
+    return g
+
+
+class Tree:
+    """A device-resident School-B tree (nodes, Morton-sorted leaves, leaf bounds)."""
+
+    def __init__(self, handle: int, ctx: "Context"):
+        self._h = handle
+        self.ctx = ctx
+        info = TreeInfo()
+        _check(load().vxl_tree_info_get(_vp(self._h), C.byref(info)))
+        self.n = info.n
+        self.k = info.k
+        self.node_count = int(info.node_count)
+        self.leaf_count = int(info.leaf_count)
+        self.level_nodes = {L: int(info.level_nodes[L]) for L in range(16) if info.level_nodes[L]}
+        self.device_bytes = int(info.device_bytes)
+
+    @property
+    def handle(self):
+        return _vp(self._h)
+
+    def download(self):
+        """(nodes [N,3] u32, leaves [L,8] u64, bounds [L] u32, codes [L] u64)."""
+        nodes = np.zeros((self.node_count, 3), np.uint32)
+        leaves = np.zeros((self.leaf_count, 8), np.uint64)
+        bounds = np.zeros(self.leaf_count, np.uint32)
+        codes = np.zeros(self.leaf_count, np.uint64)
+        _check(load().vxl_tree_download(self.handle, _ptr(nodes), _ptr(leaves), _ptr(bounds),
+                                        _ptr(codes)))
+        return nodes, leaves, bounds, codes
+
+    def free(self):
+        if self._h:
+            load().vxl_tree_free(_vp(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """A device + stream + grow-only scratch (vxl_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = _vp()
+        _check(load().vxl_ctx_create(device, C.byref(h)))
+        self._h = h.value
+
+    @property
+    def handle(self):
+        return _vp(self._h)
+
+    @property
+    def stream(self) -> int:
+        return load().vxl_ctx_stream(self.handle) or 0
+
+    def close(self):
+        if self._h:
+            load().vxl_ctx_destroy(_vp(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- voxelize ------------------------------------------------------------------------
+    def voxelize_dense(self, tris, n, origin=(0.0, 0.0, 0.0), voxel_size=1.0, eps=1e-4):
+        t = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+        words = np.zeros((n ** 3 + 31) // 32, np.uint32)
+        pairs = C.c_uint64(0)
+        g = make_grid(n, origin, voxel_size)
+        _check(load().vxl_voxelize_dense(self.handle, _ptr(t), t.shape[0], C.byref(g), eps,
+                                         _ptr(words), C.byref(pairs)))
+        return words, int(pairs.value)
+
+    def voxelize_tree(self, tris, n, origin=(0.0, 0.0, 0.0), voxel_size=1.0, eps=1e-4):
+        t = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+        out = _vp()
+        pairs = C.c_uint64(0)
+        g = make_grid(n, origin, voxel_size)
+        _check(load().vxl_voxelize_tree(self.handle, _ptr(t), t.shape[0], C.byref(g), eps,
+                                        C.byref(out), C.byref(pairs)))
+        tree = Tree(out.value, self)
+        tree.pairs = int(pairs.value)
+        return tree
+
+    def voxelize_tree_dev(self, d_tris: int, T: int, n, origin=(0.0, 0.0, 0.0), voxel_size=1.0,
+                          eps=1e-4, stream: int = 0):
+        out = _vp()
+        pairs = C.c_uint64(0)
+        g = make_grid(n, origin, voxel_size)
+        _check(load().vxl_voxelize_tree_dev(self.handle, _vp(d_tris), T, C.byref(g), eps,
+                                            C.byref(out), C.byref(pairs), _vp(stream)))
+        tree = Tree(out.value, self)
+        tree.pairs = int(pairs.value)
+        return tree
+
+    def tree_from_words(self, words, n):
+        w = np.ascontiguousarray(words, np.uint32)
+        out = _vp()
+        _check(load().vxl_tree_from_words(self.handle, _ptr(w), n, C.byref(out)))
+        return Tree(out.value, self)
+
+    def tree_upload(self, n, nodes, leaves, bounds):
+        nd = np.ascontiguousarray(nodes, np.uint32).reshape(-1, 3)
+        lv = np.ascontiguousarray(leaves, np.uint64).reshape(-1, 8)
+        bd = np.ascontiguousarray(bounds, np.uint32)
+        out = _vp()
+        _check(load().vxl_tree_upload(self.handle, n, _ptr(nd), nd.shape[0], _ptr(lv), _ptr(bd),
+                                      lv.shape[0], C.byref(out)))
+        return Tree(out.value, self)
+
+    # -- traversal -----------------------------------------------------------------------
+    def traverse(self, tree: Tree, rays, want_t=True):
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        hits = np.zeros((r.shape[0], 4), np.uint32)
+        t = np.zeros(r.shape[0], np.float64) if want_t else None
+        _check(load().vxl_traverse(self.handle, tree.handle, _ptr(r), r.shape[0], _ptr(hits),
+                                   _ptr(t) if want_t else None))
+        return (hits, t) if want_t else hits
+
+    def traverse_dev(self, tree: Tree, d_rays: int, R: int, d_hits: int, d_t: int = 0,
+                     stream: int = 0):
+        _check(load().vxl_traverse_dev(self.handle, tree.handle, _vp(d_rays), R, _vp(d_hits),
+                                       _vp(d_t) if d_t else None, _vp(stream)))
+
+    # -- timing / tuning -----------------------------------------------------------------
+    def set_timing(self, on: bool):
+        load().vxl_set_timing(self.handle, 1 if on else 0)
+
+    def last_timing(self):
+        out = (C.c_float * 8)()
+        load().vxl_last_timing(self.handle, out)
+        return list(out)
+
+    def set_tuning(self, sat_block=0, sat_pairs_per_thread=0, dda_block=0):
+        _check(load().vxl_set_tuning(self.handle, sat_block, sat_pairs_per_thread, dda_block))
+
+
+# ---- reference-shaped API (voxelizer::GpuVoxelizer, voxel_gpu::GpuTraverser) --------------
+@dataclass
+class VoxelizeOpts:
+    """voxelizer/src/core.rs:465-482 (owner/colour channels are not on this path)."""
+    epsilon: float = 1e-4
+
+
+@dataclass
+class VoxelGrid:
+    """voxelizer/src/core.rs:25-35."""
+    n: int
+    origin_world: tuple = (0.0, 0.0, 0.0)
+    voxel_size: float = 1.0
+
+
+@dataclass
+class VoxelizationOutput:
+    """voxelizer/src/core.rs:519-528: occupancy words + dispatch counters."""
+    occupancy: np.ndarray
+    triangles: int
+    voxels: int
+    pairs: int = 0
+
+
+@dataclass
+class GpuVoxelizer:
+    ctx: Context = field(default_factory=Context)
+
+    def voxelize_surface(self, triangles, grid: VoxelGrid, opts: VoxelizeOpts = VoxelizeOpts()):
+        words, pairs = self.ctx.voxelize_dense(triangles, grid.n, grid.origin_world,
+                                               grid.voxel_size, opts.epsilon)
+        t = np.asarray(triangles).reshape(-1, 9).shape[0]
+        return VoxelizationOutput(words, t, grid.n ** 3, pairs)
+
+    def voxelize_tree(self, triangles, grid: VoxelGrid, opts: VoxelizeOpts = VoxelizeOpts()):
+        return self.ctx.voxelize_tree(triangles, grid.n, grid.origin_world, grid.voxel_size,
+                                      opts.epsilon)
+
+
+class GpuTraverser:
+    """Uploads a School-B structure once, then traverses batches of rays."""
+
+    def __init__(self, ctx: Context, structure):
+        self.ctx = ctx
+        if isinstance(structure, Tree):
+            self.tree = structure
+        else:
+            n, nodes, leaves, bounds = structure
+            self.tree = ctx.tree_upload(n, nodes, leaves, bounds)
+
+    def traverse(self, rays):
+        """List of (x, y, z) or None per ray, like Vec<Option<VoxelCoord>>."""
+        hits = self.ctx.traverse(self.tree, rays, want_t=False)
+        return [tuple(map(int, h[:3])) if h[3] == 1 else None for h in hits]
+
+
+__all__ = ["Context", "Tree", "Grid", "VxlError", "load", "make_grid", "EXPORTS", "GpuVoxelizer",
+           "GpuTraverser", "VoxelGrid", "VoxelizeOpts", "VoxelizationOutput", "LIB_PATH"]

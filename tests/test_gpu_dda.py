@@ -1,0 +1,96 @@
+"""GPU parity: traversal hits are identical to the f32 mirror (mirror.rs) and t agrees
+with the f64 reference traversal (layout.rs) within 1e-6 relative on matching voxels."""
+import numpy as np
+import oracle as O
+import pytest
+from helpers import kats, single_voxel_school_b
+from vxl import workloads as W
+
+pytestmark = pytest.mark.gpu
+T_RTOL = 1e-6  # north-star tolerance on the hit distance t
+
+
+def _say(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def _check(ctx, gtree, otree, rays, label, min_hits=0):
+    rays = np.ascontiguousarray(rays, np.float32)
+    _say(f"{label}: {len(rays)} rays")
+    hits, t = ctx.traverse(gtree, rays)
+    want = otree.mirror(rays)
+    bad = np.any(hits != want, axis=1)
+    assert not bad.any(), f"{label}: {int(bad.sum())} hit mismatches, first {np.nonzero(bad)[0][:5]}"
+    assert int(hits[:, 3].sum()) >= min_hits
+    ref_h, ref_t = otree.layout(rays.astype(np.float64))
+    same = np.all(ref_h == hits, axis=1) & (hits[:, 3] == 1)
+    err = np.abs(t[same] - ref_t[same]) / np.maximum(np.abs(ref_t[same]), 1e-30)
+    zero = ref_t[same] == 0
+    assert np.all(t[same][zero] == 0)
+    assert err[~zero].max(initial=0) <= T_RTOL, f"{label}: max rel t error {err.max()}"
+    assert np.all(t[hits[:, 3] == 0] == 0)
+    return hits
+
+
+FIXTURES = [("octant", 128, (0b0110_1001,)), ("checkerboard", 128, ()), ("dust", 128, (4096,)),
+            ("wire", 128, (16, 1)), ("octant", 32, (0b1000_0001,)), ("solid", 32, ()),
+            ("single", 8, (5, 2, 2)), ("empty", 32, ())]
+
+
+@pytest.mark.parametrize("fx", FIXTURES, ids=lambda f: f"{f[0]}{f[1]}")
+def test_fixture_battery(ctx, fx):
+    """voxel-gpu/tests/differential.rs:202-266 battery (origins U[-4,n+4), dirs U[-1,1))."""
+    kind, n, params = fx
+    words = O.fixture_words(kind, n, *params)
+    ot = O.Tree.build(words, n)
+    gt = ctx.tree_from_words(words, n)
+    _check(ctx, gt, ot, W.box_rays(20000, n), f"{kind}{n}")
+
+
+def test_grazing_and_axis_aligned_rays(ctx):
+    """voxel-gpu/tests/differential.rs:97-164: rays aimed at occupied voxels, jittered to
+    graze, along axes and diagonals -- zero mismatches allowed."""
+    n = 128
+    words = O.fixture_words("dust", n, 4096)
+    ot = O.Tree.build(words, n)
+    gt = ctx.tree_from_words(words, n)
+    rng = np.random.default_rng(0x9E37)
+    occ = np.array(sorted(__import__("helpers").words_to_set(words, n)))
+    targets = occ[rng.choice(len(occ), 40, replace=False)]
+    axes = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 0], [1, 0, 1], [0, 1, 1], [1, 1, 1]],
+                    np.float64)
+    rays = []
+    for tgt in targets:
+        for d in axes:
+            aim = tgt + 0.5 + rng.uniform(-0.8, 0.8, (16, 3))
+            rays.append(np.concatenate([aim - d * 50.0, np.broadcast_to(d, aim.shape)], 1))
+    _check(ctx, gt, ot, np.concatenate(rays), "grazing", min_hits=100)
+
+
+def test_reference_grazing_cases_2048(ctx):
+    g = kats()["dda"]["grazing_cases"]
+    for v, o, d in g["cases"]:
+        nodes, leaves, bounds = single_voxel_school_b(g["n"], v)
+        ot = O.Tree.from_school_b(g["n"], nodes, leaves, bounds)
+        gt = ctx.tree_upload(g["n"], nodes, leaves, bounds)
+        _check(ctx, gt, ot, np.array([o + d]), f"grazing {v}")
+
+
+def test_sphere_tree_camera_and_random_rays(ctx):
+    tris = W.uv_sphere()
+    n, vs = W.grid_for(256)
+    words, _, _ = O.voxelize_surface(tris, n, (0, 0, 0), vs, 1e-4)
+    ot = O.Tree.build(words, n)
+    gt = ctx.voxelize_tree(tris, n, (0, 0, 0), vs, 1e-4)
+    cam = W.camera_rays(256, 480, 270)
+    _check(ctx, gt, ot, cam, "sphere camera", min_hits=1000)
+    _check(ctx, gt, ot, W.random_rays(100_000, 256), "sphere random", min_hits=1000)
+
+
+def test_traverse_empty_batch_and_empty_tree(ctx):
+    _say("empty batch / empty tree")
+    e = ctx.tree_from_words(O.fixture_words("empty", 32), 32)
+    hits, t = ctx.traverse(e, W.random_rays(1000, 32))
+    assert not hits.any() and not t.any()
+    hits, t = ctx.traverse(e, np.zeros((0, 6), np.float32))
+    assert hits.shape == (0, 4)

@@ -1,0 +1,131 @@
+"""GPU parity: dense occupancy and School-B trees from the HIP path are bit-exact with
+the oracle (voxelize_surface_cpu -> SparseTree::build -> SchoolBBuffer::from_sparse)."""
+import numpy as np
+import oracle as O
+import pytest
+import vxl
+from helpers import kats
+from vxl import workloads as W
+
+pytestmark = pytest.mark.gpu
+M = kats()["voxelize_meshes"]
+
+
+def _say(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def _cases():
+    rng = np.random.default_rng(123)
+    out = [
+        ("test_mesh_32", np.array(M["test_mesh_32"], np.float32), 32, (0, 0, 0), 1.0, 1e-4),
+        ("hand_8", np.array([M["hand_triangle"]["tri"]], np.float32), 8, (0, 0, 0), 1.0, 1e-4),
+        ("bruteforce_a_8", np.array(M["bruteforce_meshes"][0], np.float32), 8, (0, 0, 0), 1.0, 1e-4),
+        ("bruteforce_b_8", np.array(M["bruteforce_meshes"][1], np.float32), 8, (0, 0, 0), 1.0, 1e-4),
+        ("eps_0.5", np.array(M["eps_invariance_mesh"], np.float32), 8, (0, 0, 0), 1.0, 0.5),
+        ("eps_0", np.array(M["eps_invariance_mesh"], np.float32), 8, (0, 0, 0), 1.0, 0.0),
+        ("outside", np.array(M["outside_triangles"], np.float32), 8, (0, 0, 0), 1.0, 1e-4),
+        ("sphere_128", W.uv_sphere(32, 16), 128, (0, 0, 0), 1 / 128, 1e-4),
+        ("soup_128", W.soup(3000, seed=77, mu=np.log(4 / 128)), 128, (0, 0, 0), 1 / 128, 1e-4),
+        ("offset_origin_32", (rng.uniform(-0.3, 1.3, (400, 9))).astype(np.float32), 32,
+         (-0.25, 0.125, 0.3), 1 / 24.0, 1e-4),
+        ("grid_space_edges_32", np.round(rng.uniform(-2, 34, (300, 9)) * 2) / 2, 32, (0, 0, 0), 1.0,
+         1e-4),
+        ("huge_triangle_32", np.array([[-50, -40, 16.2, 90, -30, 15.7, 10, 80, 16.9]], np.float32), 32,
+         (0, 0, 0), 1.0, 1e-4),
+    ]
+    return out
+
+
+@pytest.mark.parametrize("case", _cases(), ids=lambda c: c[0])
+def test_dense_occupancy_bit_exact(ctx, case):
+    name, tris, n, origin, vs, eps = case
+    tris = np.asarray(tris, np.float32)
+    _say(f"dense {name} n={n} T={len(tris)}")
+    want, _, pairs = O.voxelize_surface(tris, n, origin, vs, eps)
+    got, gpairs = ctx.voxelize_dense(tris, n, origin, vs, eps)
+    assert gpairs == pairs
+    assert np.array_equal(got, want), f"{int((got != want).sum())} words differ"
+
+
+def _tree_equal(gt, ot):
+    gn, gl, gb, gc = gt.download()
+    on, ol, ob, oc = ot.export()
+    assert gt.leaf_count == ot.leaf_count and gt.node_count == ot.node_count
+    assert np.array_equal(gl, ol), "leaf words differ"
+    assert np.array_equal(gc, oc), "leaf codes differ"
+    assert np.array_equal(gb, ob), "leaf bounds differ"
+    assert np.array_equal(gn, on), "School-B nodes differ"
+
+
+@pytest.mark.parametrize("case", _cases(), ids=lambda c: c[0])
+def test_tree_bit_exact(ctx, case):
+    name, tris, n, origin, vs, eps = case
+    tris = np.asarray(tris, np.float32)
+    _say(f"tree {name} n={n} T={len(tris)}")
+    words, _, _ = O.voxelize_surface(tris, n, origin, vs, eps)
+    ot = O.Tree.build(words, n)
+    gt = ctx.voxelize_tree(tris, n, origin, vs, eps)
+    _tree_equal(gt, ot)
+
+
+def test_sphere_config_1_at_256_bit_exact(ctx):
+    """The 16,128-triangle seeded sphere at 256^3, embedded in Resolution(512)."""
+    tris = W.uv_sphere()
+    n, vs = W.grid_for(256)
+    _say(f"sphere C1 n={n}")
+    words, _, _ = O.voxelize_surface(tris, n, (0, 0, 0), vs, 1e-4)
+    got, _ = ctx.voxelize_dense(tris, n, (0, 0, 0), vs, 1e-4)
+    assert np.array_equal(got, words)
+    _tree_equal(ctx.voxelize_tree(tris, n, (0, 0, 0), vs, 1e-4), O.Tree.build(words, n))
+
+
+def test_empty_mesh_and_k0(ctx):
+    _say("empty mesh / k=0")
+    t = ctx.voxelize_tree(np.zeros((0, 9), np.float32), 32)
+    assert (t.leaf_count, t.node_count) == (0, 0)
+    words, _ = ctx.voxelize_dense(np.zeros((0, 9), np.float32), 32)
+    assert not words.any()
+    tri = np.array([[1.5, 1.5, 1.5, 5.5, 1.5, 1.5, 1.5, 5.5, 1.5]], np.float32)
+    w, _, _ = O.voxelize_surface(tri, 8)
+    _tree_equal(ctx.voxelize_tree(tri, 8), O.Tree.build(w, 8))
+
+
+def test_validation_errors(ctx):
+    _say("validation")
+    tri = np.zeros((1, 9), np.float32)
+    with pytest.raises(vxl.VxlError) as e:
+        ctx.voxelize_tree(tri, 1024)
+    assert e.value.name == "VXL_NOT_REPRESENTABLE"
+    with pytest.raises(vxl.VxlError) as e:
+        ctx.voxelize_tree(tri, 32, voxel_size=0.0)
+    assert e.value.name == "VXL_INVALID_ARG"
+    with pytest.raises(vxl.VxlError) as e:
+        ctx.voxelize_tree(tri, 32, eps=-1.0)
+    assert e.value.name == "VXL_INVALID_ARG"
+    bad = tri.copy()
+    bad[0, 4] = np.inf
+    with pytest.raises(vxl.VxlError) as e:
+        ctx.voxelize_tree(bad, 32)
+    assert e.value.name == "VXL_INVALID_ARG"
+
+
+def test_repeated_calls_leave_scratch_clean(ctx):
+    """The dense leaf grid is cleared by each build: calls at different n agree with fresh
+    oracle results, in any order."""
+    a = W.soup(2000, seed=5, mu=np.log(3 / 128))
+    b = W.uv_sphere(40, 20)
+    for n, tris in ((128, a), (32, b), (128, b), (512, a), (128, a)):
+        vs = 1.0 / min(n, 128)
+        _say(f"repeat n={n}")
+        words, _, _ = O.voxelize_surface(tris, n, (0, 0, 0), vs, 1e-4)
+        _tree_equal(ctx.voxelize_tree(tris, n, (0, 0, 0), vs, 1e-4), O.Tree.build(words, n))
+
+
+def test_tree_from_words_fixtures(ctx):
+    for kind, n, params in [("checkerboard", 32, ()), ("octant", 128, (0b0110_1001,)),
+                            ("dust", 128, (4096,)), ("wire", 128, (16, 1)), ("single", 8, (5, 2, 2)),
+                            ("solid", 32, ()), ("empty", 128, ())]:
+        _say(f"from_words {kind} n={n}")
+        words = O.fixture_words(kind, n, *params)
+        _tree_equal(ctx.tree_from_words(words, n), O.Tree.build(words, n))

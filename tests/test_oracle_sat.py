@@ -1,0 +1,152 @@
+"""Pins the oracle's SAT voxelizer (reference_cpu.rs:27-200) against the reference's
+analytic and brute-force tests and against an independent f64 polygon-clipping test."""
+import numpy as np
+import oracle as O
+import pytest
+from helpers import kats, overlaps_with_margin, words_to_set
+
+HALF = (0.5, 0.5, 0.5)
+M = kats()["voxelize_meshes"]
+
+
+def test_sat_analytic_cases():
+    for c in kats()["sat_analytic"]["cases"]:
+        assert O.triangle_box_overlap(c["center"], HALF, *c["tri"]) == c["overlap"]
+
+
+def test_sat_agrees_with_polygon_clipping_off_tangents():
+    """A wrong sign, axis or transposed operand in the 13-axis SAT makes it disagree with
+    clipping on many random cases; only near-tangent cases (within delta) may differ."""
+    rng = np.random.default_rng(11)
+    delta = 1e-3
+    checked = hits = 0
+    for _ in range(6000):
+        tri = rng.uniform(-2.0, 2.0, (3, 3)).astype(np.float32)
+        if rng.random() < 0.3:  # small triangles near the box
+            tri = (rng.uniform(-0.8, 0.8, 3) + rng.uniform(-0.6, 0.6, (3, 3))).astype(np.float32)
+        c = rng.uniform(-1.0, 1.0, 3).astype(np.float32)
+        inner, outer = overlaps_with_margin(tri.astype(np.float64), c.astype(np.float64), 0.5, delta)
+        got = O.triangle_box_overlap(c, HALF, tri[0], tri[1], tri[2])
+        if inner:
+            assert got, (tri, c)
+        if not outer:
+            assert not got, (tri, c)
+        checked += inner or not outer
+        hits += got
+    assert checked > 5000 and 500 < hits < 5500
+
+
+def test_cpu_voxelizes_single_triangle():
+    words, _, _ = O.voxelize_surface(np.array(M["single_triangle"]), 8)
+    assert words.any()
+
+
+def test_epsilon_does_not_change_occupancy():
+    mesh = np.array(M["eps_invariance_mesh"], np.float32)
+    ref = None
+    for eps in M["eps_values"]:
+        w, _, _ = O.voxelize_surface(mesh, 8, eps=eps)
+        if ref is None:
+            ref = w
+        assert np.array_equal(ref, w), eps
+
+
+def _bruteforce(mesh, n):
+    out = set()
+    for z in range(n):
+        for y in range(n):
+            for x in range(n):
+                c = (x + 0.5, y + 0.5, z + 0.5)
+                if any(O.triangle_box_overlap(c, HALF, t[0:3], t[3:6], t[6:9]) for t in mesh):
+                    out.add((x, y, z))
+    return out
+
+
+def test_occupancy_equals_bruteforce_sat():
+    for mesh in M["bruteforce_meshes"]:
+        mesh = np.array(mesh, np.float32)
+        w, _, _ = O.voxelize_surface(mesh, 8)
+        assert words_to_set(w, 8) == _bruteforce(mesh, 8)
+
+
+def test_owner_is_lowest_triangle_index():
+    tri = np.array(M["owner_triangle"], np.float32)
+    mesh = np.stack([tri, tri, tri])
+    w, owner, _ = O.voxelize_surface(mesh, 8, owner=True)
+    covered = words_to_set(w, 8)
+    assert covered
+    for lin in range(512):
+        v = (lin % 8, (lin // 8) % 8, lin // 64)
+        assert owner[lin] == (0 if v in covered else 0xFFFFFFFF)
+
+
+def test_fully_outside_triangles_mark_nothing():
+    for tri in M["outside_triangles"]:
+        w, _, pairs = O.voxelize_surface(np.array([tri], np.float32), 8)
+        assert not w.any() and pairs == 0
+
+
+def test_hand_triangle_marks_expected_voxels():
+    h = M["hand_triangle"]
+    w, _, _ = O.voxelize_surface(np.array([h["tri"]], np.float32), 8)
+    s = words_to_set(w, 8)
+    for v in h["occupied"]:
+        assert tuple(v) in s
+    for v in h["empty"]:
+        assert tuple(v) not in s
+
+
+def test_occupancy_is_order_invariant():
+    mesh = np.array(M["test_mesh_32"], np.float32)
+    a, _, _ = O.voxelize_surface(mesh, 32)
+    b, _, _ = O.voxelize_surface(mesh[::-1].copy(), 32)
+    assert a.any() and np.array_equal(a, b)
+
+
+def test_power_of_two_grid_transform_is_exact():
+    """With origin 0 and voxel size 2^-j the world->grid map is an exact scaling, so
+    voxelizing world-space triangles equals voxelizing the pre-scaled ones at size 1."""
+    rng = np.random.default_rng(5)
+    tris = rng.uniform(0.05, 0.95, (50, 9)).astype(np.float32)
+    for m in (8, 32, 128):
+        a, _, pa = O.voxelize_surface(tris, m, voxel_size=1.0 / m)
+        b, _, pb = O.voxelize_surface(tris * np.float32(m), m)
+        assert np.array_equal(a, b) and pa == pb
+    g = O.world_to_grid([0.3, 0.7, 0.1], [0, 0, 0], 1.0 / 64)
+    assert g.tolist() == [np.float32(0.3) * 64, np.float32(0.7) * 64, np.float32(0.1) * 64]
+
+
+def test_marked_voxels_overlap_and_candidates_count():
+    rng = np.random.default_rng(6)
+    for _ in range(60):
+        tri = rng.uniform(0.5, 7.5, (1, 9)).astype(np.float32)
+        w, _, pairs = O.voxelize_surface(tri, 8)
+        for v in words_to_set(w, 8):
+            c = (v[0] + 0.5, v[1] + 0.5, v[2] + 0.5)
+            assert O.triangle_box_overlap(c, HALF, tri[0, :3], tri[0, 3:6], tri[0, 6:])
+        g = tri.reshape(3, 3)
+        lo = np.maximum(np.floor(g.min(0) - np.float32(1e-4)), 0)
+        hi = np.minimum(np.floor(g.max(0) + np.float32(1e-4)), 7)
+        assert pairs == int(np.prod(hi - lo + 1)) == O.count_candidates(tri, 8)
+
+
+@pytest.mark.parametrize("n", [32, 128])
+def test_voxelize_matches_bruteforce_on_random_soup(n):
+    """The AABB scan never misses a voxel the SAT accepts (scan = exhaustive test),
+    checked per triangle on its padded box grown by 2 voxels."""
+    rng = np.random.default_rng(n)
+    centres = rng.uniform(3, n - 3, (40, 1, 3))
+    tris = (centres + rng.uniform(-2, 2, (40, 3, 3))).reshape(40, 9).astype(np.float32)
+    w, _, _ = O.voxelize_surface(tris, n)
+    got = words_to_set(w, n)
+    want = set()
+    for t in tris:
+        g = t.reshape(3, 3)
+        lo = np.maximum(np.floor(g.min(0)) - 2, 0).astype(int)
+        hi = np.minimum(np.floor(g.max(0)) + 2, n - 1).astype(int)
+        for z in range(lo[2], hi[2] + 1):
+            for y in range(lo[1], hi[1] + 1):
+                for x in range(lo[0], hi[0] + 1):
+                    if O.triangle_box_overlap((x + .5, y + .5, z + .5), HALF, g[0], g[1], g[2]):
+                        want.add((x, y, z))
+    assert got == want
