@@ -223,6 +223,13 @@ void vxl_ctx_destroy(vxl_ctx *ctx) {
                          &ctx->leafgrid, &ctx->bitmap,   &ctx->levelscratch, &ctx->scan_tmp,
                          &ctx->words_io, &ctx->rays_io,  &ctx->hits_io,  &ctx->t_io};
   for (auto *b : bufs) b->release();
+  // Trees outlive nothing they depend on: release their device memory now and detach
+  // them; vxl_tree_free on a detached tree only frees the handle.
+  for (vxl_tree *t : ctx->trees) {
+    vxl::tree_free_device(t);
+    t->ctx = nullptr;
+  }
+  ctx->trees.clear();
   for (auto &e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
@@ -279,6 +286,7 @@ vxl_status vxl_voxelize_tree_dev(vxl_ctx *ctx, const float *d_tris, uint64_t T,
       VXL_HIP(hipEventDestroy(ev));
     }
     *out = voxelize_tree_impl(ctx, d_tris, T, grid, k, epsilon, pairs_out);
+    ctx->trees.insert(*out);
     if (user && user != ctx->stream) {
       hipEvent_t ev;
       VXL_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
@@ -305,6 +313,7 @@ vxl_status vxl_voxelize_tree(vxl_ctx *ctx, const float *tris, uint64_t T, const 
       d = ctx->tris_in.as<float>();
     }
     *out = voxelize_tree_impl(ctx, d, T, grid, k, epsilon, pairs_out);
+    ctx->trees.insert(*out);
     VXL_HIP(hipStreamSynchronize(ctx->stream));
   });
 }
@@ -379,6 +388,7 @@ vxl_status vxl_tree_from_words(vxl_ctx *ctx, const uint32_t *words, uint32_t n, 
     vxl::prepare_grid(ctx, n);
     vxl::words_to_grid(ctx, n, ctx->words_io.as<uint32_t>());
     *out = vxl::build_tree_from_grid(ctx, n, k);
+    ctx->trees.insert(*out);
     VXL_HIP(hipStreamSynchronize(ctx->stream));
   });
 }
@@ -457,6 +467,7 @@ vxl_status vxl_tree_upload(vxl_ctx *ctx, uint32_t n, const vxl_node *nodes, uint
       throw;
     }
     t->bytes = node_count * 12 + leaf_count * 76;
+    ctx->trees.insert(t);
     *out = t;
   });
 }
@@ -468,8 +479,7 @@ void vxl_tree_free(vxl_tree *tree) {
     (void)hipSetDevice(tree->ctx->device);
     (void)hipStreamSynchronize(tree->ctx->stream);
     vxl::tree_free_device(tree);
-  } else {
-    vxl::tree_free_device(tree);
+    tree->ctx->trees.erase(tree);
   }
   delete tree;
 }

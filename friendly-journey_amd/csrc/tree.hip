@@ -290,12 +290,9 @@ void prepare_grid(vxl_ctx *ctx, uint32_t n) {
   const uint64_t bricks = nb * nb * nb;
   const size_t grid_bytes = size_t(bricks) * 64;
   const size_t w1 = bitmap_level1_words(n);
-  // bitmap area: B1 followed by the upper levels (each 1/64 of the previous)
-  size_t words = 0;
-  for (size_t w = w1;; w = div_up(w, 64)) {
-    words += w;
-    if (w <= 1) break;
-  }
+  // ctx->bitmap holds B1 only (all-zero between calls); the upper levels live in
+  // levelscratch and are fully rewritten by every build.
+  const size_t words = w1;
   bool fresh = false;
   if (ctx->leafgrid.bytes < grid_bytes) {
     ctx->leafgrid.reserve(grid_bytes);
@@ -358,25 +355,24 @@ vxl_tree *build_tree_from_grid(vxl_ctx *ctx, uint32_t n, uint32_t k) {
   hipStream_t s = ctx->stream;
   const uint32_t top = k + 1 > 2 ? k + 1 : 2;  // node levels 2..top (level 2 always built)
   Level lv[16];
-  // bitmaps: B1 at the start of ctx->bitmap, then B2, B3, ...
-  unsigned long long *bm = ctx->bitmap.as<unsigned long long>();
+  // B1 is ctx->bitmap (zero between calls); B2, B3, ... are carved from levelscratch.
   uint64_t w = bitmap_level1_words(n);
-  unsigned long long *cur = bm;
   size_t scratch = 0;
   for (uint32_t L = 2; L <= top; L++) {
     lv[L].win = w;
-    lv[L].bin = cur;
-    uint64_t wn = div_up(w, 64);
-    if (L < top) {
-      lv[L].bout = cur + w;
-      cur = cur + w;
-    }
-    w = wn;
+    w = div_up(w, 64);
     const uint64_t e = lv[L].win + 1;
-    scratch += 10 * (((e * 8) + 255) / 256) * 256;
+    scratch += 10 * (((e * 8) + 255) / 256) * 256 + ((w * 8 + 255) / 256) * 256;
   }
   ctx->levelscratch.reserve(scratch + 4096);
   char *p = static_cast<char *>(ctx->levelscratch.ptr);
+  lv[2].bin = ctx->bitmap.as<unsigned long long>();
+  for (uint32_t L = 2; L <= top; L++) {
+    if (L < top) {
+      lv[L].bout = carve<unsigned long long>(p, lv[L + 1].win);
+      lv[L + 1].bin = lv[L].bout;
+    }
+  }
   for (uint32_t L = 2; L <= top; L++) {
     const uint64_t e = lv[L].win + 1;
     lv[L].cb = carve<uint32_t>(p, e);

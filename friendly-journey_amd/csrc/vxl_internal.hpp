@@ -20,6 +20,7 @@
 #include <cstdint>
 #include <cstring>
 #include <mutex>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -93,6 +94,7 @@ struct vxl_ctx {
   bool timing = false;
   hipEvent_t ev[8] = {};  // ev[0] call start, ev[1]/ev[2] around the SAT kernel, ev[3] end
   float last_ms[8] = {0};
+  std::set<vxl_tree *> trees;  // live trees created on this context
 };
 
 struct vxl_tree {

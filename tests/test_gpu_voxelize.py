@@ -51,7 +51,13 @@ def test_dense_occupancy_bit_exact(ctx, case):
 def _tree_equal(gt, ot):
     gn, gl, gb, gc = gt.download()
     on, ol, ob, oc = ot.export()
-    assert gt.leaf_count == ot.leaf_count and gt.node_count == ot.node_count
+    if gt.leaf_count != ot.leaf_count:
+        extra = sorted(set(gc.tolist()) - set(oc.tolist()))[:8]
+        missing = sorted(set(oc.tolist()) - set(gc.tolist()))[:8]
+        raise AssertionError(f"leaves {gt.leaf_count} vs {ot.leaf_count}; extra bricks "
+                             f"{[O.morton_decode(c) for c in extra]} missing "
+                             f"{[O.morton_decode(c) for c in missing]}")
+    assert gt.node_count == ot.node_count
     assert np.array_equal(gl, ol), "leaf words differ"
     assert np.array_equal(gc, oc), "leaf codes differ"
     assert np.array_equal(gb, ob), "leaf bounds differ"
