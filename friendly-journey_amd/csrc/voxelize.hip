@@ -30,10 +30,11 @@ namespace {
 
 struct __align__(16) TriRec {
   float g[9];         // grid-space vertices
-  int32_t mn[3];      // candidate box min (inclusive)
-  uint32_t ex;        // box extent along x
-  uint32_t exy;       // ex * ey
-  uint32_t flags;     // bit 0: literal path (non-finite grid vertex); bit 1: count >= 2^32
+  uint32_t mnxy;      // candidate box min: x | y << 16
+  uint32_t mnz_ex;    // box min z | box extent along x << 16
+  uint32_t exy;       // extent x * extent y
+  uint32_t flags;     // bit 0: literal path (non-finite grid vertex); bit 1: wide decode
+  float rcp_ex, rcp_exy;  // 1/ex, 1/exy for the narrow decode
   uint32_t pad;
 };
 static_assert(sizeof(TriRec) == 64, "TriRec is one 64-byte record");
@@ -89,6 +90,7 @@ __global__ void k_compact(const float *__restrict__ tris, uint64_t T, VoxelizePa
 #pragma unroll
   for (int i = 0; i < 9; i++) finite = finite && isfinite(r.g[i]);
   uint32_t ext[3];
+  int32_t r_mn[3];
   const float hi_clamp = (float)(p.n - 1);
 #pragma unroll
   for (int a = 0; a < 3; a++) {
@@ -96,13 +98,17 @@ __global__ void k_compact(const float *__restrict__ tris, uint64_t T, VoxelizePa
     float hi = fmaxf(fmaxf(r.g[a], r.g[3 + a]), r.g[6 + a]) + p.eps;
     int32_t mn = sat_i32(fmaxf(floorf(lo), 0.0f));
     int32_t mx = sat_i32(fminf(floorf(hi), hi_clamp));
-    r.mn[a] = mn;
+    r_mn[a] = mn;
     ext[a] = uint32_t(mx - mn + 1);
   }
   uint64_t c = count[t];
-  r.ex = ext[0];
+  r.mnxy = uint32_t(r_mn[0]) | (uint32_t(r_mn[1]) << 16);
+  r.mnz_ex = uint32_t(r_mn[2]) | (ext[0] << 16);
   r.exy = ext[0] * ext[1];
-  r.flags = (finite ? 0u : 1u) | ((c >> 32) ? 2u : 0u);
+  // narrow decode: every local index < 2^22, so the float quotient is off by at most 1
+  r.flags = (finite ? 0u : 1u) | ((c >> 22) ? 2u : 0u);
+  r.rcp_ex = 1.0f / float(ext[0]);
+  r.rcp_exy = 1.0f / float(r.exy);
   r.pad = 0;
   uint32_t ci = cidx[t];
   rec[ci] = r;
@@ -121,47 +127,63 @@ __global__ void k_bstart(const uint64_t *__restrict__ coff, const uint64_t *__re
 }
 
 // ---- the SAT predicate, reference_cpu.rs:27-85 -----------------------------------------
-// Returns true when the unit box centred at (cx,cy,cz) overlaps the triangle g.
-template <bool kLiteral>
-__device__ __forceinline__ bool sat_overlap(float cx, float cy, float cz, const float *g) {
-  const float h = 0.5f;
-  const float v0x = g[0] - cx, v0y = g[1] - cy, v0z = g[2] - cz;
-  const float v1x = g[3] - cx, v1y = g[4] - cy, v1z = g[5] - cz;
-  const float v2x = g[6] - cx, v2y = g[7] - cy, v2z = g[8] - cz;
-  const float e0x = v1x - v0x, e0y = v1y - v0y, e0z = v1z - v0z;
-  const float e1x = v2x - v1x, e1y = v2y - v1y, e1z = v2z - v1z;
-  const float e2x = v0x - v2x, e2y = v0y - v2y, e2z = v0z - v2z;
+// Split in two conjuncts evaluated in separate phases: the plane test (:76-82) and the
+// rest (box faces :66-74 and the 9 edge axes :42-64). Their conjunction is the reference
+// predicate; each conjunct uses the reference's operations in the reference's order.
+struct Rel {
+  float v0x, v0y, v0z, v1x, v1y, v1z, v2x, v2y, v2z;
+};
 
-  // plane test (:76-82): normal = e0 x e1, d = -normal.dot(v0)
-  {
-    const float nx = e0y * e1z - e1y * e0z;
-    const float ny = e0z * e1x - e1z * e0x;
-    const float nz = e0x * e1y - e1x * e0y;
-    float dd = nx * v0x;
-    dd = dd + ny * v0y;
-    dd = dd + nz * v0z;
-    const float d = -dd;
-    float r = h * fabsf(nx);
-    r = r + h * fabsf(ny);
-    r = r + h * fabsf(nz);
-    float s;
-    if (kLiteral) {
-      float z = nx * 0.0f;
-      z = z + ny * 0.0f;
-      z = z + nz * 0.0f;
-      s = z + d;
-    } else {
-      // +-0 + d == d up to the sign of zero for a finite normal; NaN otherwise.
-      const bool fin = isfinite(nx) && isfinite(ny) && isfinite(nz);
-      s = fin ? d : __builtin_nanf("");
-    }
-    if (fabsf(s) > r) return false;
+__device__ __forceinline__ Rel rel_to(const float *g, float cx, float cy, float cz) {
+  Rel r;
+  r.v0x = g[0] - cx; r.v0y = g[1] - cy; r.v0z = g[2] - cz;
+  r.v1x = g[3] - cx; r.v1y = g[4] - cy; r.v1z = g[5] - cz;
+  r.v2x = g[6] - cx; r.v2y = g[7] - cy; r.v2z = g[8] - cz;
+  return r;
+}
+
+template <bool kLiteral>
+__device__ __forceinline__ bool plane_passes(const Rel &v) {
+  const float h = 0.5f;
+  const float e0x = v.v1x - v.v0x, e0y = v.v1y - v.v0y, e0z = v.v1z - v.v0z;
+  const float e1x = v.v2x - v.v1x, e1y = v.v2y - v.v1y, e1z = v.v2z - v.v1z;
+  const float nx = e0y * e1z - e1y * e0z;
+  const float ny = e0z * e1x - e1z * e0x;
+  const float nz = e0x * e1y - e1x * e0y;
+  float dd = nx * v.v0x;
+  dd = dd + ny * v.v0y;
+  dd = dd + nz * v.v0z;
+  const float d = -dd;
+  float r = h * fabsf(nx);
+  r = r + h * fabsf(ny);
+  r = r + h * fabsf(nz);
+  float s;
+  if (kLiteral) {
+    float z = nx * 0.0f;
+    z = z + ny * 0.0f;
+    z = z + nz * 0.0f;
+    s = z + d;
+  } else {
+    // +-0 + d == d up to the sign of zero for a finite normal; NaN otherwise.
+    const bool fin = isfinite(nx) && isfinite(ny) && isfinite(nz);
+    s = fin ? d : __builtin_nanf("");
   }
+  return !(fabsf(s) > r);
+}
+
+template <bool kLiteral>
+__device__ __forceinline__ bool rest_passes(const Rel &v) {
+  const float h = 0.5f;
+  const float v0x = v.v0x, v0y = v.v0y, v0z = v.v0z;
+  const float v1x = v.v1x, v1y = v.v1y, v1z = v.v1z;
+  const float v2x = v.v2x, v2y = v.v2y, v2z = v.v2z;
   // box-face axes (:66-74)
   if (fminf(v0x, fminf(v1x, v2x)) > h || fmaxf(v0x, fmaxf(v1x, v2x)) < -h) return false;
   if (fminf(v0y, fminf(v1y, v2y)) > h || fmaxf(v0y, fmaxf(v1y, v2y)) < -h) return false;
   if (fminf(v0z, fminf(v1z, v2z)) > h || fmaxf(v0z, fmaxf(v1z, v2z)) < -h) return false;
-
+  const float e0x = v1x - v0x, e0y = v1y - v0y, e0z = v1z - v0z;
+  const float e1x = v2x - v1x, e1y = v2y - v1y, e1z = v2z - v1z;
+  const float e2x = v0x - v2x, e2y = v0y - v2y, e2z = v0z - v2z;
   // the 9 edge x box-axis tests (:42-64); axis (ax, ay, az), p_i = v_i . axis
 #define VXL_AXIS(AX, AY, AZ)                                                          \
   {                                                                                   \
@@ -203,72 +225,200 @@ __device__ __forceinline__ bool sat_overlap(float cx, float cy, float cz, const 
   return true;
 }
 
-constexpr int kMaxBlock = 1024;
+__device__ __forceinline__ unsigned long long wave_or64(unsigned long long v) {
+  uint32_t lo = uint32_t(v), hi = uint32_t(v >> 32);
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    lo |= (uint32_t)__shfl_xor((int)lo, d, 64);
+    hi |= (uint32_t)__shfl_xor((int)hi, d, 64);
+  }
+  return (uint64_t(hi) << 32) | lo;
+}
 
-// One lane per (triangle, candidate voxel) pair; `chunk` = blockDim.x * ppt pairs per block.
-__global__ void k_sat(const TriRec *__restrict__ rec, const uint64_t *__restrict__ coff,
-                      const uint32_t *__restrict__ bstart, uint64_t Tc, uint64_t P, int ppt,
-                      unsigned long long *__restrict__ leafgrid,
-                      unsigned long long *__restrict__ brickbits) {
-  extern __shared__ uint32_t s_off[];  // window of chunk-relative triangle offsets
-  __shared__ uint32_t s_meta[2];
-  const uint64_t chunk = uint64_t(blockDim.x) * ppt;
-  const uint64_t base = uint64_t(blockIdx.x) * chunk;
-  if (threadIdx.x == 0) {
-    uint32_t lo = bstart[blockIdx.x];
-    uint64_t nb = (P + chunk - 1) / chunk;
-    uint32_t hi = (blockIdx.x + 1 < nb) ? bstart[blockIdx.x + 1] : uint32_t(Tc - 1);
-    s_meta[0] = lo;
-    s_meta[1] = hi - lo + 1;
-  }
-  __syncthreads();
-  const uint32_t t_lo = s_meta[0], win = s_meta[1];
-  for (uint32_t i = threadIdx.x; i < win; i += blockDim.x) {
-    uint64_t o = coff[t_lo + i];
-    s_off[i] = o <= base ? 0u : uint32_t(o - base);
-  }
-  __syncthreads();
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
+  uint32_t lo = (uint32_t)__shfl((int)uint32_t(v), src, 64);
+  uint32_t hi = (uint32_t)__shfl((int)uint32_t(v >> 32), src, 64);
+  return (uint64_t(hi) << 32) | lo;
+}
 
-  for (int it = 0; it < ppt; it++) {
-    const uint32_t rel = uint32_t(it) * blockDim.x + threadIdx.x;
-    const uint64_t pidx = base + rel;
-    if (pidx >= P) break;
-    // largest w with s_off[w] <= rel
-    uint32_t lo = 0, hi = win - 1;
-    while (lo < hi) {
-      uint32_t mid = (lo + hi + 1) >> 1;
-      if (s_off[mid] <= rel) lo = mid; else hi = mid - 1;
-    }
-    const uint32_t t = t_lo + lo;
-    const TriRec r = rec[t];
-    const uint64_t li = pidx - coff[t];
-    uint32_t x, y, z;
-    if (__builtin_expect(r.flags & 2u, 0)) {
-      uint64_t zz = li / r.exy, rem = li - zz * r.exy;
-      z = uint32_t(zz);
-      y = uint32_t(rem / r.ex);
-      x = uint32_t(rem - uint64_t(y) * r.ex);
-    } else {
-      uint32_t l32 = uint32_t(li);
-      z = l32 / r.exy;
-      uint32_t rem = l32 - z * r.exy;
-      y = rem / r.ex;
-      x = rem - y * r.ex;
-    }
-    x += uint32_t(r.mn[0]);
-    y += uint32_t(r.mn[1]);
-    z += uint32_t(r.mn[2]);
-    const float cx = float(x) + 0.5f, cy = float(y) + 0.5f, cz = float(z) + 0.5f;
-    const bool hit = (r.flags & 1u) ? sat_overlap<true>(cx, cy, cz, r.g)
-                                    : sat_overlap<false>(cx, cy, cz, r.g);
-    if (hit) {
-      const uint32_t code = dev::brick_code(x >> 3, y >> 3, z >> 3);
-      const uint32_t m = dev::morton9(x, y, z);
-      unsigned long long old =
-          atomicOr(&leafgrid[uint64_t(code) * 8 + (m >> 6)], 1ull << (m & 63));
-      if (old == 0ull) atomicOr(&brickbits[code >> 6], 1ull << (code & 63));
+// quotient of a < 2^22 by b, from a float reciprocal, corrected to the exact value
+__device__ __forceinline__ uint32_t udiv_narrow(uint32_t a, uint32_t b, float rb) {
+  uint32_t q = uint32_t(float(a) * rb);
+  const int32_t r = int32_t(a - q * b);
+  if (r < 0) q -= 1;
+  else if (uint32_t(r) >= b) q += 1;
+  return q;
+}
+
+constexpr int kWaves = 4;    // waves per block
+constexpr int kQueue = 128;  // survivor queue entries per wave
+constexpr int kTable = 256;  // leaf-word merge table slots per wave
+constexpr int kFlushAt = 128;  // flush the table once this many slots are in use
+constexpr unsigned long long kEmpty = ~0ull;
+
+// Phase B on one survivor: box faces + edge axes. On a hit, returns the leaf-grid word
+// index (brick code * 8 + sub-brick word) and the bit.
+__device__ __forceinline__ bool finish_pair(const TriRec *__restrict__ rec, uint32_t t, uint32_t xy,
+                                            uint32_t z, unsigned long long &key,
+                                            unsigned long long &bit) {
+  const TriRec r = rec[t];
+  const uint32_t x = xy & 0xffffu, y = xy >> 16;
+  const Rel v = rel_to(r.g, float(x) + 0.5f, float(y) + 0.5f, float(z) + 0.5f);
+  const bool hit = (r.flags & 1u) ? rest_passes<true>(v) : rest_passes<false>(v);
+  const uint32_t code = dev::brick_code(x >> 3, y >> 3, z >> 3);
+  const uint32_t m = dev::morton9(x, y, z);
+  key = uint64_t(code) * 8 + (m >> 6);
+  bit = 1ull << (m & 63);
+  return hit;
+}
+
+// Merges hit bits into the wave's LDS table (open addressing, linear probing).
+// Returns the number of slots this call newly occupied (wave-uniform).
+__device__ __forceinline__ uint32_t table_insert(unsigned long long *tk, unsigned long long *tv,
+                                                 bool hit, unsigned long long key,
+                                                 unsigned long long bit) {
+  bool fresh = false;
+  if (hit) {
+    uint32_t slot = (uint32_t(key) * 0x9E3779B1u ^ uint32_t(key >> 32)) >> 24;
+    for (;;) {
+      const unsigned long long old = atomicCAS(&tk[slot], kEmpty, key);
+      if (old == kEmpty || old == key) {
+        atomicOr(&tv[slot], bit);
+        fresh = old == kEmpty;
+        break;
+      }
+      slot = (slot + 1) & (kTable - 1);
     }
   }
+  __builtin_amdgcn_wave_barrier();
+  return __popcll(__ballot(fresh));
+}
+
+// Writes the table to the leaf grid and the brick bitmap with fire-and-forget atomics,
+// then empties it.
+__device__ __forceinline__ void table_flush(unsigned long long *tk, unsigned long long *tv, int lane,
+                                            unsigned long long *__restrict__ leafgrid,
+                                            unsigned long long *__restrict__ brickbits) {
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int i = 0; i < kTable / 64; i++) {
+    const int slot = i * 64 + lane;
+    const unsigned long long key = tk[slot];
+    if (key != kEmpty) {
+      atomicOr(&leafgrid[key], tv[slot]);
+      const uint64_t code = key >> 3;
+      atomicOr(&brickbits[code >> 6], 1ull << (code & 63));
+      tk[slot] = kEmpty;
+      tv[slot] = 0ull;
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Each wave walks chunks [c0, c1) of 64 consecutive pairs. The triangles of a chunk
+// are found from the 64 triangle start offsets following the chunk's first triangle:
+// a start inside the chunk marks a bit, and a lane's triangle is the first one plus
+// the number of marked bits at or below its position. Phase A evaluates the plane test
+// and queues survivors in LDS; every 64 queued survivors run phase B as a full wave.
+// Hits are merged per leaf word in an LDS table and flushed to HBM in bulk.
+__global__ __launch_bounds__(kWaves * 64) void k_sat(
+    const TriRec *__restrict__ rec, const uint64_t *__restrict__ coff,
+    const uint32_t *__restrict__ wstart, uint64_t Tc, uint64_t P, uint64_t chunks_per_wave,
+    unsigned long long *__restrict__ leafgrid, unsigned long long *__restrict__ brickbits) {
+  __shared__ uint32_t q_t[kWaves][kQueue], q_xy[kWaves][kQueue], q_z[kWaves][kQueue];
+  __shared__ unsigned long long t_key[kWaves][kTable], t_val[kWaves][kTable];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint64_t gw = uint64_t(blockIdx.x) * kWaves + wave;
+  const uint64_t nchunks = (P + 63) / 64;
+  const uint64_t c0 = gw * chunks_per_wave;
+  if (c0 >= nchunks) return;  // wave-uniform
+  const uint64_t c1 = min(c0 + chunks_per_wave, nchunks);
+  uint32_t *qt = q_t[wave], *qxy = q_xy[wave], *qz = q_z[wave];
+  unsigned long long *tk = t_key[wave], *tv = t_val[wave];
+#pragma unroll
+  for (int i = 0; i < kTable / 64; i++) {
+    tk[i * 64 + lane] = kEmpty;
+    tv[i * 64 + lane] = 0ull;
+  }
+  __builtin_amdgcn_wave_barrier();
+  uint32_t t_first = wstart[gw];
+  uint64_t start_first = coff[t_first];
+  uint32_t qn = 0, used = 0;
+  const unsigned long long below = (1ull << lane) - 1ull;
+
+  for (uint64_t c = c0; c < c1; c++) {
+    const uint64_t base = c * 64;
+    const uint64_t p = base + uint64_t(lane);
+    const bool valid = p < P;
+    const uint64_t idx = uint64_t(t_first) + 1 + uint64_t(lane);
+    const uint64_t offn = idx <= Tc ? coff[idx] : ~0ull;  // coff[Tc] == P
+    const uint64_t rpos = offn - base;                      // >= 1
+    const unsigned long long marks = wave_or64(rpos < 64 ? (1ull << rpos) : 0ull);
+    const uint32_t k = __popcll(marks & (below << 1));      // starts at positions 1..lane
+    const uint64_t kstart = shfl_u64(offn, k ? int(k) - 1 : 0);
+    const uint32_t t = t_first + k;
+    const uint64_t start = k ? kstart : start_first;
+
+    bool surv = false;
+    uint32_t x = 0, y = 0, z = 0;
+    if (valid) {
+      const TriRec r = rec[t];
+      const uint64_t li = p - start;
+      const uint32_t ex = r.mnz_ex >> 16;
+      if (!(r.flags & 2u)) {
+        const uint32_t l32 = uint32_t(li);
+        z = udiv_narrow(l32, r.exy, r.rcp_exy);
+        const uint32_t rem = l32 - z * r.exy;
+        y = udiv_narrow(rem, ex, r.rcp_ex);
+        x = rem - y * ex;
+      } else {
+        const uint64_t zz = li / r.exy, rem = li - zz * r.exy;
+        z = uint32_t(zz);
+        y = uint32_t(rem / ex);
+        x = uint32_t(rem - uint64_t(y) * ex);
+      }
+      x += r.mnxy & 0xffffu;
+      y += r.mnxy >> 16;
+      z += r.mnz_ex & 0xffffu;
+      const Rel v = rel_to(r.g, float(x) + 0.5f, float(y) + 0.5f, float(z) + 0.5f);
+      surv = (r.flags & 1u) ? plane_passes<true>(v) : plane_passes<false>(v);
+    }
+    // next chunk's first triangle: starts at positions 1..64
+    const uint32_t adv = __popcll(__ballot(rpos <= 64));
+    if (adv) {
+      start_first = shfl_u64(offn, int(adv) - 1);
+      t_first += adv;
+    }
+    // queue survivors
+    const unsigned long long sm = __ballot(surv);
+    if (surv) {
+      const uint32_t pos = qn + __popcll(sm & below);
+      qt[pos] = t;
+      qxy[pos] = x | (y << 16);
+      qz[pos] = z;
+    }
+    qn += __popcll(sm);
+    if (qn >= 64) {
+      __builtin_amdgcn_wave_barrier();
+      qn -= 64;
+      const uint32_t e = qn + lane;
+      unsigned long long key, bit;
+      const bool hit = finish_pair(rec, qt[e], qxy[e], qz[e], key, bit);
+      used += table_insert(tk, tv, hit, key, bit);
+      if (used >= kFlushAt) {
+        table_flush(tk, tv, lane, leafgrid, brickbits);
+        used = 0;
+      }
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  {
+    unsigned long long key = 0, bit = 0;
+    bool hit = false;
+    if (uint32_t(lane) < qn) hit = finish_pair(rec, qt[lane], qxy[lane], qz[lane], key, bit);
+    table_insert(tk, tv, hit, key, bit);
+  }
+  table_flush(tk, tv, lane, leafgrid, brickbits);
 }
 
 }  // namespace
@@ -307,20 +457,22 @@ uint64_t run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizePa
   const uint64_t P = host[0];
   if (P == 0) return 0;
 
-  const int block = ctx->sat_block;
-  const int ppt = ctx->sat_pairs_per_thread;
-  const uint64_t chunk = uint64_t(block) * ppt;
-  const uint64_t nchunks = (P + chunk - 1) / chunk;
-  if (nchunks > 0xffffffffull)
-    throw Error(VXL_WORKGROUPS_EXCEEDED, "voxelize: pair chunks exceed the 32-bit grid limit");
-  ctx->bstart.reserve(nchunks * 4);
-  uint32_t *bstart = ctx->bstart.as<uint32_t>();
-  hipLaunchKernelGGL(k_bstart, dim3((Tc + tb - 1) / tb), dim3(tb), 0, s, coff, ccount, Tc, chunk,
-                     nchunks, bstart);
-  const size_t lds = (chunk + 1) * sizeof(uint32_t);
+  // Waves walk contiguous runs of 64-pair chunks; wstart[w] = triangle holding the
+  // first pair of wave w (k_bstart with a chunk of chunks_per_wave * 64 pairs).
+  const uint64_t nchunks = (P + 63) / 64;
+  const uint64_t max_waves = uint64_t(ctx->cu_count > 0 ? ctx->cu_count : 256) * ctx->sat_waves_per_cu;
+  const uint64_t cpw = (nchunks + max_waves - 1) / max_waves;
+  const uint64_t waves = (nchunks + cpw - 1) / cpw;
+  const uint64_t blocks = (waves + kWaves - 1) / kWaves;
+  if (blocks > 0xffffffffull)
+    throw Error(VXL_WORKGROUPS_EXCEEDED, "voxelize: wave count exceeds the 32-bit grid limit");
+  ctx->bstart.reserve(waves * 4 + 16);
+  uint32_t *wstart = ctx->bstart.as<uint32_t>();
+  hipLaunchKernelGGL(k_bstart, dim3((Tc + tb - 1) / tb), dim3(tb), 0, s, coff, ccount, Tc,
+                     cpw * 64, waves, wstart);
   mark(ctx, 1);
-  hipLaunchKernelGGL(k_sat, dim3(uint32_t(nchunks)), dim3(block), lds, s, rec, coff, bstart, Tc,
-                     P, ppt, ctx->leafgrid.as<unsigned long long>(),
+  hipLaunchKernelGGL(k_sat, dim3(uint32_t(blocks)), dim3(kWaves * 64), 0, s, rec, coff, wstart, Tc,
+                     P, cpw, ctx->leafgrid.as<unsigned long long>(),
                      ctx->bitmap.as<unsigned long long>());
   mark(ctx, 2);
   VXL_HIP(hipGetLastError());

@@ -89,6 +89,7 @@ struct vxl_ctx {
   // tuning
   int sat_block = 256;
   int sat_pairs_per_thread = 4;
+  int sat_waves_per_cu = 32;  // SAT waves launched per CU (each walks a run of chunks)
   int dda_block = 256;
   // last-call timing (ms), filled when timing is enabled
   bool timing = false;
