@@ -101,6 +101,10 @@ void validate_mesh_host(const float *tris, uint64_t T) {
     if (!std::isfinite(tris[i])) throw Error(VXL_INVALID_ARG, "triangle contains a non-finite vertex");
 }
 
+void validate_count(uint64_t T) {
+  if (T > 0xffffffffull) throw Error(VXL_INVALID_ARG, "triangle count must be < 2^32");
+}
+
 // VoxelizeOpts::validate (core.rs:492-500)
 void validate_eps(float eps) {
   if (!std::isfinite(eps) || eps < 0.0f)
@@ -276,6 +280,7 @@ vxl_status vxl_voxelize_tree_dev(vxl_ctx *ctx, const float *d_tris, uint64_t T,
     uint32_t k = 0;
     validate_grid(grid, &k);
     if (T && !d_tris) throw Error(VXL_INVALID_ARG, "triangles pointer is null");
+    validate_count(T);
     validate_eps(epsilon);
     hipStream_t user = static_cast<hipStream_t>(stream);
     if (user && user != ctx->stream) {
@@ -304,6 +309,7 @@ vxl_status vxl_voxelize_tree(vxl_ctx *ctx, const float *tris, uint64_t T, const 
     *out = nullptr;
     uint32_t k = 0;
     validate_grid(grid, &k);
+    validate_count(T);
     validate_mesh_host(tris, T);
     validate_eps(epsilon);
     const float *d = nullptr;
@@ -325,6 +331,7 @@ vxl_status vxl_voxelize_dense_dev(vxl_ctx *ctx, const float *d_tris, uint64_t T,
     uint32_t k = 0;
     validate_grid(grid, &k);
     if (T && !d_tris) throw Error(VXL_INVALID_ARG, "triangles pointer is null");
+    validate_count(T);
     validate_eps(epsilon);
     if (!d_occ_words) throw Error(VXL_INVALID_ARG, "occupancy output is null");
     const uint64_t nn = grid->n;
@@ -349,6 +356,7 @@ vxl_status vxl_voxelize_dense(vxl_ctx *ctx, const float *tris, uint64_t T, const
   return guarded(ctx, [&] {
     uint32_t k = 0;
     validate_grid(grid, &k);
+    validate_count(T);
     validate_mesh_host(tris, T);
     validate_eps(epsilon);
     if (!occ_words) throw Error(VXL_INVALID_ARG, "occupancy output is null");

@@ -258,6 +258,182 @@ __global__ void k_words_to_grid(uint32_t n, uint64_t ngrid, const uint32_t *__re
   if (v) atomicOr(&brickbits[code >> 6], 1ull << (code & 63));
 }
 
+// ---- fast path (n <= 2048): level 2 multi-block, upper levels in one workgroup ----------
+constexpr int kSmallThreads = 1024;
+constexpr uint64_t kSmallWords = 4096;  // largest upper-level input handled by one workgroup
+
+// Level-2 pass: packed (popcount | nonzero << 32) per B1 word, ballot-packed B2.
+__global__ __launch_bounds__(kTB) void k_level2(const unsigned long long *__restrict__ bin,
+                                                  uint64_t win, uint64_t *__restrict__ pn,
+                                                  unsigned long long *__restrict__ bout) {
+  const uint64_t j = uint64_t(blockIdx.x) * kTB + threadIdx.x;
+  const unsigned long long w = j < win ? bin[j] : 0ull;
+  if (j < win) pn[j] = uint64_t(__popcll(w)) | (uint64_t(w != 0ull) << 32);
+  const unsigned long long b = __ballot(w != 0ull);
+  if (bout && (threadIdx.x & 63) == 0 && j < win) bout[j >> 6] = b;
+}
+
+__global__ __launch_bounds__(kTB) void k_emit_a2(const unsigned long long *__restrict__ bin,
+                                                   uint64_t win, const uint64_t *__restrict__ pn,
+                                                   unsigned long long *__restrict__ a_mask,
+                                                   uint32_t *__restrict__ a_base,
+                                                   uint32_t *__restrict__ a_code) {
+  const uint64_t j = uint64_t(blockIdx.x) * kTB + threadIdx.x;
+  if (j >= win) return;
+  const unsigned long long w = bin[j];
+  if (!w) return;
+  const uint64_t v = pn[j];
+  const uint32_t i = uint32_t(v >> 32);
+  a_mask[i] = w;
+  a_base[i] = uint32_t(v);
+  a_code[i] = uint32_t(j);
+}
+
+struct LevelPtrs {
+  uint64_t win[16];
+  uint32_t *cb[16], *ni[16], *desc[16], *pos[16], *bs[16];
+  unsigned long long *a_mask[16];
+  uint32_t *a_base[16], *a_code[16];
+};
+
+// Exclusive block scan of one u32 per thread (kSmallThreads threads); returns the total.
+__device__ __forceinline__ uint32_t block_scan_u32(uint32_t v, uint32_t *excl, uint32_t *sh) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = (uint32_t)__shfl_up((int)inc, d, 64);
+    if (lane >= d) inc += o;
+  }
+  if (lane == 63) sh[wave] = inc;
+  __syncthreads();
+  uint32_t before = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < kSmallThreads / 64; w++) {
+    const uint32_t t = sh[w];
+    if (w < wave) before += t;
+    total += t;
+  }
+  *excl = before + inc - v;
+  __syncthreads();
+  return total;
+}
+
+// Levels first..top (each with at most kSmallWords input words) in one workgroup:
+// popcounts, flags, scans, School-A arrays and the next bitmap, level after level.
+__global__ __launch_bounds__(kSmallThreads) void k_small_levels(
+    LevelPtrs lp, int first, int top, const unsigned long long *__restrict__ bin_first,
+    uint32_t *__restrict__ tot) {
+  __shared__ unsigned long long bcur[kSmallWords];
+  __shared__ unsigned long long bnext[kSmallWords / 64];
+  __shared__ uint32_t sh[kSmallThreads / 64];
+  const int tid = threadIdx.x;
+  for (uint64_t j = tid; j < lp.win[first]; j += kSmallThreads) bcur[j] = bin_first[j];
+  __syncthreads();
+  for (int L = first; L <= top; L++) {
+    const uint64_t win = lp.win[L];
+    const int ipt = int((win + kSmallThreads - 1) / kSmallThreads);
+    for (uint64_t j = tid; j < kSmallWords / 64; j += kSmallThreads) bnext[j] = 0ull;
+    uint32_t spop = 0, snz = 0;
+    for (int i = 0; i < ipt; i++) {
+      const uint64_t j = uint64_t(tid) * ipt + i;
+      const unsigned long long w = j < win ? bcur[j] : 0ull;
+      spop += __popcll(w);
+      snz += w != 0ull;
+    }
+    uint32_t epop, enz;
+    const uint32_t tpop = block_scan_u32(spop, &epop, sh);
+    const uint32_t tnz = block_scan_u32(snz, &enz, sh);
+    for (int i = 0; i < ipt; i++) {
+      const uint64_t j = uint64_t(tid) * ipt + i;
+      if (j >= win) break;
+      const unsigned long long w = bcur[j];
+      lp.cb[L][j] = epop;
+      lp.ni[L][j] = enz;
+      if (w) {
+        lp.a_mask[L][enz] = w;
+        lp.a_base[L][enz] = epop;
+        lp.a_code[L][enz] = uint32_t(j);
+        atomicOr(&bnext[j >> 6], 1ull << (j & 63));
+      }
+      epop += __popcll(w);
+      enz += w != 0ull;
+    }
+    if (tid == 0) {
+      lp.cb[L][win] = tpop;
+      lp.ni[L][win] = tnz;
+      tot[2 * L] = tpop;
+      tot[2 * L + 1] = tnz;
+    }
+    __syncthreads();
+    const uint64_t wnext = (win + 63) / 64;
+    for (uint64_t j = tid; j < wnext; j += kSmallThreads) bcur[j] = bnext[j];
+    __syncthreads();
+  }
+}
+
+// Subtree sizes (levels 3..k+1), the root, and School-B placement of levels k..3, in one
+// workgroup; the level-2 placement follows as a multi-block pass.
+__global__ __launch_bounds__(kSmallThreads) void k_upper_place(LevelPtrs lp, int k,
+                                                                 const uint32_t *__restrict__ tot,
+                                                                 uint32_t *__restrict__ nodes_b) {
+  __shared__ uint32_t sh[kSmallThreads / 64];
+  const int tid = threadIdx.x;
+  for (int L = 3; L <= k + 1; L++) {
+    const uint32_t nl = tot[2 * L + 1];
+    const int ipt = int((nl + kSmallThreads - 1) / kSmallThreads);
+    uint32_t local = 0;
+    for (int i = 0; i < ipt; i++) {
+      const uint32_t c = uint32_t(tid) * ipt + i;
+      if (c >= nl) break;
+      const uint32_t m = __popcll(lp.a_mask[L][c]);
+      const uint32_t b = lp.a_base[L][c];
+      const uint32_t d = m + (L > 3 ? lp.desc[L - 1][b + m] - lp.desc[L - 1][b] : 0u);
+      lp.pos[L][c] = d;  // staged desc
+      local += d;
+    }
+    uint32_t excl;
+    const uint32_t total = block_scan_u32(local, &excl, sh);
+    for (int i = 0; i < ipt; i++) {
+      const uint32_t c = uint32_t(tid) * ipt + i;
+      if (c >= nl) break;
+      const uint32_t d = lp.pos[L][c];
+      lp.desc[L][c] = excl;  // E_L: exclusive scan of desc
+      excl += d;
+    }
+    if (tid == 0) lp.desc[L][nl] = total;
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const unsigned long long m = lp.a_mask[k + 1][0];
+    nodes_b[0] = uint32_t(m);
+    nodes_b[1] = uint32_t(m >> 32);
+    nodes_b[2] = 1u;
+    lp.pos[k + 1][0] = 0;
+    lp.bs[k + 1][0] = 1;
+  }
+  __syncthreads();
+  for (int L = k; L >= 3; L--) {
+    const uint32_t nl = tot[2 * L + 1];
+    for (uint32_t c = tid; c < nl; c += kSmallThreads) {
+      const uint32_t q = lp.a_code[L][c];
+      const uint32_t p = lp.ni[L + 1][q >> 6];
+      const uint32_t pb = lp.a_base[L + 1][p];
+      const uint32_t mp = __popcll(lp.a_mask[L + 1][p]);
+      const uint32_t pbs = lp.bs[L + 1][p];
+      const uint32_t mypos = pbs + (c - pb);
+      const uint32_t mybs = pbs + mp + (lp.desc[L][c] - lp.desc[L][pb]);
+      lp.pos[L][c] = mypos;
+      lp.bs[L][c] = mybs;
+      const unsigned long long m = lp.a_mask[L][c];
+      nodes_b[3ull * mypos + 0] = uint32_t(m);
+      nodes_b[3ull * mypos + 1] = uint32_t(m >> 32);
+      nodes_b[3ull * mypos + 2] = mybs;
+    }
+    __syncthreads();
+  }
+}
+
 struct Level {
   uint64_t win = 0;        // words of the input bitmap B_{L-1}
   unsigned long long *bin = nullptr;
@@ -341,17 +517,130 @@ static void *dmalloc(size_t bytes) {
 }
 
 void tree_free_device(vxl_tree *t) {
-  if (t->nodes) (void)hipFree(t->nodes);
-  if (t->leaves) (void)hipFree(t->leaves);
-  if (t->bounds) (void)hipFree(t->bounds);
-  if (t->codes) (void)hipFree(t->codes);
+  if (t->pooled) {
+    if (t->nodes) {
+      if (t->ctx) (void)hipFreeAsync(t->nodes, t->ctx->stream);
+      else (void)hipFree(t->nodes);
+    }
+  } else {
+    if (t->nodes) (void)hipFree(t->nodes);
+    if (t->leaves) (void)hipFree(t->leaves);
+    if (t->bounds) (void)hipFree(t->bounds);
+    if (t->codes) (void)hipFree(t->codes);
+  }
   t->nodes = nullptr;
   t->leaves = nullptr;
   t->bounds = nullptr;
   t->codes = nullptr;
 }
 
+static vxl_tree *alloc_tree(vxl_ctx *ctx, uint32_t n, uint32_t k, uint64_t nodes,
+                            uint64_t leaves) {
+  vxl_tree *t = new vxl_tree();
+  t->ctx = ctx;
+  t->n = n;
+  t->k = k;
+  t->leaf_count = leaves;
+  t->node_count = nodes;
+  // one stream-ordered allocation for the four arrays
+  auto up = [](uint64_t b) { return ((b ? b : 16) + 255) / 256 * 256; };
+  const uint64_t bn = up(nodes * 12), bl = up(leaves * 64), bb = up(leaves * 4), bc = up(leaves * 8);
+  void *base = nullptr;
+  hipError_t e = hipMallocAsync(&base, bn + bl + bb + bc, ctx->stream);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    delete t;
+    throw Error(VXL_OOM, std::string("tree allocation failed: ") + hipGetErrorString(e));
+  }
+  char *c = static_cast<char *>(base);
+  t->nodes = reinterpret_cast<uint32_t *>(c);
+  t->leaves = reinterpret_cast<uint64_t *>(c + bn);
+  t->bounds = reinterpret_cast<uint32_t *>(c + bn + bl);
+  t->codes = reinterpret_cast<uint64_t *>(c + bn + bl + bb);
+  t->pooled = true;
+  t->bytes = nodes * 12 + leaves * 76;
+  return t;
+}
+
+// n <= 2048 (k <= 4): every level above 2 has at most 4096 input words.
+static vxl_tree *build_tree_fast(vxl_ctx *ctx, uint32_t n, uint32_t k) {
+  hipStream_t s = ctx->stream;
+  const int top = int(k + 1 > 2 ? k + 1 : 2);
+  LevelPtrs lp;
+  std::memset(&lp, 0, sizeof(lp));
+  uint64_t w = bitmap_level1_words(n);
+  for (int L = 2; L <= top; L++) {
+    lp.win[L] = w;
+    w = div_up(w, 64);
+  }
+  const uint64_t w1 = lp.win[2];
+  size_t scratch = 4096 + ((w1 + 1) * 8 + 255) / 256 * 256 + (lp.win[3] * 8 + 255) / 256 * 256;
+  for (int L = 2; L <= top; L++) scratch += 10 * (((lp.win[L] + 1) * 8 + 255) / 256) * 256;
+  ctx->levelscratch.reserve(scratch);
+  char *p = static_cast<char *>(ctx->levelscratch.ptr);
+  uint32_t *tot = carve<uint32_t>(p, 64);
+  uint64_t *pn2 = carve<uint64_t>(p, w1 + 1);
+  unsigned long long *b2 = top >= 3 ? carve<unsigned long long>(p, lp.win[3]) : nullptr;
+  for (int L = 2; L <= top; L++) {
+    const uint64_t e = lp.win[L] + 1;
+    lp.cb[L] = carve<uint32_t>(p, e);
+    lp.ni[L] = carve<uint32_t>(p, e);
+    lp.a_mask[L] = carve<unsigned long long>(p, e);
+    lp.a_base[L] = carve<uint32_t>(p, e);
+    lp.a_code[L] = carve<uint32_t>(p, e);
+    lp.desc[L] = carve<uint32_t>(p, e);
+    lp.pos[L] = carve<uint32_t>(p, e);
+    lp.bs[L] = carve<uint32_t>(p, e);
+  }
+  unsigned long long *b1 = ctx->bitmap.as<unsigned long long>();
+  // 1. level 2: packed popcount/flag scan over B1, B2 by ballot
+  hipLaunchKernelGGL(k_level2, dim3(div_up(w1, kTB)), dim3(kTB), 0, s, b1, w1, pn2, b2);
+  scan_exclusive_u64(pn2, pn2, w1, pn2 + w1, ctx->scan_tmp, s);
+  // 2. upper levels in one workgroup
+  if (top >= 3)
+    hipLaunchKernelGGL(k_small_levels, dim3(1), dim3(kSmallThreads), 0, s, lp, 3, top, b2, tot);
+  // 3. totals (one sync)
+  uint32_t *host = static_cast<uint32_t *>(ctx->host_pinned);
+  VXL_HIP(hipMemcpyAsync(host, pn2 + w1, 8, hipMemcpyDeviceToHost, s));
+  if (top >= 3) VXL_HIP(hipMemcpyAsync(host + 2 + 6, tot + 6, 4 * 26, hipMemcpyDeviceToHost, s));
+  VXL_HIP(hipStreamSynchronize(s));
+  const uint64_t leaves = host[0];
+  const uint64_t n2 = host[1];
+  uint64_t nodes = 0;
+  uint64_t level_nodes[16] = {0};
+  level_nodes[2] = n2;
+  for (int L = 3; L <= top; L++) level_nodes[L] = host[2 + 2 * L + 1];
+  if (k >= 1 && leaves)
+    for (int L = 2; L <= int(k) + 1; L++) nodes += level_nodes[L];
+  vxl_tree *t = alloc_tree(ctx, n, k, leaves ? nodes : 0, leaves);
+  if (k >= 1 && leaves)
+    for (int L = 2; L <= int(k) + 1; L++) t->level_nodes[L] = level_nodes[L];
+  if (leaves == 0) return t;
+  // 4. level-2 School-A arrays, leaf gather (clears the grid and B1)
+  hipLaunchKernelGGL(k_emit_a2, dim3(div_up(w1, kTB)), dim3(kTB), 0, s, b1, w1, pn2,
+                     lp.a_mask[2], lp.a_base[2], lp.a_code[2]);
+  hipLaunchKernelGGL(k_leaf_gather, dim3(div_up(n2 * 64, kTB)), dim3(kTB), 0, s, n2,
+                     lp.a_mask[2], lp.a_base[2], lp.a_code[2],
+                     ctx->leafgrid.as<unsigned long long>(), b1,
+                     reinterpret_cast<unsigned long long *>(t->leaves), t->bounds,
+                     reinterpret_cast<unsigned long long *>(t->codes));
+  // 5. School-B placement
+  if (k == 1) {
+    hipLaunchKernelGGL(k_root, dim3(1), dim3(1), 0, s, lp.a_mask[2], lp.a_base[2], 1, t->nodes,
+                       lp.pos[2], lp.bs[2]);
+  } else if (k >= 2) {
+    hipLaunchKernelGGL(k_upper_place, dim3(1), dim3(kSmallThreads), 0, s, lp, int(k), tot,
+                       t->nodes);
+    hipLaunchKernelGGL(k_place, dim3(div_up(n2, kTB)), dim3(kTB), 0, s, n2, 1, lp.a_mask[2],
+                       lp.a_base[2], lp.a_code[2], nullptr, lp.ni[3], lp.a_mask[3], lp.a_base[3],
+                       lp.bs[3], lp.pos[2], lp.bs[2], t->nodes);
+  }
+  VXL_HIP(hipGetLastError());
+  return t;
+}
+
 vxl_tree *build_tree_from_grid(vxl_ctx *ctx, uint32_t n, uint32_t k) {
+  if (k <= 4) return build_tree_fast(ctx, n, k);
   hipStream_t s = ctx->stream;
   const uint32_t top = k + 1 > 2 ? k + 1 : 2;  // node levels 2..top (level 2 always built)
   Level lv[16];

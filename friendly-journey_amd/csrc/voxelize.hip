@@ -58,32 +58,12 @@ __device__ __forceinline__ void tri_grid(const float *__restrict__ tri, const Vo
     }
 }
 
+// One pass per triangle: grid-space vertices, candidate box, candidate count, and the
+// 64-byte record the SAT kernel reads. Triangles with no candidate keep count 0.
 __global__ void k_tri_setup(const float *__restrict__ tris, uint64_t T, VoxelizeParams p,
-                            uint64_t *__restrict__ count, uint32_t *__restrict__ flag) {
-  uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+                            uint64_t *__restrict__ count, TriRec *__restrict__ rec) {
+  const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= T) return;
-  float g[9];
-  tri_grid(tris + 9 * t, p, g);
-  uint64_t c = 1;
-  const float hi_clamp = (float)(p.n - 1);
-#pragma unroll
-  for (int a = 0; a < 3; a++) {
-    float lo = fminf(fminf(g[a], g[3 + a]), g[6 + a]) - p.eps;
-    float hi = fmaxf(fmaxf(g[a], g[3 + a]), g[6 + a]) + p.eps;
-    int32_t mn = sat_i32(fmaxf(floorf(lo), 0.0f));
-    int32_t mx = sat_i32(fminf(floorf(hi), hi_clamp));
-    c = mx < mn ? 0 : c * uint64_t(int64_t(mx) - int64_t(mn) + 1);
-  }
-  count[t] = c;
-  flag[t] = c != 0;
-}
-
-__global__ void k_compact(const float *__restrict__ tris, uint64_t T, VoxelizeParams p,
-                          const uint64_t *__restrict__ count, const uint32_t *__restrict__ flag,
-                          const uint32_t *__restrict__ cidx, TriRec *__restrict__ rec,
-                          uint64_t *__restrict__ ccount) {
-  uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= T || !flag[t]) return;
   TriRec r;
   tri_grid(tris + 9 * t, p, r.g);
   bool finite = true;
@@ -91,17 +71,20 @@ __global__ void k_compact(const float *__restrict__ tris, uint64_t T, VoxelizePa
   for (int i = 0; i < 9; i++) finite = finite && isfinite(r.g[i]);
   uint32_t ext[3];
   int32_t r_mn[3];
+  uint64_t c = 1;
   const float hi_clamp = (float)(p.n - 1);
 #pragma unroll
   for (int a = 0; a < 3; a++) {
-    float lo = fminf(fminf(r.g[a], r.g[3 + a]), r.g[6 + a]) - p.eps;
-    float hi = fmaxf(fmaxf(r.g[a], r.g[3 + a]), r.g[6 + a]) + p.eps;
-    int32_t mn = sat_i32(fmaxf(floorf(lo), 0.0f));
-    int32_t mx = sat_i32(fminf(floorf(hi), hi_clamp));
-    r_mn[a] = mn;
-    ext[a] = uint32_t(mx - mn + 1);
+    const float lo = fminf(fminf(r.g[a], r.g[3 + a]), r.g[6 + a]) - p.eps;
+    const float hi = fmaxf(fmaxf(r.g[a], r.g[3 + a]), r.g[6 + a]) + p.eps;
+    const int32_t mn = sat_i32(fmaxf(floorf(lo), 0.0f));
+    const int32_t mx = sat_i32(fminf(floorf(hi), hi_clamp));
+    c = mx < mn ? 0 : c * uint64_t(int64_t(mx) - int64_t(mn) + 1);
+    r_mn[a] = mx < mn ? 0 : mn;
+    ext[a] = mx < mn ? 1u : uint32_t(mx - mn + 1);
   }
-  uint64_t c = count[t];
+  count[t] = c;
+  if (c == 0) return;  // never read
   r.mnxy = uint32_t(r_mn[0]) | (uint32_t(r_mn[1]) << 16);
   r.mnz_ex = uint32_t(r_mn[2]) | (ext[0] << 16);
   r.exy = ext[0] * ext[1];
@@ -110,12 +93,10 @@ __global__ void k_compact(const float *__restrict__ tris, uint64_t T, VoxelizePa
   r.rcp_ex = 1.0f / float(ext[0]);
   r.rcp_exy = 1.0f / float(r.exy);
   r.pad = 0;
-  uint32_t ci = cidx[t];
-  rec[ci] = r;
-  ccount[ci] = c;
+  rec[t] = r;
 }
 
-// bstart[b] = compacted triangle holding pair b * chunk.
+// bstart[b] = triangle holding pair b * chunk (triangles with no candidate never do).
 __global__ void k_bstart(const uint64_t *__restrict__ coff, const uint64_t *__restrict__ ccount,
                          uint64_t Tc, uint64_t chunk, uint64_t nchunks,
                          uint32_t *__restrict__ bstart) {
@@ -315,15 +296,15 @@ __device__ __forceinline__ void table_flush(unsigned long long *tk, unsigned lon
   __builtin_amdgcn_wave_barrier();
 }
 
-// Each wave walks chunks [c0, c1) of 64 consecutive pairs. The triangles of a chunk
-// are found from the 64 triangle start offsets following the chunk's first triangle:
-// a start inside the chunk marks a bit, and a lane's triangle is the first one plus
-// the number of marked bits at or below its position. Phase A evaluates the plane test
-// and queues survivors in LDS; every 64 queued survivors run phase B as a full wave.
-// Hits are merged per leaf word in an LDS table and flushed to HBM in bulk.
+// Each wave walks chunks [c0, c1) of 64 consecutive pairs. A lane's triangle is the
+// last triangle whose first pair is at or before the lane's pair: the starts of the 64
+// triangles after the chunk's first triangle are loaded one per lane and searched
+// across lanes. Phase A evaluates the plane test and queues survivors in LDS; every 64
+// queued survivors run phase B as a full wave. Hits are merged per leaf word in an LDS
+// table and flushed to HBM in bulk.
 __global__ __launch_bounds__(kWaves * 64) void k_sat(
     const TriRec *__restrict__ rec, const uint64_t *__restrict__ coff,
-    const uint32_t *__restrict__ wstart, uint64_t Tc, uint64_t P, uint64_t chunks_per_wave,
+    const uint32_t *__restrict__ wstart, uint64_t T, uint64_t P, uint64_t chunks_per_wave,
     unsigned long long *__restrict__ leafgrid, unsigned long long *__restrict__ brickbits) {
   __shared__ uint32_t q_t[kWaves][kQueue], q_xy[kWaves][kQueue], q_z[kWaves][kQueue];
   __shared__ unsigned long long t_key[kWaves][kTable], t_val[kWaves][kTable];
@@ -351,13 +332,29 @@ __global__ __launch_bounds__(kWaves * 64) void k_sat(
     const uint64_t p = base + uint64_t(lane);
     const bool valid = p < P;
     const uint64_t idx = uint64_t(t_first) + 1 + uint64_t(lane);
-    const uint64_t offn = idx <= Tc ? coff[idx] : ~0ull;  // coff[Tc] == P
-    const uint64_t rpos = offn - base;                      // >= 1
-    const unsigned long long marks = wave_or64(rpos < 64 ? (1ull << rpos) : 0ull);
-    const uint32_t k = __popcll(marks & (below << 1));      // starts at positions 1..lane
+    const uint64_t offn = idx <= T ? coff[idx] : ~0ull;  // coff[T] == P; non-decreasing
+    // k = number of the next 64 triangles starting at or before p (cross-lane search)
+    uint32_t k = 0;
+#pragma unroll
+    for (int step = 32; step >= 1; step >>= 1) {
+      const uint64_t v = shfl_u64(offn, int(k + step) - 1);
+      if (v <= p) k += step;
+    }
+    if (shfl_u64(offn, 63) <= p) k = 64;  // the search above tops out at 63
+    // every lane takes part in the shuffle (a lane that is inactive in a cross-lane read
+    // would supply 0); the selection happens afterwards
     const uint64_t kstart = shfl_u64(offn, k ? int(k) - 1 : 0);
-    const uint32_t t = t_first + k;
-    const uint64_t start = k ? kstart : start_first;
+    uint32_t t = t_first + k;
+    uint64_t start = k ? kstart : start_first;
+    if (k == 64 && valid) {  // 64 starts inside one chunk (zero-candidate triangles)
+      uint64_t lo = t, hi = T - 1;  // last triangle with start <= p
+      while (lo < hi) {
+        const uint64_t mid = (lo + hi + 1) >> 1;
+        if (coff[mid] <= p) lo = mid; else hi = mid - 1;
+      }
+      t = uint32_t(lo);
+      start = coff[lo];
+    }
 
     bool surv = false;
     uint32_t x = 0, y = 0, z = 0;
@@ -383,11 +380,22 @@ __global__ __launch_bounds__(kWaves * 64) void k_sat(
       const Rel v = rel_to(r.g, float(x) + 0.5f, float(y) + 0.5f, float(z) + 0.5f);
       surv = (r.flags & 1u) ? plane_passes<true>(v) : plane_passes<false>(v);
     }
-    // next chunk's first triangle: starts at positions 1..64
-    const uint32_t adv = __popcll(__ballot(rpos <= 64));
-    if (adv) {
-      start_first = shfl_u64(offn, int(adv) - 1);
-      t_first += adv;
+    // next chunk's first triangle: the last one starting at or before base + 64
+    {
+      const uint64_t nb = base + 64;
+      const uint32_t adv = __popcll(__ballot(offn <= nb));
+      if (adv == 64 && nb < P) {  // saturated window: search (wave-uniform)
+        uint64_t lo = uint64_t(t_first) + 64, hi = T - 1;
+        while (lo < hi) {
+          const uint64_t mid = (lo + hi + 1) >> 1;
+          if (coff[mid] <= nb) lo = mid; else hi = mid - 1;
+        }
+        t_first = uint32_t(lo);
+        start_first = coff[lo];
+      } else if (adv) {
+        start_first = shfl_u64(offn, int(adv) - 1);
+        t_first += adv;
+      }
     }
     // queue survivors
     const unsigned long long sm = __ballot(surv);
@@ -427,35 +435,21 @@ uint64_t run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizePa
   hipStream_t s = ctx->stream;
   const int tb = 256;
   ctx->tri_count.reserve(T * 8);
-  ctx->tri_flag.reserve(T * 4);
-  ctx->tri_cidx.reserve(T * 4 + 8);
+  ctx->rec.reserve(T * sizeof(TriRec));
+  ctx->coff.reserve(T * 8 + 8);
   uint64_t *count = ctx->tri_count.as<uint64_t>();
-  uint32_t *flag = ctx->tri_flag.as<uint32_t>();
-  uint32_t *cidx = ctx->tri_cidx.as<uint32_t>();
-  uint64_t *host = static_cast<uint64_t *>(ctx->host_pinned);
-  uint32_t *total_dev = cidx + T;  // one extra slot (8-byte aligned region)
-
-  hipLaunchKernelGGL(k_tri_setup, dim3((T + tb - 1) / tb), dim3(tb), 0, s, d_tris, T, p, count,
-                     flag);
-  scan_exclusive_u32(flag, cidx, T, total_dev, ctx->scan_tmp, s);
-  VXL_HIP(hipMemcpyAsync(host, total_dev, 4, hipMemcpyDeviceToHost, s));
-  VXL_HIP(hipStreamSynchronize(s));
-  const uint64_t Tc = static_cast<uint32_t *>(ctx->host_pinned)[0];
-  if (Tc == 0) return 0;
-
-  ctx->rec.reserve(Tc * sizeof(TriRec));
-  ctx->ccount.reserve(Tc * 8);
-  ctx->coff.reserve(Tc * 8 + 8);
   TriRec *rec = ctx->rec.as<TriRec>();
-  uint64_t *ccount = ctx->ccount.as<uint64_t>();
   uint64_t *coff = ctx->coff.as<uint64_t>();
-  hipLaunchKernelGGL(k_compact, dim3((T + tb - 1) / tb), dim3(tb), 0, s, d_tris, T, p, count,
-                     flag, cidx, rec, ccount);
-  scan_exclusive_u64(ccount, coff, Tc, coff + Tc, ctx->scan_tmp, s);
-  VXL_HIP(hipMemcpyAsync(host, coff + Tc, 8, hipMemcpyDeviceToHost, s));
+  uint64_t *host = static_cast<uint64_t *>(ctx->host_pinned);
+  hipLaunchKernelGGL(k_tri_setup, dim3((T + tb - 1) / tb), dim3(tb), 0, s, d_tris, T, p, count,
+                     rec);
+  scan_exclusive_u64(count, coff, T, coff + T, ctx->scan_tmp, s);
+  VXL_HIP(hipMemcpyAsync(host, coff + T, 8, hipMemcpyDeviceToHost, s));
   VXL_HIP(hipStreamSynchronize(s));
   const uint64_t P = host[0];
   if (P == 0) return 0;
+  uint64_t *ccount = count;
+  const uint64_t Tc = T;
 
   // Waves walk contiguous runs of 64-pair chunks; wstart[w] = triangle holding the
   // first pair of wave w (k_bstart with a chunk of chunks_per_wave * 64 pairs).
@@ -471,7 +465,7 @@ uint64_t run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizePa
   hipLaunchKernelGGL(k_bstart, dim3((Tc + tb - 1) / tb), dim3(tb), 0, s, coff, ccount, Tc,
                      cpw * 64, waves, wstart);
   mark(ctx, 1);
-  hipLaunchKernelGGL(k_sat, dim3(uint32_t(blocks)), dim3(kWaves * 64), 0, s, rec, coff, wstart, Tc,
+  hipLaunchKernelGGL(k_sat, dim3(uint32_t(blocks)), dim3(kWaves * 64), 0, s, rec, coff, wstart, T,
                      P, cpw, ctx->leafgrid.as<unsigned long long>(),
                      ctx->bitmap.as<unsigned long long>());
   mark(ctx, 2);

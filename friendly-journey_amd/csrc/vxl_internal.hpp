@@ -108,6 +108,7 @@ struct vxl_tree {
   uint64_t *codes = nullptr;   // 1 x u64 per leaf (>= 1 entry allocated)
   uint64_t level_nodes[16] = {0};
   size_t bytes = 0;
+  bool pooled = false;  // one stream-ordered allocation starting at `nodes`
 };
 
 namespace vxl {
