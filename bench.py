@@ -157,6 +157,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dda", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this path has no multi-GPU form here "
@@ -168,6 +170,14 @@ def main():
 
     if not torch.cuda.is_available():
         raise SystemExit("no GPU visible")
+    if args.copy_probe:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": src.numel() * 4, "copies": 3}), flush=True)
+        return
     ctx = vxl.Context(0)
     vox, tris = bench_voxelize(ctx, torch, W, args.steps, args.warmup)
     ms_per_step = vox["elapsed"] / vox["steps"] * 1e3

@@ -252,16 +252,18 @@ void vxl_last_timing(vxl_ctx *ctx, float out_ms[8]) {
   for (int i = 0; i < 8; i++) out_ms[i] = ctx->last_ms[i];
 }
 
-vxl_status vxl_set_tuning(vxl_ctx *ctx, int sat_block, int sat_ppt, int dda_block) {
+vxl_status vxl_set_tuning(vxl_ctx *ctx, int sat_waves_per_cu, int sat_units_per_wave,
+                          int dda_block) {
   return guarded(ctx, [&] {
-    if (sat_block) {
-      if (sat_block % 64 || sat_block < 64 || sat_block > 1024)
-        throw Error(VXL_INVALID_ARG, "sat_block must be a multiple of 64 in [64, 1024]");
-      ctx->sat_block = sat_block;
+    if (sat_waves_per_cu) {
+      if (sat_waves_per_cu < 1 || sat_waves_per_cu > 64)
+        throw Error(VXL_INVALID_ARG, "sat_waves_per_cu must be in [1, 64]");
+      ctx->sat_waves_per_cu = sat_waves_per_cu;
     }
-    if (sat_ppt) {
-      if (sat_ppt < 1 || sat_ppt > 64) throw Error(VXL_INVALID_ARG, "sat_pairs_per_thread in [1,64]");
-      ctx->sat_pairs_per_thread = sat_ppt;
+    if (sat_units_per_wave) {
+      if (sat_units_per_wave < 1 || sat_units_per_wave > 1024)
+        throw Error(VXL_INVALID_ARG, "sat_units_per_wave must be in [1, 1024]");
+      ctx->sat_units_per_wave = sat_units_per_wave;
     }
     if (dda_block) {
       if (dda_block % 64 || dda_block < 64 || dda_block > 256)

@@ -216,6 +216,30 @@ __device__ __forceinline__ unsigned long long wave_or64(unsigned long long v) {
   return (uint64_t(hi) << 32) | lo;
 }
 
+// OR over the 64 lanes of the wave, returned uniform: Hillis-Steele within each row of 16
+// lanes (row_shr 1, 2, 4, 8), then row_bcast 15 and 31 carry the row totals up to lane 63.
+__device__ __forceinline__ uint32_t wave_or_u32_dpp(uint32_t v) {
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, true);
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, true);
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+__device__ __forceinline__ unsigned long long wave_or64_dpp(unsigned long long v) {
+  const uint32_t lo = wave_or_u32_dpp(uint32_t(v));
+  const uint32_t hi = wave_or_u32_dpp(uint32_t(v >> 32));
+  return (uint64_t(hi) << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int lane) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)uint32_t(v), lane);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)uint32_t(v >> 32), lane);
+  return (uint64_t(hi) << 32) | lo;
+}
+
 __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
   uint32_t lo = (uint32_t)__shfl((int)uint32_t(v), src, 64);
   uint32_t hi = (uint32_t)__shfl((int)uint32_t(v >> 32), src, 64);
@@ -235,6 +259,7 @@ constexpr int kWaves = 4;    // waves per block
 constexpr int kQueue = 128;  // survivor queue entries per wave
 constexpr int kTable = 256;  // leaf-word merge table slots per wave
 constexpr int kFlushAt = 128;  // flush the table once this many slots are in use
+constexpr uint64_t kChunksPerUnit = 16;  // at least 1024 pairs per work unit
 constexpr unsigned long long kEmpty = ~0ull;
 
 // Phase B on one survivor: box faces + edge axes. On a hit, returns the leaf-grid word
@@ -304,16 +329,14 @@ __device__ __forceinline__ void table_flush(unsigned long long *tk, unsigned lon
 // table and flushed to HBM in bulk.
 __global__ __launch_bounds__(kWaves * 64) void k_sat(
     const TriRec *__restrict__ rec, const uint64_t *__restrict__ coff,
-    const uint32_t *__restrict__ wstart, uint64_t T, uint64_t P, uint64_t chunks_per_wave,
-    unsigned long long *__restrict__ leafgrid, unsigned long long *__restrict__ brickbits) {
+    const uint32_t *__restrict__ ustart, uint64_t T, uint64_t P, uint64_t chunks_per_unit,
+    unsigned int *__restrict__ next_unit, unsigned long long *__restrict__ leafgrid,
+    unsigned long long *__restrict__ brickbits) {
   __shared__ uint32_t q_t[kWaves][kQueue], q_xy[kWaves][kQueue], q_z[kWaves][kQueue];
   __shared__ unsigned long long t_key[kWaves][kTable], t_val[kWaves][kTable];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const uint64_t gw = uint64_t(blockIdx.x) * kWaves + wave;
   const uint64_t nchunks = (P + 63) / 64;
-  const uint64_t c0 = gw * chunks_per_wave;
-  if (c0 >= nchunks) return;  // wave-uniform
-  const uint64_t c1 = min(c0 + chunks_per_wave, nchunks);
+  const uint64_t nunits = (nchunks + chunks_per_unit - 1) / chunks_per_unit;
   uint32_t *qt = q_t[wave], *qxy = q_xy[wave], *qz = q_z[wave];
   unsigned long long *tk = t_key[wave], *tv = t_val[wave];
 #pragma unroll
@@ -322,38 +345,86 @@ __global__ __launch_bounds__(kWaves * 64) void k_sat(
     tv[i * 64 + lane] = 0ull;
   }
   __builtin_amdgcn_wave_barrier();
-  uint32_t t_first = wstart[gw];
-  uint64_t start_first = coff[t_first];
   uint32_t qn = 0, used = 0;
   const unsigned long long below = (1ull << lane) - 1ull;
 
+  // units of chunks_per_unit chunks are handed out by an atomic counter
+  for (;;) {
+  uint32_t unit = 0;
+  if (lane == 0) unit = atomicAdd(next_unit, 1u);
+  unit = __shfl(unit, 0, 64);
+  if (uint64_t(unit) >= nunits) break;  // wave-uniform exit: every wave ends here
+  const uint64_t c0 = uint64_t(unit) * chunks_per_unit;
+  const uint64_t c1 = min(c0 + chunks_per_unit, nchunks);
+  uint32_t t_first = ustart[unit];
+  uint64_t start_first = coff[t_first];
+  uint64_t offn;  // start offset of triangle t_first + 1 + lane (P past the last triangle)
+  {
+    const uint64_t idx = uint64_t(t_first) + 1 + uint64_t(lane);
+    offn = idx <= T ? coff[idx] : ~0ull;
+  }
   for (uint64_t c = c0; c < c1; c++) {
     const uint64_t base = c * 64;
     const uint64_t p = base + uint64_t(lane);
     const bool valid = p < P;
-    const uint64_t idx = uint64_t(t_first) + 1 + uint64_t(lane);
-    const uint64_t offn = idx <= T ? coff[idx] : ~0ull;  // coff[T] == P; non-decreasing
-    // k = number of the next 64 triangles starting at or before p (cross-lane search)
-    uint32_t k = 0;
+    const uint64_t rpos = offn - base;  // >= 1
+    const unsigned long long marks = wave_or64_dpp(rpos < 64 ? (1ull << rpos) : 0ull);
+    const uint32_t starts_in = __popcll(__ballot(rpos < 64));
+    uint32_t t;
+    uint64_t start;
+    if (__popcll(marks) == starts_in) {
+      // one start per marked position: lane's triangle = t_first + marks at positions 1..lane,
+      // and its first pair is the highest of those positions
+      const unsigned long long m = marks & (below << 1);
+      const uint32_t k = __popcll(m);
+      t = t_first + k;
+      start = k ? base + uint64_t(63 - __clzll(m)) : start_first;
+    } else {
+      // several triangles start at one position (triangles without candidates)
+      uint32_t k = 0;
 #pragma unroll
-    for (int step = 32; step >= 1; step >>= 1) {
-      const uint64_t v = shfl_u64(offn, int(k + step) - 1);
-      if (v <= p) k += step;
-    }
-    if (shfl_u64(offn, 63) <= p) k = 64;  // the search above tops out at 63
-    // every lane takes part in the shuffle (a lane that is inactive in a cross-lane read
-    // would supply 0); the selection happens afterwards
-    const uint64_t kstart = shfl_u64(offn, k ? int(k) - 1 : 0);
-    uint32_t t = t_first + k;
-    uint64_t start = k ? kstart : start_first;
-    if (k == 64 && valid) {  // 64 starts inside one chunk (zero-candidate triangles)
-      uint64_t lo = t, hi = T - 1;  // last triangle with start <= p
-      while (lo < hi) {
-        const uint64_t mid = (lo + hi + 1) >> 1;
-        if (coff[mid] <= p) lo = mid; else hi = mid - 1;
+      for (int step = 32; step >= 1; step >>= 1) {
+        const uint64_t v = shfl_u64(offn, int(k + step) - 1);
+        if (v <= p) k += step;
       }
-      t = uint32_t(lo);
-      start = coff[lo];
+      if (shfl_u64(offn, 63) <= p) k = 64;  // the search above tops out at 63
+      const uint64_t kstart = shfl_u64(offn, k ? int(k) - 1 : 0);
+      t = t_first + k;
+      start = k ? kstart : start_first;
+      if (k == 64 && valid) {  // 64 starts inside one chunk
+        uint64_t lo = t, hi = T - 1;  // last triangle with start <= p
+        while (lo < hi) {
+          const uint64_t mid = (lo + hi + 1) >> 1;
+          if (coff[mid] <= p) lo = mid; else hi = mid - 1;
+        }
+        t = uint32_t(lo);
+        start = coff[lo];
+      }
+    }
+    // next chunk's first triangle (the last one starting at or before base + 64) and the
+    // prefetch of its window of starts
+    uint32_t t_next = t_first;
+    uint64_t start_next = start_first;
+    {
+      const uint64_t nb = base + 64;
+      const uint32_t adv = __popcll(__ballot(offn <= nb));
+      if (adv == 64 && nb < P) {  // saturated window: search (wave-uniform)
+        uint64_t lo = uint64_t(t_first) + 64, hi = T - 1;
+        while (lo < hi) {
+          const uint64_t mid = (lo + hi + 1) >> 1;
+          if (coff[mid] <= nb) lo = mid; else hi = mid - 1;
+        }
+        t_next = uint32_t(lo);
+        start_next = coff[lo];
+      } else if (adv) {
+        start_next = readlane_u64(offn, int(adv) - 1);
+        t_next = t_first + adv;
+      }
+    }
+    uint64_t offn_next = ~0ull;
+    if (c + 1 < c1) {
+      const uint64_t idx = uint64_t(t_next) + 1 + uint64_t(lane);
+      offn_next = idx <= T ? coff[idx] : ~0ull;
     }
 
     bool surv = false;
@@ -380,23 +451,6 @@ __global__ __launch_bounds__(kWaves * 64) void k_sat(
       const Rel v = rel_to(r.g, float(x) + 0.5f, float(y) + 0.5f, float(z) + 0.5f);
       surv = (r.flags & 1u) ? plane_passes<true>(v) : plane_passes<false>(v);
     }
-    // next chunk's first triangle: the last one starting at or before base + 64
-    {
-      const uint64_t nb = base + 64;
-      const uint32_t adv = __popcll(__ballot(offn <= nb));
-      if (adv == 64 && nb < P) {  // saturated window: search (wave-uniform)
-        uint64_t lo = uint64_t(t_first) + 64, hi = T - 1;
-        while (lo < hi) {
-          const uint64_t mid = (lo + hi + 1) >> 1;
-          if (coff[mid] <= nb) lo = mid; else hi = mid - 1;
-        }
-        t_first = uint32_t(lo);
-        start_first = coff[lo];
-      } else if (adv) {
-        start_first = shfl_u64(offn, int(adv) - 1);
-        t_first += adv;
-      }
-    }
     // queue survivors
     const unsigned long long sm = __ballot(surv);
     if (surv) {
@@ -418,6 +472,10 @@ __global__ __launch_bounds__(kWaves * 64) void k_sat(
         used = 0;
       }
     }
+    offn = offn_next;
+    t_first = t_next;
+    start_first = start_next;
+  }
   }
   __builtin_amdgcn_wave_barrier();
   {
@@ -451,22 +509,30 @@ uint64_t run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizePa
   uint64_t *ccount = count;
   const uint64_t Tc = T;
 
-  // Waves walk contiguous runs of 64-pair chunks; wstart[w] = triangle holding the
-  // first pair of wave w (k_bstart with a chunk of chunks_per_wave * 64 pairs).
+  // Work units of kChunksPerUnit chunks of 64 pairs, handed out to a persistent grid of
+  // waves by an atomic counter; ustart[u] = triangle holding the first pair of unit u.
+  // Units are long contiguous runs (about sat_units_per_wave per resident wave) so that
+  // each wave's merge table sees spatially coherent hits.
   const uint64_t nchunks = (P + 63) / 64;
-  const uint64_t max_waves = uint64_t(ctx->cu_count > 0 ? ctx->cu_count : 256) * ctx->sat_waves_per_cu;
-  const uint64_t cpw = (nchunks + max_waves - 1) / max_waves;
-  const uint64_t waves = (nchunks + cpw - 1) / cpw;
+  const uint64_t max_waves =
+      uint64_t(ctx->cu_count > 0 ? ctx->cu_count : 256) * ctx->sat_waves_per_cu;
+  const uint64_t want_units = max_waves * uint64_t(ctx->sat_units_per_wave);
+  uint64_t cpu = (nchunks + want_units - 1) / want_units;
+  if (cpu < kChunksPerUnit) cpu = kChunksPerUnit;
+  const uint64_t nunits = (nchunks + cpu - 1) / cpu;
+  const uint64_t waves = nunits < max_waves ? nunits : max_waves;
   const uint64_t blocks = (waves + kWaves - 1) / kWaves;
-  if (blocks > 0xffffffffull)
-    throw Error(VXL_WORKGROUPS_EXCEEDED, "voxelize: wave count exceeds the 32-bit grid limit");
-  ctx->bstart.reserve(waves * 4 + 16);
-  uint32_t *wstart = ctx->bstart.as<uint32_t>();
+  if (nunits > 0xffffffffull)
+    throw Error(VXL_WORKGROUPS_EXCEEDED, "voxelize: pair units exceed the 32-bit limit");
+  ctx->bstart.reserve(nunits * 4 + 64);
+  uint32_t *ustart = ctx->bstart.as<uint32_t>();
+  unsigned int *counter = reinterpret_cast<unsigned int *>(ustart + nunits + 8);
   hipLaunchKernelGGL(k_bstart, dim3((Tc + tb - 1) / tb), dim3(tb), 0, s, coff, ccount, Tc,
-                     cpw * 64, waves, wstart);
+                     cpu * 64, nunits, ustart);
+  VXL_HIP(hipMemsetAsync(counter, 0, 4, s));
   mark(ctx, 1);
-  hipLaunchKernelGGL(k_sat, dim3(uint32_t(blocks)), dim3(kWaves * 64), 0, s, rec, coff, wstart, T,
-                     P, cpw, ctx->leafgrid.as<unsigned long long>(),
+  hipLaunchKernelGGL(k_sat, dim3(uint32_t(blocks)), dim3(kWaves * 64), 0, s, rec, coff, ustart, T,
+                     P, cpu, counter, ctx->leafgrid.as<unsigned long long>(),
                      ctx->bitmap.as<unsigned long long>());
   mark(ctx, 2);
   VXL_HIP(hipGetLastError());

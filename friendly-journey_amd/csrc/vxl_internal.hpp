@@ -87,9 +87,8 @@ struct vxl_ctx {
   vxl::DevBuf rays_io, hits_io, t_io;
   void *host_pinned = nullptr;  // small readback area
   // tuning
-  int sat_block = 256;
-  int sat_pairs_per_thread = 4;
-  int sat_waves_per_cu = 32;  // SAT waves launched per CU (each walks a run of chunks)
+  int sat_waves_per_cu = 28;  // resident SAT waves per CU (persistent, work-queue fed)
+  int sat_units_per_wave = 1;  // work units per resident wave
   int dda_block = 256;
   // last-call timing (ms), filled when timing is enabled
   bool timing = false;

@@ -265,8 +265,8 @@ class Context:
         load().vxl_last_timing(self.handle, out)
         return list(out)
 
-    def set_tuning(self, sat_block=0, sat_pairs_per_thread=0, dda_block=0):
-        _check(load().vxl_set_tuning(self.handle, sat_block, sat_pairs_per_thread, dda_block))
+    def set_tuning(self, sat_waves_per_cu=0, sat_units_per_wave=0, dda_block=0):
+        _check(load().vxl_set_tuning(self.handle, sat_waves_per_cu, sat_units_per_wave, dda_block))
 
 
 # ---- reference-shaped API (voxelizer::GpuVoxelizer, voxel_gpu::GpuTraverser) --------------

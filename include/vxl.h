@@ -135,9 +135,13 @@ vxl_status vxl_traverse_dev(vxl_ctx *ctx, const vxl_tree *tree, const float *d_r
 void vxl_set_timing(vxl_ctx *ctx, int enable);
 void vxl_last_timing(vxl_ctx *ctx, float out_ms[8]);
 
-/* ---- tuning knobs (autotuner surface): block size and pairs per thread for the SAT
- * kernel, block size for the DDA kernel. 0 keeps the current value. */
-vxl_status vxl_set_tuning(vxl_ctx *ctx, int sat_block, int sat_pairs_per_thread, int dda_block);
+/* ---- tuning knobs (the autotuner's surface; no reference counterpart: the reference's
+ * GpuVoxelizerConfig{workgroup_size, tiles_per_workgroup}, voxelizer/src/gpu/mod.rs:61-75,
+ * plays this role for its WGSL kernel). sat_waves_per_cu: resident SAT waves per CU;
+ * sat_units_per_wave: contiguous work units per resident wave; dda_block: DDA workgroup
+ * size. 0 keeps the current value. */
+vxl_status vxl_set_tuning(vxl_ctx *ctx, int sat_waves_per_cu, int sat_units_per_wave,
+                          int dda_block);
 
 #ifdef __cplusplus
 }
