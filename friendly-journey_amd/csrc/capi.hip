@@ -12,6 +12,8 @@ void set_error(const std::string &msg) { g_last_error = msg; }
 
 void DevBuf::reserve(size_t want) {
   if (want <= bytes && ptr) return;
+  // kernels enqueued earlier may still read the old buffer: let them finish first
+  if (ptr) (void)hipDeviceSynchronize();
   release();
   size_t b = want < 256 ? 256 : want;
   hipError_t e = hipMalloc(&ptr, b);
@@ -68,10 +70,12 @@ vxl_status guarded(vxl_ctx *ctx, F &&f) {
   } catch (const Error &e) {
     vxl::set_error(e.msg);
     ctx->leafgrid_dirty = true;  // a failed call may leave scratch non-zero
+    ctx->pool_dirty = true;
     return e.code;
   } catch (const std::exception &e) {
     vxl::set_error(e.what());
     ctx->leafgrid_dirty = true;
+    ctx->pool_dirty = true;
     return VXL_DEVICE_ERROR;
   }
 }
@@ -162,6 +166,68 @@ vxl_tree *voxelize_tree_impl(vxl_ctx *ctx, const float *d_tris, uint64_t T, cons
   return t;
 }
 
+// Validated batch voxelize on ctx->stream: empty meshes get empty trees, meshes the
+// batched tree build covers (k = 2..4, n = 128..2048) go through grouped launches, the
+// rest through the single-mesh path. Every grid and the epsilon are checked before any
+// device work.
+void voxelize_batch_impl(vxl_ctx *ctx, const float *d_tris, const uint64_t *off, uint32_t count,
+                         const vxl_grid *grids, float eps, vxl_tree **out, uint64_t *pairs_out) {
+  std::vector<uint32_t> ks(count);
+  for (uint32_t g = 0; g < count; g++) {
+    validate_grid(&grids[g], &ks[g]);
+    if (off[g + 1] < off[g]) throw Error(VXL_INVALID_ARG, "tri_offsets must be non-decreasing");
+    validate_count(off[g + 1] - off[g]);
+  }
+  if (count && off[count] && !d_tris) throw Error(VXL_INVALID_ARG, "triangles pointer is null");
+  validate_eps(eps);
+  for (uint32_t g = 0; g < count; g++) out[g] = nullptr;
+  try {
+    std::vector<vxl::BatchInput> batch;
+    for (uint32_t g = 0; g < count; g++) {
+      const uint64_t T = off[g + 1] - off[g];
+      if (T == 0) {
+        out[g] = empty_tree(ctx, grids[g].n, ks[g]);
+        ctx->trees.insert(out[g]);
+        if (pairs_out) pairs_out[g] = 0;
+      } else if (ks[g] >= 2 && ks[g] <= 4) {
+        batch.push_back(vxl::BatchInput{g, off[g], T, grids[g].n, ks[g], make_params(&grids[g], eps)});
+      } else {
+        out[g] = voxelize_tree_impl(ctx, d_tris + 9 * off[g], T, &grids[g], ks[g], eps,
+                                    pairs_out ? pairs_out + g : nullptr);
+        ctx->trees.insert(out[g]);
+      }
+    }
+    vxl::voxelize_batch(ctx, d_tris, batch, out, pairs_out);
+  } catch (...) {
+    for (uint32_t g = 0; g < count; g++)
+      if (out[g]) {
+        ctx->trees.erase(out[g]);
+        vxl::tree_free_device(out[g]);
+        delete out[g];
+        out[g] = nullptr;
+      }
+    throw;
+  }
+}
+
+void traverse_batch_checks(const vxl_tree *const *trees, uint32_t count, const uint64_t *off) {
+  if (count && (!trees || !off)) throw Error(VXL_INVALID_ARG, "null trees or ray offsets");
+  for (uint32_t g = 0; g < count; g++) {
+    if (!trees[g]) throw Error(VXL_INVALID_ARG, "null tree in batch");
+    if (off[g + 1] < off[g]) throw Error(VXL_INVALID_ARG, "ray_offsets must be non-decreasing");
+  }
+}
+
+// Makes `to` wait for the work enqueued on `from` so far (no-op for null or equal streams).
+void join_streams(hipStream_t from, hipStream_t to) {
+  if (!from || !to || from == to) return;
+  hipEvent_t ev;
+  VXL_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  VXL_HIP(hipEventRecord(ev, from));
+  VXL_HIP(hipStreamWaitEvent(to, ev, 0));
+  VXL_HIP(hipEventDestroy(ev));
+}
+
 }  // namespace
 
 extern "C" {
@@ -225,7 +291,9 @@ void vxl_ctx_destroy(vxl_ctx *ctx) {
   vxl::DevBuf *bufs[] = {&ctx->tris_in, &ctx->tri_count, &ctx->tri_flag, &ctx->tri_cidx,
                          &ctx->rec,     &ctx->ccount,    &ctx->coff,     &ctx->bstart,
                          &ctx->leafgrid, &ctx->bitmap,   &ctx->levelscratch, &ctx->scan_tmp,
-                         &ctx->words_io, &ctx->rays_io,  &ctx->hits_io,  &ctx->t_io};
+                         &ctx->words_io, &ctx->rays_io,  &ctx->hits_io,  &ctx->t_io,
+                         &ctx->pool_grid, &ctx->pool_b1, &ctx->batch_scratch, &ctx->batch_l2,
+                         &ctx->batch_desc, &ctx->trav_desc};
   for (auto *b : bufs) b->release();
   // Trees outlive nothing they depend on: release their device memory now and detach
   // them; vxl_tree_free on a detached tree only frees the handle.
@@ -237,6 +305,9 @@ void vxl_ctx_destroy(vxl_ctx *ctx) {
   for (auto &e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
+  if (ctx->batch_host) (void)hipHostFree(ctx->batch_host);
+  if (ctx->staging_free) (void)hipEventDestroy(ctx->staging_free);
+  if (ctx->trav_done) (void)hipEventDestroy(ctx->trav_done);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -507,6 +578,197 @@ vxl_status vxl_traverse_dev(vxl_ctx *ctx, const vxl_tree *tree, const float *d_r
       VXL_HIP(hipEventSynchronize(ctx->ev[5]));
       VXL_HIP(hipEventElapsedTime(&ctx->last_ms[3], ctx->ev[4], ctx->ev[5]));
     }
+  });
+}
+
+vxl_status vxl_voxelize_batch_dev(vxl_ctx *ctx, const float *d_tris, const uint64_t *tri_offsets,
+                                  uint32_t count, const vxl_grid *grids, float epsilon,
+                                  vxl_tree **out, uint64_t *pairs_out, void *stream) {
+  return guarded(ctx, [&] {
+    if (count && (!tri_offsets || !grids || !out))
+      throw Error(VXL_INVALID_ARG, "null offsets, grids or out");
+    hipStream_t user = static_cast<hipStream_t>(stream);
+    join_streams(user, ctx->stream);
+    if (ctx->timing) VXL_HIP(hipEventRecord(ctx->ev[6], ctx->stream));
+    voxelize_batch_impl(ctx, d_tris, tri_offsets, count, grids, epsilon, out, pairs_out);
+    if (ctx->timing) {
+      VXL_HIP(hipEventRecord(ctx->ev[7], ctx->stream));
+      VXL_HIP(hipEventSynchronize(ctx->ev[7]));
+      VXL_HIP(hipEventElapsedTime(&ctx->last_ms[4], ctx->ev[6], ctx->ev[7]));
+    }
+    join_streams(ctx->stream, user);
+  });
+}
+
+vxl_status vxl_voxelize_batch(vxl_ctx *ctx, const float *tris, const uint64_t *tri_offsets,
+                              uint32_t count, const vxl_grid *grids, float epsilon, vxl_tree **out,
+                              uint64_t *pairs_out) {
+  return guarded(ctx, [&] {
+    if (count && (!tri_offsets || !grids || !out))
+      throw Error(VXL_INVALID_ARG, "null offsets, grids or out");
+    const uint64_t T = count ? tri_offsets[count] - tri_offsets[0] : 0;
+    if (count && tri_offsets[0] != 0) throw Error(VXL_INVALID_ARG, "tri_offsets[0] must be 0");
+    validate_mesh_host(tris, T);
+    const float *d = nullptr;
+    if (T) {
+      ctx->tris_in.reserve(T * 36);
+      VXL_HIP(hipMemcpyAsync(ctx->tris_in.ptr, tris, T * 36, hipMemcpyHostToDevice, ctx->stream));
+      d = ctx->tris_in.as<float>();
+    }
+    voxelize_batch_impl(ctx, d, tri_offsets, count, grids, epsilon, out, pairs_out);
+    VXL_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+vxl_status vxl_traverse_batch_dev(vxl_ctx *ctx, const vxl_tree *const *trees, uint32_t count,
+                                  const float *d_rays, const uint64_t *ray_offsets,
+                                  uint32_t *d_hits, double *d_t, void *stream) {
+  return guarded(ctx, [&] {
+    traverse_batch_checks(trees, count, ray_offsets);
+    if (count && ray_offsets[count] > ray_offsets[0] && (!d_rays || !d_hits))
+      throw Error(VXL_INVALID_ARG, "null ray or hit buffer");
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    if (ctx->timing) VXL_HIP(hipEventRecord(ctx->ev[4], s));
+    vxl::launch_traverse_batch(ctx, trees, count, ray_offsets, d_rays, d_hits, d_t, s);
+    if (ctx->timing) {
+      VXL_HIP(hipEventRecord(ctx->ev[5], s));
+      VXL_HIP(hipEventSynchronize(ctx->ev[5]));
+      VXL_HIP(hipEventElapsedTime(&ctx->last_ms[5], ctx->ev[4], ctx->ev[5]));
+    }
+  });
+}
+
+vxl_status vxl_traverse_batch(vxl_ctx *ctx, const vxl_tree *const *trees, uint32_t count,
+                              const float *rays, const uint64_t *ray_offsets, uint32_t *hits,
+                              double *t) {
+  return guarded(ctx, [&] {
+    traverse_batch_checks(trees, count, ray_offsets);
+    if (count && ray_offsets[0] != 0) throw Error(VXL_INVALID_ARG, "ray_offsets[0] must be 0");
+    const uint64_t R = count ? ray_offsets[count] : 0;
+    if (R == 0) return;
+    if (!rays || !hits) throw Error(VXL_INVALID_ARG, "null ray or hit buffer");
+    hipStream_t s = ctx->stream;
+    ctx->rays_io.reserve(R * 24);
+    ctx->hits_io.reserve(R * 16);
+    if (t) ctx->t_io.reserve(R * 8);
+    VXL_HIP(hipMemcpyAsync(ctx->rays_io.ptr, rays, R * 24, hipMemcpyHostToDevice, s));
+    vxl::launch_traverse_batch(ctx, trees, count, ray_offsets, ctx->rays_io.as<float>(),
+                               ctx->hits_io.as<uint32_t>(), t ? ctx->t_io.as<double>() : nullptr,
+                               s);
+    VXL_HIP(hipMemcpyAsync(hits, ctx->hits_io.ptr, R * 16, hipMemcpyDeviceToHost, s));
+    if (t) VXL_HIP(hipMemcpyAsync(t, ctx->t_io.ptr, R * 8, hipMemcpyDeviceToHost, s));
+    VXL_HIP(hipStreamSynchronize(s));
+  });
+}
+
+vxl_status vxl_set_batch_budget(vxl_ctx *ctx, uint64_t bytes) {
+  return guarded(ctx, [&] {
+    if (bytes < (1ull << 30) || bytes > (128ull << 30))
+      throw Error(VXL_INVALID_ARG, "batch budget must be in [1 GiB, 128 GiB]");
+    ctx->pool_budget = bytes;
+  });
+}
+
+vxl_status vxl_autotune_batch(vxl_ctx *ctx, const float *d_tris, const uint64_t *tri_offsets,
+                              uint32_t count, const vxl_grid *grids, float epsilon,
+                              const float *d_rays, const uint64_t *ray_offsets, uint32_t *d_hits,
+                              double *d_t, vxl_tuning *result) {
+  return guarded(ctx, [&] {
+    if (!result) throw Error(VXL_INVALID_ARG, "result is null");
+    if (count && (!tri_offsets || !grids)) throw Error(VXL_INVALID_ARG, "null offsets or grids");
+    std::vector<vxl_tree *> trees(count, nullptr);
+    hipEvent_t e0, e1;
+    VXL_HIP(hipEventCreate(&e0));
+    VXL_HIP(hipEventCreate(&e1));
+    auto free_trees = [&] {
+      VXL_HIP(hipStreamSynchronize(ctx->stream));
+      for (auto &t : trees)
+        if (t) {
+          ctx->trees.erase(t);
+          vxl::tree_free_device(t);
+          delete t;
+          t = nullptr;
+        }
+    };
+    auto time_voxelize = [&] {
+      free_trees();
+      VXL_HIP(hipEventRecord(e0, ctx->stream));
+      voxelize_batch_impl(ctx, d_tris, tri_offsets, count, grids, epsilon, trees.data(), nullptr);
+      VXL_HIP(hipEventRecord(e1, ctx->stream));
+      VXL_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      VXL_HIP(hipEventElapsedTime(&ms, e0, e1));
+      return ms;
+    };
+    const int waves[] = {16, 20, 24, 28, 32};
+    const int units[] = {1, 2, 4};
+    const int blocks[] = {64, 128, 256};
+    const int keep_w = ctx->sat_waves_per_cu, keep_u = ctx->sat_units_per_wave;
+    try {
+      time_voxelize();  // warm: scratch and pools reach their size
+      float best = 1e30f;
+      int bw = keep_w, bu = keep_u;
+      for (int w : waves)
+        for (int u : units) {
+          ctx->sat_waves_per_cu = w;
+          ctx->sat_units_per_wave = u;
+          const float ms = time_voxelize();
+          if (ms < best) {
+            best = ms;
+            bw = w;
+            bu = u;
+          }
+        }
+      ctx->sat_waves_per_cu = bw;
+      ctx->sat_units_per_wave = bu;
+      result->sat_waves_per_cu = bw;
+      result->sat_units_per_wave = bu;
+      result->voxelize_ms = time_voxelize();
+      result->dda_block = ctx->dda_block;
+      result->traverse_ms = 0;
+      const bool rays = count && ray_offsets && ray_offsets[count] > ray_offsets[0] && d_rays &&
+                        d_hits;
+      if (rays) {
+        VXL_HIP(hipStreamSynchronize(ctx->stream));
+        float bt = 1e30f;
+        int bb = ctx->dda_block;
+        for (int b : blocks) {
+          ctx->dda_block = b;
+          vxl::launch_traverse_batch(ctx, trees.data(), count, ray_offsets, d_rays, d_hits, d_t,
+                                     ctx->stream);  // warm
+          VXL_HIP(hipEventRecord(e0, ctx->stream));
+          vxl::launch_traverse_batch(ctx, trees.data(), count, ray_offsets, d_rays, d_hits, d_t,
+                                     ctx->stream);
+          VXL_HIP(hipEventRecord(e1, ctx->stream));
+          VXL_HIP(hipEventSynchronize(e1));
+          float ms = 0;
+          VXL_HIP(hipEventElapsedTime(&ms, e0, e1));
+          if (ms < bt) {
+            bt = ms;
+            bb = b;
+          }
+        }
+        ctx->dda_block = bb;
+        result->dda_block = bb;
+        result->traverse_ms = bt;
+      }
+      free_trees();
+    } catch (...) {
+      ctx->sat_waves_per_cu = keep_w;
+      ctx->sat_units_per_wave = keep_u;
+      (void)hipStreamSynchronize(ctx->stream);
+      for (auto &t : trees)
+        if (t) {
+          ctx->trees.erase(t);
+          vxl::tree_free_device(t);
+          delete t;
+        }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
   });
 }
 

@@ -102,12 +102,10 @@ __device__ __forceinline__ bool leaf_reaches(uint32_t packed, const Ray &r, uint
   return t_far >= t_enter;
 }
 
-__global__ __launch_bounds__(256) void k_traverse(
-    const float *__restrict__ rays, uint64_t R, const uint32_t *__restrict__ nodes,
+__device__ __forceinline__ void traverse_ray(
+    uint64_t ri, const float *__restrict__ rays, const uint32_t *__restrict__ nodes,
     const uint32_t *__restrict__ leafw, const uint32_t *__restrict__ bounds, uint32_t n,
     uint32_t k, uint64_t leaf_count, uint4 *__restrict__ hits, double *__restrict__ tout) {
-  const uint64_t ri = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (ri >= R) return;
   Ray r;
   {
     const float2 a = *reinterpret_cast<const float2 *>(rays + 6 * ri);
@@ -324,12 +322,40 @@ __global__ __launch_bounds__(256) void k_traverse(
   }
 }
 
+__global__ __launch_bounds__(256) void k_traverse(
+    const float *__restrict__ rays, uint64_t R, const uint32_t *__restrict__ nodes,
+    const uint32_t *__restrict__ leafw, const uint32_t *__restrict__ bounds, uint32_t n,
+    uint32_t k, uint64_t leaf_count, uint4 *__restrict__ hits, double *__restrict__ tout) {
+  const uint64_t ri = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (ri >= R) return;
+  traverse_ray(ri, rays, nodes, leafw, bounds, n, k, leaf_count, hits, tout);
+}
+
+// One segment of a batched traversal: rays [ray_begin, ray_end) against one tree.
+struct TraverseSeg {
+  const uint32_t *nodes, *leafw, *bounds;
+  uint64_t leaf_count, ray_begin, ray_end;
+  uint32_t n, k;
+};
+
+// Segment g owns blocks [blk_begin[g], blk_begin[g+1]), so every block reads one tree.
+__global__ __launch_bounds__(256) void k_traverse_batch(
+    const TraverseSeg *__restrict__ seg, const uint32_t *__restrict__ blk_begin, uint32_t S,
+    const float *__restrict__ rays, uint4 *__restrict__ hits, double *__restrict__ tout) {
+  const uint32_t g = dev::upper_index(blk_begin, S, blockIdx.x);
+  const TraverseSeg &sg = seg[g];
+  const uint64_t ri =
+      sg.ray_begin + uint64_t(blockIdx.x - blk_begin[g]) * blockDim.x + threadIdx.x;
+  if (ri >= sg.ray_end) return;
+  traverse_ray(ri, rays, sg.nodes, sg.leafw, sg.bounds, sg.n, sg.k, sg.leaf_count, hits, tout);
+}
+
 }  // namespace
 
 void launch_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, uint64_t R,
                      uint32_t *d_hits, double *d_t, hipStream_t s) {
   if (R == 0) return;
-  const int tb = 256;
+  const int tb = ctx->dda_block;
   const uint64_t blocks = (R + tb - 1) / tb;
   if (blocks > 0xffffffffull)
     throw Error(VXL_WORKGROUPS_EXCEEDED, "traverse: ray batch exceeds the grid limit");
@@ -337,6 +363,44 @@ void launch_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, ui
                      reinterpret_cast<const uint32_t *>(tree->leaves), tree->bounds, tree->n,
                      tree->k, tree->leaf_count, reinterpret_cast<uint4 *>(d_hits), d_t);
   VXL_HIP(hipGetLastError());
+}
+
+void launch_traverse_batch(vxl_ctx *ctx, const vxl_tree *const *trees, uint32_t S,
+                           const uint64_t *ray_off, const float *d_rays, uint32_t *d_hits,
+                           double *d_t, hipStream_t s) {
+  if (S == 0 || ray_off[S] == ray_off[0]) return;
+  const uint32_t tb = uint32_t(ctx->dda_block);
+  const size_t seg_bytes = (S * sizeof(TraverseSeg) + 255) / 256 * 256;
+  const size_t blk_bytes = ((S + 1) * 4 + 255) / 256 * 256;
+  staging_begin(ctx, seg_bytes + blk_bytes);
+  char *h = static_cast<char *>(staging_take(ctx, seg_bytes + blk_bytes));
+  TraverseSeg *hs = reinterpret_cast<TraverseSeg *>(h);
+  uint32_t *hb = reinterpret_cast<uint32_t *>(h + seg_bytes);
+  uint64_t blocks = 0;
+  for (uint32_t g = 0; g < S; g++) {
+    const vxl_tree *t = trees[g];
+    const uint64_t R = ray_off[g + 1] - ray_off[g];
+    hs[g] = TraverseSeg{t->nodes, reinterpret_cast<const uint32_t *>(t->leaves), t->bounds,
+                        t->leaf_count, ray_off[g], ray_off[g + 1], t->n, t->k};
+    hb[g] = uint32_t(blocks);
+    blocks += (R + tb - 1) / tb;
+    if (blocks > 0xffffffffull)
+      throw Error(VXL_WORKGROUPS_EXCEEDED, "traverse batch exceeds the grid limit");
+  }
+  hb[S] = uint32_t(blocks);
+  ctx->trav_desc.reserve(seg_bytes + blk_bytes);
+  char *d = static_cast<char *>(ctx->trav_desc.ptr);
+  // the previous batch's kernel (possibly on another stream) reads the descriptors
+  if (ctx->trav_done) VXL_HIP(hipStreamWaitEvent(s, ctx->trav_done, 0));
+  else VXL_HIP(hipEventCreateWithFlags(&ctx->trav_done, hipEventDisableTiming));
+  VXL_HIP(hipMemcpyAsync(d, h, seg_bytes + blk_bytes, hipMemcpyHostToDevice, s));
+  staging_end(ctx, s);
+  hipLaunchKernelGGL(k_traverse_batch, dim3(uint32_t(blocks)), dim3(tb), 0, s,
+                     reinterpret_cast<const TraverseSeg *>(d),
+                     reinterpret_cast<const uint32_t *>(d + seg_bytes), S, d_rays,
+                     reinterpret_cast<uint4 *>(d_hits), d_t);
+  VXL_HIP(hipGetLastError());
+  VXL_HIP(hipEventRecord(ctx->trav_done, s));
 }
 
 }  // namespace vxl

@@ -104,17 +104,20 @@ __device__ __forceinline__ void word_extent(unsigned long long w, int axis, uint
 }
 
 // One wave per level-2 node: 8 children at a time, 8 lanes (one per u64 word) per child.
+// In a batch group (n2_beg non-null) the stored brick code is relative to the mesh's region.
 __global__ __launch_bounds__(kTB) void k_leaf_gather(
     uint64_t n2, const unsigned long long *__restrict__ a_mask, const uint32_t *__restrict__ a_base,
     const uint32_t *__restrict__ a_code, unsigned long long *__restrict__ leafgrid,
     unsigned long long *__restrict__ brickbits, unsigned long long *__restrict__ leaves,
-    uint32_t *__restrict__ bounds, unsigned long long *__restrict__ codes) {
+    uint32_t *__restrict__ bounds, unsigned long long *__restrict__ codes,
+    const uint64_t *__restrict__ n2_beg, uint32_t G, const BatchMesh *__restrict__ mesh) {
   const uint64_t node = (uint64_t(blockIdx.x) * kTB + threadIdx.x) >> 6;
   if (node >= n2) return;
   const int lane = threadIdx.x & 63;
   const unsigned long long mask = a_mask[node];
   const uint32_t base = a_base[node];
   const uint32_t j = a_code[node];
+  const uint64_t code_bias = n2_beg ? mesh[dev::upper_index(n2_beg, G, node)].b1_begin << 6 : 0;
   const uint32_t nch = __popcll(mask);
   const int sub = lane >> 3, w = lane & 7;
   for (uint32_t g = 0; g < nch; g += 8) {
@@ -158,7 +161,7 @@ __global__ __launch_bounds__(kTB) void k_leaf_gather(
                        : (mn[0] | (mn[1] << 3) | (mn[2] << 6) | (mx[0] << 9) | (mx[1] << 12) |
                           (mx[2] << 15));
       bounds[base + s] = b;
-      codes[base + s] = code;
+      codes[base + s] = code - code_bias;
     }
   }
   if (lane == 0) brickbits[j] = 0ull;  // B1 back to all-zero
@@ -289,13 +292,6 @@ __global__ __launch_bounds__(kTB) void k_emit_a2(const unsigned long long *__res
   a_code[i] = uint32_t(j);
 }
 
-struct LevelPtrs {
-  uint64_t win[16];
-  uint32_t *cb[16], *ni[16], *desc[16], *pos[16], *bs[16];
-  unsigned long long *a_mask[16];
-  uint32_t *a_base[16], *a_code[16];
-};
-
 // Exclusive block scan of one u32 per thread (kSmallThreads threads); returns the total.
 __device__ __forceinline__ uint32_t block_scan_u32(uint32_t v, uint32_t *excl, uint32_t *sh) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -321,9 +317,9 @@ __device__ __forceinline__ uint32_t block_scan_u32(uint32_t v, uint32_t *excl, u
 
 // Levels first..top (each with at most kSmallWords input words) in one workgroup:
 // popcounts, flags, scans, School-A arrays and the next bitmap, level after level.
-__global__ __launch_bounds__(kSmallThreads) void k_small_levels(
-    LevelPtrs lp, int first, int top, const unsigned long long *__restrict__ bin_first,
-    uint32_t *__restrict__ tot) {
+__device__ __forceinline__ void small_levels(const LevelPtrs &lp, int first, int top,
+                                             const unsigned long long *__restrict__ bin_first,
+                                             uint32_t *__restrict__ tot) {
   __shared__ unsigned long long bcur[kSmallWords];
   __shared__ unsigned long long bnext[kSmallWords / 64];
   __shared__ uint32_t sh[kSmallThreads / 64];
@@ -372,11 +368,26 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_levels(
   }
 }
 
+__global__ __launch_bounds__(kSmallThreads) void k_small_levels(
+    LevelPtrs lp, int first, int top, const unsigned long long *__restrict__ bin_first,
+    uint32_t *__restrict__ tot) {
+  small_levels(lp, first, top, bin_first, tot);
+}
+
+// One workgroup per mesh of a batch group; its upper bitmap words start at b1_begin / 64.
+__global__ __launch_bounds__(kSmallThreads) void k_small_levels_batch(
+    const BatchMesh *__restrict__ mesh, const LevelPtrs *__restrict__ lps,
+    const unsigned long long *__restrict__ b2) {
+  const BatchMesh &m = mesh[blockIdx.x];
+  const LevelPtrs &lp = lps[blockIdx.x];
+  small_levels(lp, 3, int(m.k) + 1, b2 + m.b1_begin / 64, lp.tot);
+}
+
 // Subtree sizes (levels 3..k+1), the root, and School-B placement of levels k..3, in one
 // workgroup; the level-2 placement follows as a multi-block pass.
-__global__ __launch_bounds__(kSmallThreads) void k_upper_place(LevelPtrs lp, int k,
-                                                                 const uint32_t *__restrict__ tot,
-                                                                 uint32_t *__restrict__ nodes_b) {
+__device__ __forceinline__ void upper_place(const LevelPtrs &lp, int k,
+                                            const uint32_t *__restrict__ tot,
+                                            uint32_t *__restrict__ nodes_b) {
   __shared__ uint32_t sh[kSmallThreads / 64];
   const int tid = threadIdx.x;
   for (int L = 3; L <= k + 1; L++) {
@@ -432,6 +443,70 @@ __global__ __launch_bounds__(kSmallThreads) void k_upper_place(LevelPtrs lp, int
     }
     __syncthreads();
   }
+}
+
+__global__ __launch_bounds__(kSmallThreads) void k_upper_place(LevelPtrs lp, int k,
+                                                                 const uint32_t *__restrict__ tot,
+                                                                 uint32_t *__restrict__ nodes_b) {
+  upper_place(lp, k, tot, nodes_b);
+}
+
+// ---- batch groups -------------------------------------------------------------------
+// Group-wide arrays: pn2 is the packed scan over every mesh's B1 region in order, so a
+// level-2 node's global index and leaf base are group-wide; per mesh they start at
+// n2_beg[g] and leaf_beg[g]. Upper levels are per mesh (one workgroup each).
+__global__ void k_batch_totals(const BatchMesh *__restrict__ mesh, const LevelPtrs *__restrict__ lps,
+                               uint32_t G, const uint64_t *__restrict__ pn2,
+                               const uint64_t *__restrict__ coff, BatchTotals *__restrict__ out,
+                               uint64_t *__restrict__ n2_beg) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  const BatchMesh &m = mesh[g];
+  const uint64_t lo = pn2[m.b1_begin], hi = pn2[m.b1_begin + m.w1];
+  BatchTotals t;
+  t.leaves = uint32_t(hi) - uint32_t(lo);
+  t.n2 = uint32_t(hi >> 32) - uint32_t(lo >> 32);
+  t.leaf_begin = uint32_t(lo);
+  t.n2_begin = lo >> 32;
+  t.pairs = coff[m.slot_end] - coff[m.slot_begin];
+  for (int L = 3; L <= 5; L++)
+    t.nodes[L - 3] = int(L) <= int(m.k) + 1 ? lps[g].tot[2 * L + 1] : 0u;
+  out[g] = t;
+  n2_beg[g] = t.n2_begin;
+  if (g == G - 1) n2_beg[G] = t.n2_begin + t.n2;
+}
+
+__global__ __launch_bounds__(kSmallThreads) void k_upper_place_batch(
+    const BatchMesh *__restrict__ mesh, const LevelPtrs *__restrict__ lps,
+    const BatchTotals *__restrict__ tot, uint32_t *const *__restrict__ nodes_ptr) {
+  const uint32_t g = blockIdx.x;
+  if (tot[g].leaves == 0) return;
+  upper_place(lps[g], int(mesh[g].k), lps[g].tot, nodes_ptr[g]);
+}
+
+// Level-2 placement under level-3 parents, for every level-2 node of the group.
+__global__ void k_place2_batch(uint64_t N2, uint32_t G, const uint64_t *__restrict__ n2_beg,
+                               const BatchMesh *__restrict__ mesh,
+                               const LevelPtrs *__restrict__ lps,
+                               const BatchTotals *__restrict__ tot,
+                               uint32_t *const *__restrict__ nodes_ptr,
+                               const unsigned long long *__restrict__ a_mask,
+                               const uint32_t *__restrict__ a_base,
+                               const uint32_t *__restrict__ a_code) {
+  const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= N2) return;
+  const uint32_t g = dev::upper_index(n2_beg, G, c);
+  const LevelPtrs &lp = lps[g];
+  const uint32_t cl = uint32_t(c - n2_beg[g]);
+  const uint32_t q = uint32_t(a_code[c] - mesh[g].b1_begin);
+  const uint32_t p = lp.ni[3][q >> 6];
+  const uint32_t pb = lp.a_base[3][p];
+  const uint32_t mypos = lp.bs[3][p] + (cl - pb);
+  uint32_t *nodes_b = nodes_ptr[g];
+  const unsigned long long m = a_mask[c];
+  nodes_b[3ull * mypos + 0] = uint32_t(m);
+  nodes_b[3ull * mypos + 1] = uint32_t(m >> 32);
+  nodes_b[3ull * mypos + 2] = uint32_t(a_base[c] - tot[g].leaf_begin);
 }
 
 struct Level {
@@ -517,7 +592,14 @@ static void *dmalloc(size_t bytes) {
 }
 
 void tree_free_device(vxl_tree *t) {
-  if (t->pooled) {
+  if (t->shared) {
+    if (--t->shared->refs == 0) {
+      if (t->ctx) (void)hipFreeAsync(t->shared->base, t->ctx->stream);
+      else (void)hipFree(t->shared->base);
+      delete t->shared;
+    }
+    t->shared = nullptr;
+  } else if (t->pooled) {
     if (t->nodes) {
       if (t->ctx) (void)hipFreeAsync(t->nodes, t->ctx->stream);
       else (void)hipFree(t->nodes);
@@ -623,7 +705,7 @@ static vxl_tree *build_tree_fast(vxl_ctx *ctx, uint32_t n, uint32_t k) {
                      lp.a_mask[2], lp.a_base[2], lp.a_code[2],
                      ctx->leafgrid.as<unsigned long long>(), b1,
                      reinterpret_cast<unsigned long long *>(t->leaves), t->bounds,
-                     reinterpret_cast<unsigned long long *>(t->codes));
+                     reinterpret_cast<unsigned long long *>(t->codes), nullptr, 0u, nullptr);
   // 5. School-B placement
   if (k == 1) {
     hipLaunchKernelGGL(k_root, dim3(1), dim3(1), 0, s, lp.a_mask[2], lp.a_base[2], 1, t->nodes,
@@ -637,6 +719,149 @@ static vxl_tree *build_tree_fast(vxl_ctx *ctx, uint32_t n, uint32_t k) {
   }
   VXL_HIP(hipGetLastError());
   return t;
+}
+
+void build_trees_batch(vxl_ctx *ctx, const std::vector<BatchMesh> &meshes,
+                       const BatchMesh *d_mesh, uint64_t W1, vxl_tree **trees, uint64_t *pairs) {
+  hipStream_t s = ctx->stream;
+  const uint32_t G = uint32_t(meshes.size());
+  auto up = [](uint64_t b) { return (b + 255) / 256 * 256; };
+  // scratch: group arrays, then per-mesh upper levels
+  size_t need = up((W1 + 1) * 8) + up(W1 / 64 * 8 + 8) + up(G * sizeof(BatchTotals)) +
+                up((G + 1) * 8) + up(G * 8) + up(G * sizeof(LevelPtrs));
+  std::vector<LevelPtrs> lps(G);
+  for (uint32_t g = 0; g < G; g++) {
+    const BatchMesh &m = meshes[g];
+    std::memset(&lps[g], 0, sizeof(LevelPtrs));
+    uint64_t w = m.w1;
+    for (int L = 2; L <= int(m.k) + 1; L++) {
+      lps[g].win[L] = w;
+      w = div_up(w, 64);
+      if (L >= 3) need += 8 * up((lps[g].win[L] + 1) * 8);
+    }
+    need += up(64 * 4);
+  }
+  ctx->batch_scratch.reserve(need);
+  ctx->batch_l2.reserve(up(W1 * 8 + 8) + 2 * up(W1 * 4 + 4));
+  char *p = static_cast<char *>(ctx->batch_scratch.ptr);
+  uint64_t *pn2 = carve<uint64_t>(p, W1 + 1);
+  unsigned long long *b2 = carve<unsigned long long>(p, W1 / 64 + 1);
+  BatchTotals *d_tot = carve<BatchTotals>(p, G);
+  uint64_t *n2_beg = carve<uint64_t>(p, G + 1);
+  uint32_t **nodes_ptr = carve<uint32_t *>(p, G);
+  LevelPtrs *d_lps = carve<LevelPtrs>(p, G);
+  for (uint32_t g = 0; g < G; g++) {
+    LevelPtrs &lp = lps[g];
+    for (int L = 3; L <= int(meshes[g].k) + 1; L++) {
+      const uint64_t e = lp.win[L] + 1;
+      lp.cb[L] = carve<uint32_t>(p, e * 2);
+      lp.ni[L] = carve<uint32_t>(p, e * 2);
+      lp.a_mask[L] = carve<unsigned long long>(p, e);
+      lp.a_base[L] = carve<uint32_t>(p, e * 2);
+      lp.a_code[L] = carve<uint32_t>(p, e * 2);
+      lp.desc[L] = carve<uint32_t>(p, e * 2);
+      lp.pos[L] = carve<uint32_t>(p, e * 2);
+      lp.bs[L] = carve<uint32_t>(p, e * 2);
+    }
+    lp.tot = carve<uint32_t>(p, 64);
+  }
+  char *q = static_cast<char *>(ctx->batch_l2.ptr);
+  unsigned long long *a_mask2 = carve<unsigned long long>(q, W1 + 1);
+  uint32_t *a_base2 = carve<uint32_t>(q, W1 + 1);
+  uint32_t *a_code2 = carve<uint32_t>(q, W1 + 1);
+
+  LevelPtrs *h_lps = static_cast<LevelPtrs *>(staging_take(ctx, G * sizeof(LevelPtrs)));
+  std::memcpy(h_lps, lps.data(), G * sizeof(LevelPtrs));
+  VXL_HIP(hipMemcpyAsync(d_lps, h_lps, G * sizeof(LevelPtrs), hipMemcpyHostToDevice, s));
+
+  unsigned long long *b1 = ctx->pool_b1.as<unsigned long long>();
+  // 1. level 2 over the whole group, then the per-mesh upper levels
+  hipLaunchKernelGGL(k_level2, dim3(div_up(W1, kTB)), dim3(kTB), 0, s, b1, W1, pn2, b2);
+  scan_exclusive_u64(pn2, pn2, W1, pn2 + W1, ctx->scan_tmp, s);
+  hipLaunchKernelGGL(k_small_levels_batch, dim3(G), dim3(kSmallThreads), 0, s, d_mesh, d_lps, b2);
+  hipLaunchKernelGGL(k_batch_totals, dim3(div_up(G, 64)), dim3(64), 0, s, d_mesh, d_lps, G, pn2,
+                     ctx->coff.as<uint64_t>(), d_tot, n2_beg);
+  VXL_HIP(hipGetLastError());
+  // 2. per-mesh sizes (one sync)
+  BatchTotals *h_tot = static_cast<BatchTotals *>(staging_take(ctx, G * sizeof(BatchTotals)));
+  VXL_HIP(hipMemcpyAsync(h_tot, d_tot, G * sizeof(BatchTotals), hipMemcpyDeviceToHost, s));
+  VXL_HIP(hipStreamSynchronize(s));
+  // 3. one allocation for the group's trees: node arrays, then leaves / bounds / codes of
+  //    every mesh in group order (a mesh's leaves start at its leaf_begin)
+  uint64_t total_leaves = 0, node_bytes = 0, N2 = 0, live = 0;
+  std::vector<uint64_t> nodes(G, 0), node_off(G, 0);
+  for (uint32_t g = 0; g < G; g++) {
+    const BatchTotals &t = h_tot[g];
+    if (pairs) pairs[g] = t.pairs;
+    total_leaves += t.leaves;
+    N2 += t.n2;
+    if (!t.leaves) continue;
+    live++;
+    nodes[g] = t.n2;
+    for (int L = 3; L <= int(meshes[g].k) + 1; L++) nodes[g] += t.nodes[L - 3];
+    node_off[g] = node_bytes;
+    node_bytes += up(nodes[g] * 12);
+  }
+  SharedBlock *blk = nullptr;
+  char *base = nullptr;
+  if (live) {
+    const uint64_t bytes = node_bytes + up(total_leaves * 64) + up(total_leaves * 4) +
+                           up(total_leaves * 8);
+    void *mem = nullptr;
+    hipError_t e = hipMallocAsync(&mem, bytes, s);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw Error(VXL_OOM, std::string("batch tree allocation failed: ") + hipGetErrorString(e));
+    }
+    blk = new SharedBlock();
+    blk->base = mem;
+    base = static_cast<char *>(mem);
+  }
+  char *leaves_base = base + node_bytes;
+  char *bounds_base = leaves_base + up(total_leaves * 64);
+  char *codes_base = bounds_base + up(total_leaves * 4);
+  uint32_t **h_nodes = static_cast<uint32_t **>(staging_take(ctx, G * sizeof(uint32_t *)));
+  for (uint32_t g = 0; g < G; g++) {
+    const BatchTotals &t = h_tot[g];
+    const BatchMesh &m = meshes[g];
+    h_nodes[g] = nullptr;
+    if (!t.leaves) {
+      trees[g] = alloc_tree(ctx, m.n, m.k, 0, 0);
+      continue;
+    }
+    vxl_tree *tr = new vxl_tree();
+    tr->ctx = ctx;
+    tr->n = m.n;
+    tr->k = m.k;
+    tr->leaf_count = t.leaves;
+    tr->node_count = nodes[g];
+    tr->level_nodes[2] = t.n2;
+    for (int L = 3; L <= int(m.k) + 1; L++) tr->level_nodes[L] = t.nodes[L - 3];
+    tr->nodes = reinterpret_cast<uint32_t *>(base + node_off[g]);
+    tr->leaves = reinterpret_cast<uint64_t *>(leaves_base) + t.leaf_begin * 8;
+    tr->bounds = reinterpret_cast<uint32_t *>(bounds_base) + t.leaf_begin;
+    tr->codes = reinterpret_cast<uint64_t *>(codes_base) + t.leaf_begin;
+    tr->bytes = nodes[g] * 12 + t.leaves * 76;
+    tr->shared = blk;
+    blk->refs++;
+    trees[g] = tr;
+    h_nodes[g] = tr->nodes;
+  }
+  if (!live) return;
+  VXL_HIP(hipMemcpyAsync(nodes_ptr, h_nodes, G * sizeof(uint32_t *), hipMemcpyHostToDevice, s));
+  // 4. level-2 School-A arrays, leaf gather (clears the pooled grid and B1), placement
+  hipLaunchKernelGGL(k_emit_a2, dim3(div_up(W1, kTB)), dim3(kTB), 0, s, b1, W1, pn2, a_mask2,
+                     a_base2, a_code2);
+  hipLaunchKernelGGL(k_leaf_gather, dim3(div_up(N2 * 64, kTB)), dim3(kTB), 0, s, N2, a_mask2,
+                     a_base2, a_code2, ctx->pool_grid.as<unsigned long long>(), b1,
+                     reinterpret_cast<unsigned long long *>(leaves_base),
+                     reinterpret_cast<uint32_t *>(bounds_base),
+                     reinterpret_cast<unsigned long long *>(codes_base), n2_beg, G, d_mesh);
+  hipLaunchKernelGGL(k_upper_place_batch, dim3(G), dim3(kSmallThreads), 0, s, d_mesh, d_lps,
+                     d_tot, nodes_ptr);
+  hipLaunchKernelGGL(k_place2_batch, dim3(div_up(N2, kTB)), dim3(kTB), 0, s, N2, G, n2_beg,
+                     d_mesh, d_lps, d_tot, nodes_ptr, a_mask2, a_base2, a_code2);
+  VXL_HIP(hipGetLastError());
 }
 
 vxl_tree *build_tree_from_grid(vxl_ctx *ctx, uint32_t n, uint32_t k) {
@@ -731,7 +956,7 @@ vxl_tree *build_tree_from_grid(vxl_ctx *ctx, uint32_t n, uint32_t k) {
                        lv[2].a_mask, lv[2].a_base, lv[2].a_code,
                        ctx->leafgrid.as<unsigned long long>(), lv[2].bin,
                        reinterpret_cast<unsigned long long *>(t->leaves), t->bounds,
-                       reinterpret_cast<unsigned long long *>(t->codes));
+                       reinterpret_cast<unsigned long long *>(t->codes), nullptr, 0u, nullptr);
   }
   if (k >= 1) {
     // 5. subtree sizes bottom-up: desc, then E = exclusive scan of desc (in place)

@@ -1,19 +1,22 @@
 // Conservative triangle/box SAT voxelization into the dense leaf grid.
 //
-// Stage 1 (k_tri_setup): world->grid transform (voxelizer/src/core.rs:101-107) and the
-//   epsilon-padded, clamped candidate box of reference_cpu.rs:135-150; count of
-//   candidate voxels per triangle.
-// Stage 2 (scans + k_compact): drop triangles with no candidates; exclusive scan of
-//   candidate counts gives every (triangle, voxel) pair a global index.
-// Stage 3 (k_bstart): for each block-sized chunk of pairs, the triangle holding its
-//   first pair.
-// Stage 4 (k_sat): one lane per pair. The lane's triangle is found by a binary search
-//   in LDS over the chunk's window of triangle offsets; the pair decodes to its voxel
-//   in the reference's scan order (x fastest, reference_cpu.rs:152-154); the lane runs
-//   the 13-axis SAT of reference_cpu.rs:27-85 with the same binary32 operations in
-//   the same order (no FMA contraction: built -ffp-contract=off); an overlapping voxel
-//   sets its bit in the dense leaf grid with a 64-bit atomic OR, and the first bit of
-//   a 4^3 sub-brick word sets the brick's bit in the brick bitmap.
+// Stage 1 (k_tri_setup / k_tri_setup_batch): world->grid transform
+//   (voxelizer/src/core.rs:101-107), the epsilon-padded, clamped candidate box of
+//   reference_cpu.rs:135-150, the candidate-voxel count and a 64-byte record per triangle.
+// Stage 2: exclusive scan of the counts gives every (triangle, voxel) pair a global index;
+//   the total is the only value read back before the SAT launch.
+// Stage 3 (k_bstart): the triangle holding the first pair of each work unit.
+// Stage 4 (k_sat): a persistent grid of waves takes work units from an atomic counter and
+//   walks them 64 pairs at a time, one pair per lane. The lane's triangle comes from the
+//   starts of the next 64 triangles, held one per lane: a DPP OR-reduce marks the chunk
+//   positions where triangles start, and the lane's triangle is the number of marks at
+//   or below it. The pair decodes to its voxel in the reference's scan order (x fastest,
+//   reference_cpu.rs:152-154), the plane test runs first, survivors queue in LDS, and
+//   every 64 queued survivors run the remaining 12 axes as a full wave. Each hit merges
+//   into a per-wave LDS table keyed by leaf-grid word; the table is flushed with 64-bit
+//   atomic ORs into the leaf grid and the brick bitmap (bit per 8^3 brick).
+// Many meshes share one launch: each record carries its mesh's leaf-grid base, so the
+//   pair loop is the same for one mesh or hundreds.
 //
 // Bit-exactness notes:
 //  * predicate order is free: the result is a pure conjunction without side effects;
@@ -35,7 +38,7 @@ struct __align__(16) TriRec {
   uint32_t exy;       // extent x * extent y
   uint32_t flags;     // bit 0: literal path (non-finite grid vertex); bit 1: wide decode
   float rcp_ex, rcp_exy;  // 1/ex, 1/exy for the narrow decode
-  uint32_t pad;
+  uint32_t grid_base;  // mesh's first leaf-grid word / 512 (0 for a single mesh)
 };
 static_assert(sizeof(TriRec) == 64, "TriRec is one 64-byte record");
 
@@ -60,12 +63,11 @@ __device__ __forceinline__ void tri_grid(const float *__restrict__ tri, const Vo
 
 // One pass per triangle: grid-space vertices, candidate box, candidate count, and the
 // 64-byte record the SAT kernel reads. Triangles with no candidate keep count 0.
-__global__ void k_tri_setup(const float *__restrict__ tris, uint64_t T, VoxelizeParams p,
-                            uint64_t *__restrict__ count, TriRec *__restrict__ rec) {
-  const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= T) return;
+__device__ __forceinline__ void setup_one(const float *__restrict__ tri, const VoxelizeParams &p,
+                                          uint32_t grid_base, uint64_t *__restrict__ count,
+                                          TriRec *__restrict__ rec) {
   TriRec r;
-  tri_grid(tris + 9 * t, p, r.g);
+  tri_grid(tri, p, r.g);
   bool finite = true;
 #pragma unroll
   for (int i = 0; i < 9; i++) finite = finite && isfinite(r.g[i]);
@@ -83,7 +85,7 @@ __global__ void k_tri_setup(const float *__restrict__ tris, uint64_t T, Voxelize
     r_mn[a] = mx < mn ? 0 : mn;
     ext[a] = mx < mn ? 1u : uint32_t(mx - mn + 1);
   }
-  count[t] = c;
+  *count = c;
   if (c == 0) return;  // never read
   r.mnxy = uint32_t(r_mn[0]) | (uint32_t(r_mn[1]) << 16);
   r.mnz_ex = uint32_t(r_mn[2]) | (ext[0] << 16);
@@ -92,8 +94,33 @@ __global__ void k_tri_setup(const float *__restrict__ tris, uint64_t T, Voxelize
   r.flags = (finite ? 0u : 1u) | ((c >> 22) ? 2u : 0u);
   r.rcp_ex = 1.0f / float(ext[0]);
   r.rcp_exy = 1.0f / float(r.exy);
-  r.pad = 0;
-  rec[t] = r;
+  r.grid_base = grid_base;
+  *rec = r;
+}
+
+__global__ void k_tri_setup(const float *__restrict__ tris, uint64_t T, VoxelizeParams p,
+                            uint64_t *__restrict__ count, TriRec *__restrict__ rec) {
+  const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  setup_one(tris + 9 * t, p, 0u, count + t, rec + t);
+}
+
+// Many meshes: mesh g owns setup blocks [sb_begin[g], sb_begin[g+1]) and record slots
+// from sb_begin[g] * kSetupTB on; the slots past its triangle count hold count 0.
+constexpr int kSetupTB = 256;
+__global__ __launch_bounds__(kSetupTB) void k_tri_setup_batch(
+    const float *__restrict__ tris, const BatchMesh *__restrict__ mesh,
+    const uint32_t *__restrict__ sb_begin, uint32_t G, uint64_t *__restrict__ count,
+    TriRec *__restrict__ rec) {
+  const uint32_t g = dev::upper_index(sb_begin, G, blockIdx.x);
+  const BatchMesh &m = mesh[g];
+  const uint64_t tl = uint64_t(blockIdx.x - sb_begin[g]) * kSetupTB + threadIdx.x;
+  const uint64_t slot = uint64_t(blockIdx.x) * kSetupTB + threadIdx.x;
+  if (tl >= m.T) {
+    count[slot] = 0;
+    return;
+  }
+  setup_one(tris + 9 * (m.tri_begin + tl), m.p, uint32_t(m.b1_begin), count + slot, rec + slot);
 }
 
 // bstart[b] = triangle holding pair b * chunk (triangles with no candidate never do).
@@ -273,7 +300,7 @@ __device__ __forceinline__ bool finish_pair(const TriRec *__restrict__ rec, uint
   const bool hit = (r.flags & 1u) ? rest_passes<true>(v) : rest_passes<false>(v);
   const uint32_t code = dev::brick_code(x >> 3, y >> 3, z >> 3);
   const uint32_t m = dev::morton9(x, y, z);
-  key = uint64_t(code) * 8 + (m >> 6);
+  key = (uint64_t(r.grid_base) << 9) + uint64_t(code) * 8 + (m >> 6);
   bit = 1ull << (m & 63);
   return hit;
 }
@@ -489,30 +516,28 @@ __global__ __launch_bounds__(kWaves * 64) void k_sat(
 
 }  // namespace
 
-uint64_t run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizeParams &p) {
+namespace {
+
+// Scan of the Tp per-slot candidate counts in ctx->tri_count, sizing, and the SAT launch
+// into `leafgrid` / `brickbits`. Returns the pair count.
+uint64_t sat_after_setup(vxl_ctx *ctx, uint64_t Tp, unsigned long long *leafgrid,
+                         unsigned long long *brickbits) {
   hipStream_t s = ctx->stream;
   const int tb = 256;
-  ctx->tri_count.reserve(T * 8);
-  ctx->rec.reserve(T * sizeof(TriRec));
-  ctx->coff.reserve(T * 8 + 8);
   uint64_t *count = ctx->tri_count.as<uint64_t>();
   TriRec *rec = ctx->rec.as<TriRec>();
   uint64_t *coff = ctx->coff.as<uint64_t>();
   uint64_t *host = static_cast<uint64_t *>(ctx->host_pinned);
-  hipLaunchKernelGGL(k_tri_setup, dim3((T + tb - 1) / tb), dim3(tb), 0, s, d_tris, T, p, count,
-                     rec);
-  scan_exclusive_u64(count, coff, T, coff + T, ctx->scan_tmp, s);
-  VXL_HIP(hipMemcpyAsync(host, coff + T, 8, hipMemcpyDeviceToHost, s));
+  scan_exclusive_u64(count, coff, Tp, coff + Tp, ctx->scan_tmp, s);
+  VXL_HIP(hipMemcpyAsync(host, coff + Tp, 8, hipMemcpyDeviceToHost, s));
   VXL_HIP(hipStreamSynchronize(s));
   const uint64_t P = host[0];
   if (P == 0) return 0;
-  uint64_t *ccount = count;
-  const uint64_t Tc = T;
 
-  // Work units of kChunksPerUnit chunks of 64 pairs, handed out to a persistent grid of
-  // waves by an atomic counter; ustart[u] = triangle holding the first pair of unit u.
-  // Units are long contiguous runs (about sat_units_per_wave per resident wave) so that
-  // each wave's merge table sees spatially coherent hits.
+  // Work units of at least kChunksPerUnit chunks of 64 pairs, handed out to a persistent
+  // grid of waves by an atomic counter; ustart[u] = triangle holding the first pair of
+  // unit u. Units are long contiguous runs (about sat_units_per_wave per resident wave)
+  // so that each wave's merge table sees spatially coherent hits.
   const uint64_t nchunks = (P + 63) / 64;
   const uint64_t max_waves =
       uint64_t(ctx->cu_count > 0 ? ctx->cu_count : 256) * ctx->sat_waves_per_cu;
@@ -527,16 +552,46 @@ uint64_t run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizePa
   ctx->bstart.reserve(nunits * 4 + 64);
   uint32_t *ustart = ctx->bstart.as<uint32_t>();
   unsigned int *counter = reinterpret_cast<unsigned int *>(ustart + nunits + 8);
-  hipLaunchKernelGGL(k_bstart, dim3((Tc + tb - 1) / tb), dim3(tb), 0, s, coff, ccount, Tc,
+  hipLaunchKernelGGL(k_bstart, dim3((Tp + tb - 1) / tb), dim3(tb), 0, s, coff, count, Tp,
                      cpu * 64, nunits, ustart);
   VXL_HIP(hipMemsetAsync(counter, 0, 4, s));
   mark(ctx, 1);
-  hipLaunchKernelGGL(k_sat, dim3(uint32_t(blocks)), dim3(kWaves * 64), 0, s, rec, coff, ustart, T,
-                     P, cpu, counter, ctx->leafgrid.as<unsigned long long>(),
-                     ctx->bitmap.as<unsigned long long>());
+  hipLaunchKernelGGL(k_sat, dim3(uint32_t(blocks)), dim3(kWaves * 64), 0, s, rec, coff, ustart,
+                     Tp, P, cpu, counter, leafgrid, brickbits);
   mark(ctx, 2);
   VXL_HIP(hipGetLastError());
   return P;
 }
+
+void reserve_records(vxl_ctx *ctx, uint64_t Tp) {
+  ctx->tri_count.reserve(Tp * 8);
+  ctx->rec.reserve(Tp * sizeof(TriRec));
+  ctx->coff.reserve(Tp * 8 + 8);
+}
+
+}  // namespace
+
+uint64_t run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizeParams &p) {
+  const int tb = 256;
+  reserve_records(ctx, T);
+  hipLaunchKernelGGL(k_tri_setup, dim3((T + tb - 1) / tb), dim3(tb), 0, ctx->stream, d_tris, T,
+                     p, ctx->tri_count.as<uint64_t>(), ctx->rec.as<TriRec>());
+  return sat_after_setup(ctx, T, ctx->leafgrid.as<unsigned long long>(),
+                         ctx->bitmap.as<unsigned long long>());
+}
+
+uint64_t run_sat_batch(vxl_ctx *ctx, const float *d_tris, const BatchMesh *d_mesh,
+                       const uint32_t *d_sb_begin, uint32_t G, uint32_t setup_blocks,
+                       unsigned long long *leafgrid, unsigned long long *brickbits) {
+  const uint64_t Tp = uint64_t(setup_blocks) * kSetupTB;
+  reserve_records(ctx, Tp);
+  if (setup_blocks)
+    hipLaunchKernelGGL(k_tri_setup_batch, dim3(setup_blocks), dim3(kSetupTB), 0, ctx->stream,
+                       d_tris, d_mesh, d_sb_begin, G, ctx->tri_count.as<uint64_t>(),
+                       ctx->rec.as<TriRec>());
+  return sat_after_setup(ctx, Tp, leafgrid, brickbits);
+}
+
+uint64_t setup_slots_per_block() { return kSetupTB; }
 
 }  // namespace vxl

@@ -95,7 +95,24 @@ struct vxl_ctx {
   hipEvent_t ev[8] = {};  // ev[0] call start, ev[1]/ev[2] around the SAT kernel, ev[3] end
   float last_ms[8] = {0};
   std::set<vxl_tree *> trees;  // live trees created on this context
+  // batch scratch: pooled leaf grids / brick bitmaps of a group of meshes (zero between
+  // calls), per-group descriptors, pinned staging for their upload
+  vxl::DevBuf pool_grid, pool_b1, batch_scratch, batch_l2, batch_desc, trav_desc;
+  bool pool_dirty = false;
+  uint64_t pool_budget = 16ull << 30;  // leaf-grid bytes one batch group may use
+  void *batch_host = nullptr;
+  size_t batch_host_bytes = 0, batch_host_used = 0;
+  hipEvent_t staging_free = nullptr;  // recorded after the last upload from batch_host
+  hipEvent_t trav_done = nullptr;     // recorded after the last batched traversal kernel
 };
+
+namespace vxl {
+// One device allocation shared by the trees of one batch group; freed with the last tree.
+struct SharedBlock {
+  void *base = nullptr;
+  uint32_t refs = 0;
+};
+}  // namespace vxl
 
 struct vxl_tree {
   vxl_ctx *ctx = nullptr;
@@ -108,6 +125,7 @@ struct vxl_tree {
   uint64_t level_nodes[16] = {0};
   size_t bytes = 0;
   bool pooled = false;  // one stream-ordered allocation starting at `nodes`
+  vxl::SharedBlock *shared = nullptr;  // set for trees of a batch: arrays live in `shared`
 };
 
 namespace vxl {
@@ -136,8 +154,51 @@ struct VoxelizeParams {
   float tx[3];     // inv * (-origin)
   float eps;
 };
+
+// One mesh of a batch group, as the device sees it. Mesh g owns the leaf-grid words
+// [b1_begin * 512, (b1_begin + w1r) * 512) of the pooled grid and brick-bitmap words
+// [b1_begin, b1_begin + w1r), with w1r = w1 rounded up to 64 so that every upper bitmap
+// word belongs to one mesh; its records occupy slots [slot_begin, slot_end).
+struct BatchMesh {
+  uint64_t tri_begin;   // first triangle in the input array
+  uint64_t T;           // triangles
+  uint64_t b1_begin;    // first brick-bitmap word of the mesh's region
+  uint64_t w1;          // brick-bitmap words in use: ceil((n/8)^3 / 64)
+  uint64_t slot_begin, slot_end;
+  uint32_t n, k;
+  VoxelizeParams p;
+};
+
+// Per-level arrays of one tree's upper levels (levels 2..5, n <= 2048).
+constexpr int kLevels = 6;
+struct LevelPtrs {
+  uint64_t win[kLevels];
+  uint32_t *cb[kLevels], *ni[kLevels], *desc[kLevels], *pos[kLevels], *bs[kLevels];
+  unsigned long long *a_mask[kLevels];
+  uint32_t *a_base[kLevels], *a_code[kLevels];
+  uint32_t *tot;  // per level L: tot[2L] = children, tot[2L+1] = nodes
+};
+
+// Per-mesh sizes of a batch group, read back once before the trees are allocated.
+struct BatchTotals {
+  uint64_t leaves, n2, leaf_begin, n2_begin, pairs, nodes[3];  // nodes of levels 3, 4, 5
+};
+
+// Pinned staging a batch group of G meshes uses (descriptors, block starts, level
+// pointers, sizes, node-array pointers), each part 256-byte aligned.
+inline size_t batch_staging_bytes(uint32_t G) {
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  return up(G * sizeof(BatchMesh)) + up((G + 1) * 4) + up(G * sizeof(LevelPtrs)) +
+         up(G * sizeof(BatchTotals)) + up(G * 8) + 1024;
+}
 // Runs setup + SAT into ctx->leafgrid / ctx->bitmap (level-1). Returns pair count.
 uint64_t run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizeParams &p);
+// Setup + SAT of a batch group into the pooled grid; mesh g owns setup blocks
+// [sb_begin[g], sb_begin[g+1]) (setup_slots_per_block() record slots each).
+uint64_t run_sat_batch(vxl_ctx *ctx, const float *d_tris, const BatchMesh *d_mesh,
+                       const uint32_t *d_sb_begin, uint32_t G, uint32_t setup_blocks,
+                       unsigned long long *leafgrid, unsigned long long *brickbits);
+uint64_t setup_slots_per_block();
 
 // tree.hip
 // Builds a School-B tree from the brick bitmap + leafgrid in ctx (clearing both).
@@ -149,10 +210,37 @@ void words_to_grid(vxl_ctx *ctx, uint32_t n, const uint32_t *d_words);
 // Ensures leafgrid/bitmap capacity for n; clears them if dirty.
 void prepare_grid(vxl_ctx *ctx, uint32_t n);
 size_t bitmap_level1_words(uint32_t n);
+// Trees of a batch group from the pooled grid (cleared again on the way). meshes[g] as
+// uploaded to d_mesh; W1 = brick-bitmap words of the group. Writes trees[g] and, when
+// non-null, pairs[g]. One host sync.
+void build_trees_batch(vxl_ctx *ctx, const std::vector<BatchMesh> &meshes,
+                       const BatchMesh *d_mesh, uint64_t W1, vxl_tree **trees, uint64_t *pairs);
 
 // dda.hip
 void launch_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, uint64_t R,
                      uint32_t *d_hits, double *d_t, hipStream_t s);
+// One launch over S ray segments; segment i traverses trees[i] with rays
+// [ray_off[i], ray_off[i+1]) of d_rays.
+void launch_traverse_batch(vxl_ctx *ctx, const vxl_tree *const *trees, uint32_t S,
+                           const uint64_t *ray_off, const float *d_rays, uint32_t *d_hits,
+                           double *d_t, hipStream_t s);
+
+// batch.hip
+// Pinned host staging for descriptor uploads: staging_begin waits until earlier uploads
+// from it are done and hands out `bytes` (more via staging_take); staging_end records
+// the point after which it may be reused.
+void *staging_begin(vxl_ctx *ctx, size_t bytes);
+void *staging_take(vxl_ctx *ctx, size_t bytes);
+void staging_end(vxl_ctx *ctx, hipStream_t s);
+// One mesh handed to voxelize_batch: out[index] / pairs[index] receive its results.
+struct BatchInput {
+  uint32_t index;
+  uint64_t tri_begin, T;
+  uint32_t n, k;  // 2 <= k <= 4
+  VoxelizeParams p;
+};
+void voxelize_batch(vxl_ctx *ctx, const float *d_tris, const std::vector<BatchInput> &in,
+                    vxl_tree **out, uint64_t *pairs);
 
 void tree_free_device(vxl_tree *t);
 
@@ -189,6 +277,18 @@ __host__ __device__ __forceinline__ uint32_t brick_code(uint32_t bx, uint32_t by
 __host__ __device__ __forceinline__ uint32_t morton9(uint32_t x, uint32_t y, uint32_t z) {
   return (x & 1) | ((y & 1) << 1) | ((z & 1) << 2) | ((x & 2) << 2) | ((y & 2) << 3) |
          ((z & 2) << 4) | ((x & 4) << 4) | ((y & 4) << 5) | ((z & 4) << 6);
+}
+// Largest g in [0, G) with begin[g] <= x, for a non-decreasing begin[] (the segment
+// holding x when segment g spans [begin[g], begin[g+1])). Callers pass a wave-uniform x.
+template <class T>
+__device__ __forceinline__ uint32_t upper_index(const T *__restrict__ begin, uint32_t G, T x) {
+  uint32_t lo = 0, hi = G - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (begin[mid] <= x) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
 }
 // 6-bit child index of a 4^3 child (voxel-core/src/node.rs:78-83).
 __host__ __device__ __forceinline__ uint32_t child_bit(uint32_t x, uint32_t y, uint32_t z) {

@@ -35,7 +35,8 @@ EXPORTS = [
     "vxl_voxelize_dense", "vxl_voxelize_dense_dev", "vxl_voxelize_tree", "vxl_voxelize_tree_dev",
     "vxl_tree_from_words", "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload",
     "vxl_tree_free", "vxl_traverse", "vxl_traverse_dev", "vxl_set_timing", "vxl_last_timing",
-    "vxl_set_tuning",
+    "vxl_set_tuning", "vxl_voxelize_batch", "vxl_voxelize_batch_dev", "vxl_traverse_batch",
+    "vxl_traverse_batch_dev", "vxl_set_batch_budget", "vxl_autotune_batch",
 ]
 
 
@@ -56,6 +57,11 @@ class TreeInfo(C.Structure):
     _fields_ = [("n", C.c_uint32), ("k", C.c_uint32), ("node_count", C.c_uint64),
                 ("leaf_count", C.c_uint64), ("level_nodes", C.c_uint64 * 16),
                 ("device_bytes", C.c_uint64)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("sat_waves_per_cu", C.c_int), ("sat_units_per_wave", C.c_int),
+                ("dda_block", C.c_int), ("voxelize_ms", C.c_float), ("traverse_ms", C.c_float)]
 
 
 _lib = None
@@ -100,10 +106,20 @@ def load(path: str = LIB_PATH):
     L.vxl_last_timing.argtypes = [_vp, C.POINTER(C.c_float)]
     L.vxl_last_timing.restype = None
     L.vxl_set_tuning.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
+    L.vxl_voxelize_batch.argtypes = [_vp, _vp, _vp, C.c_uint32, _vp, C.c_float, _vp, _vp]
+    L.vxl_voxelize_batch_dev.argtypes = [_vp, _vp, _vp, C.c_uint32, _vp, C.c_float, _vp, _vp,
+                                         _vp]
+    L.vxl_traverse_batch.argtypes = [_vp, _vp, C.c_uint32, _vp, _vp, _vp, _vp]
+    L.vxl_traverse_batch_dev.argtypes = [_vp, _vp, C.c_uint32, _vp, _vp, _vp, _vp, _vp]
+    L.vxl_set_batch_budget.argtypes = [_vp, _u64]
+    L.vxl_autotune_batch.argtypes = [_vp, _vp, _vp, C.c_uint32, _vp, C.c_float, _vp, _vp, _vp,
+                                     _vp, C.POINTER(Tuning)]
     for name in ("vxl_ctx_create", "vxl_voxelize_dense", "vxl_voxelize_dense_dev",
                  "vxl_voxelize_tree", "vxl_voxelize_tree_dev", "vxl_tree_from_words",
                  "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload", "vxl_traverse",
-                 "vxl_traverse_dev", "vxl_set_tuning"):
+                 "vxl_traverse_dev", "vxl_set_tuning", "vxl_voxelize_batch",
+                 "vxl_voxelize_batch_dev", "vxl_traverse_batch", "vxl_traverse_batch_dev",
+                 "vxl_set_batch_budget", "vxl_autotune_batch"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -268,6 +284,90 @@ class Context:
     def set_tuning(self, sat_waves_per_cu=0, sat_units_per_wave=0, dda_block=0):
         _check(load().vxl_set_tuning(self.handle, sat_waves_per_cu, sat_units_per_wave, dda_block))
 
+    # -- batches ---------------------------------------------------------------------------
+    @staticmethod
+    def _grids(grids):
+        arr = (Grid * max(len(grids), 1))()
+        for i, g in enumerate(grids):
+            n, origin, vs = g
+            arr[i] = make_grid(n, origin, vs)
+        return arr
+
+    def _batch_trees(self, out, pairs, count):
+        trees = []
+        for i in range(count):
+            t = Tree(out[i], self)
+            t.pairs = int(pairs[i])
+            trees.append(t)
+        return trees
+
+    def voxelize_batch(self, meshes, grids, eps=1e-4):
+        """meshes: list of (T_i, 9) f32 arrays; grids: list of (n, origin, voxel_size)."""
+        count = len(meshes)
+        off = np.zeros(count + 1, np.uint64)
+        off[1:] = np.cumsum([len(m) for m in meshes]) if count else []
+        tris = (np.ascontiguousarray(np.concatenate([np.asarray(m, np.float32).reshape(-1, 9)
+                                                     for m in meshes]), np.float32)
+                if count else np.zeros((0, 9), np.float32))
+        out = (_vp * max(count, 1))()
+        pairs = np.zeros(max(count, 1), np.uint64)
+        _check(load().vxl_voxelize_batch(self.handle, _ptr(tris), _ptr(off), count,
+                                         self._grids(grids), eps, out, _ptr(pairs)))
+        return self._batch_trees(out, pairs, count)
+
+    def voxelize_batch_dev(self, d_tris: int, tri_offsets, grids, eps=1e-4, stream: int = 0):
+        off = np.ascontiguousarray(tri_offsets, np.uint64)
+        count = len(off) - 1
+        out = (_vp * max(count, 1))()
+        pairs = np.zeros(max(count, 1), np.uint64)
+        _check(load().vxl_voxelize_batch_dev(self.handle, _vp(d_tris), _ptr(off), count,
+                                             self._grids(grids), eps, out, _ptr(pairs),
+                                             _vp(stream)))
+        return self._batch_trees(out, pairs, count)
+
+    def traverse_batch(self, trees, rays_list, want_t=True):
+        """rays_list[i] (R_i x 6 f32) against trees[i]; returns per-segment (hits, t)."""
+        count = len(trees)
+        off = np.zeros(count + 1, np.uint64)
+        off[1:] = np.cumsum([len(r) for r in rays_list]) if count else []
+        rays = (np.ascontiguousarray(np.concatenate([np.asarray(r, np.float32).reshape(-1, 6)
+                                                     for r in rays_list]), np.float32)
+                if count else np.zeros((0, 6), np.float32))
+        R = rays.shape[0]
+        hits = np.zeros((R, 4), np.uint32)
+        t = np.zeros(R, np.float64) if want_t else None
+        handles = (_vp * max(count, 1))(*[tr._h for tr in trees])
+        _check(load().vxl_traverse_batch(self.handle, handles, count, _ptr(rays), _ptr(off),
+                                         _ptr(hits), _ptr(t) if want_t else None))
+        segs = [(int(off[i]), int(off[i + 1])) for i in range(count)]
+        return [(hits[a:b], t[a:b] if want_t else None) for a, b in segs]
+
+    def traverse_batch_dev(self, trees, d_rays: int, ray_offsets, d_hits: int, d_t: int = 0,
+                           stream: int = 0):
+        off = np.ascontiguousarray(ray_offsets, np.uint64)
+        count = len(trees)
+        handles = (_vp * max(count, 1))(*[tr._h for tr in trees])
+        _check(load().vxl_traverse_batch_dev(self.handle, handles, count, _vp(d_rays), _ptr(off),
+                                             _vp(d_hits), _vp(d_t) if d_t else None,
+                                             _vp(stream)))
+
+    def set_batch_budget(self, nbytes: int):
+        _check(load().vxl_set_batch_budget(self.handle, int(nbytes)))
+
+    def autotune_batch(self, d_tris: int, tri_offsets, grids, d_rays: int = 0, ray_offsets=None,
+                       d_hits: int = 0, d_t: int = 0, eps=1e-4):
+        off = np.ascontiguousarray(tri_offsets, np.uint64)
+        roff = np.ascontiguousarray(ray_offsets, np.uint64) if ray_offsets is not None else None
+        res = Tuning()
+        _check(load().vxl_autotune_batch(self.handle, _vp(d_tris), _ptr(off), len(off) - 1,
+                                         self._grids(grids), eps, _vp(d_rays) if d_rays else None,
+                                         _ptr(roff) if roff is not None else None,
+                                         _vp(d_hits) if d_hits else None,
+                                         _vp(d_t) if d_t else None, C.byref(res)))
+        return dict(sat_waves_per_cu=res.sat_waves_per_cu,
+                    sat_units_per_wave=res.sat_units_per_wave, dda_block=res.dda_block,
+                    voxelize_ms=res.voxelize_ms, traverse_ms=res.traverse_ms)
+
 
 # ---- reference-shaped API (voxelizer::GpuVoxelizer, voxel_gpu::GpuTraverser) --------------
 @dataclass
@@ -325,5 +425,5 @@ class GpuTraverser:
         return [tuple(map(int, h[:3])) if h[3] == 1 else None for h in hits]
 
 
-__all__ = ["Context", "Tree", "Grid", "VxlError", "load", "make_grid", "EXPORTS", "GpuVoxelizer",
+__all__ = ["Context", "Tree", "Grid", "Tuning", "VxlError", "load", "make_grid", "EXPORTS", "GpuVoxelizer",
            "GpuTraverser", "VoxelGrid", "VoxelizeOpts", "VoxelizationOutput", "LIB_PATH"]

@@ -129,9 +129,56 @@ vxl_status vxl_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *rays, u
 vxl_status vxl_traverse_dev(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, uint64_t R,
                             uint32_t *d_hits, double *d_t, void *stream);
 
+/* ---- batches (no single reference counterpart: each replaces a host loop of the calls
+ *      above, e.g. the per-mesh GpuVoxelizer::voxelize_surface -> to_school_b ->
+ *      GpuTraverser::new/traverse sequence of voxel-cli/src/main.rs:1288-1300 and
+ *      :785-792, over many meshes and ray batches).
+ * Meshes: d_tris holds every mesh's triangles back to back (T x 9 f32, device);
+ * tri_offsets[count+1] (host) delimits mesh g = triangles [tri_offsets[g],
+ * tri_offsets[g+1]); grids[count]; out[count] receives one tree per mesh; pairs_out
+ * (nullable) one pair count per mesh. Each tree equals the one vxl_voxelize_tree_dev
+ * returns for that mesh alone. Meshes run in groups whose dense grids fit the batch
+ * budget (vxl_set_batch_budget): per group one setup, one SAT and one tree-build pass
+ * over all its meshes, with two host syncs in total. Trees of one group share one device
+ * allocation, released when the last of them is freed. */
+vxl_status vxl_voxelize_batch_dev(vxl_ctx *ctx, const float *d_tris, const uint64_t *tri_offsets,
+                                  uint32_t count, const vxl_grid *grids, float epsilon,
+                                  vxl_tree **out, uint64_t *pairs_out, void *stream);
+/* Host-pointer form; tri_offsets[0] must be 0. Synchronous. */
+vxl_status vxl_voxelize_batch(vxl_ctx *ctx, const float *tris, const uint64_t *tri_offsets,
+                              uint32_t count, const vxl_grid *grids, float epsilon, vxl_tree **out,
+                              uint64_t *pairs_out);
+/* Rays: segment g = rays [ray_offsets[g], ray_offsets[g+1]) of d_rays (R x 6 f32) against
+ * trees[g], in ONE kernel launch; hits / t as for vxl_traverse_dev, indexed like the rays.
+ * Results equal per-segment vxl_traverse_dev calls. */
+vxl_status vxl_traverse_batch_dev(vxl_ctx *ctx, const vxl_tree *const *trees, uint32_t count,
+                                  const float *d_rays, const uint64_t *ray_offsets,
+                                  uint32_t *d_hits, double *d_t, void *stream);
+/* Host-pointer form; ray_offsets[0] must be 0. Synchronous. */
+vxl_status vxl_traverse_batch(vxl_ctx *ctx, const vxl_tree *const *trees, uint32_t count,
+                              const float *rays, const uint64_t *ray_offsets, uint32_t *hits,
+                              double *t);
+/* Dense-grid bytes one batch group may use (default 16 GiB; 1..128 GiB). */
+vxl_status vxl_set_batch_budget(vxl_ctx *ctx, uint64_t bytes);
+
+typedef struct vxl_tuning {
+  int sat_waves_per_cu, sat_units_per_wave, dda_block;
+  float voxelize_ms, traverse_ms; /* batch times with the chosen values */
+} vxl_tuning;
+/* Autotuner: times vxl_voxelize_batch_dev on the given meshes for each SAT launch shape
+ * (resident waves per CU x work units per wave), keeps the fastest, then times
+ * vxl_traverse_batch_dev on the given rays (nullable) for each DDA workgroup size; the
+ * chosen values stay set on ctx and are returned in *result. Trees made while tuning are
+ * freed; d_hits / d_t are overwritten. */
+vxl_status vxl_autotune_batch(vxl_ctx *ctx, const float *d_tris, const uint64_t *tri_offsets,
+                              uint32_t count, const vxl_grid *grids, float epsilon,
+                              const float *d_rays, const uint64_t *ray_offsets, uint32_t *d_hits,
+                              double *d_t, vxl_tuning *result);
+
 /* ---- timing: when enabled, the last voxelize/traverse call records per-stage
  * device times (ms) measured with HIP events on the stream the kernels ran on.
- * stage 0 = setup+scan, 1 = SAT, 2 = tree build, 3 = traverse. */
+ * stage 0 = setup+scan, 1 = SAT, 2 = tree build, 3 = traverse, 4 = whole voxelize
+ * batch, 5 = traverse batch. */
 void vxl_set_timing(vxl_ctx *ctx, int enable);
 void vxl_last_timing(vxl_ctx *ctx, float out_ms[8]);
 
