@@ -115,6 +115,116 @@ def bench_dda(ctx, torch, W, reps=5):
     return out
 
 
+def bench_batch(ctx, torch, W, reps=3):
+    """512 meshes, each followed by its ray batch: 512 plain per-mesh launches
+    (vxl_voxelize_tree_dev + vxl_traverse_dev per mesh) against the batched form
+    (vxl_voxelize_batch_dev + vxl_traverse_batch_dev over all of them), after one
+    autotuner pass; inputs resident in HBM; results compared tree for tree, hit for hit."""
+    import numpy as np
+    plan = W.batch_plan()
+    log(f"batch: generating {len(plan)} meshes + ray batches")
+    meshes = [W.batch_mesh(p) for p in plan]
+    rays = [W.batch_rays(p) for p in plan]
+    toff = np.zeros(len(plan) + 1, np.uint64)
+    toff[1:] = np.cumsum([len(m) for m in meshes])
+    roff = np.zeros(len(plan) + 1, np.uint64)
+    roff[1:] = np.cumsum([len(r) for r in rays])
+    T, R = int(toff[-1]), int(roff[-1])
+    d_tris = torch.from_numpy(np.concatenate(meshes)).to("cuda")
+    d_rays = torch.from_numpy(np.concatenate(rays)).to("cuda")
+    del meshes, rays
+    grids = [(p["n"], (0, 0, 0), p["voxel_size"]) for p in plan]
+    hb = torch.empty((R, 4), dtype=torch.int32, device="cuda")
+    tb = torch.empty(R, dtype=torch.float64, device="cuda")
+    hp = torch.empty((R, 4), dtype=torch.int32, device="cuda")
+    tp = torch.empty(R, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    log(f"batch: autotune (T={T}, R={R})")
+    tune = ctx.autotune_batch(d_tris.data_ptr(), toff, grids, d_rays.data_ptr(), roff,
+                              hb.data_ptr(), tb.data_ptr())
+    tp0, rp0, hp0, tpp0 = d_tris.data_ptr(), d_rays.data_ptr(), hp.data_ptr(), tp.data_ptr()
+
+    def plain(vox=True, trav=True, trees=None):
+        out = trees or []
+        for i, p in enumerate(plan):
+            if vox:
+                a, z = int(toff[i]), int(toff[i + 1])
+                out.append(ctx.voxelize_tree_dev(tp0 + 36 * a, z - a, p["n"], (0, 0, 0),
+                                                 p["voxel_size"]))
+            if trav:
+                a, z = int(roff[i]), int(roff[i + 1])
+                ctx.traverse_dev(out[i], rp0 + 24 * a, z - a, hp0 + 16 * a, tpp0 + 8 * a)
+        torch.cuda.synchronize()
+        return out
+
+    def batched(vox=True, trav=True, trees=None):
+        out = trees
+        if vox:
+            out = ctx.voxelize_batch_dev(tp0, toff, grids)
+        if trav:
+            ctx.traverse_batch_dev(out, rp0, roff, hb.data_ptr(), tb.data_ptr())
+        torch.cuda.synchronize()
+        return out
+
+    def timed(fn, **kw):
+        t0 = time.perf_counter()
+        out = fn(**kw)
+        return (time.perf_counter() - t0) * 1e3, out
+
+    res = {k: [] for k in ("plain", "batched", "plain_vox", "batched_vox", "plain_trav",
+                           "batched_trav")}
+    for f in plain(), batched():  # warm-up: scratch, pools
+        for t in f:
+            t.free()
+    keep_p = keep_b = None
+    for rep in range(reps):
+        log(f"batch: rep {rep}")
+        for name, fn in (("plain", plain), ("batched", batched)):
+            ms, trees = timed(fn)
+            res[name].append(ms)
+            ms, _ = timed(fn, vox=False, trees=trees)
+            res[name + "_trav"].append(ms)
+            if rep == reps - 1:
+                if name == "plain":
+                    keep_p = trees
+                else:
+                    keep_b = trees
+            else:
+                for t in trees:
+                    t.free()
+            ms, trees = timed(fn, trav=False)
+            res[name + "_vox"].append(ms)
+            for t in trees:
+                t.free()
+    same_hits = bool(torch.equal(hp, hb)) and bool(torch.equal(tp, tb))
+    same_trees = True
+    for i, (a, b) in enumerate(zip(keep_p, keep_b)):
+        if (a.leaf_count, a.node_count, a.pairs) != (b.leaf_count, b.node_count, b.pairs):
+            same_trees = False
+        elif i % 8 == 0:
+            for x, y in zip(a.download(), b.download()):
+                same_trees = same_trees and np.array_equal(x, y)
+    for t in keep_p + keep_b:
+        t.free()
+    med = {k: sorted(v)[len(v) // 2] for k, v in res.items()}
+    return dict(
+        workload=(f"{len(plan)} meshes (T log-U[100, 2e5], sphere/soup coin, m in "
+                  "{64,...,1024}) each followed by log-U[1k, 1M] random rays"),
+        meshes=len(plan), triangles=T, rays=R,
+        plain_ms=round(med["plain"], 2), batched_ms=round(med["batched"], 2),
+        speedup=round(med["plain"] / med["batched"], 3),
+        voxelize={"plain_ms": round(med["plain_vox"], 2),
+                  "batched_ms": round(med["batched_vox"], 2),
+                  "speedup": round(med["plain_vox"] / med["batched_vox"], 3),
+                  "batched_mtris_per_s": round(T / med["batched_vox"] / 1e3, 1)},
+        traverse={"plain_ms": round(med["plain_trav"], 2),
+                  "batched_ms": round(med["batched_trav"], 2),
+                  "speedup": round(med["plain_trav"] / med["batched_trav"], 3),
+                  "batched_mrays_per_s": round(R / med["batched_trav"] / 1e3, 1)},
+        tuning=tune, identical_trees=same_trees, identical_hits=same_hits,
+        timing="wall clock, median of %d, inputs resident in HBM" % reps)
+
+
 def dda_bytes_per_ray(tree, rays, sample=20000):
     """Algorithmic HBM bytes per ray = 24 (ray) + 16 (hit) + 8 (t) + 68 per leaf descent +
     12 per node descent below the root, with descents counted by the oracle's mirror."""
@@ -157,6 +267,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dda", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
@@ -216,6 +327,8 @@ def main():
                                "node_descents": dn, "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                                "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
         line["dda"] = dda
+    if not args.no_batch:
+        line["batched"] = bench_batch(ctx, torch, W)
     if not args.no_cpu:
         log("cpu baseline (oracle) on the full workload")
         model, _ = host_cpu()

@@ -219,3 +219,39 @@ def box_rays(count: int, n: int, seed=0x1357_9BDF_2468_ACE0) -> np.ndarray:
         keep = np.linalg.norm(d, axis=1) >= 1e-3
         out.append(np.concatenate([o, d], 1)[keep])
     return np.concatenate(out)[:count]
+
+
+def _log_uniform_int(rng: SplitMix64, lo: int, hi: int) -> int:
+    u = float(rng.unit(1)[0])
+    return int(round(math.exp(math.log(lo) + u * (math.log(hi) - math.log(lo)))))
+
+
+def batch_plan(count=512, seed=0x5EED_0007, tris=(100, 200_000), rays=(1_000, 1_000_000),
+               resolutions=(64, 128, 256, 512, 1024)):
+    """The many-mesh workload: per mesh a triangle count log-uniform in ``tris``, a seeded
+    coin between a UV sphere of about that many triangles and a soup (edge lengths
+    log-normal about 2 voxels of the mesh's resolution), a resolution m uniform over
+    ``resolutions`` (voxel size 1/m, in Resolution grid_for(m)), and a ray batch of size
+    log-uniform in ``rays`` with origins U[0,m)^3 and directions uniform on S^2.
+    Returns one dict per mesh (kind, T, m, n, voxel_size, R, seeds), no arrays."""
+    rng = SplitMix64(seed)
+    plan = []
+    for i in range(count):
+        T = _log_uniform_int(rng, *tris)
+        sphere = bool(int(rng.u64(1)[0]) & 1)
+        m = resolutions[int(rng.u64(1)[0] % np.uint64(len(resolutions)))]
+        R = _log_uniform_int(rng, *rays)
+        n, vs = grid_for(m)
+        plan.append(dict(kind="sphere" if sphere else "soup", T=T, m=m, n=n, voxel_size=vs, R=R,
+                         mesh_seed=seed + 1000 + i, ray_seed=seed + 100_000 + i))
+    return plan
+
+
+def batch_mesh(entry) -> np.ndarray:
+    if entry["kind"] == "sphere":
+        return sphere_with_tris(entry["T"], seed=entry["mesh_seed"])
+    return soup(entry["T"], seed=entry["mesh_seed"], mu=math.log(2.0 / entry["m"]))
+
+
+def batch_rays(entry) -> np.ndarray:
+    return random_rays(entry["R"], entry["m"], seed=entry["ray_seed"])
