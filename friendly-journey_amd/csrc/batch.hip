@@ -65,8 +65,8 @@ void ensure_pool(vxl_ctx *ctx, uint64_t W1) {
   }
 }
 
-void run_group(vxl_ctx *ctx, const float *d_tris, const BatchInput *in, uint32_t G,
-               vxl_tree **out, uint64_t *pairs) {
+void run_group(vxl_ctx *ctx, const BatchInput *in, uint32_t G, vxl_tree **out,
+               uint64_t *pairs) {
   hipStream_t s = ctx->stream;
   const uint64_t per_block = setup_slots_per_block();
   staging_begin(ctx, batch_staging_bytes(G));
@@ -76,7 +76,7 @@ void run_group(vxl_ctx *ctx, const float *d_tris, const BatchInput *in, uint32_t
   uint64_t W1 = 0, blocks = 0;
   for (uint32_t g = 0; g < G; g++) {
     BatchMesh &m = meshes[g];
-    m.tri_begin = in[g].tri_begin;
+    m.tris = in[g].tris;
     m.T = in[g].T;
     m.b1_begin = W1;
     m.w1 = bitmap_level1_words(in[g].n);
@@ -99,7 +99,7 @@ void run_group(vxl_ctx *ctx, const float *d_tris, const BatchInput *in, uint32_t
   char *d = static_cast<char *>(ctx->batch_desc.ptr);
   VXL_HIP(hipMemcpyAsync(d, h_mesh, mesh_bytes + (G + 1) * 4, hipMemcpyHostToDevice, s));
   const BatchMesh *d_mesh = reinterpret_cast<const BatchMesh *>(d);
-  run_sat_batch(ctx, d_tris, d_mesh, reinterpret_cast<const uint32_t *>(d + mesh_bytes), G,
+  run_sat_batch(ctx, d_mesh, reinterpret_cast<const uint32_t *>(d + mesh_bytes), G,
                 uint32_t(blocks), ctx->pool_grid.as<unsigned long long>(),
                 ctx->pool_b1.as<unsigned long long>());
   std::vector<vxl_tree *> trees(G, nullptr);
@@ -115,8 +115,8 @@ void run_group(vxl_ctx *ctx, const float *d_tris, const BatchInput *in, uint32_t
 
 }  // namespace
 
-void voxelize_batch(vxl_ctx *ctx, const float *d_tris, const std::vector<BatchInput> &in,
-                    vxl_tree **out, uint64_t *pairs) {
+void voxelize_batch(vxl_ctx *ctx, const std::vector<BatchInput> &in, vxl_tree **out,
+                    uint64_t *pairs) {
   size_t i = 0;
   while (i < in.size()) {
     uint64_t words = 0;
@@ -127,7 +127,7 @@ void voxelize_batch(vxl_ctx *ctx, const float *d_tris, const std::vector<BatchIn
       words += w;
       j++;
     }
-    run_group(ctx, d_tris, in.data() + i, uint32_t(j - i), out, pairs);
+    run_group(ctx, in.data() + i, uint32_t(j - i), out, pairs);
     i = j;
   }
 }

@@ -190,14 +190,15 @@ void voxelize_batch_impl(vxl_ctx *ctx, const float *d_tris, const uint64_t *off,
         ctx->trees.insert(out[g]);
         if (pairs_out) pairs_out[g] = 0;
       } else if (ks[g] >= 2 && ks[g] <= 4) {
-        batch.push_back(vxl::BatchInput{g, off[g], T, grids[g].n, ks[g], make_params(&grids[g], eps)});
+        batch.push_back(vxl::BatchInput{g, d_tris + 9 * off[g], T, grids[g].n, ks[g],
+                                        make_params(&grids[g], eps)});
       } else {
         out[g] = voxelize_tree_impl(ctx, d_tris + 9 * off[g], T, &grids[g], ks[g], eps,
                                     pairs_out ? pairs_out + g : nullptr);
         ctx->trees.insert(out[g]);
       }
     }
-    vxl::voxelize_batch(ctx, d_tris, batch, out, pairs_out);
+    vxl::voxelize_batch(ctx, batch, out, pairs_out);
   } catch (...) {
     for (uint32_t g = 0; g < count; g++)
       if (out[g]) {
@@ -208,6 +209,19 @@ void voxelize_batch_impl(vxl_ctx *ctx, const float *d_tris, const uint64_t *off,
       }
     throw;
   }
+}
+
+// Segments of a batched traversal over one ray array delimited by offsets.
+std::vector<vxl::RaySegment> segments(const vxl_tree *const *trees, uint32_t count,
+                                      const uint64_t *off, const float *rays, uint32_t *hits,
+                                      double *t) {
+  std::vector<vxl::RaySegment> segs(count);
+  for (uint32_t g = 0; g < count; g++) {
+    const uint64_t a = off[g];
+    segs[g] = vxl::RaySegment{trees[g], rays + 6 * a, off[g + 1] - off[g], hits + 4 * a,
+                              t ? t + a : nullptr};
+  }
+  return segs;
 }
 
 void traverse_batch_checks(const vxl_tree *const *trees, uint32_t count, const uint64_t *off) {
@@ -229,6 +243,17 @@ void join_streams(hipStream_t from, hipStream_t to) {
 }
 
 }  // namespace
+
+namespace vxl {
+uint32_t check_voxelize_args(const vxl_grid *grid, uint64_t T, float eps) {
+  uint32_t k = 0;
+  validate_grid(grid, &k);
+  validate_count(T);
+  validate_eps(eps);
+  return k;
+}
+VoxelizeParams voxelize_params(const vxl_grid *grid, float eps) { return make_params(grid, eps); }
+}  // namespace vxl
 
 extern "C" {
 
@@ -629,7 +654,8 @@ vxl_status vxl_traverse_batch_dev(vxl_ctx *ctx, const vxl_tree *const *trees, ui
       throw Error(VXL_INVALID_ARG, "null ray or hit buffer");
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
     if (ctx->timing) VXL_HIP(hipEventRecord(ctx->ev[4], s));
-    vxl::launch_traverse_batch(ctx, trees, count, ray_offsets, d_rays, d_hits, d_t, s);
+    const auto segs = segments(trees, count, ray_offsets, d_rays, d_hits, d_t);
+    vxl::launch_traverse_batch(ctx, segs.data(), count, s);
     if (ctx->timing) {
       VXL_HIP(hipEventRecord(ctx->ev[5], s));
       VXL_HIP(hipEventSynchronize(ctx->ev[5]));
@@ -652,9 +678,9 @@ vxl_status vxl_traverse_batch(vxl_ctx *ctx, const vxl_tree *const *trees, uint32
     ctx->hits_io.reserve(R * 16);
     if (t) ctx->t_io.reserve(R * 8);
     VXL_HIP(hipMemcpyAsync(ctx->rays_io.ptr, rays, R * 24, hipMemcpyHostToDevice, s));
-    vxl::launch_traverse_batch(ctx, trees, count, ray_offsets, ctx->rays_io.as<float>(),
-                               ctx->hits_io.as<uint32_t>(), t ? ctx->t_io.as<double>() : nullptr,
-                               s);
+    const auto segs = segments(trees, count, ray_offsets, ctx->rays_io.as<float>(),
+                               ctx->hits_io.as<uint32_t>(), t ? ctx->t_io.as<double>() : nullptr);
+    vxl::launch_traverse_batch(ctx, segs.data(), count, s);
     VXL_HIP(hipMemcpyAsync(hits, ctx->hits_io.ptr, R * 16, hipMemcpyDeviceToHost, s));
     if (t) VXL_HIP(hipMemcpyAsync(t, ctx->t_io.ptr, R * 8, hipMemcpyDeviceToHost, s));
     VXL_HIP(hipStreamSynchronize(s));
@@ -732,13 +758,12 @@ vxl_status vxl_autotune_batch(vxl_ctx *ctx, const float *d_tris, const uint64_t 
         VXL_HIP(hipStreamSynchronize(ctx->stream));
         float bt = 1e30f;
         int bb = ctx->dda_block;
+        const auto segs = segments(trees.data(), count, ray_offsets, d_rays, d_hits, d_t);
         for (int b : blocks) {
           ctx->dda_block = b;
-          vxl::launch_traverse_batch(ctx, trees.data(), count, ray_offsets, d_rays, d_hits, d_t,
-                                     ctx->stream);  // warm
+          vxl::launch_traverse_batch(ctx, segs.data(), count, ctx->stream);  // warm
           VXL_HIP(hipEventRecord(e0, ctx->stream));
-          vxl::launch_traverse_batch(ctx, trees.data(), count, ray_offsets, d_rays, d_hits, d_t,
-                                     ctx->stream);
+          vxl::launch_traverse_batch(ctx, segs.data(), count, ctx->stream);
           VXL_HIP(hipEventRecord(e1, ctx->stream));
           VXL_HIP(hipEventSynchronize(e1));
           float ms = 0;

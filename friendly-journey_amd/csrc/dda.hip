@@ -331,23 +331,26 @@ __global__ __launch_bounds__(256) void k_traverse(
   traverse_ray(ri, rays, nodes, leafw, bounds, n, k, leaf_count, hits, tout);
 }
 
-// One segment of a batched traversal: rays [ray_begin, ray_end) against one tree.
+// One segment of a batched traversal, as the kernel sees it.
 struct TraverseSeg {
   const uint32_t *nodes, *leafw, *bounds;
-  uint64_t leaf_count, ray_begin, ray_end;
+  const float *rays;
+  uint4 *hits;
+  double *t;
+  uint64_t leaf_count, R;
   uint32_t n, k;
 };
 
 // Segment g owns blocks [blk_begin[g], blk_begin[g+1]), so every block reads one tree.
-__global__ __launch_bounds__(256) void k_traverse_batch(
-    const TraverseSeg *__restrict__ seg, const uint32_t *__restrict__ blk_begin, uint32_t S,
-    const float *__restrict__ rays, uint4 *__restrict__ hits, double *__restrict__ tout) {
+__global__ __launch_bounds__(256) void k_traverse_batch(const TraverseSeg *__restrict__ seg,
+                                                        const uint32_t *__restrict__ blk_begin,
+                                                        uint32_t S) {
   const uint32_t g = dev::upper_index(blk_begin, S, blockIdx.x);
   const TraverseSeg &sg = seg[g];
-  const uint64_t ri =
-      sg.ray_begin + uint64_t(blockIdx.x - blk_begin[g]) * blockDim.x + threadIdx.x;
-  if (ri >= sg.ray_end) return;
-  traverse_ray(ri, rays, sg.nodes, sg.leafw, sg.bounds, sg.n, sg.k, sg.leaf_count, hits, tout);
+  const uint64_t ri = uint64_t(blockIdx.x - blk_begin[g]) * blockDim.x + threadIdx.x;
+  if (ri >= sg.R) return;
+  traverse_ray(ri, sg.rays, sg.nodes, sg.leafw, sg.bounds, sg.n, sg.k, sg.leaf_count, sg.hits,
+               sg.t);
 }
 
 }  // namespace
@@ -365,27 +368,27 @@ void launch_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, ui
   VXL_HIP(hipGetLastError());
 }
 
-void launch_traverse_batch(vxl_ctx *ctx, const vxl_tree *const *trees, uint32_t S,
-                           const uint64_t *ray_off, const float *d_rays, uint32_t *d_hits,
-                           double *d_t, hipStream_t s) {
-  if (S == 0 || ray_off[S] == ray_off[0]) return;
+void launch_traverse_batch(vxl_ctx *ctx, const RaySegment *segs, uint32_t S, hipStream_t s) {
   const uint32_t tb = uint32_t(ctx->dda_block);
+  uint64_t blocks = 0;
+  for (uint32_t g = 0; g < S; g++) blocks += (segs[g].R + tb - 1) / tb;
+  if (blocks == 0) return;
+  if (blocks > 0xffffffffull)
+    throw Error(VXL_WORKGROUPS_EXCEEDED, "traverse batch exceeds the grid limit");
   const size_t seg_bytes = (S * sizeof(TraverseSeg) + 255) / 256 * 256;
   const size_t blk_bytes = ((S + 1) * 4 + 255) / 256 * 256;
   staging_begin(ctx, seg_bytes + blk_bytes);
   char *h = static_cast<char *>(staging_take(ctx, seg_bytes + blk_bytes));
   TraverseSeg *hs = reinterpret_cast<TraverseSeg *>(h);
   uint32_t *hb = reinterpret_cast<uint32_t *>(h + seg_bytes);
-  uint64_t blocks = 0;
+  blocks = 0;
   for (uint32_t g = 0; g < S; g++) {
-    const vxl_tree *t = trees[g];
-    const uint64_t R = ray_off[g + 1] - ray_off[g];
+    const vxl_tree *t = segs[g].tree;
     hs[g] = TraverseSeg{t->nodes, reinterpret_cast<const uint32_t *>(t->leaves), t->bounds,
-                        t->leaf_count, ray_off[g], ray_off[g + 1], t->n, t->k};
+                        segs[g].rays, reinterpret_cast<uint4 *>(segs[g].hits), segs[g].t,
+                        t->leaf_count, segs[g].R, t->n, t->k};
     hb[g] = uint32_t(blocks);
-    blocks += (R + tb - 1) / tb;
-    if (blocks > 0xffffffffull)
-      throw Error(VXL_WORKGROUPS_EXCEEDED, "traverse batch exceeds the grid limit");
+    blocks += (segs[g].R + tb - 1) / tb;
   }
   hb[S] = uint32_t(blocks);
   ctx->trav_desc.reserve(seg_bytes + blk_bytes);
@@ -397,8 +400,7 @@ void launch_traverse_batch(vxl_ctx *ctx, const vxl_tree *const *trees, uint32_t 
   staging_end(ctx, s);
   hipLaunchKernelGGL(k_traverse_batch, dim3(uint32_t(blocks)), dim3(tb), 0, s,
                      reinterpret_cast<const TraverseSeg *>(d),
-                     reinterpret_cast<const uint32_t *>(d + seg_bytes), S, d_rays,
-                     reinterpret_cast<uint4 *>(d_hits), d_t);
+                     reinterpret_cast<const uint32_t *>(d + seg_bytes), S);
   VXL_HIP(hipGetLastError());
   VXL_HIP(hipEventRecord(ctx->trav_done, s));
 }

@@ -721,6 +721,10 @@ static vxl_tree *build_tree_fast(vxl_ctx *ctx, uint32_t n, uint32_t k) {
   return t;
 }
 
+vxl_tree *empty_tree_device(vxl_ctx *ctx, uint32_t n, uint32_t k) {
+  return alloc_tree(ctx, n, k, 0, 0);
+}
+
 void build_trees_batch(vxl_ctx *ctx, const std::vector<BatchMesh> &meshes,
                        const BatchMesh *d_mesh, uint64_t W1, vxl_tree **trees, uint64_t *pairs) {
   hipStream_t s = ctx->stream;

@@ -109,7 +109,7 @@ __global__ void k_tri_setup(const float *__restrict__ tris, uint64_t T, Voxelize
 // from sb_begin[g] * kSetupTB on; the slots past its triangle count hold count 0.
 constexpr int kSetupTB = 256;
 __global__ __launch_bounds__(kSetupTB) void k_tri_setup_batch(
-    const float *__restrict__ tris, const BatchMesh *__restrict__ mesh,
+    const BatchMesh *__restrict__ mesh,
     const uint32_t *__restrict__ sb_begin, uint32_t G, uint64_t *__restrict__ count,
     TriRec *__restrict__ rec) {
   const uint32_t g = dev::upper_index(sb_begin, G, blockIdx.x);
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(kSetupTB) void k_tri_setup_batch(
     count[slot] = 0;
     return;
   }
-  setup_one(tris + 9 * (m.tri_begin + tl), m.p, uint32_t(m.b1_begin), count + slot, rec + slot);
+  setup_one(m.tris + 9 * tl, m.p, uint32_t(m.b1_begin), count + slot, rec + slot);
 }
 
 // bstart[b] = triangle holding pair b * chunk (triangles with no candidate never do).
@@ -580,14 +580,14 @@ uint64_t run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizePa
                          ctx->bitmap.as<unsigned long long>());
 }
 
-uint64_t run_sat_batch(vxl_ctx *ctx, const float *d_tris, const BatchMesh *d_mesh,
+uint64_t run_sat_batch(vxl_ctx *ctx, const BatchMesh *d_mesh,
                        const uint32_t *d_sb_begin, uint32_t G, uint32_t setup_blocks,
                        unsigned long long *leafgrid, unsigned long long *brickbits) {
   const uint64_t Tp = uint64_t(setup_blocks) * kSetupTB;
   reserve_records(ctx, Tp);
   if (setup_blocks)
     hipLaunchKernelGGL(k_tri_setup_batch, dim3(setup_blocks), dim3(kSetupTB), 0, ctx->stream,
-                       d_tris, d_mesh, d_sb_begin, G, ctx->tri_count.as<uint64_t>(),
+                       d_mesh, d_sb_begin, G, ctx->tri_count.as<uint64_t>(),
                        ctx->rec.as<TriRec>());
   return sat_after_setup(ctx, Tp, leafgrid, brickbits);
 }

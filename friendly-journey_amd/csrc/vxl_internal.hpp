@@ -160,7 +160,7 @@ struct VoxelizeParams {
 // [b1_begin, b1_begin + w1r), with w1r = w1 rounded up to 64 so that every upper bitmap
 // word belongs to one mesh; its records occupy slots [slot_begin, slot_end).
 struct BatchMesh {
-  uint64_t tri_begin;   // first triangle in the input array
+  const float *tris;    // the mesh's triangles (T x 9 f32, device)
   uint64_t T;           // triangles
   uint64_t b1_begin;    // first brick-bitmap word of the mesh's region
   uint64_t w1;          // brick-bitmap words in use: ceil((n/8)^3 / 64)
@@ -195,7 +195,7 @@ inline size_t batch_staging_bytes(uint32_t G) {
 uint64_t run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizeParams &p);
 // Setup + SAT of a batch group into the pooled grid; mesh g owns setup blocks
 // [sb_begin[g], sb_begin[g+1]) (setup_slots_per_block() record slots each).
-uint64_t run_sat_batch(vxl_ctx *ctx, const float *d_tris, const BatchMesh *d_mesh,
+uint64_t run_sat_batch(vxl_ctx *ctx, const BatchMesh *d_mesh,
                        const uint32_t *d_sb_begin, uint32_t G, uint32_t setup_blocks,
                        unsigned long long *leafgrid, unsigned long long *brickbits);
 uint64_t setup_slots_per_block();
@@ -219,11 +219,24 @@ void build_trees_batch(vxl_ctx *ctx, const std::vector<BatchMesh> &meshes,
 // dda.hip
 void launch_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, uint64_t R,
                      uint32_t *d_hits, double *d_t, hipStream_t s);
-// One launch over S ray segments; segment i traverses trees[i] with rays
-// [ray_off[i], ray_off[i+1]) of d_rays.
-void launch_traverse_batch(vxl_ctx *ctx, const vxl_tree *const *trees, uint32_t S,
-                           const uint64_t *ray_off, const float *d_rays, uint32_t *d_hits,
-                           double *d_t, hipStream_t s);
+// One segment of a batched traversal: R rays at `rays` (6 f32 each) against `tree`,
+// results to hits (4 u32 each) and t (nullable).
+struct RaySegment {
+  const vxl_tree *tree;
+  const float *rays;
+  uint64_t R;
+  uint32_t *hits;
+  double *t;
+};
+// All segments in one kernel launch on stream s.
+void launch_traverse_batch(vxl_ctx *ctx, const RaySegment *segs, uint32_t S, hipStream_t s);
+
+// capi.hip: argument checks of a voxelize call (grid, triangle count, epsilon; returns k)
+// and the world-to-grid parameters.
+uint32_t check_voxelize_args(const vxl_grid *grid, uint64_t T, float eps);
+VoxelizeParams voxelize_params(const vxl_grid *grid, float eps);
+// tree.hip: an empty tree (one padded entry per array, like upload_structure's padding)
+vxl_tree *empty_tree_device(vxl_ctx *ctx, uint32_t n, uint32_t k);
 
 // batch.hip
 // Pinned host staging for descriptor uploads: staging_begin waits until earlier uploads
@@ -235,12 +248,13 @@ void staging_end(vxl_ctx *ctx, hipStream_t s);
 // One mesh handed to voxelize_batch: out[index] / pairs[index] receive its results.
 struct BatchInput {
   uint32_t index;
-  uint64_t tri_begin, T;
+  const float *tris;  // device
+  uint64_t T;
   uint32_t n, k;  // 2 <= k <= 4
   VoxelizeParams p;
 };
-void voxelize_batch(vxl_ctx *ctx, const float *d_tris, const std::vector<BatchInput> &in,
-                    vxl_tree **out, uint64_t *pairs);
+void voxelize_batch(vxl_ctx *ctx, const std::vector<BatchInput> &in, vxl_tree **out,
+                    uint64_t *pairs);
 
 void tree_free_device(vxl_tree *t);
 

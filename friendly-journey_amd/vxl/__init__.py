@@ -37,6 +37,9 @@ EXPORTS = [
     "vxl_tree_free", "vxl_traverse", "vxl_traverse_dev", "vxl_set_timing", "vxl_last_timing",
     "vxl_set_tuning", "vxl_voxelize_batch", "vxl_voxelize_batch_dev", "vxl_traverse_batch",
     "vxl_traverse_batch_dev", "vxl_set_batch_budget", "vxl_autotune_batch",
+    "vxl_server_create", "vxl_server_destroy", "vxl_server_hold", "vxl_server_submit_rays",
+    "vxl_server_submit_voxelize", "vxl_server_poll", "vxl_server_wait", "vxl_server_stats_get",
+    "vxl_server_reset_stats",
 ]
 
 
@@ -62,6 +65,21 @@ class TreeInfo(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("sat_waves_per_cu", C.c_int), ("sat_units_per_wave", C.c_int),
                 ("dda_block", C.c_int), ("voxelize_ms", C.c_float), ("traverse_ms", C.c_float)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("budget_bytes", C.c_uint64), ("batching", C.c_uint32),
+                ("max_batch_requests", C.c_uint32), ("max_batch_rays", C.c_uint64),
+                ("max_batch_tris", C.c_uint64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("batches", C.c_uint64), ("mean_batch_requests", C.c_double),
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double),
+                ("max_ms", C.c_double), ("ray_p50_ms", C.c_double), ("ray_p99_ms", C.c_double),
+                ("vox_p50_ms", C.c_double), ("vox_p99_ms", C.c_double),
+                ("throughput_rps", C.c_double), ("peak_bytes", C.c_uint64)]
 
 
 _lib = None
@@ -114,6 +132,20 @@ def load(path: str = LIB_PATH):
     L.vxl_set_batch_budget.argtypes = [_vp, _u64]
     L.vxl_autotune_batch.argtypes = [_vp, _vp, _vp, C.c_uint32, _vp, C.c_float, _vp, _vp, _vp,
                                      _vp, C.POINTER(Tuning)]
+    L.vxl_server_create.argtypes = [C.c_int, C.POINTER(ServerConfig), C.POINTER(_vp)]
+    L.vxl_server_destroy.argtypes = [_vp]
+    L.vxl_server_destroy.restype = None
+    L.vxl_server_hold.argtypes = [_vp, C.c_int64]
+    L.vxl_server_submit_rays.argtypes = [_vp, _vp, _vp, _u64, _vp, _vp, _pu64]
+    L.vxl_server_submit_voxelize.argtypes = [_vp, _vp, _u64, C.POINTER(Grid), C.c_float, _pu64]
+    L.vxl_server_poll.argtypes = [_vp, _u64, C.POINTER(C.c_int)]
+    L.vxl_server_wait.argtypes = [_vp, _u64, C.POINTER(_vp), _pu64, C.POINTER(C.c_double)]
+    L.vxl_server_stats_get.argtypes = [_vp, C.POINTER(ServerStats)]
+    L.vxl_server_reset_stats.argtypes = [_vp]
+    for name in ("vxl_server_create", "vxl_server_hold", "vxl_server_submit_rays",
+                 "vxl_server_submit_voxelize", "vxl_server_poll", "vxl_server_wait",
+                 "vxl_server_stats_get", "vxl_server_reset_stats"):
+        getattr(L, name).restype = C.c_int
     for name in ("vxl_ctx_create", "vxl_voxelize_dense", "vxl_voxelize_dense_dev",
                  "vxl_voxelize_tree", "vxl_voxelize_tree_dev", "vxl_tree_from_words",
                  "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload", "vxl_traverse",
@@ -369,6 +401,78 @@ class Context:
                     voxelize_ms=res.voxelize_ms, traverse_ms=res.traverse_ms)
 
 
+class Server:
+    """The request loop (continuous batching of ray queries and voxelize requests)."""
+
+    def __init__(self, device: int = 0, budget_bytes: int = 0, batching: bool = True,
+                 max_batch_requests: int = 0, max_batch_rays: int = 0, max_batch_tris: int = 0):
+        cfg = ServerConfig(budget_bytes, 1 if batching else 0, max_batch_requests,
+                           max_batch_rays, max_batch_tris)
+        h = _vp()
+        _check(load().vxl_server_create(device, C.byref(cfg), C.byref(h)))
+        self._h = h.value
+        self._ctx = _ServerLaneOwner(self)
+
+    def close(self):
+        if self._h:
+            load().vxl_server_destroy(_vp(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def hold(self, delta_bytes: int):
+        _check(load().vxl_server_hold(_vp(self._h), int(delta_bytes)))
+
+    def submit_rays(self, tree: "Tree", d_rays: int, R: int, d_hits: int, d_t: int = 0) -> int:
+        rid = C.c_uint64(0)
+        _check(load().vxl_server_submit_rays(_vp(self._h), tree.handle, _vp(d_rays), R,
+                                             _vp(d_hits), _vp(d_t) if d_t else None,
+                                             C.byref(rid)))
+        return int(rid.value)
+
+    def submit_voxelize(self, d_tris: int, T: int, n: int, origin=(0.0, 0.0, 0.0),
+                        voxel_size=1.0, eps=1e-4) -> int:
+        rid = C.c_uint64(0)
+        g = make_grid(n, origin, voxel_size)
+        _check(load().vxl_server_submit_voxelize(_vp(self._h), _vp(d_tris), T, C.byref(g), eps,
+                                                 C.byref(rid)))
+        return int(rid.value)
+
+    def poll(self, rid: int) -> bool:
+        d = C.c_int(0)
+        _check(load().vxl_server_poll(_vp(self._h), rid, C.byref(d)))
+        return bool(d.value)
+
+    def wait(self, rid: int, keep_tree: bool = True):
+        """(tree or None, pairs, latency_ms); the tree is freed unless keep_tree."""
+        out = _vp()
+        pairs = C.c_uint64(0)
+        lat = C.c_double(0)
+        _check(load().vxl_server_wait(_vp(self._h), rid, C.byref(out) if keep_tree else None,
+                                      C.byref(pairs), C.byref(lat)))
+        tree = Tree(out.value, self._ctx) if out.value else None
+        return tree, int(pairs.value), float(lat.value)
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(load().vxl_server_stats_get(_vp(self._h), C.byref(st)))
+        return {name: getattr(st, name) for name, _ in ServerStats._fields_}
+
+    def reset_stats(self):
+        _check(load().vxl_server_reset_stats(_vp(self._h)))
+
+
+class _ServerLaneOwner:
+    """Stands in for the Context a Tree keeps a reference to: the server's lane owns it."""
+
+    def __init__(self, server: Server):
+        self.server = server
+
+
 # ---- reference-shaped API (voxelizer::GpuVoxelizer, voxel_gpu::GpuTraverser) --------------
 @dataclass
 class VoxelizeOpts:
@@ -425,5 +529,5 @@ class GpuTraverser:
         return [tuple(map(int, h[:3])) if h[3] == 1 else None for h in hits]
 
 
-__all__ = ["Context", "Tree", "Grid", "Tuning", "VxlError", "load", "make_grid", "EXPORTS", "GpuVoxelizer",
+__all__ = ["Context", "Tree", "Grid", "Tuning", "Server", "VxlError", "load", "make_grid", "EXPORTS", "GpuVoxelizer",
            "GpuTraverser", "VoxelGrid", "VoxelizeOpts", "VoxelizationOutput", "LIB_PATH"]

@@ -255,3 +255,32 @@ def batch_mesh(entry) -> np.ndarray:
 
 def batch_rays(entry) -> np.ndarray:
     return random_rays(entry["R"], entry["m"], seed=entry["ray_seed"])
+
+
+def serve_plan(duration_s=2.0, rate=2000.0, ray_share=0.7, trees=64, seed=0x5EED_0008,
+               ray_range=(1_000, 500_000), tree_resolutions=(256, 512, 1024), mesh_pool=64):
+    """The serving workload: Poisson arrivals at ``rate`` per second for ``duration_s``;
+    each request is a ray query (probability ``ray_share``: log-uniform ray count in
+    ``ray_range`` against one of ``trees`` resident trees at resolutions
+    ``tree_resolutions``) or a voxelize request (a mesh of the many-mesh distribution,
+    drawn from a pool of ``mesh_pool``). Returns (tree_plan, mesh_plan, requests) where a
+    request is (t_arrival_s, "rays", tree_index, R, ray_offset) or
+    (t_arrival_s, "vox", mesh_index)."""
+    rng = SplitMix64(seed)
+    tree_plan = [e for e in batch_plan(trees * 4, seed=seed + 1, resolutions=tree_resolutions)
+                 ][:trees]
+    mesh_plan = batch_plan(mesh_pool, seed=seed + 2)
+    reqs = []
+    t = 0.0
+    while True:
+        t += -math.log1p(-float(rng.unit(1)[0])) / rate
+        if t >= duration_s:
+            break
+        if float(rng.unit(1)[0]) < ray_share:
+            ti = int(rng.u64(1)[0] % np.uint64(trees))
+            R = _log_uniform_int(rng, *ray_range)
+            off = int(float(rng.unit(1)[0]) * (ray_range[1] * 2 - R))
+            reqs.append((t, "rays", ti, R, off))
+        else:
+            reqs.append((t, "vox", int(rng.u64(1)[0] % np.uint64(mesh_pool))))
+    return tree_plan, mesh_plan, reqs

@@ -175,6 +175,51 @@ vxl_status vxl_autotune_batch(vxl_ctx *ctx, const float *d_tris, const uint64_t 
                               const float *d_rays, const uint64_t *ray_offsets, uint32_t *d_hits,
                               double *d_t, vxl_tuning *result);
 
+/* ---- serving (no reference counterpart: the reference runs one blocking call at a time;
+ *      this is a request loop over the calls above). Trees stay resident in HBM; ray
+ *      queries and voxelize requests are admitted by device-byte footprint under a budget,
+ *      batched as they arrive (continuous batching) and overlapped on the device. Each
+ *      result equals the one the request gets alone through the per-request entry points.
+ * Footprints: a ray query R * 48 bytes (rays, hits, t); a voxelize request
+ * T * 116 bytes + its dense-grid region + an output-tree estimate. */
+typedef struct vxl_server vxl_server;
+typedef struct vxl_server_config {
+  uint64_t budget_bytes;       /* admission budget over in-flight footprints + held bytes
+                                  (0 = 200e9) */
+  uint32_t batching;           /* 1: continuous batching on two lanes (rays, voxelize);
+                                  0: plain, one request per launch in arrival order */
+  uint32_t max_batch_requests; /* requests per launch (0 = 1024) */
+  uint64_t max_batch_rays;     /* rays per traversal launch (0 = 64 Mi) */
+  uint64_t max_batch_tris;     /* triangles per voxelize batch (0 = 16 Mi) */
+} vxl_server_config;
+typedef struct vxl_server_stats {
+  uint64_t submitted, completed, failed, batches;
+  double mean_batch_requests;
+  double p50_ms, p99_ms, mean_ms, max_ms; /* submit -> completion, all requests */
+  double ray_p50_ms, ray_p99_ms, vox_p50_ms, vox_p99_ms;
+  double throughput_rps;                  /* completed / (last completion - first submit) */
+  uint64_t peak_bytes;                    /* peak held + in-flight footprint */
+} vxl_server_stats;
+vxl_status vxl_server_create(int device, const vxl_server_config *cfg, vxl_server **out);
+/* Serves every pending request, then stops. Voxelize results not yet collected with
+ * vxl_server_wait are released. */
+void vxl_server_destroy(vxl_server *server);
+/* Adds (or, negative, releases) bytes held resident, e.g. the trees queries run against. */
+vxl_status vxl_server_hold(vxl_server *server, int64_t delta_bytes);
+/* Enqueue; device buffers must stay valid until the request completes. VXL_STORAGE_EXCEEDED
+ * if the footprint can never fit the budget. */
+vxl_status vxl_server_submit_rays(vxl_server *server, const vxl_tree *tree, const float *d_rays,
+                                  uint64_t R, uint32_t *d_hits, double *d_t, uint64_t *id);
+vxl_status vxl_server_submit_voxelize(vxl_server *server, const float *d_tris, uint64_t T,
+                                      const vxl_grid *grid, float epsilon, uint64_t *id);
+vxl_status vxl_server_poll(vxl_server *server, uint64_t id, int *done);
+/* Blocks until request id completes and retires it: its status is returned; a voxelize
+ * request's tree goes to *tree_out (freed when tree_out is NULL). */
+vxl_status vxl_server_wait(vxl_server *server, uint64_t id, vxl_tree **tree_out,
+                           uint64_t *pairs_out, double *latency_ms);
+vxl_status vxl_server_stats_get(vxl_server *server, vxl_server_stats *out);
+vxl_status vxl_server_reset_stats(vxl_server *server);
+
 /* ---- timing: when enabled, the last voxelize/traverse call records per-stage
  * device times (ms) measured with HIP events on the stream the kernels ran on.
  * stage 0 = setup+scan, 1 = SAT, 2 = tree build, 3 = traverse, 4 = whole voxelize
