@@ -31,16 +31,34 @@
 namespace vxl {
 namespace {
 
+constexpr uint32_t kLiteral = 1u;    // a grid vertex is not finite: every product is formed
+constexpr uint32_t kWide = 2u;       // more than 2^22 candidates: 64-bit decode
+constexpr uint32_t kPlaneHoist = 4u; // exact per-triangle plane normal and radius (below)
+
 struct __align__(16) TriRec {
   float g[9];         // grid-space vertices
+  float nrm[3];       // e0 x e1 from the grid vertices (valid with kPlaneHoist)
+  float rp;           // 0.5 |n|_1, summed as the reference sums it (valid with kPlaneHoist)
   uint32_t mnxy;      // candidate box min: x | y << 16
   uint32_t mnz_ex;    // box min z | box extent along x << 16
   uint32_t exy;       // extent x * extent y
-  uint32_t flags;     // bit 0: literal path (non-finite grid vertex); bit 1: wide decode
+  uint32_t flags;     // kLiteral | kWide | kPlaneHoist
   float rcp_ex, rcp_exy;  // 1/ex, 1/exy for the narrow decode
   uint32_t grid_base;  // mesh's first leaf-grid word / 512 (0 for a single mesh)
 };
-static_assert(sizeof(TriRec) == 64, "TriRec is one 64-byte record");
+static_assert(sizeof(TriRec) == 80, "TriRec is one 80-byte record");
+
+// True when g - c is exact in binary32 for every candidate centre c = m + 1/2, lo <= m <= hi.
+// g is a multiple of u = 2^(E-23) (E its exponent); with E <= 22 the centres are multiples
+// of u too, so g - c is a multiple of u, and any multiple of u below 2^24 u = 2^(E+1) in
+// magnitude is a binary32 number. The largest |g - c| is at an end of the range.
+__device__ __forceinline__ bool sub_exact(float g, int32_t lo, int32_t hi) {
+  if (g == 0.0f) return true;
+  const int32_t e = int32_t((__float_as_uint(g) >> 23) & 0xffu) - 127;
+  if (e < -126 || e > 22) return false;  // subnormal, or ulp(g) above 1/2
+  const double lim = ldexp(1.0, e + 1), gd = double(g);
+  return fabs(gd - (double(lo) + 0.5)) < lim && fabs(gd - (double(hi) + 0.5)) < lim;
+}
 
 __device__ __forceinline__ int32_t sat_i32(float v) {
   // Rust `f32 as i32`: saturating, NaN -> 0
@@ -91,7 +109,30 @@ __device__ __forceinline__ void setup_one(const float *__restrict__ tri, const V
   r.mnz_ex = uint32_t(r_mn[2]) | (ext[0] << 16);
   r.exy = ext[0] * ext[1];
   // narrow decode: every local index < 2^22, so the float quotient is off by at most 1
-  r.flags = (finite ? 0u : 1u) | ((c >> 22) ? 2u : 0u);
+  r.flags = (finite ? 0u : kLiteral) | ((c >> 22) ? kWide : 0u);
+  // When every v_i = g_i - c is exact for all candidate centres, the reference's
+  // centre-relative edges (v1 - v0, v2 - v1) round to the same values as g1 - g0 and
+  // g2 - g1, so the plane normal and its radius are per-triangle constants; only
+  // d = -(n . v0) depends on the voxel.
+  bool hoist = finite;
+#pragma unroll
+  for (int v = 0; v < 3; v++)
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+      hoist = hoist && sub_exact(r.g[3 * v + a], r_mn[a], r_mn[a] + int32_t(ext[a]) - 1);
+  {
+    const float e0x = r.g[3] - r.g[0], e0y = r.g[4] - r.g[1], e0z = r.g[5] - r.g[2];
+    const float e1x = r.g[6] - r.g[3], e1y = r.g[7] - r.g[4], e1z = r.g[8] - r.g[5];
+    r.nrm[0] = e0y * e1z - e1y * e0z;
+    r.nrm[1] = e0z * e1x - e1z * e0x;
+    r.nrm[2] = e0x * e1y - e1x * e0y;
+    float rp = 0.5f * fabsf(r.nrm[0]);
+    rp = rp + 0.5f * fabsf(r.nrm[1]);
+    rp = rp + 0.5f * fabsf(r.nrm[2]);
+    r.rp = rp;
+    hoist = hoist && isfinite(r.nrm[0]) && isfinite(r.nrm[1]) && isfinite(r.nrm[2]);
+  }
+  if (hoist) r.flags |= kPlaneHoist;
   r.rcp_ex = 1.0f / float(ext[0]);
   r.rcp_exy = 1.0f / float(r.exy);
   r.grid_base = grid_base;
@@ -297,7 +338,7 @@ __device__ __forceinline__ bool finish_pair(const TriRec *__restrict__ rec, uint
   const TriRec r = rec[t];
   const uint32_t x = xy & 0xffffu, y = xy >> 16;
   const Rel v = rel_to(r.g, float(x) + 0.5f, float(y) + 0.5f, float(z) + 0.5f);
-  const bool hit = (r.flags & 1u) ? rest_passes<true>(v) : rest_passes<false>(v);
+  const bool hit = (r.flags & kLiteral) ? rest_passes<true>(v) : rest_passes<false>(v);
   const uint32_t code = dev::brick_code(x >> 3, y >> 3, z >> 3);
   const uint32_t m = dev::morton9(x, y, z);
   key = (uint64_t(r.grid_base) << 9) + uint64_t(code) * 8 + (m >> 6);
@@ -460,7 +501,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_sat(
       const TriRec r = rec[t];
       const uint64_t li = p - start;
       const uint32_t ex = r.mnz_ex >> 16;
-      if (!(r.flags & 2u)) {
+      if (!(r.flags & kWide)) {
         const uint32_t l32 = uint32_t(li);
         z = udiv_narrow(l32, r.exy, r.rcp_exy);
         const uint32_t rem = l32 - z * r.exy;
@@ -475,8 +516,19 @@ __global__ __launch_bounds__(kWaves * 64) void k_sat(
       x += r.mnxy & 0xffffu;
       y += r.mnxy >> 16;
       z += r.mnz_ex & 0xffffu;
-      const Rel v = rel_to(r.g, float(x) + 0.5f, float(y) + 0.5f, float(z) + 0.5f);
-      surv = (r.flags & 1u) ? plane_passes<true>(v) : plane_passes<false>(v);
+      const float cx = float(x) + 0.5f, cy = float(y) + 0.5f, cz = float(z) + 0.5f;
+      if (r.flags & kPlaneHoist) {
+        // the plane test of reference_cpu.rs:76-82 with per-triangle n and r:
+        // |normal.dot(ZERO) + d| = |n . v0| for a finite normal
+        const float v0x = r.g[0] - cx, v0y = r.g[1] - cy, v0z = r.g[2] - cz;
+        float dd = r.nrm[0] * v0x;
+        dd = dd + r.nrm[1] * v0y;
+        dd = dd + r.nrm[2] * v0z;
+        surv = !(fabsf(dd) > r.rp);
+      } else {
+        const Rel v = rel_to(r.g, cx, cy, cz);
+        surv = (r.flags & kLiteral) ? plane_passes<true>(v) : plane_passes<false>(v);
+      }
     }
     // queue survivors
     const unsigned long long sm = __ballot(surv);
