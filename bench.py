@@ -31,6 +31,10 @@ sys.path[:0] = [os.path.join(ROOT, "friendly-journey_amd")]
 SAT_OPS_PER_PAIR = 218          # VALU lane-ops of the full 13-axis predicate per pair
 VALU_PEAK_TOPS = 39.3           # 256 CU x 64 lanes x 2.4 GHz, non-packed, no FMA
 HBM_PEAK_GBS = 8000.0
+# HBM bytes one k_sat launch moves on terrain_1024, from two rocprofv3 --pmc passes
+# (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB per dispatch, factors from a 1 GiB copy kernel);
+# profiles/k_sat__terrain_1024.md holds the commands and the raw counter values.
+K_SAT_TRAFFIC_BYTES = 272_661_818
 
 
 def log(msg):
@@ -419,7 +423,14 @@ def main():
                       "tree_build": round(vox["build_ms"], 4)},
         "roofline": {"kernel": "k_sat", "bound": "valu", "achieved": round(achieved, 3),
                      "peak": VALU_PEAK_TOPS, "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4),
-                     "traffic": None, "algorithmic": f"{SAT_OPS_PER_PAIR} ops x {vox['pairs']} pairs"},
+                     "traffic": K_SAT_TRAFFIC_BYTES,
+                     "traffic_gbs": round(K_SAT_TRAFFIC_BYTES / (vox["sat_ms"] / 1e3) / 1e9, 1),
+                     "traffic_frac_of_hbm": round(K_SAT_TRAFFIC_BYTES / (vox["sat_ms"] / 1e3) / 1e9
+                                                  / HBM_PEAK_GBS, 4),
+                     "algorithmic": f"{SAT_OPS_PER_PAIR} ops x {vox['pairs']} pairs",
+                     "peak_note": "non-packed FP32/int VALU lane-ops without FMA (the SAT has no "
+                                  "FMA-contractible or packable pairs under its exact operation order); "
+                                  "the 157.3 TFLOP/s FP32 vector figure counts packed FMA as 4 ops"},
     }
     if not args.no_dda:
         dda = bench_dda(ctx, torch, W)
