@@ -89,7 +89,7 @@ struct vxl_ctx {
   // tuning
   int sat_waves_per_cu = 28;  // resident SAT waves per CU (persistent, work-queue fed)
   int sat_units_per_wave = 1;  // work units per resident wave
-  int dda_block = 256;
+  int dda_block = 64;  // one-wave workgroups: a finished wave frees its slot at once
   // last-call timing (ms), filled when timing is enabled
   bool timing = false;
   hipEvent_t ev[8] = {};  // ev[0] call start, ev[1]/ev[2] around the SAT kernel, ev[3] end

@@ -156,7 +156,7 @@ def test_batch_traverse_dev_block_sizes(ctx):
                 a, z = int(off[i]), int(off[i + 1])
                 assert np.array_equal(h[a:z], wh) and np.array_equal(t[a:z], wt), (b, i)
     finally:
-        ctx.set_tuning(dda_block=256)
+        ctx.set_tuning(dda_block=64)
     for t in trees:
         t.free()
 
@@ -190,7 +190,7 @@ def test_autotune_keeps_results(ctx):
         for (name, *_), a, b in zip(mix, before, after):
             _assert_same_tree(a, b, name)
     finally:
-        ctx.set_tuning(sat_waves_per_cu=28, sat_units_per_wave=1, dda_block=256)
+        ctx.set_tuning(sat_waves_per_cu=28, sat_units_per_wave=1, dda_block=64)
     for t in before + after:
         t.free()
 
