@@ -4,8 +4,8 @@
 // mesh gets a region of the pooled leaf grid (its (n/8)^3 bricks, rounded up to a whole
 // upper-bitmap word of 64 x 64 bricks) and the matching words of the pooled brick bitmap.
 // Per group: one setup launch over every triangle, one scan, one persistent SAT launch
-// over every (triangle, voxel) pair, one batched tree build -- two host syncs in all,
-// whatever the number of meshes. The pooled grid and bitmap are all-zero between groups
+// over every (triangle, voxel) pair, one batched tree build -- one host round trip (the
+// per-mesh tree sizes), whatever the number of meshes. The pooled grid and bitmap are all-zero between groups
 // (the leaf gather clears what it reads), like the single-mesh scratch.
 #include "vxl_internal.hpp"
 
@@ -20,6 +20,7 @@ void *staging_begin(vxl_ctx *ctx, size_t bytes) {
     const size_t b = bytes < (64u << 10) ? (64u << 10) : (bytes + 4095) / 4096 * 4096;
     VXL_HIP(hipHostMalloc(&ctx->batch_host, b, hipHostMallocDefault));
     ctx->batch_host_bytes = b;
+    VXL_HIP(hipHostGetDevicePointer(&ctx->batch_host_dev, ctx->batch_host, 0));
   }
   ctx->batch_host_used = 0;
   return ctx->batch_host;
@@ -32,6 +33,11 @@ void *staging_take(vxl_ctx *ctx, size_t bytes) {
   void *p = static_cast<char *>(ctx->batch_host) + ctx->batch_host_used;
   ctx->batch_host_used += b;
   return p;
+}
+
+void *staging_dev(vxl_ctx *ctx, void *host) {
+  return static_cast<char *>(ctx->batch_host_dev) +
+         (static_cast<char *>(host) - static_cast<char *>(ctx->batch_host));
 }
 
 void staging_end(vxl_ctx *ctx, hipStream_t s) {
@@ -101,7 +107,7 @@ void run_group(vxl_ctx *ctx, const BatchInput *in, uint32_t G, vxl_tree **out,
   const BatchMesh *d_mesh = reinterpret_cast<const BatchMesh *>(d);
   run_sat_batch(ctx, d_mesh, reinterpret_cast<const uint32_t *>(d + mesh_bytes), G,
                 uint32_t(blocks), ctx->pool_grid.as<unsigned long long>(),
-                ctx->pool_b1.as<unsigned long long>());
+                ctx->pool_b1.as<unsigned long long>());  // no host round trip
   std::vector<vxl_tree *> trees(G, nullptr);
   std::vector<uint64_t> pr(G, 0);
   build_trees_batch(ctx, meshes, d_mesh, W1, trees.data(), pr.data());

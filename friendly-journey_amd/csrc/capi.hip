@@ -33,6 +33,19 @@ void DevBuf::release() {
   bytes = 0;
 }
 
+void wait_stream(vxl_ctx *ctx, hipStream_t s) {
+  if (!ctx->ev_wait) VXL_HIP(hipEventCreateWithFlags(&ctx->ev_wait, hipEventDisableTiming));
+  VXL_HIP(hipEventRecord(ctx->ev_wait, s));
+  for (;;) {
+    const hipError_t e = hipEventQuery(ctx->ev_wait);
+    if (e == hipSuccess) return;
+    if (e != hipErrorNotReady) {
+      (void)hipGetLastError();
+      throw Error(VXL_DEVICE_ERROR, std::string("stream wait failed: ") + hipGetErrorString(e));
+    }
+  }
+}
+
 bool resolution_k(uint32_t n, uint32_t *k) {
   // voxel-core/src/resolution.rs:76-99: n = 8 * 4^k
   if (n < 8 || n % 8) return false;
@@ -131,6 +144,14 @@ vxl::VoxelizeParams make_params(const vxl_grid *g, float eps) {
 }
 
 
+// The pair count of the last SAT launch (one host round trip).
+uint64_t read_pairs(vxl_ctx *ctx) {
+  VXL_HIP(hipMemcpyAsync(ctx->host_pinned, vxl::sat_pairs_dev(ctx), 8, hipMemcpyDeviceToHost,
+                         ctx->stream));
+  vxl::wait_stream(ctx, ctx->stream);
+  return *static_cast<uint64_t *>(ctx->host_pinned);
+}
+
 vxl_tree *empty_tree(vxl_ctx *ctx, uint32_t n, uint32_t k) {
   vxl_tree *t = new vxl_tree();
   t->ctx = ctx;
@@ -153,9 +174,10 @@ vxl_tree *voxelize_tree_impl(vxl_ctx *ctx, const float *d_tris, uint64_t T, cons
   }
   vxl::mark(ctx, 0);
   vxl::prepare_grid(ctx, g->n);
-  const uint64_t P = vxl::run_sat(ctx, d_tris, T, make_params(g, eps));
+  vxl::run_sat(ctx, d_tris, T, make_params(g, eps));
+  uint64_t P = 0;
+  vxl_tree *t = vxl::build_tree_from_grid(ctx, g->n, k, vxl::sat_pairs_dev(ctx), &P);
   if (pairs_out) *pairs_out = P;
-  vxl_tree *t = vxl::build_tree_from_grid(ctx, g->n, k);
   vxl::mark(ctx, 3);
   if (ctx->timing && P) {
     VXL_HIP(hipEventSynchronize(ctx->ev[3]));
@@ -296,6 +318,7 @@ vxl_status vxl_ctx_create(int device, vxl_ctx **out) {
     VXL_HIP(hipSetDevice(device));
     VXL_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     VXL_HIP(hipHostMalloc(&ctx->host_pinned, 4096, hipHostMallocDefault));
+    VXL_HIP(hipHostGetDevicePointer(&ctx->host_pinned_dev, ctx->host_pinned, 0));
     hipDeviceProp_t prop;
     VXL_HIP(hipGetDeviceProperties(&prop, device));
     ctx->cu_count = prop.multiProcessorCount;
@@ -327,12 +350,14 @@ void vxl_ctx_destroy(vxl_ctx *ctx) {
     t->ctx = nullptr;
   }
   ctx->trees.clear();
+  vxl::arena_destroy(ctx);
   for (auto &e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
   if (ctx->batch_host) (void)hipHostFree(ctx->batch_host);
   if (ctx->staging_free) (void)hipEventDestroy(ctx->staging_free);
   if (ctx->trav_done) (void)hipEventDestroy(ctx->trav_done);
+  if (ctx->ev_wait) (void)hipEventDestroy(ctx->ev_wait);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -441,8 +466,8 @@ vxl_status vxl_voxelize_dense_dev(vxl_ctx *ctx, const float *d_tris, uint64_t T,
       if (pairs_out) *pairs_out = 0;
     } else {
       vxl::prepare_grid(ctx, grid->n);
-      const uint64_t P = vxl::run_sat(ctx, d_tris, T, make_params(grid, epsilon));
-      if (pairs_out) *pairs_out = P;
+      vxl::run_sat(ctx, d_tris, T, make_params(grid, epsilon));
+      if (pairs_out) *pairs_out = read_pairs(ctx);
       vxl::grid_to_words(ctx, grid->n, d_occ_words);
     }
     if (user && user != ctx->stream) VXL_HIP(hipStreamSynchronize(ctx->stream));
@@ -469,8 +494,8 @@ vxl_status vxl_voxelize_dense(vxl_ctx *ctx, const float *tris, uint64_t T, const
     ctx->words_io.reserve(nwords * 4);
     VXL_HIP(hipMemcpyAsync(ctx->tris_in.ptr, tris, T * 36, hipMemcpyHostToDevice, ctx->stream));
     vxl::prepare_grid(ctx, grid->n);
-    const uint64_t P = vxl::run_sat(ctx, ctx->tris_in.as<float>(), T, make_params(grid, epsilon));
-    if (pairs_out) *pairs_out = P;
+    vxl::run_sat(ctx, ctx->tris_in.as<float>(), T, make_params(grid, epsilon));
+    if (pairs_out) *pairs_out = read_pairs(ctx);
     vxl::grid_to_words(ctx, grid->n, ctx->words_io.as<uint32_t>());
     VXL_HIP(hipMemcpyAsync(occ_words, ctx->words_io.ptr, nwords * 4, hipMemcpyDeviceToHost,
                            ctx->stream));

@@ -147,8 +147,8 @@ vxl_tree *voxelize_one(vxl_ctx *ctx, const Request *r, uint64_t *pairs) {
     return empty_tree_device(ctx, r->n, r->k);
   }
   prepare_grid(ctx, r->n);
-  *pairs = run_sat(ctx, r->tris, r->T, r->p);
-  return build_tree_from_grid(ctx, r->n, r->k);
+  run_sat(ctx, r->tris, r->T, r->p);
+  return build_tree_from_grid(ctx, r->n, r->k, sat_pairs_dev(ctx), pairs);
 }
 
 // Enqueues one batch on the lane's stream; returns its completion event. The plain mode

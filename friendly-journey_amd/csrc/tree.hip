@@ -476,6 +476,12 @@ __global__ void k_batch_totals(const BatchMesh *__restrict__ mesh, const LevelPt
   if (g == G - 1) n2_beg[G] = t.n2_begin + t.n2;
 }
 
+__global__ void k_copy_totals(const BatchTotals *__restrict__ in, uint32_t G,
+                              BatchTotals *__restrict__ out) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < G) out[g] = in[g];
+}
+
 __global__ __launch_bounds__(kSmallThreads) void k_upper_place_batch(
     const BatchMesh *__restrict__ mesh, const LevelPtrs *__restrict__ lps,
     const BatchTotals *__restrict__ tot, uint32_t *const *__restrict__ nodes_ptr) {
@@ -591,19 +597,69 @@ static void *dmalloc(size_t bytes) {
   return p;
 }
 
+// ---- tree memory: a context-owned slab arena ------------------------------------------
+// Trees are carved from slabs of at least kSlabBytes with a bump pointer; a slab whose
+// trees are all freed is reused from its start. This keeps device allocation off the
+// critical path of each voxelize call (a fresh stream-ordered allocation per tree cost
+// tens of microseconds of idle GPU). Callers free trees only after the context's stream
+// has drained (vxl_tree_free synchronises first), so reuse never races a reader.
+constexpr size_t kSlabBytes = size_t(256) << 20;
+
+void *arena_alloc(vxl_ctx *ctx, size_t bytes, int *slab_out) {
+  bytes = (bytes + 255) / 256 * 256;
+  for (size_t i = 0; i < ctx->slabs.size(); i++) {
+    Slab &sl = ctx->slabs[i];
+    if (sl.base && sl.bytes - sl.used >= bytes) {
+      void *p = sl.base + sl.used;
+      sl.used += bytes;
+      sl.live++;
+      *slab_out = int(i);
+      return p;
+    }
+  }
+  Slab sl;
+  sl.bytes = bytes > kSlabBytes ? (bytes + (size_t(2) << 20) - 1) / (size_t(2) << 20) * (size_t(2) << 20)
+                                : kSlabBytes;
+  sl.base = static_cast<char *>(dmalloc(sl.bytes));
+  sl.used = bytes;
+  sl.live = 1;
+  ctx->slabs.push_back(sl);
+  *slab_out = int(ctx->slabs.size() - 1);
+  return sl.base;
+}
+
+void arena_release(vxl_ctx *ctx, int slab) {
+  if (!ctx || slab < 0 || size_t(slab) >= ctx->slabs.size()) return;
+  Slab &sl = ctx->slabs[slab];
+  if (sl.live && --sl.live == 0) {
+    sl.used = 0;
+    // keep at most two empty slabs; give larger idle ones back
+    size_t empty = 0;
+    for (const Slab &o : ctx->slabs) empty += (o.base && o.live == 0) ? 1 : 0;
+    if (empty > 2) {
+      (void)hipFree(sl.base);
+      sl.base = nullptr;
+      sl.bytes = 0;
+    }
+  }
+}
+
+void arena_destroy(vxl_ctx *ctx) {
+  for (Slab &sl : ctx->slabs)
+    if (sl.base) (void)hipFree(sl.base);
+  ctx->slabs.clear();
+}
+
 void tree_free_device(vxl_tree *t) {
   if (t->shared) {
     if (--t->shared->refs == 0) {
-      if (t->ctx) (void)hipFreeAsync(t->shared->base, t->ctx->stream);
-      else (void)hipFree(t->shared->base);
+      if (t->ctx) arena_release(t->ctx, t->shared->slab);
       delete t->shared;
     }
     t->shared = nullptr;
-  } else if (t->pooled) {
-    if (t->nodes) {
-      if (t->ctx) (void)hipFreeAsync(t->nodes, t->ctx->stream);
-      else (void)hipFree(t->nodes);
-    }
+  } else if (t->slab >= 0) {
+    if (t->ctx) arena_release(t->ctx, t->slab);
+    t->slab = -1;
   } else {
     if (t->nodes) (void)hipFree(t->nodes);
     if (t->leaves) (void)hipFree(t->leaves);
@@ -624,28 +680,47 @@ static vxl_tree *alloc_tree(vxl_ctx *ctx, uint32_t n, uint32_t k, uint64_t nodes
   t->k = k;
   t->leaf_count = leaves;
   t->node_count = nodes;
-  // one stream-ordered allocation for the four arrays
+  // one arena block for the four arrays
   auto up = [](uint64_t b) { return ((b ? b : 16) + 255) / 256 * 256; };
   const uint64_t bn = up(nodes * 12), bl = up(leaves * 64), bb = up(leaves * 4), bc = up(leaves * 8);
-  void *base = nullptr;
-  hipError_t e = hipMallocAsync(&base, bn + bl + bb + bc, ctx->stream);
-  if (e != hipSuccess) {
-    (void)hipGetLastError();
+  char *c = nullptr;
+  try {
+    c = static_cast<char *>(arena_alloc(ctx, bn + bl + bb + bc, &t->slab));
+  } catch (...) {
     delete t;
-    throw Error(VXL_OOM, std::string("tree allocation failed: ") + hipGetErrorString(e));
+    throw;
   }
-  char *c = static_cast<char *>(base);
   t->nodes = reinterpret_cast<uint32_t *>(c);
   t->leaves = reinterpret_cast<uint64_t *>(c + bn);
   t->bounds = reinterpret_cast<uint32_t *>(c + bn + bl);
   t->codes = reinterpret_cast<uint64_t *>(c + bn + bl + bb);
-  t->pooled = true;
   t->bytes = nodes * 12 + leaves * 76;
   return t;
 }
 
+// Sizes a build needs before the tree is allocated, written by the device straight into
+// mapped pinned memory (no copy call): the packed level-2 total, the per-level totals of
+// levels >= 3, and the SAT pair count.
+struct BuildSizes {
+  uint64_t pn2_total;
+  uint64_t pairs;
+  uint32_t tot[32];
+};
+
+__global__ void k_build_sizes(const uint64_t *__restrict__ pn2_total,
+                              const uint32_t *__restrict__ tot, int top,
+                              const uint64_t *__restrict__ pairs, BuildSizes *__restrict__ out) {
+  const int i = threadIdx.x;
+  if (i == 0) {
+    out->pn2_total = *pn2_total;
+    out->pairs = pairs ? *pairs : 0;
+  }
+  if (i < 32) out->tot[i] = (top >= 3 && i >= 6 && i < 2 * top + 2) ? tot[i] : 0u;
+}
+
 // n <= 2048 (k <= 4): every level above 2 has at most 4096 input words.
-static vxl_tree *build_tree_fast(vxl_ctx *ctx, uint32_t n, uint32_t k) {
+static vxl_tree *build_tree_fast(vxl_ctx *ctx, uint32_t n, uint32_t k, const uint64_t *pairs_dev,
+                                 uint64_t *pairs) {
   hipStream_t s = ctx->stream;
   const int top = int(k + 1 > 2 ? k + 1 : 2);
   LevelPtrs lp;
@@ -681,17 +756,18 @@ static vxl_tree *build_tree_fast(vxl_ctx *ctx, uint32_t n, uint32_t k) {
   // 2. upper levels in one workgroup
   if (top >= 3)
     hipLaunchKernelGGL(k_small_levels, dim3(1), dim3(kSmallThreads), 0, s, lp, 3, top, b2, tot);
-  // 3. totals (one sync)
-  uint32_t *host = static_cast<uint32_t *>(ctx->host_pinned);
-  VXL_HIP(hipMemcpyAsync(host, pn2 + w1, 8, hipMemcpyDeviceToHost, s));
-  if (top >= 3) VXL_HIP(hipMemcpyAsync(host + 2 + 6, tot + 6, 4 * 26, hipMemcpyDeviceToHost, s));
-  VXL_HIP(hipStreamSynchronize(s));
-  const uint64_t leaves = host[0];
-  const uint64_t n2 = host[1];
+  // 3. totals (the one host round trip of the call)
+  BuildSizes *hs = static_cast<BuildSizes *>(ctx->host_pinned);
+  hipLaunchKernelGGL(k_build_sizes, dim3(1), dim3(64), 0, s, pn2 + w1, tot, top, pairs_dev,
+                     static_cast<BuildSizes *>(ctx->host_pinned_dev));
+  wait_stream(ctx, s);
+  const uint64_t leaves = uint32_t(hs->pn2_total);
+  const uint64_t n2 = hs->pn2_total >> 32;
+  if (pairs) *pairs = hs->pairs;
   uint64_t nodes = 0;
   uint64_t level_nodes[16] = {0};
   level_nodes[2] = n2;
-  for (int L = 3; L <= top; L++) level_nodes[L] = host[2 + 2 * L + 1];
+  for (int L = 3; L <= top; L++) level_nodes[L] = hs->tot[2 * L + 1];
   if (k >= 1 && leaves)
     for (int L = 2; L <= int(k) + 1; L++) nodes += level_nodes[L];
   vxl_tree *t = alloc_tree(ctx, n, k, leaves ? nodes : 0, leaves);
@@ -786,10 +862,12 @@ void build_trees_batch(vxl_ctx *ctx, const std::vector<BatchMesh> &meshes,
   hipLaunchKernelGGL(k_batch_totals, dim3(div_up(G, 64)), dim3(64), 0, s, d_mesh, d_lps, G, pn2,
                      ctx->coff.as<uint64_t>(), d_tot, n2_beg);
   VXL_HIP(hipGetLastError());
-  // 2. per-mesh sizes (one sync)
+  // 2. per-mesh sizes (the one host round trip of the group): k_batch_totals also wrote
+  //    them into mapped pinned staging
   BatchTotals *h_tot = static_cast<BatchTotals *>(staging_take(ctx, G * sizeof(BatchTotals)));
-  VXL_HIP(hipMemcpyAsync(h_tot, d_tot, G * sizeof(BatchTotals), hipMemcpyDeviceToHost, s));
-  VXL_HIP(hipStreamSynchronize(s));
+  hipLaunchKernelGGL(k_copy_totals, dim3(div_up(G, 64)), dim3(64), 0, s, d_tot, G,
+                     static_cast<BatchTotals *>(staging_dev(ctx, h_tot)));
+  wait_stream(ctx, s);
   // 3. one allocation for the group's trees: node arrays, then leaves / bounds / codes of
   //    every mesh in group order (a mesh's leaves start at its leaf_begin)
   uint64_t total_leaves = 0, node_bytes = 0, N2 = 0, live = 0;
@@ -811,14 +889,11 @@ void build_trees_batch(vxl_ctx *ctx, const std::vector<BatchMesh> &meshes,
   if (live) {
     const uint64_t bytes = node_bytes + up(total_leaves * 64) + up(total_leaves * 4) +
                            up(total_leaves * 8);
-    void *mem = nullptr;
-    hipError_t e = hipMallocAsync(&mem, bytes, s);
-    if (e != hipSuccess) {
-      (void)hipGetLastError();
-      throw Error(VXL_OOM, std::string("batch tree allocation failed: ") + hipGetErrorString(e));
-    }
+    int slab = -1;
+    void *mem = arena_alloc(ctx, bytes, &slab);
     blk = new SharedBlock();
     blk->base = mem;
+    blk->slab = slab;
     base = static_cast<char *>(mem);
   }
   char *leaves_base = base + node_bytes;
@@ -868,8 +943,17 @@ void build_trees_batch(vxl_ctx *ctx, const std::vector<BatchMesh> &meshes,
   VXL_HIP(hipGetLastError());
 }
 
-vxl_tree *build_tree_from_grid(vxl_ctx *ctx, uint32_t n, uint32_t k) {
-  if (k <= 4) return build_tree_fast(ctx, n, k);
+vxl_tree *build_tree_from_grid(vxl_ctx *ctx, uint32_t n, uint32_t k, const uint64_t *pairs_dev,
+                               uint64_t *pairs) {
+  if (k <= 4) return build_tree_fast(ctx, n, k, pairs_dev, pairs);
+  if (pairs) {
+    *pairs = 0;
+    if (pairs_dev) {
+      VXL_HIP(hipMemcpyAsync(ctx->host_pinned, pairs_dev, 8, hipMemcpyDeviceToHost, ctx->stream));
+      wait_stream(ctx, ctx->stream);
+      *pairs = *static_cast<uint64_t *>(ctx->host_pinned);
+    }
+  }
   hipStream_t s = ctx->stream;
   const uint32_t top = k + 1 > 2 ? k + 1 : 2;  // node levels 2..top (level 2 always built)
   Level lv[16];
@@ -916,7 +1000,7 @@ vxl_tree *build_tree_from_grid(vxl_ctx *ctx, uint32_t n, uint32_t k) {
     VXL_HIP(hipMemcpyAsync(host + 2 * L, lv[L].cb + lv[L].win, 4, hipMemcpyDeviceToHost, s));
     VXL_HIP(hipMemcpyAsync(host + 2 * L + 1, lv[L].ni + lv[L].win, 4, hipMemcpyDeviceToHost, s));
   }
-  VXL_HIP(hipStreamSynchronize(s));
+  wait_stream(ctx, s);
   for (uint32_t L = 2; L <= top; L++) {
     lv[L].children = host[2 * L];
     lv[L].nodes = host[2 * L + 1];

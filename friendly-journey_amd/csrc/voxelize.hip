@@ -164,15 +164,34 @@ __global__ __launch_bounds__(kSetupTB) void k_tri_setup_batch(
   setup_one(m.tris + 9 * tl, m.p, uint32_t(m.b1_begin), count + slot, rec + slot);
 }
 
-// bstart[b] = triangle holding pair b * chunk (triangles with no candidate never do).
+constexpr uint64_t kChunksPerUnit = 16;  // at least 1024 pairs per work unit
+
+// The SAT launch plan, computed on the device from the pair total so that no host
+// read-back sits between the scan and the SAT kernel: work units of `cpu` chunks of 64
+// pairs (at least kChunksPerUnit, about `want` units in all), and the work counter.
+__global__ void k_sat_plan(const uint64_t *__restrict__ total, uint64_t want,
+                           SatPlan *__restrict__ plan) {
+  const uint64_t P = *total;
+  const uint64_t nchunks = (P + 63) / 64;
+  uint64_t cpu = (nchunks + want - 1) / want;
+  if (cpu < kChunksPerUnit) cpu = kChunksPerUnit;
+  plan->P = P;
+  plan->cpu = cpu;
+  plan->nunits = (nchunks + cpu - 1) / cpu;
+  plan->counter = 0;
+}
+
+// ustart[u] = triangle holding the first pair of unit u (triangles with no candidate never
+// do).
 __global__ void k_bstart(const uint64_t *__restrict__ coff, const uint64_t *__restrict__ ccount,
-                         uint64_t Tc, uint64_t chunk, uint64_t nchunks,
-                         uint32_t *__restrict__ bstart) {
+                         uint64_t Tc, const SatPlan *__restrict__ plan,
+                         uint32_t *__restrict__ ustart) {
   uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= Tc) return;
+  const uint64_t chunk = plan->cpu * 64, nunits = plan->nunits;
   uint64_t lo = coff[t], hi = lo + ccount[t];
   uint64_t b = (lo + chunk - 1) / chunk;
-  for (; b * chunk < hi && b < nchunks; b++) bstart[b] = uint32_t(t);
+  for (; b * chunk < hi && b < nunits; b++) ustart[b] = uint32_t(t);
 }
 
 // ---- the SAT predicate, reference_cpu.rs:27-85 -----------------------------------------
@@ -327,7 +346,6 @@ constexpr int kWaves = 4;    // waves per block
 constexpr int kQueue = 128;  // survivor queue entries per wave
 constexpr int kTable = 256;  // leaf-word merge table slots per wave
 constexpr int kFlushAt = 128;  // flush the table once this many slots are in use
-constexpr uint64_t kChunksPerUnit = 16;  // at least 1024 pairs per work unit
 constexpr unsigned long long kEmpty = ~0ull;
 
 // Phase B on one survivor: box faces + edge axes. On a hit, returns the leaf-grid word
@@ -397,16 +415,22 @@ __device__ __forceinline__ void table_flush(unsigned long long *tk, unsigned lon
 // table and flushed to HBM in bulk.
 __global__ __launch_bounds__(kWaves * 64) void k_sat(
     const TriRec *__restrict__ rec, const uint64_t *__restrict__ coff,
-    const uint32_t *__restrict__ ustart, uint64_t T, uint64_t P, uint64_t chunks_per_unit,
-    unsigned int *__restrict__ next_unit, unsigned long long *__restrict__ leafgrid,
-    unsigned long long *__restrict__ brickbits) {
+    const uint32_t *__restrict__ ustart, uint64_t T, SatPlan *__restrict__ plan,
+    unsigned long long *__restrict__ leafgrid, unsigned long long *__restrict__ brickbits) {
   __shared__ uint32_t q_t[kWaves][kQueue], q_xy[kWaves][kQueue], q_z[kWaves][kQueue];
   __shared__ unsigned long long t_key[kWaves][kTable], t_val[kWaves][kTable];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint64_t P = plan->P, chunks_per_unit = plan->cpu, nunits = plan->nunits;
   const uint64_t nchunks = (P + 63) / 64;
-  const uint64_t nunits = (nchunks + chunks_per_unit - 1) / chunks_per_unit;
+  unsigned int *next_unit = &plan->counter;
   uint32_t *qt = q_t[wave], *qxy = q_xy[wave], *qz = q_z[wave];
   unsigned long long *tk = t_key[wave], *tv = t_val[wave];
+  // units of chunks_per_unit chunks are handed out by an atomic counter; a wave that finds
+  // none at its first try leaves before touching LDS (the grid is sized for the device)
+  uint32_t unit = 0;
+  if (lane == 0) unit = atomicAdd(next_unit, 1u);
+  unit = __shfl(unit, 0, 64);
+  if (uint64_t(unit) >= nunits) return;  // wave-uniform
 #pragma unroll
   for (int i = 0; i < kTable / 64; i++) {
     tk[i * 64 + lane] = kEmpty;
@@ -416,12 +440,12 @@ __global__ __launch_bounds__(kWaves * 64) void k_sat(
   uint32_t qn = 0, used = 0;
   const unsigned long long below = (1ull << lane) - 1ull;
 
-  // units of chunks_per_unit chunks are handed out by an atomic counter
-  for (;;) {
-  uint32_t unit = 0;
-  if (lane == 0) unit = atomicAdd(next_unit, 1u);
-  unit = __shfl(unit, 0, 64);
-  if (uint64_t(unit) >= nunits) break;  // wave-uniform exit: every wave ends here
+  for (bool first = true;; first = false) {
+  if (!first) {
+    if (lane == 0) unit = atomicAdd(next_unit, 1u);
+    unit = __shfl(unit, 0, 64);
+    if (uint64_t(unit) >= nunits) break;  // wave-uniform exit: every wave ends here
+  }
   const uint64_t c0 = uint64_t(unit) * chunks_per_unit;
   const uint64_t c1 = min(c0 + chunks_per_unit, nchunks);
   uint32_t t_first = ustart[unit];
@@ -570,49 +594,33 @@ __global__ __launch_bounds__(kWaves * 64) void k_sat(
 
 namespace {
 
-// Scan of the Tp per-slot candidate counts in ctx->tri_count, sizing, and the SAT launch
-// into `leafgrid` / `brickbits`. Returns the pair count.
-uint64_t sat_after_setup(vxl_ctx *ctx, uint64_t Tp, unsigned long long *leafgrid,
-                         unsigned long long *brickbits) {
+// Scan of the Tp per-slot candidate counts in ctx->tri_count, the device-side plan, and
+// the SAT launch into `leafgrid` / `brickbits`, all enqueued without a host round trip. The
+// persistent grid is sized for the device (waves that find no unit exit at once).
+void sat_after_setup(vxl_ctx *ctx, uint64_t Tp, unsigned long long *leafgrid,
+                     unsigned long long *brickbits) {
   hipStream_t s = ctx->stream;
   const int tb = 256;
   uint64_t *count = ctx->tri_count.as<uint64_t>();
   TriRec *rec = ctx->rec.as<TriRec>();
   uint64_t *coff = ctx->coff.as<uint64_t>();
-  uint64_t *host = static_cast<uint64_t *>(ctx->host_pinned);
   scan_exclusive_u64(count, coff, Tp, coff + Tp, ctx->scan_tmp, s);
-  VXL_HIP(hipMemcpyAsync(host, coff + Tp, 8, hipMemcpyDeviceToHost, s));
-  VXL_HIP(hipStreamSynchronize(s));
-  const uint64_t P = host[0];
-  if (P == 0) return 0;
-
-  // Work units of at least kChunksPerUnit chunks of 64 pairs, handed out to a persistent
-  // grid of waves by an atomic counter; ustart[u] = triangle holding the first pair of
-  // unit u. Units are long contiguous runs (about sat_units_per_wave per resident wave)
-  // so that each wave's merge table sees spatially coherent hits.
-  const uint64_t nchunks = (P + 63) / 64;
   const uint64_t max_waves =
       uint64_t(ctx->cu_count > 0 ? ctx->cu_count : 256) * ctx->sat_waves_per_cu;
-  const uint64_t want_units = max_waves * uint64_t(ctx->sat_units_per_wave);
-  uint64_t cpu = (nchunks + want_units - 1) / want_units;
-  if (cpu < kChunksPerUnit) cpu = kChunksPerUnit;
-  const uint64_t nunits = (nchunks + cpu - 1) / cpu;
-  const uint64_t waves = nunits < max_waves ? nunits : max_waves;
-  const uint64_t blocks = (waves + kWaves - 1) / kWaves;
-  if (nunits > 0xffffffffull)
-    throw Error(VXL_WORKGROUPS_EXCEEDED, "voxelize: pair units exceed the 32-bit limit");
-  ctx->bstart.reserve(nunits * 4 + 64);
-  uint32_t *ustart = ctx->bstart.as<uint32_t>();
-  unsigned int *counter = reinterpret_cast<unsigned int *>(ustart + nunits + 8);
-  hipLaunchKernelGGL(k_bstart, dim3((Tp + tb - 1) / tb), dim3(tb), 0, s, coff, count, Tp,
-                     cpu * 64, nunits, ustart);
-  VXL_HIP(hipMemsetAsync(counter, 0, 4, s));
+  const uint64_t want = max_waves * uint64_t(ctx->sat_units_per_wave);
+  ctx->bstart.reserve(256 + (want + 64) * 4);
+  SatPlan *plan = ctx->bstart.as<SatPlan>();
+  uint32_t *ustart = reinterpret_cast<uint32_t *>(ctx->bstart.as<char>() + 256);
+  hipLaunchKernelGGL(k_sat_plan, dim3(1), dim3(1), 0, s, coff + Tp, want, plan);
+  if (Tp)
+    hipLaunchKernelGGL(k_bstart, dim3((Tp + tb - 1) / tb), dim3(tb), 0, s, coff, count, Tp, plan,
+                       ustart);
   mark(ctx, 1);
+  const uint64_t blocks = (max_waves + kWaves - 1) / kWaves;
   hipLaunchKernelGGL(k_sat, dim3(uint32_t(blocks)), dim3(kWaves * 64), 0, s, rec, coff, ustart,
-                     Tp, P, cpu, counter, leafgrid, brickbits);
+                     Tp, plan, leafgrid, brickbits);
   mark(ctx, 2);
   VXL_HIP(hipGetLastError());
-  return P;
 }
 
 void reserve_records(vxl_ctx *ctx, uint64_t Tp) {
@@ -623,25 +631,27 @@ void reserve_records(vxl_ctx *ctx, uint64_t Tp) {
 
 }  // namespace
 
-uint64_t run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizeParams &p) {
+void run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizeParams &p) {
   const int tb = 256;
   reserve_records(ctx, T);
   hipLaunchKernelGGL(k_tri_setup, dim3((T + tb - 1) / tb), dim3(tb), 0, ctx->stream, d_tris, T,
                      p, ctx->tri_count.as<uint64_t>(), ctx->rec.as<TriRec>());
-  return sat_after_setup(ctx, T, ctx->leafgrid.as<unsigned long long>(),
-                         ctx->bitmap.as<unsigned long long>());
+  sat_after_setup(ctx, T, ctx->leafgrid.as<unsigned long long>(),
+                  ctx->bitmap.as<unsigned long long>());
 }
 
-uint64_t run_sat_batch(vxl_ctx *ctx, const BatchMesh *d_mesh,
-                       const uint32_t *d_sb_begin, uint32_t G, uint32_t setup_blocks,
-                       unsigned long long *leafgrid, unsigned long long *brickbits) {
+const uint64_t *sat_pairs_dev(vxl_ctx *ctx) { return &ctx->bstart.as<SatPlan>()->P; }
+
+void run_sat_batch(vxl_ctx *ctx, const BatchMesh *d_mesh, const uint32_t *d_sb_begin,
+                   uint32_t G, uint32_t setup_blocks, unsigned long long *leafgrid,
+                   unsigned long long *brickbits) {
   const uint64_t Tp = uint64_t(setup_blocks) * kSetupTB;
   reserve_records(ctx, Tp);
   if (setup_blocks)
     hipLaunchKernelGGL(k_tri_setup_batch, dim3(setup_blocks), dim3(kSetupTB), 0, ctx->stream,
                        d_mesh, d_sb_begin, G, ctx->tri_count.as<uint64_t>(),
                        ctx->rec.as<TriRec>());
-  return sat_after_setup(ctx, Tp, leafgrid, brickbits);
+  sat_after_setup(ctx, Tp, leafgrid, brickbits);
 }
 
 uint64_t setup_slots_per_block() { return kSetupTB; }

@@ -69,6 +69,21 @@ struct vxl_ctx_impl;
 
 }  // namespace vxl
 
+namespace vxl {
+// One device allocation shared by the trees of one batch group; freed with the last tree.
+struct SharedBlock {
+  void *base = nullptr;
+  uint32_t refs = 0;
+  int slab = -1;  // arena slab holding base
+};
+// A device slab of the tree arena (tree.hip).
+struct Slab {
+  char *base = nullptr;
+  size_t bytes = 0, used = 0;
+  uint32_t live = 0;  // trees (or batch groups) allocated in it and not yet freed
+};
+}  // namespace vxl
+
 // Opaque handle types of the C ABI.
 struct vxl_ctx {
   int device = 0;
@@ -85,7 +100,9 @@ struct vxl_ctx {
   vxl::DevBuf scan_tmp;
   vxl::DevBuf words_io;  // staging for dense words
   vxl::DevBuf rays_io, hits_io, t_io;
-  void *host_pinned = nullptr;  // small readback area
+  void *host_pinned = nullptr;      // small readback area (mapped: kernels write it)
+  void *host_pinned_dev = nullptr;  // its device address
+  std::vector<vxl::Slab> slabs;     // tree arena
   // tuning
   int sat_waves_per_cu = 28;  // resident SAT waves per CU (persistent, work-queue fed)
   int sat_units_per_wave = 1;  // work units per resident wave
@@ -100,19 +117,13 @@ struct vxl_ctx {
   vxl::DevBuf pool_grid, pool_b1, batch_scratch, batch_l2, batch_desc, trav_desc;
   bool pool_dirty = false;
   uint64_t pool_budget = 16ull << 30;  // leaf-grid bytes one batch group may use
-  void *batch_host = nullptr;
+  void *batch_host = nullptr, *batch_host_dev = nullptr;  // mapped: kernels may write it
   size_t batch_host_bytes = 0, batch_host_used = 0;
   hipEvent_t staging_free = nullptr;  // recorded after the last upload from batch_host
   hipEvent_t trav_done = nullptr;     // recorded after the last batched traversal kernel
+  hipEvent_t ev_wait = nullptr;       // wait_stream's completion marker
 };
 
-namespace vxl {
-// One device allocation shared by the trees of one batch group; freed with the last tree.
-struct SharedBlock {
-  void *base = nullptr;
-  uint32_t refs = 0;
-};
-}  // namespace vxl
 
 struct vxl_tree {
   vxl_ctx *ctx = nullptr;
@@ -124,7 +135,7 @@ struct vxl_tree {
   uint64_t *codes = nullptr;   // 1 x u64 per leaf (>= 1 entry allocated)
   uint64_t level_nodes[16] = {0};
   size_t bytes = 0;
-  bool pooled = false;  // one stream-ordered allocation starting at `nodes`
+  int slab = -1;  // arena slab holding the four arrays (-1: separate device allocations)
   vxl::SharedBlock *shared = nullptr;  // set for trees of a batch: arrays live in `shared`
 };
 
@@ -140,6 +151,11 @@ inline void mark(vxl_ctx *ctx, int i) {
 // ---------------------------------------------------------------------------
 bool resolution_k(uint32_t n, uint32_t *k);  // n = 8*4^k ?
 bool is_device_ptr(const void *p);
+
+// Waits until every operation enqueued on s so far has completed, polling an event
+// rather than sleeping in hipStreamSynchronize: the sizing read-backs between the
+// voxelize stages sit on the critical path of every call.
+void wait_stream(vxl_ctx *ctx, hipStream_t s);
 
 // scan.hip: exclusive scans on `stream`; total (if non-null) is written to device memory.
 void scan_exclusive_u64(const uint64_t *in, uint64_t *out, uint64_t count, uint64_t *total_dev,
@@ -191,18 +207,33 @@ inline size_t batch_staging_bytes(uint32_t G) {
   return up(G * sizeof(BatchMesh)) + up((G + 1) * 4) + up(G * sizeof(LevelPtrs)) +
          up(G * sizeof(BatchTotals)) + up(G * 8) + 1024;
 }
-// Runs setup + SAT into ctx->leafgrid / ctx->bitmap (level-1). Returns pair count.
-uint64_t run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizeParams &p);
+// The SAT launch plan the device derives from the pair total (voxelize.hip).
+struct SatPlan {
+  uint64_t P;       // candidate pairs
+  uint64_t cpu;     // chunks of 64 pairs per work unit
+  uint64_t nunits;  // work units
+  unsigned int counter;  // next unit to hand out
+};
+// Enqueues setup + SAT into ctx->leafgrid / ctx->bitmap (level-1); no host round trip.
+// The pair count stays on the device at sat_pairs_dev(ctx) until the next call.
+void run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizeParams &p);
+const uint64_t *sat_pairs_dev(vxl_ctx *ctx);
 // Setup + SAT of a batch group into the pooled grid; mesh g owns setup blocks
 // [sb_begin[g], sb_begin[g+1]) (setup_slots_per_block() record slots each).
-uint64_t run_sat_batch(vxl_ctx *ctx, const BatchMesh *d_mesh,
-                       const uint32_t *d_sb_begin, uint32_t G, uint32_t setup_blocks,
-                       unsigned long long *leafgrid, unsigned long long *brickbits);
+void run_sat_batch(vxl_ctx *ctx, const BatchMesh *d_mesh, const uint32_t *d_sb_begin,
+                   uint32_t G, uint32_t setup_blocks, unsigned long long *leafgrid,
+                   unsigned long long *brickbits);
 uint64_t setup_slots_per_block();
 
 // tree.hip
-// Builds a School-B tree from the brick bitmap + leafgrid in ctx (clearing both).
-vxl_tree *build_tree_from_grid(vxl_ctx *ctx, uint32_t n, uint32_t k);
+// Builds a School-B tree from the brick bitmap + leafgrid in ctx (clearing both). When
+// pairs is non-null it receives *pairs_dev (0 if pairs_dev is null), read with the sizes.
+vxl_tree *build_tree_from_grid(vxl_ctx *ctx, uint32_t n, uint32_t k,
+                               const uint64_t *pairs_dev = nullptr, uint64_t *pairs = nullptr);
+// Tree arena (tree.hip)
+void *arena_alloc(vxl_ctx *ctx, size_t bytes, int *slab_out);
+void arena_release(vxl_ctx *ctx, int slab);
+void arena_destroy(vxl_ctx *ctx);
 // Converts the leafgrid to x-major dense words (n^3/32 u32) and clears grid + bitmap.
 void grid_to_words(vxl_ctx *ctx, uint32_t n, uint32_t *d_words);
 // Loads x-major words into the leafgrid + brick bitmap.
@@ -245,6 +276,7 @@ vxl_tree *empty_tree_device(vxl_ctx *ctx, uint32_t n, uint32_t k);
 void *staging_begin(vxl_ctx *ctx, size_t bytes);
 void *staging_take(vxl_ctx *ctx, size_t bytes);
 void staging_end(vxl_ctx *ctx, hipStream_t s);
+void *staging_dev(vxl_ctx *ctx, void *host);  // device address of a staging pointer
 // One mesh handed to voxelize_batch: out[index] / pairs[index] receive its results.
 struct BatchInput {
   uint32_t index;
