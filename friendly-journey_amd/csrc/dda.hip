@@ -169,30 +169,29 @@ __device__ __forceinline__ void traverse_ray(
     te = (TENTER);                                                                         \
   }
 
-    // Walker::step (mirror.rs:87-116) on the active frame; returns false when exhausted.
+    // Walker::step (mirror.rs:87-116) on the active frame; OK = false when exhausted. The
+    // axis choice (lowest t_max, ties to the lowest axis) and the update are selects, so
+    // lanes stepping along different axes do not split the wave.
 #define VXL_STEP(OK)                                                                       \
   {                                                                                        \
     const uint32_t dim_ = level == 1 ? 8u : 4u;                                            \
-    int a_ = 0;                                                                            \
-    float best_ = tm0;                                                                     \
-    if (tm1 < best_) { a_ = 1; best_ = tm1; }                                              \
-    if (tm2 < best_) { a_ = 2; best_ = tm2; }                                              \
-    OK = false;                                                                            \
-    if (isfinite(best_)) {                                                                 \
-      if (a_ == 0) {                                                                       \
-        if (r.d[0] > 0.0f) { if (cx + 1 < dim_) { cx += 1; OK = true; } }                  \
-        else if (r.d[0] < 0.0f) { if (cx != 0) { cx -= 1; OK = true; } }                   \
-        if (OK) tm0 = tm0 + td0;                                                           \
-      } else if (a_ == 1) {                                                                \
-        if (r.d[1] > 0.0f) { if (cy + 1 < dim_) { cy += 1; OK = true; } }                  \
-        else if (r.d[1] < 0.0f) { if (cy != 0) { cy -= 1; OK = true; } }                   \
-        if (OK) tm1 = tm1 + td1;                                                           \
-      } else {                                                                             \
-        if (r.d[2] > 0.0f) { if (cz + 1 < dim_) { cz += 1; OK = true; } }                  \
-        else if (r.d[2] < 0.0f) { if (cz != 0) { cz -= 1; OK = true; } }                   \
-        if (OK) tm2 = tm2 + td2;                                                           \
-      }                                                                                    \
-      if (OK) te = best_;                                                                  \
+    const bool a1_ = tm1 < tm0;                                                            \
+    float best_ = a1_ ? tm1 : tm0;                                                         \
+    const bool a2_ = tm2 < best_;                                                          \
+    best_ = a2_ ? tm2 : best_;                                                             \
+    const bool on0_ = !a1_ && !a2_, on1_ = a1_ && !a2_;                                    \
+    const float da_ = a2_ ? r.d[2] : (a1_ ? r.d[1] : r.d[0]);                              \
+    const uint32_t ca_ = a2_ ? cz : (a1_ ? cy : cx);                                       \
+    OK = isfinite(best_) && ((da_ > 0.0f && ca_ + 1 < dim_) || (da_ < 0.0f && ca_ != 0));  \
+    if (OK) {                                                                              \
+      const uint32_t inc_ = da_ > 0.0f ? 1u : 0xFFFFFFFFu;                                 \
+      cx += on0_ ? inc_ : 0u;                                                              \
+      cy += on1_ ? inc_ : 0u;                                                              \
+      cz += a2_ ? inc_ : 0u;                                                               \
+      tm0 = on0_ ? tm0 + td0 : tm0;                                                        \
+      tm1 = on1_ ? tm1 + td1 : tm1;                                                        \
+      tm2 = a2_ ? tm2 + td2 : tm2;                                                         \
+      te = best_;                                                                          \
     }                                                                                      \
   }
 
@@ -224,28 +223,32 @@ __device__ __forceinline__ void traverse_ray(
         break;
       }
       bool stepped;
+      // the active cell: a voxel bit of the leaf (one word load) or a child bit of the node
+      // record held in registers; one step site serves both kinds of frame
+      uint32_t word, bitpos;
       if (level == 1) {
         const uint32_t m = dev::morton9(cx, cy, cz);
-        const uint32_t word = leafw[uint64_t(node) * 16 + (m >> 5)];
-        if ((word >> (m & 31)) & 1u) {
-          result = make_uint4(ox + cx, oy + cy, oz + cz, 1u);
-          hit = true;
-          break;
-        }
-        VXL_STEP(stepped);
+        word = leafw[uint64_t(node) * 16 + (m >> 5)];
+        bitpos = m & 31;
       } else {
         const uint32_t bit = dev::child_bit(cx, cy, cz);
-        const bool has = bit < 32 ? ((mlo >> bit) & 1u) : ((mhi >> (bit - 32)) & 1u);
+        word = bit < 32 ? mlo : mhi;
+        bitpos = bit & 31;
+      }
+      const bool has = (word >> bitpos) & 1u;
+      if (level == 1 && has) {
+        result = make_uint4(ox + cx, oy + cy, oz + cz, 1u);
+        hit = true;
+        break;
+      }
+      if (level > 1 && has) {
+        const uint32_t bit = dev::child_bit(cx, cy, cz);
         const uint32_t size = frame_cell_size(level);
         const uint32_t cox = ox + cx * size, coy = oy + cy * size, coz = oz + cz * size;
-        bool descend = has;
-        uint32_t slot = 0;
-        if (has) {
-          const uint32_t below = bit < 32 ? __popc(mlo & ((1u << bit) - 1u))
-                                          : __popc(mlo) + __popc(mhi & ((1u << (bit - 32)) - 1u));
-          slot = cbase + below;
-          if (level == 2) descend = leaf_reaches(bounds[slot], r, cox, coy, coz, te);
-        }
+        const uint32_t below = bit < 32 ? __popc(mlo & ((1u << bit) - 1u))
+                                        : __popc(mlo) + __popc(mhi & ((1u << (bit - 32)) - 1u));
+        const uint32_t slot = cbase + below;
+        const bool descend = level != 2 || leaf_reaches(bounds[slot], r, cox, coy, coz, te);
         if (descend) {
           switch (level) {
             VXL_SAVE_CASE(2)
@@ -269,8 +272,8 @@ __device__ __forceinline__ void traverse_ray(
           }
           continue;
         }
-        VXL_STEP(stepped);
       }
+      VXL_STEP(stepped);
       // ascend (mirror.rs:155-165): pop parents until one steps to a new cell
       while (!stepped) {
         if (level == root_level) {
