@@ -45,7 +45,9 @@ struct Ray {
   bool big[3];
 };
 
-// Walker::enter (mirror.rs:48-85) for one axis.
+// Walker::enter (mirror.rs:48-85) for one axis. The d > 0 and d < 0 cases differ only in
+// which cell face is next (idx + 1 or idx), so one division serves both; a lane with
+// d == 0 (or NaN) keeps t_max = t_delta = +inf.
 __device__ __forceinline__ void axis_enter(const Ray &r, int a, float org, uint32_t dim, float cs,
                                            float inv_cs, float t_enter, uint32_t &cell,
                                            float &tmax, float &tdel) {
@@ -54,17 +56,13 @@ __device__ __forceinline__ void axis_enter(const Ray &r, int a, float org, uint3
   const float local = (entry - org) * inv_cs;
   const uint32_t idx = floor_to_cell(local, dim);
   cell = idx;
-  tmax = INFINITY;
-  tdel = INFINITY;
-  if (d > 0.0f) {
-    const float next = org + (float(idx) + 1.0f) * cs;
-    tmax = t_enter + (next - entry) / d;
-    tdel = r.big[a] ? cs / d : cs * r.rinv[a];
-  } else if (d < 0.0f) {
-    const float next = org + float(idx) * cs;
-    tmax = t_enter + (next - entry) / d;
-    tdel = -(r.big[a] ? cs / d : cs * r.rinv[a]);
-  }
+  const float face = d > 0.0f ? float(idx) + 1.0f : float(idx);
+  const float next = org + face * cs;
+  const float tm = t_enter + (next - entry) / d;
+  float q = cs * r.rinv[a];
+  if (r.big[a]) q = cs / d;
+  tmax = (d > 0.0f || d < 0.0f) ? tm : INFINITY;  // a NaN component keeps +inf, as in enter
+  tdel = d > 0.0f ? q : (d < 0.0f ? -q : INFINITY);
 }
 
 __device__ __forceinline__ uint32_t frame_cell_size(uint32_t level) {

@@ -94,3 +94,30 @@ def test_traverse_empty_batch_and_empty_tree(ctx):
     assert not hits.any() and not t.any()
     hits, t = ctx.traverse(e, np.zeros((0, 6), np.float32))
     assert hits.shape == (0, 4)
+
+
+def test_non_finite_and_degenerate_rays(ctx):
+    """Directions with NaN, +-inf, zero and huge components, origins with NaN / inf:
+    hits identical to the mirror (which keeps t_max = +inf for a NaN or zero component)."""
+    n = 128
+    words = O.fixture_words("checkerboard", n)
+    ot = O.Tree.build(words, n)
+    gt = ctx.tree_from_words(words, n)
+    nan, inf = np.nan, np.inf
+    rng = np.random.default_rng(5)
+    rows = []
+    specials = [nan, inf, -inf, 0.0, -0.0, 1e30, -1e30, 3e38, 1e-30, 1.0, -1.0]
+    for _ in range(3000):
+        o = rng.uniform(-10, n + 10, 3)
+        d = rng.normal(size=3)
+        for a in range(3):
+            if rng.random() < 0.3:
+                d[a] = specials[rng.integers(len(specials))]
+        if rng.random() < 0.05:
+            o[rng.integers(3)] = specials[rng.integers(3)]
+        rows.append(np.concatenate([o, d]))
+    rays = np.array(rows, np.float32)
+    hits, _ = ctx.traverse(gt, rays)
+    want = ot.mirror(rays)
+    bad = np.any(hits != want, axis=1)
+    assert not bad.any(), f"{int(bad.sum())} mismatches, first rays {rays[np.nonzero(bad)[0][:3]]}"
