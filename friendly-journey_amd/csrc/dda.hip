@@ -26,10 +26,17 @@ namespace {
 constexpr uint32_t kFull = (7u << 9) | (7u << 12) | (7u << 15);
 constexpr uint32_t kMaxIter = 1u << 22;
 
-struct Saved {
-  uint32_t node, cell;
-  float tm0, tm1, tm2, td0, td1, td2, te;
-};
+// A suspended ancestor frame lives in LDS, one row per (level, field), one column per
+// thread: node, packed cell, t_max x 3. Its t_delta is not kept (a function of the frame's
+// cell size and the ray, recomputed with the same operations on restore), nor its t_entry
+// (a restored frame steps at once, which overwrites it, or the walk ends).
+constexpr int kFrameWords = 5;
+
+__device__ __forceinline__ float axis_delta(float cs, float d, float rinv, bool big) {
+  float q = cs * rinv;
+  if (big) q = cs / d;
+  return d > 0.0f ? q : (d < 0.0f ? -q : INFINITY);
+}
 
 __device__ __forceinline__ uint32_t floor_to_cell(float v, uint32_t dim) {
   // mirror.rs:31-33: v.floor().clamp(0, dim-1) as u32 (NaN passes the clamp -> 0)
@@ -59,10 +66,8 @@ __device__ __forceinline__ void axis_enter(const Ray &r, int a, float org, uint3
   const float face = d > 0.0f ? float(idx) + 1.0f : float(idx);
   const float next = org + face * cs;
   const float tm = t_enter + (next - entry) / d;
-  float q = cs * r.rinv[a];
-  if (r.big[a]) q = cs / d;
   tmax = (d > 0.0f || d < 0.0f) ? tm : INFINITY;  // a NaN component keeps +inf, as in enter
-  tdel = d > 0.0f ? q : (d < 0.0f ? -q : INFINITY);
+  tdel = axis_delta(cs, d, r.rinv[a], r.big[a]);
 }
 
 __device__ __forceinline__ uint32_t frame_cell_size(uint32_t level) {
@@ -153,7 +158,9 @@ __device__ __forceinline__ void traverse_ray(
     float tm0, tm1, tm2, td0, td1, td2;
     float te = fmaxf(t0, 0.0f);
     uint32_t mlo = 0, mhi = 0, cbase = 0;  // cached node record of a node frame
-    Saved s2, s3, s4, s5, s6;
+    extern __shared__ uint32_t frames_lds[];
+    uint32_t *const frames = frames_lds + threadIdx.x;  // frames[((L-2)*5 + f) * blockDim.x]
+    const uint32_t fstride = blockDim.x;
 
 #define VXL_MAKE_FRAME(LEVEL, ORGX, ORGY, ORGZ, TENTER)                                    \
   {                                                                                        \
@@ -192,20 +199,6 @@ __device__ __forceinline__ void traverse_ray(
       te = best_;                                                                          \
     }                                                                                      \
   }
-
-#define VXL_SAVE_CASE(L)                                                                   \
-  case L:                                                                                  \
-    s##L.node = node; s##L.cell = cx | (cy << 8) | (cz << 16);                             \
-    s##L.tm0 = tm0; s##L.tm1 = tm1; s##L.tm2 = tm2;                                        \
-    s##L.td0 = td0; s##L.td1 = td1; s##L.td2 = td2; s##L.te = te;                          \
-    break;
-#define VXL_LOAD_CASE(L)                                                                   \
-  case L:                                                                                  \
-    node = s##L.node; cx = s##L.cell & 0xffu; cy = (s##L.cell >> 8) & 0xffu;               \
-    cz = s##L.cell >> 16;                                                                  \
-    tm0 = s##L.tm0; tm1 = s##L.tm1; tm2 = s##L.tm2;                                        \
-    td0 = s##L.td0; td1 = s##L.td1; td2 = s##L.td2; te = s##L.te;                          \
-    break;
 
     VXL_MAKE_FRAME(root_level, 0u, 0u, 0u, te);
     if (level >= 2) {
@@ -248,13 +241,13 @@ __device__ __forceinline__ void traverse_ray(
         const uint32_t slot = cbase + below;
         const bool descend = level != 2 || leaf_reaches(bounds[slot], r, cox, coy, coz, te);
         if (descend) {
-          switch (level) {
-            VXL_SAVE_CASE(2)
-            VXL_SAVE_CASE(3)
-            VXL_SAVE_CASE(4)
-            VXL_SAVE_CASE(5)
-            VXL_SAVE_CASE(6)
-            default: break;
+          {
+            uint32_t *f = frames + (level - 2) * kFrameWords * fstride;
+            f[0] = node;
+            f[fstride] = cx | (cy << 8) | (cz << 16);
+            f[2 * fstride] = __float_as_uint(tm0);
+            f[3 * fstride] = __float_as_uint(tm1);
+            f[4 * fstride] = __float_as_uint(tm2);
           }
           const float t_in = te;
           level -= 1;
@@ -279,15 +272,22 @@ __device__ __forceinline__ void traverse_ray(
           break;
         }
         level += 1;
-        switch (level) {
-          VXL_LOAD_CASE(2)
-          VXL_LOAD_CASE(3)
-          VXL_LOAD_CASE(4)
-          VXL_LOAD_CASE(5)
-          VXL_LOAD_CASE(6)
-          default: break;
-        }
         const uint32_t size = frame_cell_size(level);
+        {
+          const uint32_t *f = frames + (level - 2) * kFrameWords * fstride;
+          node = f[0];
+          const uint32_t cell = f[fstride];
+          cx = cell & 0xffu;
+          cy = (cell >> 8) & 0xffu;
+          cz = cell >> 16;
+          tm0 = __uint_as_float(f[2 * fstride]);
+          tm1 = __uint_as_float(f[3 * fstride]);
+          tm2 = __uint_as_float(f[4 * fstride]);
+          const float cs = float(size);
+          td0 = axis_delta(cs, r.d[0], r.rinv[0], r.big[0]);
+          td1 = axis_delta(cs, r.d[1], r.rinv[1], r.big[1]);
+          td2 = axis_delta(cs, r.d[2], r.rinv[2], r.big[2]);
+        }
         ox -= cx * size;
         oy -= cy * size;
         oz -= cz * size;
@@ -299,8 +299,6 @@ __device__ __forceinline__ void traverse_ray(
     }
 #undef VXL_MAKE_FRAME
 #undef VXL_STEP
-#undef VXL_SAVE_CASE
-#undef VXL_LOAD_CASE
   }
   hits[ri] = result;
   if (tout) {
@@ -354,6 +352,12 @@ __global__ __launch_bounds__(256) void k_traverse_batch(const TraverseSeg *__res
                sg.t);
 }
 
+// LDS for the suspended frames: levels 2..k+1 (the walk saves a frame when it descends
+// from it), kFrameWords words per thread each.
+size_t frame_lds_bytes(uint32_t k, uint64_t lanes) {
+  return size_t(k >= 1 ? k : 1) * kFrameWords * 4 * lanes;
+}
+
 }  // namespace
 
 void launch_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, uint64_t R,
@@ -363,7 +367,8 @@ void launch_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, ui
   const uint64_t blocks = (R + tb - 1) / tb;
   if (blocks > 0xffffffffull)
     throw Error(VXL_WORKGROUPS_EXCEEDED, "traverse: ray batch exceeds the grid limit");
-  hipLaunchKernelGGL(k_traverse, dim3(uint32_t(blocks)), dim3(tb), 0, s, d_rays, R, tree->nodes,
+  hipLaunchKernelGGL(k_traverse, dim3(uint32_t(blocks)), dim3(tb), frame_lds_bytes(tree->k, tb), s,
+                     d_rays, R, tree->nodes,
                      reinterpret_cast<const uint32_t *>(tree->leaves), tree->bounds, tree->n,
                      tree->k, tree->leaf_count, reinterpret_cast<uint4 *>(d_hits), d_t);
   VXL_HIP(hipGetLastError());
@@ -399,7 +404,10 @@ void launch_traverse_batch(vxl_ctx *ctx, const RaySegment *segs, uint32_t S, hip
   else VXL_HIP(hipEventCreateWithFlags(&ctx->trav_done, hipEventDisableTiming));
   VXL_HIP(hipMemcpyAsync(d, h, seg_bytes + blk_bytes, hipMemcpyHostToDevice, s));
   staging_end(ctx, s);
-  hipLaunchKernelGGL(k_traverse_batch, dim3(uint32_t(blocks)), dim3(tb), 0, s,
+  uint32_t kmax = 0;
+  for (uint32_t g = 0; g < S; g++) kmax = segs[g].tree->k > kmax ? segs[g].tree->k : kmax;
+  hipLaunchKernelGGL(k_traverse_batch, dim3(uint32_t(blocks)), dim3(tb),
+                     frame_lds_bytes(kmax, tb), s,
                      reinterpret_cast<const TraverseSeg *>(d),
                      reinterpret_cast<const uint32_t *>(d + seg_bytes), S);
   VXL_HIP(hipGetLastError());
