@@ -239,6 +239,49 @@ __device__ __forceinline__ bool plane_passes(const Rel &v) {
   return !(fabsf(s) > r);
 }
 
+// Finite vertices: the 12 axes as one straight-line conjunction (a wave rarely has all
+// lanes fail the same early axis, so early exits only cost branches). In the reference's
+// radius 0.5|ax| + 0.5|ay| + 0.5|az| one axis component is the constant 0; 0.5|0| = +0
+// and adding +0 to a value that is >= +0 or NaN returns it unchanged, so each radius is
+// the sum of two of the nine per-edge terms 0.5|e_c|, formed once.
+__device__ __forceinline__ bool rest_passes_finite(const Rel &v) {
+  const float h = 0.5f;
+  const float v0x = v.v0x, v0y = v.v0y, v0z = v.v0z;
+  const float v1x = v.v1x, v1y = v.v1y, v1z = v.v1z;
+  const float v2x = v.v2x, v2y = v.v2y, v2z = v.v2z;
+  bool ok = true;
+  // box-face axes (:66-74)
+  ok &= !(fminf(v0x, fminf(v1x, v2x)) > h || fmaxf(v0x, fmaxf(v1x, v2x)) < -h);
+  ok &= !(fminf(v0y, fminf(v1y, v2y)) > h || fmaxf(v0y, fmaxf(v1y, v2y)) < -h);
+  ok &= !(fminf(v0z, fminf(v1z, v2z)) > h || fmaxf(v0z, fmaxf(v1z, v2z)) < -h);
+  const float e0x = v1x - v0x, e0y = v1y - v0y, e0z = v1z - v0z;
+  const float e1x = v2x - v1x, e1y = v2y - v1y, e1z = v2z - v1z;
+  const float e2x = v0x - v2x, e2y = v0y - v2y, e2z = v0z - v2z;
+  const float h0x = h * fabsf(e0x), h0y = h * fabsf(e0y), h0z = h * fabsf(e0z);
+  const float h1x = h * fabsf(e1x), h1y = h * fabsf(e1y), h1z = h * fabsf(e1z);
+  const float h2x = h * fabsf(e2x), h2y = h * fabsf(e2y), h2z = h * fabsf(e2z);
+  // the 9 edge x box-axis tests (:42-64), p_i = v_i . axis without the zero component
+#define VXL_AXIS2(P0, P1, P2, R)                                                      \
+  {                                                                                   \
+    const float p0 = (P0), p1 = (P1), p2 = (P2), r = (R);                             \
+    ok &= !(fminf(p0, fminf(p1, p2)) > r || fmaxf(p0, fmaxf(p1, p2)) < -r);           \
+  }
+  // axes (0, -e.z, e.y)
+  VXL_AXIS2(v0y * -e0z + v0z * e0y, v1y * -e0z + v1z * e0y, v2y * -e0z + v2z * e0y, h0z + h0y)
+  VXL_AXIS2(v0y * -e1z + v0z * e1y, v1y * -e1z + v1z * e1y, v2y * -e1z + v2z * e1y, h1z + h1y)
+  VXL_AXIS2(v0y * -e2z + v0z * e2y, v1y * -e2z + v1z * e2y, v2y * -e2z + v2z * e2y, h2z + h2y)
+  // axes (e.z, 0, -e.x)
+  VXL_AXIS2(v0x * e0z + v0z * -e0x, v1x * e0z + v1z * -e0x, v2x * e0z + v2z * -e0x, h0z + h0x)
+  VXL_AXIS2(v0x * e1z + v0z * -e1x, v1x * e1z + v1z * -e1x, v2x * e1z + v2z * -e1x, h1z + h1x)
+  VXL_AXIS2(v0x * e2z + v0z * -e2x, v1x * e2z + v1z * -e2x, v2x * e2z + v2z * -e2x, h2z + h2x)
+  // axes (-e.y, e.x, 0)
+  VXL_AXIS2(v0x * -e0y + v0y * e0x, v1x * -e0y + v1y * e0x, v2x * -e0y + v2y * e0x, h0y + h0x)
+  VXL_AXIS2(v0x * -e1y + v0y * e1x, v1x * -e1y + v1y * e1x, v2x * -e1y + v2y * e1x, h1y + h1x)
+  VXL_AXIS2(v0x * -e2y + v0y * e2x, v1x * -e2y + v1y * e2x, v2x * -e2y + v2y * e2x, h2y + h2x)
+#undef VXL_AXIS2
+  return ok;
+}
+
 template <bool kLiteral>
 __device__ __forceinline__ bool rest_passes(const Rel &v) {
   const float h = 0.5f;
@@ -356,7 +399,7 @@ __device__ __forceinline__ bool finish_pair(const TriRec *__restrict__ rec, uint
   const TriRec r = rec[t];
   const uint32_t x = xy & 0xffffu, y = xy >> 16;
   const Rel v = rel_to(r.g, float(x) + 0.5f, float(y) + 0.5f, float(z) + 0.5f);
-  const bool hit = (r.flags & kLiteral) ? rest_passes<true>(v) : rest_passes<false>(v);
+  const bool hit = (r.flags & kLiteral) ? rest_passes<true>(v) : rest_passes_finite(v);
   const uint32_t code = dev::brick_code(x >> 3, y >> 3, z >> 3);
   const uint32_t m = dev::morton9(x, y, z);
   key = (uint64_t(r.grid_base) << 9) + uint64_t(code) * 8 + (m >> 6);
