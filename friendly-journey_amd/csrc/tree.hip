@@ -81,6 +81,19 @@ __device__ __forceinline__ uint32_t select_bit(unsigned long long m, uint32_t s)
   return pos;
 }
 
+// Coordinates 0..3 along `axis` occupied in a 4^3 sub-brick word, as a 4-bit mask
+// (bit = x0|y0<<1|z0<<2|x1<<3|y1<<4|z1<<5).
+__device__ __forceinline__ uint32_t word_coords(unsigned long long w, int axis) {
+  const unsigned long long lo0[3] = {0x5555555555555555ull, 0x3333333333333333ull,
+                                     0x0F0F0F0F0F0F0F0Full};
+  const unsigned long long lo1[3] = {0x00FF00FF00FF00FFull, 0x0000FFFF0000FFFFull,
+                                     0x00000000FFFFFFFFull};
+  const unsigned long long m0 = lo0[axis] & lo1[axis], m1 = ~lo0[axis] & lo1[axis];
+  const unsigned long long m2 = lo0[axis] & ~lo1[axis], m3 = ~lo0[axis] & ~lo1[axis];
+  return ((w & m0) ? 1u : 0u) | ((w & m1) ? 2u : 0u) | ((w & m2) ? 4u : 0u) |
+         ((w & m3) ? 8u : 0u);
+}
+
 // Coordinate masks of a 4^3 sub-brick word (bit = x0|y0<<1|z0<<2|x1<<3|y1<<4|z1<<5).
 __device__ __forceinline__ void word_extent(unsigned long long w, int axis, uint32_t &mn,
                                             uint32_t &mx) {
@@ -131,29 +144,25 @@ __global__ __launch_bounds__(kTB) void k_leaf_gather(
       leafgrid[code * 8 + w] = 0ull;
       leaves[(uint64_t(base) + s) * 8 + w] = v;
     }
-    // LeafBounds over the 8 words of this leaf (8-lane groups)
+    // LeafBounds over the 8 words of this leaf (8-lane groups): each lane marks the
+    // coordinates its 4^3 sub-brick word occupies along x, y, z (one byte each, shifted to
+    // the word's octant), then three xor-swizzle rounds OR the marks and add the counts.
     uint32_t cnt = __popcll(v);
+    uint32_t occ = 0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) occ |= (word_coords(v, a) << ((w >> a) & 1) * 4) << (8 * a);
+    cnt += (uint32_t)__builtin_amdgcn_ds_swizzle((int)cnt, 0x041f);  // lane ^ 1
+    occ |= (uint32_t)__builtin_amdgcn_ds_swizzle((int)occ, 0x041f);
+    cnt += (uint32_t)__builtin_amdgcn_ds_swizzle((int)cnt, 0x081f);  // lane ^ 2
+    occ |= (uint32_t)__builtin_amdgcn_ds_swizzle((int)occ, 0x081f);
+    cnt += (uint32_t)__builtin_amdgcn_ds_swizzle((int)cnt, 0x101f);  // lane ^ 4
+    occ |= (uint32_t)__builtin_amdgcn_ds_swizzle((int)occ, 0x101f);
     uint32_t mn[3], mx[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-      word_extent(v, a, mn[a], mx[a]);
-      const uint32_t off = ((w >> a) & 1) * 4;
-      if (v) {
-        mn[a] += off;
-        mx[a] += off;
-      } else {
-        mn[a] = 7;
-        mx[a] = 0;
-      }
-    }
-#pragma unroll
-    for (int d = 1; d < 8; d <<= 1) {
-      cnt += __shfl_xor(cnt, d, 64);
-#pragma unroll
-      for (int a = 0; a < 3; a++) {
-        mn[a] = min(mn[a], (uint32_t)__shfl_xor((int)mn[a], d, 64));
-        mx[a] = max(mx[a], (uint32_t)__shfl_xor((int)mx[a], d, 64));
-      }
+      const uint32_t m = (occ >> (8 * a)) & 0xffu;
+      mn[a] = m ? uint32_t(__ffs(m) - 1) : 7u;
+      mx[a] = m ? 31u - uint32_t(__clz(m)) : 0u;
     }
     if (act && w == 0) {
       uint32_t b = (cnt > 64 || cnt == 0)
