@@ -34,7 +34,11 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes one k_sat launch moves on terrain_1024, from two rocprofv3 --pmc passes
 # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB per dispatch, factors from a 1 GiB copy kernel);
 # profiles/k_sat__terrain_1024.md holds the commands and the raw counter values.
-K_SAT_TRAFFIC_BYTES = 272_661_818
+K_SAT_TRAFFIC_BYTES = 271_399_899
+# Algorithmic VALU lane-ops of one ray walk, counted on the reference's f32 walker: a cell
+# visit (bit test + Walker::step) ~22, a descent (Walker::enter on 3 axes + the frame save)
+# ~50, ray setup and clip ~60, the f64 t of a hit ~20.
+DDA_OPS_VISIT, DDA_OPS_DESCENT, DDA_OPS_RAY = 22, 50, 80
 
 
 def log(msg):
@@ -338,9 +342,10 @@ def bench_serve(ctx, torch, W, duration_s=2.0):
     return out
 
 
-def dda_bytes_per_ray(tree, rays, sample=20000):
-    """Algorithmic HBM bytes per ray = 24 (ray) + 16 (hit) + 8 (t) + 68 per leaf descent +
-    12 per node descent below the root, with descents counted by the oracle's mirror."""
+def dda_walk_stats(tree, rays, sample=20000):
+    """Per-ray walk statistics counted by the oracle's mirror on a ray sample: leaf and node
+    descents and cell visits. Algorithmic HBM bytes per ray = 24 (ray) + 16 (hit) + 8 (t)
+    + 68 per leaf descent + 12 per node descent below the root."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as O
@@ -348,8 +353,10 @@ def dda_bytes_per_ray(tree, rays, sample=20000):
     ot = O.Tree.from_school_b(tree.n, nodes, leaves, bounds)
     idx = np.linspace(0, len(rays) - 1, sample).astype(np.int64)
     _, st = ot.mirror(rays[idx], stats=True)
-    d_leaf, d_node = st[:, 0].mean(), st[:, 1].mean()
-    return 24 + 16 + 8 + 68 * d_leaf + 12 * d_node, float(d_leaf), float(d_node)
+    d_leaf, d_node, visits = (float(v) for v in st.mean(0))
+    return dict(bytes=24 + 16 + 8 + 68 * d_leaf + 12 * d_node, leaf_descents=d_leaf,
+                node_descents=d_node, visits=visits,
+                ops=DDA_OPS_RAY + DDA_OPS_VISIT * visits + DDA_OPS_DESCENT * (d_leaf + d_node))
 
 
 def cpu_baseline(tris, W):
@@ -443,11 +450,20 @@ def main():
     if not args.no_dda:
         dda = bench_dda(ctx, torch, W)
         for name, rec in dda.items():
-            bpr, dl, dn = dda_bytes_per_ray(rec.pop("_tree"), rec.pop("_rays"))
-            gbs = bpr * rec["rays"] / (rec["kernel_ms"] / 1e3) / 1e9
-            rec["roofline"] = {"bound": "hbm", "bytes_per_ray": round(bpr, 1), "leaf_descents": dl,
-                               "node_descents": dn, "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
-                               "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
+            ws = dda_walk_stats(rec.pop("_tree"), rec.pop("_rays"))
+            secs = rec["kernel_ms"] / 1e3
+            gbs = ws["bytes"] * rec["rays"] / secs / 1e9
+            tops = ws["ops"] * rec["rays"] / secs / 1e12
+            rec["walk"] = {k: round(ws[k], 3) for k in ("visits", "leaf_descents", "node_descents")}
+            rec["roofline"] = {"bound": "valu", "ops_per_ray": round(ws["ops"], 1),
+                               "achieved": round(tops, 3), "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
+                               "frac": round(tops / VALU_PEAK_TOPS, 4),
+                               "hbm": {"bytes_per_ray": round(ws["bytes"], 1),
+                                       "achieved_gbs": round(gbs, 1),
+                                       "frac": round(gbs / HBM_PEAK_GBS, 4)},
+                               "note": "walks diverge: 9-14% of lanes active per VALU instruction "
+                                       "(VALUUtilization), so the issued work is several times "
+                                       "the algorithmic work"}
         line["dda"] = dda
     if not args.no_batch:
         line["batched"] = bench_batch(ctx, torch, W)

@@ -429,8 +429,17 @@ __device__ __forceinline__ uint32_t table_insert(unsigned long long *tk, unsigne
   return __popcll(__ballot(fresh));
 }
 
-// Writes the table to the leaf grid and the brick bitmap with fire-and-forget atomics,
-// then empties it.
+// ORs `bits` into *p unless a coherent read already shows them all. During the SAT bits
+// only ever go from 0 to 1, so a read can only under-report what is set: skipping is
+// exact, and it keeps hot words (dense meshes on small grids hit a few thousand leaf words
+// and a handful of bitmap words from every wave) off the atomic units.
+__device__ __forceinline__ void or_if_missing(unsigned long long *p, unsigned long long bits) {
+  const unsigned long long cur =
+      __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if ((cur & bits) != bits) atomicOr(p, bits);
+}
+
+// Writes the table to the leaf grid and the brick bitmap, then empties it.
 __device__ __forceinline__ void table_flush(unsigned long long *tk, unsigned long long *tv, int lane,
                                             unsigned long long *__restrict__ leafgrid,
                                             unsigned long long *__restrict__ brickbits) {
@@ -440,9 +449,9 @@ __device__ __forceinline__ void table_flush(unsigned long long *tk, unsigned lon
     const int slot = i * 64 + lane;
     const unsigned long long key = tk[slot];
     if (key != kEmpty) {
-      atomicOr(&leafgrid[key], tv[slot]);
+      or_if_missing(&leafgrid[key], tv[slot]);
       const uint64_t code = key >> 3;
-      atomicOr(&brickbits[code >> 6], 1ull << (code & 63));
+      or_if_missing(&brickbits[code >> 6], 1ull << (code & 63));
       tk[slot] = kEmpty;
       tv[slot] = 0ull;
     }
