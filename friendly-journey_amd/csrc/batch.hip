@@ -78,6 +78,8 @@ void run_group(vxl_ctx *ctx, const BatchInput *in, uint32_t G, vxl_tree **out,
   staging_begin(ctx, batch_staging_bytes(G));
   BatchMesh *h_mesh = static_cast<BatchMesh *>(staging_take(ctx, G * sizeof(BatchMesh)));
   uint32_t *h_sb = static_cast<uint32_t *>(staging_take(ctx, (G + 1) * 4));
+  uint64_t *h_ref = static_cast<uint64_t *>(staging_take(ctx, G * 8));  // zeroed counters
+  std::memset(h_ref, 0, G * 8);
   std::vector<BatchMesh> meshes(G);
   uint64_t W1 = 0, blocks = 0;
   for (uint32_t g = 0; g < G; g++) {
@@ -101,16 +103,19 @@ void run_group(vxl_ctx *ctx, const BatchInput *in, uint32_t G, vxl_tree **out,
   std::memcpy(h_mesh, meshes.data(), G * sizeof(BatchMesh));
   ensure_pool(ctx, W1);
   const size_t mesh_bytes = (G * sizeof(BatchMesh) + 255) / 256 * 256;
-  ctx->batch_desc.reserve(mesh_bytes + (G + 1) * 4 + 256);
+  const size_t sb_bytes = ((G + 1) * 4 + 255) / 256 * 256;
+  const size_t desc_bytes = mesh_bytes + sb_bytes + G * 8;
+  ctx->batch_desc.reserve(desc_bytes + 256);
   char *d = static_cast<char *>(ctx->batch_desc.ptr);
-  VXL_HIP(hipMemcpyAsync(d, h_mesh, mesh_bytes + (G + 1) * 4, hipMemcpyHostToDevice, s));
+  VXL_HIP(hipMemcpyAsync(d, h_mesh, desc_bytes, hipMemcpyHostToDevice, s));
   const BatchMesh *d_mesh = reinterpret_cast<const BatchMesh *>(d);
+  unsigned long long *d_ref = reinterpret_cast<unsigned long long *>(d + mesh_bytes + sb_bytes);
   run_sat_batch(ctx, d_mesh, reinterpret_cast<const uint32_t *>(d + mesh_bytes), G,
                 uint32_t(blocks), ctx->pool_grid.as<unsigned long long>(),
-                ctx->pool_b1.as<unsigned long long>());  // no host round trip
+                ctx->pool_b1.as<unsigned long long>(), d_ref);  // no host round trip
   std::vector<vxl_tree *> trees(G, nullptr);
   std::vector<uint64_t> pr(G, 0);
-  build_trees_batch(ctx, meshes, d_mesh, W1, trees.data(), pr.data());
+  build_trees_batch(ctx, meshes, d_mesh, W1, trees.data(), pr.data(), d_ref);
   staging_end(ctx, s);
   for (uint32_t g = 0; g < G; g++) {
     out[in[g].index] = trees[g];

@@ -466,8 +466,9 @@ __global__ __launch_bounds__(kSmallThreads) void k_upper_place(LevelPtrs lp, int
 // n2_beg[g] and leaf_beg[g]. Upper levels are per mesh (one workgroup each).
 __global__ void k_batch_totals(const BatchMesh *__restrict__ mesh, const LevelPtrs *__restrict__ lps,
                                uint32_t G, const uint64_t *__restrict__ pn2,
-                               const uint64_t *__restrict__ coff, BatchTotals *__restrict__ out,
-                               uint64_t *__restrict__ n2_beg) {
+                               const uint64_t *__restrict__ coff,
+                               const unsigned long long *__restrict__ skipped,
+                               BatchTotals *__restrict__ out, uint64_t *__restrict__ n2_beg) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= G) return;
   const BatchMesh &m = mesh[g];
@@ -477,7 +478,7 @@ __global__ void k_batch_totals(const BatchMesh *__restrict__ mesh, const LevelPt
   t.n2 = uint32_t(hi >> 32) - uint32_t(lo >> 32);
   t.leaf_begin = uint32_t(lo);
   t.n2_begin = lo >> 32;
-  t.pairs = coff[m.slot_end] - coff[m.slot_begin];
+  t.pairs = coff[m.slot_end] - coff[m.slot_begin] + skipped[g];  // the reference's count
   for (int L = 3; L <= 5; L++)
     t.nodes[L - 3] = int(L) <= int(m.k) + 1 ? lps[g].tot[2 * L + 1] : 0u;
   out[g] = t;
@@ -811,7 +812,8 @@ vxl_tree *empty_tree_device(vxl_ctx *ctx, uint32_t n, uint32_t k) {
 }
 
 void build_trees_batch(vxl_ctx *ctx, const std::vector<BatchMesh> &meshes,
-                       const BatchMesh *d_mesh, uint64_t W1, vxl_tree **trees, uint64_t *pairs) {
+                       const BatchMesh *d_mesh, uint64_t W1, vxl_tree **trees, uint64_t *pairs,
+                       const unsigned long long *d_ref_pairs) {
   hipStream_t s = ctx->stream;
   const uint32_t G = uint32_t(meshes.size());
   auto up = [](uint64_t b) { return (b + 255) / 256 * 256; };
@@ -869,7 +871,7 @@ void build_trees_batch(vxl_ctx *ctx, const std::vector<BatchMesh> &meshes,
   scan_exclusive_u64(pn2, pn2, W1, pn2 + W1, ctx->scan_tmp, s);
   hipLaunchKernelGGL(k_small_levels_batch, dim3(G), dim3(kSmallThreads), 0, s, d_mesh, d_lps, b2);
   hipLaunchKernelGGL(k_batch_totals, dim3(div_up(G, 64)), dim3(64), 0, s, d_mesh, d_lps, G, pn2,
-                     ctx->coff.as<uint64_t>(), d_tot, n2_beg);
+                     ctx->coff.as<uint64_t>(), d_ref_pairs, d_tot, n2_beg);
   VXL_HIP(hipGetLastError());
   // 2. per-mesh sizes (the one host round trip of the group): k_batch_totals also wrote
   //    them into mapped pinned staging

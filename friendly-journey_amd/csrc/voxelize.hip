@@ -34,19 +34,30 @@ namespace {
 constexpr uint32_t kLiteral = 1u;    // a grid vertex is not finite: every product is formed
 constexpr uint32_t kWide = 2u;       // more than 2^22 candidates: 64-bit decode
 constexpr uint32_t kPlaneHoist = 4u; // exact per-triangle plane normal and radius (below)
+constexpr uint32_t kWindow = 8u;     // candidates: a window of W cells per column (below)
 
+// The candidates a triangle enumerates are the cells of a box decoded x fastest: its
+// clamped bounding box, or, for a windowed triangle, (W, ext_a, ext_b) -- W cells along
+// the normal's largest axis c per (a, b) column, starting at floor(k0 + ka*a + kb*b).
 struct __align__(16) TriRec {
   float g[9];         // grid-space vertices
   float nrm[3];       // e0 x e1 from the grid vertices (valid with kPlaneHoist)
   float rp;           // 0.5 |n|_1, summed as the reference sums it (valid with kPlaneHoist)
-  uint32_t mnxy;      // candidate box min: x | y << 16
-  uint32_t mnz_ex;    // box min z | box extent along x << 16
-  uint32_t exy;       // extent x * extent y
-  uint32_t flags;     // kLiteral | kWide | kPlaneHoist
+  uint32_t mnxy;      // bounding box min: x | y << 16
+  uint32_t mnz_ex;    // bounding box min z | decoded box extent along its first axis << 16
+  uint32_t exy;       // decoded box: extent of the first two axes multiplied
+  uint32_t flags;     // kLiteral | kWide | kPlaneHoist | kWindow
   float rcp_ex, rcp_exy;  // 1/ex, 1/exy for the narrow decode
   uint32_t grid_base;  // mesh's first leaf-grid word / 512 (0 for a single mesh)
 };
 static_assert(sizeof(TriRec) == 80, "TriRec is one 80-byte record");
+
+// The window of a kWindow triangle, in a side array beside the records (written for
+// windowed triangles only, so the others cost no bytes).
+struct __align__(16) WinRec {
+  float k0, ka, kb;   // window start along c
+  uint32_t cinfo;     // c | W << 2 | bounding-box extent along c << 6
+};
 
 // True when g - c is exact in binary32 for every candidate centre c = m + 1/2, lo <= m <= hi.
 // g is a multiple of u = 2^(E-23) (E its exponent); with E <= 22 the centres are multiples
@@ -81,9 +92,14 @@ __device__ __forceinline__ void tri_grid(const float *__restrict__ tri, const Vo
 
 // One pass per triangle: grid-space vertices, candidate box, candidate count, and the
 // 64-byte record the SAT kernel reads. Triangles with no candidate keep count 0.
-__device__ __forceinline__ void setup_one(const float *__restrict__ tri, const VoxelizeParams &p,
-                                          uint32_t grid_base, uint64_t *__restrict__ count,
-                                          TriRec *__restrict__ rec) {
+// *count receives the number of candidates this record decodes; returns how many fewer
+// that is than the reference's count (the clamped bounding-box volume, as
+// reference_cpu.rs:139-154 enumerates): 0 unless the triangle is windowed.
+__device__ __forceinline__ uint64_t setup_one(const float *__restrict__ tri,
+                                              const VoxelizeParams &p, uint32_t grid_base,
+                                              uint64_t *__restrict__ count,
+                                              TriRec *__restrict__ rec,
+                                              WinRec *__restrict__ wrec) {
   TriRec r;
   tri_grid(tri, p, r.g);
   bool finite = true;
@@ -104,12 +120,11 @@ __device__ __forceinline__ void setup_one(const float *__restrict__ tri, const V
     ext[a] = mx < mn ? 1u : uint32_t(mx - mn + 1);
   }
   *count = c;
-  if (c == 0) return;  // never read
+  if (c == 0) return 0;  // never read
   r.mnxy = uint32_t(r_mn[0]) | (uint32_t(r_mn[1]) << 16);
   r.mnz_ex = uint32_t(r_mn[2]) | (ext[0] << 16);
   r.exy = ext[0] * ext[1];
-  // narrow decode: every local index < 2^22, so the float quotient is off by at most 1
-  r.flags = (finite ? 0u : kLiteral) | ((c >> 22) ? kWide : 0u);
+  r.flags = finite ? 0u : kLiteral;
   // When every v_i = g_i - c is exact for all candidate centres, the reference's
   // centre-relative edges (v1 - v0, v2 - v1) round to the same values as g1 - g0 and
   // g2 - g1, so the plane normal and its radius are per-triangle constants; only
@@ -133,17 +148,90 @@ __device__ __forceinline__ void setup_one(const float *__restrict__ tri, const V
     hoist = hoist && isfinite(r.nrm[0]) && isfinite(r.nrm[1]) && isfinite(r.nrm[2]);
   }
   if (hoist) r.flags |= kPlaneHoist;
-  r.rcp_ex = 1.0f / float(ext[0]);
+  uint64_t decoded = c;
+  // Window (hoisted triangles whose box is at least twice the window): with c the axis of
+  // the normal's largest component, a cell passes the plane test only if its centre lies
+  // within hw = R / |n_c| of the plane along c, where R bounds the test's radius plus the
+  // rounding of |d| (relative 1e-4 of rp and of sum |n_i| |v0_i| over the box, orders of
+  // magnitude above the few-ulp errors of the binary32 sums), and hw carries one more cell of
+  // slack for the binary32 evaluation of the window start (values below 2^15, so its
+  // rounding stays below 0.01 of a cell). So the cells floor(c0 - hw - 1/2) .. + W - 1 of
+  // each column, W = ceil(2 hw) + 2, hold every candidate that can pass; the rest of the box
+  // is never enumerated. Cells outside the bounding box are skipped when decoded. A window
+  // never has fewer than 4 cells (hw > 1), so boxes shorter than 8 along c keep the box.
+  // (axis picks are selects: a runtime index into r or ext would put them in scratch)
+  const float an0 = fabsf(r.nrm[0]), an1 = fabsf(r.nrm[1]), an2 = fabsf(r.nrm[2]);
+  const bool c1 = an1 > an0 && an1 >= an2, c2 = !c1 && an2 > an0 && an2 > an1;
+  const uint32_t ext_c = c1 ? ext[1] : (c2 ? ext[2] : ext[0]);
+  if (hoist && ext_c >= 8) {
+    const float nc = c1 ? r.nrm[1] : (c2 ? r.nrm[2] : r.nrm[0]);
+    if (nc != 0.0f) {
+      float S = 0.0f;
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        const float lo = (float(r_mn[i]) + 0.5f) - r.g[i];
+        const float hi = (float(r_mn[i] + int32_t(ext[i]) - 1) + 0.5f) - r.g[i];
+        S += fabsf(r.nrm[i]) * fmaxf(fabsf(lo), fabsf(hi));
+      }
+      const float R = r.rp * (1.0f + 1e-4f) + 1e-4f * S;
+      const float hw = R / fabsf(nc) + 1.0f;
+      const float wd = ceilf(2.0f * hw) + 2.0f;
+      // the other two axes in increasing order: a = first, b = second
+      const bool c0 = !c1 && !c2;
+      const uint32_t ext_a = c0 ? ext[1] : ext[0], ext_b = c2 ? ext[1] : ext[2];
+      if (wd <= 15.0f) {
+        const uint32_t W = uint32_t(wd);
+        const uint64_t decw = uint64_t(W) * ext_a * ext_b;
+        if (2 * decw <= c) {
+          const float na = c0 ? r.nrm[1] : r.nrm[0], nb = c2 ? r.nrm[1] : r.nrm[2];
+          const float ga = c0 ? r.g[1] : r.g[0], gb = c2 ? r.g[1] : r.g[2];
+          const float gc = c1 ? r.g[1] : (c2 ? r.g[2] : r.g[0]);
+          const uint32_t ax = c1 ? 1u : (c2 ? 2u : 0u);
+          WinRec w;
+          w.ka = -na / nc;
+          w.kb = -nb / nc;
+          w.k0 = gc - (na * (0.5f - ga) + nb * (0.5f - gb)) / nc - hw - 0.5f;
+          w.cinfo = ax | (W << 2) | (ext_c << 6);
+          *wrec = w;
+          r.mnz_ex = uint32_t(r_mn[2]) | (W << 16);
+          r.exy = W * ext_a;
+          r.flags |= kWindow;
+          decoded = decw;
+        }
+      }
+    }
+  }
+  // narrow decode: every local index < 2^22, so the float quotient is off by at most 1
+  if (decoded >> 22) r.flags |= kWide;
+  *count = decoded;
+  r.rcp_ex = 1.0f / float(r.mnz_ex >> 16);
   r.rcp_exy = 1.0f / float(r.exy);
   r.grid_base = grid_base;
   *rec = r;
+  return c - decoded;
+}
+
+// The cells the wave's windows skip (0 for a lane without a windowed triangle), added once
+// to *acc; a wave with any windowed triangle also stamps *win with the call's epoch.
+__device__ __forceinline__ void note_windows(unsigned long long *acc, unsigned int *win,
+                                             uint32_t epoch, uint64_t v) {
+  if (__ballot(v != 0) == 0) return;  // wave-uniform: the common case
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) v += (uint64_t)__shfl_xor((long long)v, d, 64);
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(acc, (unsigned long long)v);
+    __hip_atomic_store(win, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 __global__ void k_tri_setup(const float *__restrict__ tris, uint64_t T, VoxelizeParams p,
-                            uint64_t *__restrict__ count, TriRec *__restrict__ rec) {
+                            uint64_t *__restrict__ count, TriRec *__restrict__ rec,
+                            WinRec *__restrict__ wrec, unsigned long long *__restrict__ skipped_acc,
+                            unsigned int *__restrict__ win, uint32_t epoch) {
   const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= T) return;
-  setup_one(tris + 9 * t, p, 0u, count + t, rec + t);
+  uint64_t skipped = 0;
+  if (t < T) skipped = setup_one(tris + 9 * t, p, 0u, count + t, rec + t, wrec + t);
+  note_windows(skipped_acc, win, epoch, skipped);
 }
 
 // Many meshes: mesh g owns setup blocks [sb_begin[g], sb_begin[g+1]) and record slots
@@ -152,16 +240,18 @@ constexpr int kSetupTB = 256;
 __global__ __launch_bounds__(kSetupTB) void k_tri_setup_batch(
     const BatchMesh *__restrict__ mesh,
     const uint32_t *__restrict__ sb_begin, uint32_t G, uint64_t *__restrict__ count,
-    TriRec *__restrict__ rec) {
+    TriRec *__restrict__ rec, WinRec *__restrict__ wrec,
+    unsigned long long *__restrict__ ref_pairs, unsigned int *__restrict__ win, uint32_t epoch) {
   const uint32_t g = dev::upper_index(sb_begin, G, blockIdx.x);
   const BatchMesh &m = mesh[g];
   const uint64_t tl = uint64_t(blockIdx.x - sb_begin[g]) * kSetupTB + threadIdx.x;
   const uint64_t slot = uint64_t(blockIdx.x) * kSetupTB + threadIdx.x;
-  if (tl >= m.T) {
-    count[slot] = 0;
-    return;
-  }
-  setup_one(m.tris + 9 * tl, m.p, uint32_t(m.b1_begin), count + slot, rec + slot);
+  uint64_t skipped = 0;
+  if (tl >= m.T) count[slot] = 0;
+  else
+    skipped = setup_one(m.tris + 9 * tl, m.p, uint32_t(m.b1_begin), count + slot, rec + slot,
+                        wrec + slot);
+  note_windows(ref_pairs + g, win, epoch, skipped);  // a block belongs to one mesh
 }
 
 constexpr uint64_t kChunksPerUnit = 16;  // at least 1024 pairs per work unit
@@ -170,12 +260,14 @@ constexpr uint64_t kChunksPerUnit = 16;  // at least 1024 pairs per work unit
 // read-back sits between the scan and the SAT kernel: work units of `cpu` chunks of 64
 // pairs (at least kChunksPerUnit, about `want` units in all), and the work counter.
 __global__ void k_sat_plan(const uint64_t *__restrict__ total, uint64_t want,
-                           SatPlan *__restrict__ plan) {
+                           SatPlan *__restrict__ plan, uint32_t epoch) {
   const uint64_t P = *total;
   const uint64_t nchunks = (P + 63) / 64;
   uint64_t cpu = (nchunks + want - 1) / want;
   if (cpu < kChunksPerUnit) cpu = kChunksPerUnit;
   plan->P = P;
+  plan->P_ref = P + plan->skipped[epoch & 1];
+  plan->skipped[(epoch + 1) & 1] = 0;
   plan->cpu = cpu;
   plan->nunits = (nchunks + cpu - 1) / cpu;
   plan->counter = 0;
@@ -465,10 +557,20 @@ __device__ __forceinline__ void table_flush(unsigned long long *tk, unsigned lon
 // across lanes. Phase A evaluates the plane test and queues survivors in LDS; every 64
 // queued survivors run phase B as a full wave. Hits are merged per leaf word in an LDS
 // table and flushed to HBM in bulk.
-__global__ __launch_bounds__(kWaves * 64) void k_sat(
-    const TriRec *__restrict__ rec, const uint64_t *__restrict__ coff,
+//
+// Two builds: kWin decodes windowed triangles; both are launched and the one that does not
+// match the call (windowed triangles present or not) leaves at once, so calls without a
+// windowed triangle run the build that carries no window decode. Both are held to 72
+// VGPRs (7 waves per SIMD): the few values spilled live in the per-unit loop, and the
+// extra wave hides more of the record loads than the spills cost.
+template <bool kWin>
+__global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(7))) void k_sat(
+    const TriRec *__restrict__ rec, const WinRec *__restrict__ wrec,
+    const uint64_t *__restrict__ coff,
     const uint32_t *__restrict__ ustart, uint64_t T, SatPlan *__restrict__ plan,
-    unsigned long long *__restrict__ leafgrid, unsigned long long *__restrict__ brickbits) {
+    unsigned long long *__restrict__ leafgrid, unsigned long long *__restrict__ brickbits,
+    uint32_t epoch) {
+  if ((plan->win_epoch == epoch) != kWin) return;  // uniform over the grid
   __shared__ uint32_t q_t[kWaves][kQueue], q_xy[kWaves][kQueue], q_z[kWaves][kQueue];
   __shared__ unsigned long long t_key[kWaves][kTable], t_val[kWaves][kTable];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -574,7 +676,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_sat(
     bool surv = false;
     uint32_t x = 0, y = 0, z = 0;
     if (valid) {
-      const TriRec r = rec[t];
+      const TriRec r = rec[t];  // one copy: the record's loads issue together
       const uint64_t li = p - start;
       const uint32_t ex = r.mnz_ex >> 16;
       if (!(r.flags & kWide)) {
@@ -589,11 +691,31 @@ __global__ __launch_bounds__(kWaves * 64) void k_sat(
         y = uint32_t(rem / ex);
         x = uint32_t(rem - uint64_t(y) * ex);
       }
-      x += r.mnxy & 0xffffu;
-      y += r.mnxy >> 16;
-      z += r.mnz_ex & 0xffffu;
+      bool inside = true;
+      if (kWin && __builtin_expect((r.flags & kWindow) != 0, 0)) {
+        // (w, la, lb) -> the cell w of the window of column (a, b) along c
+        const WinRec wf = wrec[t];
+        const uint32_t cinfo = wf.cinfo;
+        const uint32_t ax = cinfo & 3u, ec = cinfo >> 6;
+        const uint32_t m0 = r.mnxy & 0xffffu, m1 = r.mnxy >> 16, m2 = r.mnz_ex & 0xffffu;
+        const uint32_t ma = ax == 0 ? m1 : m0, mb = ax == 2 ? m1 : m2;
+        const uint32_t mc = ax == 0 ? m0 : (ax == 1 ? m1 : m2);
+        const uint32_t ca = ma + y, cb = mb + z;
+        const int32_t cc = int32_t(floorf(wf.k0 + wf.ka * float(ca) + wf.kb * float(cb))) + int32_t(x);
+        inside = cc >= int32_t(mc) && cc < int32_t(mc + ec);
+        const uint32_t ucc = uint32_t(cc);
+        x = ax == 0 ? ucc : ca;
+        y = ax == 1 ? ucc : (ax == 0 ? ca : cb);
+        z = ax == 2 ? ucc : cb;
+      } else {
+        x += r.mnxy & 0xffffu;
+        y += r.mnxy >> 16;
+        z += r.mnz_ex & 0xffffu;
+      }
       const float cx = float(x) + 0.5f, cy = float(y) + 0.5f, cz = float(z) + 0.5f;
-      if (r.flags & kPlaneHoist) {
+      if (!inside) {
+        surv = false;
+      } else if (r.flags & kPlaneHoist) {
         // the plane test of reference_cpu.rs:76-82 with per-triangle n and r:
         // |normal.dot(ZERO) + d| = |n . v0| for a finite normal
         const float v0x = r.g[0] - cx, v0y = r.g[1] - cy, v0z = r.g[2] - cz;
@@ -649,35 +771,59 @@ namespace {
 // Scan of the Tp per-slot candidate counts in ctx->tri_count, the device-side plan, and
 // the SAT launch into `leafgrid` / `brickbits`, all enqueued without a host round trip. The
 // persistent grid is sized for the device (waves that find no unit exit at once).
+// The Tp window records follow the Tp triangle records in ctx->rec.
+WinRec *win_records(vxl_ctx *ctx, uint64_t Tp) {
+  return reinterpret_cast<WinRec *>(ctx->rec.as<char>() + Tp * sizeof(TriRec));
+}
+
+uint64_t plan_want(vxl_ctx *ctx, uint64_t *max_waves) {
+  *max_waves = uint64_t(ctx->cu_count > 0 ? ctx->cu_count : 256) * ctx->sat_waves_per_cu;
+  return *max_waves * uint64_t(ctx->sat_units_per_wave);
+}
+
+// The plan area (SatPlan, then the unit starts), sized before setup writes into it; a new
+// area starts with both skipped-cell slots clear.
+SatPlan *plan_area(vxl_ctx *ctx) {
+  uint64_t mw = 0;
+  const uint64_t want = plan_want(ctx, &mw);
+  const size_t had = ctx->bstart.bytes;
+  ctx->bstart.reserve(256 + (want + 64) * 4);
+  SatPlan *plan = ctx->bstart.as<SatPlan>();
+  if (ctx->bstart.bytes != had)
+    VXL_HIP(hipMemsetAsync(plan->skipped, 0, sizeof(plan->skipped), ctx->stream));
+  return plan;
+}
+
 void sat_after_setup(vxl_ctx *ctx, uint64_t Tp, unsigned long long *leafgrid,
-                     unsigned long long *brickbits) {
+                     unsigned long long *brickbits, uint32_t epoch) {
   hipStream_t s = ctx->stream;
   const int tb = 256;
   uint64_t *count = ctx->tri_count.as<uint64_t>();
   TriRec *rec = ctx->rec.as<TriRec>();
+  const WinRec *wrec = win_records(ctx, Tp);
   uint64_t *coff = ctx->coff.as<uint64_t>();
   scan_exclusive_u64(count, coff, Tp, coff + Tp, ctx->scan_tmp, s);
-  const uint64_t max_waves =
-      uint64_t(ctx->cu_count > 0 ? ctx->cu_count : 256) * ctx->sat_waves_per_cu;
-  const uint64_t want = max_waves * uint64_t(ctx->sat_units_per_wave);
-  ctx->bstart.reserve(256 + (want + 64) * 4);
-  SatPlan *plan = ctx->bstart.as<SatPlan>();
+  uint64_t max_waves = 0;
+  const uint64_t want = plan_want(ctx, &max_waves);
+  SatPlan *plan = plan_area(ctx);
   uint32_t *ustart = reinterpret_cast<uint32_t *>(ctx->bstart.as<char>() + 256);
-  hipLaunchKernelGGL(k_sat_plan, dim3(1), dim3(1), 0, s, coff + Tp, want, plan);
+  hipLaunchKernelGGL(k_sat_plan, dim3(1), dim3(1), 0, s, coff + Tp, want, plan, epoch);
   if (Tp)
     hipLaunchKernelGGL(k_bstart, dim3((Tp + tb - 1) / tb), dim3(tb), 0, s, coff, count, Tp, plan,
                        ustart);
   mark(ctx, 1);
   const uint64_t blocks = (max_waves + kWaves - 1) / kWaves;
-  hipLaunchKernelGGL(k_sat, dim3(uint32_t(blocks)), dim3(kWaves * 64), 0, s, rec, coff, ustart,
-                     Tp, plan, leafgrid, brickbits);
+  hipLaunchKernelGGL(k_sat<false>, dim3(uint32_t(blocks)), dim3(kWaves * 64), 0, s, rec, wrec,
+                     coff, ustart, Tp, plan, leafgrid, brickbits, epoch);
+  hipLaunchKernelGGL(k_sat<true>, dim3(uint32_t(blocks)), dim3(kWaves * 64), 0, s, rec, wrec,
+                     coff, ustart, Tp, plan, leafgrid, brickbits, epoch);
   mark(ctx, 2);
   VXL_HIP(hipGetLastError());
 }
 
 void reserve_records(vxl_ctx *ctx, uint64_t Tp) {
   ctx->tri_count.reserve(Tp * 8);
-  ctx->rec.reserve(Tp * sizeof(TriRec));
+  ctx->rec.reserve(Tp * (sizeof(TriRec) + sizeof(WinRec)));
   ctx->coff.reserve(Tp * 8 + 8);
 }
 
@@ -686,24 +832,30 @@ void reserve_records(vxl_ctx *ctx, uint64_t Tp) {
 void run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizeParams &p) {
   const int tb = 256;
   reserve_records(ctx, T);
+  SatPlan *plan = plan_area(ctx);
+  const uint32_t e = ++ctx->sat_epoch;
   hipLaunchKernelGGL(k_tri_setup, dim3((T + tb - 1) / tb), dim3(tb), 0, ctx->stream, d_tris, T,
-                     p, ctx->tri_count.as<uint64_t>(), ctx->rec.as<TriRec>());
+                     p, ctx->tri_count.as<uint64_t>(), ctx->rec.as<TriRec>(), win_records(ctx, T),
+                     &plan->skipped[e & 1], &plan->win_epoch, e);
   sat_after_setup(ctx, T, ctx->leafgrid.as<unsigned long long>(),
-                  ctx->bitmap.as<unsigned long long>());
+                  ctx->bitmap.as<unsigned long long>(), e);
 }
 
-const uint64_t *sat_pairs_dev(vxl_ctx *ctx) { return &ctx->bstart.as<SatPlan>()->P; }
+const uint64_t *sat_pairs_dev(vxl_ctx *ctx) { return &ctx->bstart.as<SatPlan>()->P_ref; }
 
 void run_sat_batch(vxl_ctx *ctx, const BatchMesh *d_mesh, const uint32_t *d_sb_begin,
                    uint32_t G, uint32_t setup_blocks, unsigned long long *leafgrid,
-                   unsigned long long *brickbits) {
+                   unsigned long long *brickbits, unsigned long long *d_ref_pairs) {
   const uint64_t Tp = uint64_t(setup_blocks) * kSetupTB;
   reserve_records(ctx, Tp);
+  SatPlan *plan = plan_area(ctx);
+  const uint32_t e = ++ctx->sat_epoch;
   if (setup_blocks)
     hipLaunchKernelGGL(k_tri_setup_batch, dim3(setup_blocks), dim3(kSetupTB), 0, ctx->stream,
                        d_mesh, d_sb_begin, G, ctx->tri_count.as<uint64_t>(),
-                       ctx->rec.as<TriRec>());
-  sat_after_setup(ctx, Tp, leafgrid, brickbits);
+                       ctx->rec.as<TriRec>(), win_records(ctx, Tp), d_ref_pairs,
+                       &plan->win_epoch, e);
+  sat_after_setup(ctx, Tp, leafgrid, brickbits, e);
 }
 
 uint64_t setup_slots_per_block() { return kSetupTB; }

@@ -104,6 +104,7 @@ struct vxl_ctx {
   void *host_pinned_dev = nullptr;  // its device address
   std::vector<vxl::Slab> slabs;     // tree arena
   // tuning
+  uint32_t sat_epoch = 0;  // setup + SAT calls enqueued (numbers the plan's parity slots)
   int sat_waves_per_cu = 28;  // resident SAT waves per CU (persistent, work-queue fed)
   int sat_units_per_wave = 1;  // work units per resident wave
   int dda_block = 64;  // one-wave workgroups: a finished wave frees its slot at once
@@ -204,25 +205,31 @@ struct BatchTotals {
 // pointers, sizes, node-array pointers), each part 256-byte aligned.
 inline size_t batch_staging_bytes(uint32_t G) {
   auto up = [](size_t b) { return (b + 255) / 256 * 256; };
-  return up(G * sizeof(BatchMesh)) + up((G + 1) * 4) + up(G * sizeof(LevelPtrs)) +
+  return up(G * sizeof(BatchMesh)) + up((G + 1) * 4) + up(G * 8) + up(G * sizeof(LevelPtrs)) +
          up(G * sizeof(BatchTotals)) + up(G * 8) + 1024;
 }
 // The SAT launch plan the device derives from the pair total (voxelize.hip).
 struct SatPlan {
-  uint64_t P;       // candidate pairs
+  uint64_t P;       // decoded pairs (what the SAT kernel walks)
   uint64_t cpu;     // chunks of 64 pairs per work unit
   uint64_t nunits;  // work units
-  unsigned int counter;  // next unit to hand out
+  uint64_t P_ref;   // the reference's candidate pairs (bounding-box cells), reported
+  // cells the windows of call e skip accumulate in skipped[e & 1]; the plan of call e
+  // reads it and clears skipped[(e + 1) & 1] for the next call on the stream
+  unsigned long long skipped[2];
+  unsigned int counter;    // next unit to hand out
+  unsigned int win_epoch;  // == e when call e holds a windowed triangle
 };
 // Enqueues setup + SAT into ctx->leafgrid / ctx->bitmap (level-1); no host round trip.
-// The pair count stays on the device at sat_pairs_dev(ctx) until the next call.
+// The reference's pair count stays on the device at sat_pairs_dev(ctx) until the next call.
 void run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizeParams &p);
 const uint64_t *sat_pairs_dev(vxl_ctx *ctx);
 // Setup + SAT of a batch group into the pooled grid; mesh g owns setup blocks
 // [sb_begin[g], sb_begin[g+1]) (setup_slots_per_block() record slots each).
+// d_ref_pairs: G zeroed counters that receive the candidates each mesh's windows skip.
 void run_sat_batch(vxl_ctx *ctx, const BatchMesh *d_mesh, const uint32_t *d_sb_begin,
                    uint32_t G, uint32_t setup_blocks, unsigned long long *leafgrid,
-                   unsigned long long *brickbits);
+                   unsigned long long *brickbits, unsigned long long *d_ref_pairs);
 uint64_t setup_slots_per_block();
 
 // tree.hip
@@ -245,7 +252,8 @@ size_t bitmap_level1_words(uint32_t n);
 // uploaded to d_mesh; W1 = brick-bitmap words of the group. Writes trees[g] and, when
 // non-null, pairs[g]. One host sync.
 void build_trees_batch(vxl_ctx *ctx, const std::vector<BatchMesh> &meshes,
-                       const BatchMesh *d_mesh, uint64_t W1, vxl_tree **trees, uint64_t *pairs);
+                       const BatchMesh *d_mesh, uint64_t W1, vxl_tree **trees, uint64_t *pairs,
+                       const unsigned long long *d_ref_pairs);
 
 // dda.hip
 void launch_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, uint64_t R,

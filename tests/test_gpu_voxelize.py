@@ -33,6 +33,13 @@ def _cases():
          1e-4),
         ("huge_triangle_32", np.array([[-50, -40, 16.2, 90, -30, 15.7, 10, 80, 16.9]], np.float32), 32,
          (0, 0, 0), 1.0, 1e-4),
+        # large triangles: candidate windows along the normal's main axis instead of the box
+        ("big_sphere_512", W.sphere_with_tris(100, seed=3), 512, (0, 0, 0), 1 / 256, 1e-4),
+        ("big_random_128", rng.uniform(0.0, 1.0, (40, 9)).astype(np.float32), 128, (0, 0, 0),
+         1 / 128, 1e-4),
+        ("big_steep_128", np.array([[0.1, 0.1, 0.05, 0.9, 0.2, 0.95, 0.3, 0.9, 0.5],
+                                    [0.05, 0.9, 0.1, 0.95, 0.1, 0.2, 0.5, 0.5, 0.99]], np.float32),
+         128, (0, 0, 0), 1 / 128, 1e-4),
     ]
     return out
 
