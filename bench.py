@@ -122,9 +122,13 @@ def bench_dda(ctx, torch, W, reps=5):
         ctx.set_timing(False)
         k_ms = sum(ms) / len(ms)
         hits = int((dh[:, 3] == 1).sum().item())
+        walked = tree.leaf_count * 68 + tree.node_count * 12
         out[name] = dict(rays=R, kernel_ms=round(k_ms, 4), mrays_per_s=round(R / k_ms / 1e3, 1),
                          hit_fraction=round(hits / R, 4), leaves=tree.leaf_count,
-                         nodes=tree.node_count, tree_mb=round(tree.device_bytes / 2 ** 20, 1))
+                         nodes=tree.node_count, tree_mb=round(tree.device_bytes / 2 ** 20, 1),
+                         lane_schedule=("persistent lanes, refill at 8 idle (tree above the "
+                                        "256 MiB Infinity Cache)" if walked > 256 * 2 ** 20
+                                        else "one lane per ray (cache-resident tree)"))
         out[name]["_tree"] = tree
         out[name]["_rays"] = rays
     return out

@@ -341,7 +341,7 @@ void vxl_ctx_destroy(vxl_ctx *ctx) {
                          &ctx->leafgrid, &ctx->bitmap,   &ctx->levelscratch, &ctx->scan_tmp,
                          &ctx->words_io, &ctx->rays_io,  &ctx->hits_io,  &ctx->t_io,
                          &ctx->pool_grid, &ctx->pool_b1, &ctx->batch_scratch, &ctx->batch_l2,
-                         &ctx->batch_desc, &ctx->trav_desc};
+                         &ctx->batch_desc, &ctx->trav_desc,  &ctx->trav_ctr};
   for (auto *b : bufs) b->release();
   // Trees outlive nothing they depend on: release their device memory now and detach
   // them; vxl_tree_free on a detached tree only frees the handle.
@@ -391,6 +391,14 @@ vxl_status vxl_set_tuning(vxl_ctx *ctx, int sat_waves_per_cu, int sat_units_per_
         throw Error(VXL_INVALID_ARG, "dda_block must be a multiple of 64 in [64, 256]");
       ctx->dda_block = dda_block;
     }
+  });
+}
+
+vxl_status vxl_set_dda_refill(vxl_ctx *ctx, int refill_min) {
+  return guarded(ctx, [&] {
+    if (refill_min < -1 || refill_min > 64)
+      throw Error(VXL_INVALID_ARG, "refill_min must be in [-1, 64]");
+    ctx->dda_refill = refill_min;
   });
 }
 

@@ -9,10 +9,9 @@
 //    power of two commutes with rounding unless 1/d is subnormal, which that bound
 //    excludes. Otherwise the division is done as written.
 //
-// Frame storage: the active frame lives in registers. Each ancestor frame is kept in
-// a register slot named by its level (the stack index of a frame is root_level - level,
-// so "the frame at level L" is unique at any time); saving and restoring go through a
-// switch on the level, so no frame is ever addressed dynamically.
+// Frame storage: the active frame lives in registers. Each ancestor frame is kept in an
+// LDS row named by its level (the stack index of a frame is root_level - level, so "the
+// frame at level L" is unique at any time), one column per lane.
 //
 // The hit's ray parameter t is the entry parameter of the f64 slab of the hit voxel for
 // the ray widened to f64: max(0, max_a min((v_a - o_a)/d_a, (v_a + 1 - o_a)/d_a)) over
@@ -105,27 +104,33 @@ __device__ __forceinline__ bool leaf_reaches(uint32_t packed, const Ray &r, uint
   return t_far >= t_enter;
 }
 
-__device__ __forceinline__ void traverse_ray(
-    uint64_t ri, const float *__restrict__ rays, const uint32_t *__restrict__ nodes,
-    const uint32_t *__restrict__ leafw, const uint32_t *__restrict__ bounds, uint32_t n,
-    uint32_t k, uint64_t leaf_count, uint4 *__restrict__ hits, double *__restrict__ tout) {
-  Ray r;
-  {
-    const float2 a = *reinterpret_cast<const float2 *>(rays + 6 * ri);
-    const float2 b = *reinterpret_cast<const float2 *>(rays + 6 * ri + 2);
-    const float2 c = *reinterpret_cast<const float2 *>(rays + 6 * ri + 4);
-    r.o[0] = a.x; r.o[1] = a.y; r.o[2] = b.x;
-    r.d[0] = b.y; r.d[1] = c.x; r.d[2] = c.y;
-  }
+// The walk of one lane: the active frame (level, node, its origin, cell, t_max, t_delta,
+// t_entry) and, for a node frame, the node record held in registers. Ancestor frames are
+// in LDS (above).
+struct Walk {
+  uint32_t level, node, ox, oy, oz, cx, cy, cz;
+  float tm0, tm1, tm2, td0, td1, td2, te;
+  uint32_t mlo, mhi, cbase;
+  uint32_t root_level, iter;
+};
+
+__device__ __forceinline__ void load_ray(const float *__restrict__ rays, uint64_t ri, Ray &r) {
+  const float2 a = *reinterpret_cast<const float2 *>(rays + 6 * ri);
+  const float2 b = *reinterpret_cast<const float2 *>(rays + 6 * ri + 2);
+  const float2 c = *reinterpret_cast<const float2 *>(rays + 6 * ri + 4);
+  r.o[0] = a.x; r.o[1] = a.y; r.o[2] = b.x;
+  r.d[0] = b.y; r.d[1] = c.x; r.d[2] = c.y;
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     r.rinv[a] = 1.0f / r.d[a];
     r.big[a] = !(fabsf(r.d[a]) < 0x1p100f);
   }
-  uint4 result = make_uint4(0u, 0u, 0u, 0u);
-  bool hit = false;
+}
 
-  // grid clip, ray_aabb_f32 (mirror.rs:169-192) + t1 < 0 check (:273-277)
+// Grid clip, ray_aabb_f32 (mirror.rs:169-192) + the t1 < 0 check (:273-277); false when
+// the ray misses the grid or the tree is empty, else t0 is the entry parameter.
+__device__ __forceinline__ bool clip_grid(const Ray &r, uint32_t n, uint64_t leaf_count,
+                                          float &t0_out) {
   const float nf = float(n);
   float t0 = -INFINITY, t1 = INFINITY;
   bool missed = leaf_count == 0;
@@ -148,158 +153,167 @@ __device__ __forceinline__ void traverse_ray(
     }
   }
   if (!missed && t1 < 0.0f) missed = true;
+  t0_out = t0;
+  return !missed;
+}
 
-  if (!missed) {
-    const uint32_t root_level = k == 0 ? 1u : k + 1;
-    // active frame
-    uint32_t level = root_level, node = 0;
-    uint32_t ox = 0, oy = 0, oz = 0;
-    uint32_t cx, cy, cz;
-    float tm0, tm1, tm2, td0, td1, td2;
-    float te = fmaxf(t0, 0.0f);
-    uint32_t mlo = 0, mhi = 0, cbase = 0;  // cached node record of a node frame
-    extern __shared__ uint32_t frames_lds[];
-    uint32_t *const frames = frames_lds + threadIdx.x;  // frames[((L-2)*5 + f) * blockDim.x]
-    const uint32_t fstride = blockDim.x;
+// Walker::enter (mirror.rs:48-85) of a frame at `level` with origin (ox, oy, oz).
+__device__ __forceinline__ void make_frame(const Ray &r, Walk &w, uint32_t level, float t_enter) {
+  const uint32_t dim = level == 1 ? 8u : 4u;
+  const uint32_t csu = frame_cell_size(level);
+  const float cs = float(csu);
+  const float ics = __uint_as_float((127u - (31u - __clz(csu))) << 23);  // 2^-e
+  axis_enter(r, 0, float(w.ox), dim, cs, ics, t_enter, w.cx, w.tm0, w.td0);
+  axis_enter(r, 1, float(w.oy), dim, cs, ics, t_enter, w.cy, w.tm1, w.td1);
+  axis_enter(r, 2, float(w.oz), dim, cs, ics, t_enter, w.cz, w.tm2, w.td2);
+  w.te = t_enter;
+}
 
-#define VXL_MAKE_FRAME(LEVEL, ORGX, ORGY, ORGZ, TENTER)                                    \
-  {                                                                                        \
-    const uint32_t dim_ = (LEVEL) == 1 ? 8u : 4u;                                          \
-    const uint32_t csu_ = frame_cell_size(LEVEL);                                          \
-    const float cs_ = float(csu_);                                                         \
-    const float ics_ = __uint_as_float((127u - (31u - __clz(csu_))) << 23); /* 2^-e */       \
-    axis_enter(r, 0, float(ORGX), dim_, cs_, ics_, TENTER, cx, tm0, td0);                  \
-    axis_enter(r, 1, float(ORGY), dim_, cs_, ics_, TENTER, cy, tm1, td1);                  \
-    axis_enter(r, 2, float(ORGZ), dim_, cs_, ics_, TENTER, cz, tm2, td2);                  \
-    te = (TENTER);                                                                         \
+// Walker::step (mirror.rs:87-116) on the active frame; false when exhausted. The axis
+// choice (lowest t_max, ties to the lowest axis) and the update are selects, so lanes
+// stepping along different axes do not split the wave.
+// The fields are read into locals first: a conditional expression over struct fields would
+// become a load from a selected address, which keeps the walk state out of registers.
+__device__ __forceinline__ bool step_frame(const Ray &r, Walk &w) {
+  const uint32_t dim = w.level == 1 ? 8u : 4u;
+  const float tm0 = w.tm0, tm1 = w.tm1, tm2 = w.tm2;
+  const float d0 = r.d[0], d1 = r.d[1], d2 = r.d[2];
+  const uint32_t cx = w.cx, cy = w.cy, cz = w.cz;
+  const bool a1 = tm1 < tm0;
+  float best = a1 ? tm1 : tm0;
+  const bool a2 = tm2 < best;
+  best = a2 ? tm2 : best;
+  const bool on0 = !a1 && !a2, on1 = a1 && !a2;
+  const float da = a2 ? d2 : (a1 ? d1 : d0);
+  const uint32_t ca = a2 ? cz : (a1 ? cy : cx);
+  const bool ok = isfinite(best) && ((da > 0.0f && ca + 1 < dim) || (da < 0.0f && ca != 0));
+  if (ok) {
+    const uint32_t inc = da > 0.0f ? 1u : 0xFFFFFFFFu;
+    const float td0 = w.td0, td1 = w.td1, td2 = w.td2;
+    w.cx = cx + (on0 ? inc : 0u);
+    w.cy = cy + (on1 ? inc : 0u);
+    w.cz = cz + (a2 ? inc : 0u);
+    w.tm0 = on0 ? tm0 + td0 : tm0;
+    w.tm1 = on1 ? tm1 + td1 : tm1;
+    w.tm2 = a2 ? tm2 + td2 : tm2;
+    w.te = best;
   }
+  return ok;
+}
 
-    // Walker::step (mirror.rs:87-116) on the active frame; OK = false when exhausted. The
-    // axis choice (lowest t_max, ties to the lowest axis) and the update are selects, so
-    // lanes stepping along different axes do not split the wave.
-#define VXL_STEP(OK)                                                                       \
-  {                                                                                        \
-    const uint32_t dim_ = level == 1 ? 8u : 4u;                                            \
-    const bool a1_ = tm1 < tm0;                                                            \
-    float best_ = a1_ ? tm1 : tm0;                                                         \
-    const bool a2_ = tm2 < best_;                                                          \
-    best_ = a2_ ? tm2 : best_;                                                             \
-    const bool on0_ = !a1_ && !a2_, on1_ = a1_ && !a2_;                                    \
-    const float da_ = a2_ ? r.d[2] : (a1_ ? r.d[1] : r.d[0]);                              \
-    const uint32_t ca_ = a2_ ? cz : (a1_ ? cy : cx);                                       \
-    OK = isfinite(best_) && ((da_ > 0.0f && ca_ + 1 < dim_) || (da_ < 0.0f && ca_ != 0));  \
-    if (OK) {                                                                              \
-      const uint32_t inc_ = da_ > 0.0f ? 1u : 0xFFFFFFFFu;                                 \
-      cx += on0_ ? inc_ : 0u;                                                              \
-      cy += on1_ ? inc_ : 0u;                                                              \
-      cz += a2_ ? inc_ : 0u;                                                               \
-      tm0 = on0_ ? tm0 + td0 : tm0;                                                        \
-      tm1 = on1_ ? tm1 + td1 : tm1;                                                        \
-      tm2 = a2_ ? tm2 + td2 : tm2;                                                         \
-      te = best_;                                                                          \
-    }                                                                                      \
+// The root frame of a ray that entered the grid at t0 (mirror.rs:278-283).
+__device__ __forceinline__ void begin_walk(const Ray &r, Walk &w, uint32_t k, float t0,
+                                           const uint32_t *__restrict__ nodes) {
+  w.root_level = k == 0 ? 1u : k + 1;
+  w.level = w.root_level;
+  w.node = 0;
+  w.ox = w.oy = w.oz = 0;
+  w.iter = 0;
+  w.mlo = w.mhi = w.cbase = 0;
+  make_frame(r, w, w.level, fmaxf(t0, 0.0f));
+  if (w.level >= 2) {
+    w.mlo = nodes[0];
+    w.mhi = nodes[1];
+    w.cbase = nodes[2];
   }
+}
 
-    VXL_MAKE_FRAME(root_level, 0u, 0u, 0u, te);
-    if (level >= 2) {
-      mlo = nodes[0];
-      mhi = nodes[1];
-      cbase = nodes[2];
+constexpr uint32_t kWalking = 0, kHit = 1, kLeft = 2, kCapped = 3;
+
+// One visit of the walk (mirror.rs:284-325): test the active cell -- a voxel bit of the
+// leaf (one word load) or a child bit of the node record in registers; on a hit voxel
+// stop; on an occupied child whose leaf the ray reaches descend; else step, popping
+// parents until one steps to a new cell (mirror.rs:155-165). `frames` is this lane's
+// column of the LDS frame rows, `fstride` the row pitch.
+__device__ __forceinline__ uint32_t walk_visit(const Ray &r, Walk &w,
+                                               const uint32_t *__restrict__ nodes,
+                                               const uint32_t *__restrict__ leafw,
+                                               const uint32_t *__restrict__ bounds,
+                                               uint32_t *frames, uint32_t fstride) {
+  if (++w.iter > kMaxIter) return kCapped;  // bounded: a legitimate walk never comes close
+  uint32_t word, bitpos;
+  const uint32_t mlo = w.mlo, mhi = w.mhi;
+  if (w.level == 1) {
+    const uint32_t m = dev::morton9(w.cx, w.cy, w.cz);
+    word = leafw[uint64_t(w.node) * 16 + (m >> 5)];
+    bitpos = m & 31;
+  } else {
+    const uint32_t bit = dev::child_bit(w.cx, w.cy, w.cz);
+    word = bit < 32 ? mlo : mhi;
+    bitpos = bit & 31;
+  }
+  const bool has = (word >> bitpos) & 1u;
+  if (w.level == 1 && has) return kHit;
+  if (w.level > 1 && has) {
+    const uint32_t bit = dev::child_bit(w.cx, w.cy, w.cz);
+    const uint32_t size = frame_cell_size(w.level);
+    const uint32_t cox = w.ox + w.cx * size, coy = w.oy + w.cy * size, coz = w.oz + w.cz * size;
+    const uint32_t below = bit < 32 ? __popc(mlo & ((1u << bit) - 1u))
+                                    : __popc(mlo) + __popc(mhi & ((1u << (bit - 32)) - 1u));
+    const uint32_t slot = w.cbase + below;
+    const bool descend = w.level != 2 || leaf_reaches(bounds[slot], r, cox, coy, coz, w.te);
+    if (descend) {
+      {
+        uint32_t *f = frames + (w.level - 2) * kFrameWords * fstride;
+        f[0] = w.node;
+        f[fstride] = w.cx | (w.cy << 8) | (w.cz << 16);
+        f[2 * fstride] = __float_as_uint(w.tm0);
+        f[3 * fstride] = __float_as_uint(w.tm1);
+        f[4 * fstride] = __float_as_uint(w.tm2);
+      }
+      w.level -= 1;
+      w.node = slot;
+      w.ox = cox;
+      w.oy = coy;
+      w.oz = coz;
+      make_frame(r, w, w.level, w.te);
+      if (w.level >= 2) {
+        w.mlo = nodes[3ull * w.node];
+        w.mhi = nodes[3ull * w.node + 1];
+        w.cbase = nodes[3ull * w.node + 2];
+      }
+      return kWalking;
     }
-    bool done = false;
-    uint32_t iter = 0;
-    while (!done) {
-      if (++iter > kMaxIter) {  // bounded: a legitimate walk never comes close
-        result = make_uint4(0u, 0u, 0u, 0xFFFFFFFFu);
-        break;
-      }
-      bool stepped;
-      // the active cell: a voxel bit of the leaf (one word load) or a child bit of the node
-      // record held in registers; one step site serves both kinds of frame
-      uint32_t word, bitpos;
-      if (level == 1) {
-        const uint32_t m = dev::morton9(cx, cy, cz);
-        word = leafw[uint64_t(node) * 16 + (m >> 5)];
-        bitpos = m & 31;
-      } else {
-        const uint32_t bit = dev::child_bit(cx, cy, cz);
-        word = bit < 32 ? mlo : mhi;
-        bitpos = bit & 31;
-      }
-      const bool has = (word >> bitpos) & 1u;
-      if (level == 1 && has) {
-        result = make_uint4(ox + cx, oy + cy, oz + cz, 1u);
-        hit = true;
-        break;
-      }
-      if (level > 1 && has) {
-        const uint32_t bit = dev::child_bit(cx, cy, cz);
-        const uint32_t size = frame_cell_size(level);
-        const uint32_t cox = ox + cx * size, coy = oy + cy * size, coz = oz + cz * size;
-        const uint32_t below = bit < 32 ? __popc(mlo & ((1u << bit) - 1u))
-                                        : __popc(mlo) + __popc(mhi & ((1u << (bit - 32)) - 1u));
-        const uint32_t slot = cbase + below;
-        const bool descend = level != 2 || leaf_reaches(bounds[slot], r, cox, coy, coz, te);
-        if (descend) {
-          {
-            uint32_t *f = frames + (level - 2) * kFrameWords * fstride;
-            f[0] = node;
-            f[fstride] = cx | (cy << 8) | (cz << 16);
-            f[2 * fstride] = __float_as_uint(tm0);
-            f[3 * fstride] = __float_as_uint(tm1);
-            f[4 * fstride] = __float_as_uint(tm2);
-          }
-          const float t_in = te;
-          level -= 1;
-          node = slot;
-          ox = cox;
-          oy = coy;
-          oz = coz;
-          VXL_MAKE_FRAME(level, ox, oy, oz, t_in);
-          if (level >= 2) {
-            mlo = nodes[3ull * node];
-            mhi = nodes[3ull * node + 1];
-            cbase = nodes[3ull * node + 2];
-          }
-          continue;
-        }
-      }
-      VXL_STEP(stepped);
-      // ascend (mirror.rs:155-165): pop parents until one steps to a new cell
-      while (!stepped) {
-        if (level == root_level) {
-          done = true;
-          break;
-        }
-        level += 1;
-        const uint32_t size = frame_cell_size(level);
-        {
-          const uint32_t *f = frames + (level - 2) * kFrameWords * fstride;
-          node = f[0];
-          const uint32_t cell = f[fstride];
-          cx = cell & 0xffu;
-          cy = (cell >> 8) & 0xffu;
-          cz = cell >> 16;
-          tm0 = __uint_as_float(f[2 * fstride]);
-          tm1 = __uint_as_float(f[3 * fstride]);
-          tm2 = __uint_as_float(f[4 * fstride]);
-          const float cs = float(size);
-          td0 = axis_delta(cs, r.d[0], r.rinv[0], r.big[0]);
-          td1 = axis_delta(cs, r.d[1], r.rinv[1], r.big[1]);
-          td2 = axis_delta(cs, r.d[2], r.rinv[2], r.big[2]);
-        }
-        ox -= cx * size;
-        oy -= cy * size;
-        oz -= cz * size;
-        mlo = nodes[3ull * node];
-        mhi = nodes[3ull * node + 1];
-        cbase = nodes[3ull * node + 2];
-        VXL_STEP(stepped);
-      }
-    }
-#undef VXL_MAKE_FRAME
-#undef VXL_STEP
   }
+  bool stepped = step_frame(r, w);
+  while (!stepped) {
+    if (w.level == w.root_level) return kLeft;
+    w.level += 1;
+    const uint32_t size = frame_cell_size(w.level);
+    {
+      const uint32_t *f = frames + (w.level - 2) * kFrameWords * fstride;
+      w.node = f[0];
+      const uint32_t cell = f[fstride];
+      w.cx = cell & 0xffu;
+      w.cy = (cell >> 8) & 0xffu;
+      w.cz = cell >> 16;
+      w.tm0 = __uint_as_float(f[2 * fstride]);
+      w.tm1 = __uint_as_float(f[3 * fstride]);
+      w.tm2 = __uint_as_float(f[4 * fstride]);
+      const float cs = float(size);
+      w.td0 = axis_delta(cs, r.d[0], r.rinv[0], r.big[0]);
+      w.td1 = axis_delta(cs, r.d[1], r.rinv[1], r.big[1]);
+      w.td2 = axis_delta(cs, r.d[2], r.rinv[2], r.big[2]);
+    }
+    w.ox -= w.cx * size;
+    w.oy -= w.cy * size;
+    w.oz -= w.cz * size;
+    w.mlo = nodes[3ull * w.node];
+    w.mhi = nodes[3ull * w.node + 1];
+    w.cbase = nodes[3ull * w.node + 2];
+    stepped = step_frame(r, w);
+  }
+  return kWalking;
+}
+
+// Writes ray ri's result: (x, y, z, 1) for a hit, (0, 0, 0, 0) for none, (0, 0, 0, ~0) for
+// a walk over the iteration bound; and t.
+__device__ __forceinline__ void store_result(uint64_t ri, const Ray &r, const Walk &w,
+                                             uint32_t status, uint4 *__restrict__ hits,
+                                             double *__restrict__ tout) {
+  const bool hit = status == kHit;
+  uint4 result = make_uint4(0u, 0u, 0u, status == kCapped ? 0xFFFFFFFFu : 0u);
+  if (hit) result = make_uint4(w.ox + w.cx, w.oy + w.cy, w.oz + w.cz, 1u);
   hits[ri] = result;
   if (tout) {
     double t = 0.0;
@@ -321,6 +335,26 @@ __device__ __forceinline__ void traverse_ray(
   }
 }
 
+__device__ __forceinline__ void traverse_ray(
+    uint64_t ri, const float *__restrict__ rays, const uint32_t *__restrict__ nodes,
+    const uint32_t *__restrict__ leafw, const uint32_t *__restrict__ bounds, uint32_t n,
+    uint32_t k, uint64_t leaf_count, uint4 *__restrict__ hits, double *__restrict__ tout) {
+  extern __shared__ uint32_t frames_lds[];
+  Ray r;
+  load_ray(rays, ri, r);
+  Walk w;
+  w.ox = w.oy = w.oz = w.cx = w.cy = w.cz = 0;
+  uint32_t status = kLeft;
+  float t0;
+  if (clip_grid(r, n, leaf_count, t0)) {
+    begin_walk(r, w, k, t0, nodes);
+    do {
+      status = walk_visit(r, w, nodes, leafw, bounds, frames_lds + threadIdx.x, blockDim.x);
+    } while (status == kWalking);
+  }
+  store_result(ri, r, w, status, hits, tout);
+}
+
 __global__ __launch_bounds__(256) void k_traverse(
     const float *__restrict__ rays, uint64_t R, const uint32_t *__restrict__ nodes,
     const uint32_t *__restrict__ leafw, const uint32_t *__restrict__ bounds, uint32_t n,
@@ -328,6 +362,71 @@ __global__ __launch_bounds__(256) void k_traverse(
   const uint64_t ri = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (ri >= R) return;
   traverse_ray(ri, rays, nodes, leafw, bounds, n, k, leaf_count, hits, tout);
+}
+
+// Persistent lanes: a grid sized to the device's resident one-wave blocks; each wave takes
+// runs of kRayRun consecutive rays from a counter (one atomic per run), and whenever at
+// least refill_min of its lanes are idle those lanes take the next rays of its run, so a
+// wave is not held by its longest walk. Every ray's walk is the same as in k_traverse;
+// only the lane that runs it differs.
+constexpr uint32_t kRayRun = 256;
+
+__global__ __launch_bounds__(64) void k_traverse_pl(
+    const float *__restrict__ rays, uint64_t R, const uint32_t *__restrict__ nodes,
+    const uint32_t *__restrict__ leafw, const uint32_t *__restrict__ bounds, uint32_t n,
+    uint32_t k, uint64_t leaf_count, uint4 *__restrict__ hits, double *__restrict__ tout,
+    unsigned long long *__restrict__ counter, uint32_t refill_min) {
+  extern __shared__ uint32_t frames_lds[];
+  uint32_t *const frames = frames_lds + threadIdx.x;
+  const uint32_t lane = threadIdx.x;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  Ray r;
+  Walk w;
+  w.ox = w.oy = w.oz = w.cx = w.cy = w.cz = 0;
+  uint64_t ri = 0;
+  uint64_t run = 0, run_end = 0;  // the wave's rays not yet handed out (wave-uniform)
+  bool active = false, exhausted = false;
+  for (;;) {
+    const unsigned long long idle = __ballot(!active);
+    const uint32_t ni = __popcll(idle);
+    if (!exhausted && ni >= refill_min) {  // wave-uniform
+      if (run == run_end) {
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(counter, (unsigned long long)kRayRun);
+        run = __shfl(base, 0, 64);
+        run_end = run + kRayRun < R ? run + kRayRun : R;
+        if (run >= R) {
+          exhausted = true;
+          run = run_end = R;
+        }
+      }
+      const uint64_t take = run_end - run < ni ? run_end - run : ni;
+      if (!active) {
+        const uint32_t rank = __popcll(idle & below);
+        if (rank < take) {
+          ri = run + rank;
+          load_ray(rays, ri, r);
+          float t0;
+          if (clip_grid(r, n, leaf_count, t0)) {
+            begin_walk(r, w, k, t0, nodes);
+            active = true;
+          } else {
+            store_result(ri, r, w, kLeft, hits, tout);
+          }
+        }
+      }
+      run += take;
+    } else if (ni == 64) {
+      break;  // no rays left and every lane idle (wave-uniform)
+    }
+    if (active) {
+      const uint32_t status = walk_visit(r, w, nodes, leafw, bounds, frames, 64);
+      if (status != kWalking) {
+        store_result(ri, r, w, status, hits, tout);
+        active = false;
+      }
+    }
+  }
 }
 
 // One segment of a batched traversal, as the kernel sees it.
@@ -358,11 +457,50 @@ size_t frame_lds_bytes(uint32_t k, uint64_t lanes) {
   return size_t(k >= 1 ? k : 1) * kFrameWords * 4 * lanes;
 }
 
+// Work counters of persistent traversals in flight at once on one context (at most).
+constexpr uint32_t kCounterSlots = 1024;
+
+// The automatic lane schedule: a tree whose walked arrays (64-byte leaves, 4-byte bounds,
+// 12-byte nodes) exceed the 256 MiB Infinity Cache is walked from HBM, where a wave's
+// lanes wait on leaf loads and refilling idle lanes pays (soup_2048: 3.44 -> 2.74 ms); a
+// cache-resident tree is bound by instruction issue, where the refills' partial-wave ray
+// setups cost more than the idle lanes (sphere_256: 0.98 vs 1.15 ms).
+constexpr uint64_t kCacheBytes = 256ull << 20;
+constexpr int kAutoRefill = 8;
+
+int refill_for(const vxl_ctx *ctx, const vxl_tree *tree) {
+  if (ctx->dda_refill >= 0) return ctx->dda_refill;
+  const uint64_t walked = tree->leaf_count * 68 + tree->node_count * 12;
+  return walked > kCacheBytes ? kAutoRefill : 0;
+}
+
 }  // namespace
 
 void launch_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, uint64_t R,
                      uint32_t *d_hits, double *d_t, hipStream_t s) {
   if (R == 0) return;
+  const size_t lds64 = frame_lds_bytes(tree->k, 64);
+  const int refill = refill_for(ctx, tree);
+  if (refill > 0) {
+    // persistent lanes: as many one-wave blocks as the device holds at once, each with a
+    // counter slot of its own launch (a ring, so launches on other streams keep theirs)
+    int per_cu = 0;
+    VXL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_traverse_pl, 64, lds64));
+    if (per_cu < 1) per_cu = 1;
+    const uint64_t waves = (R + 63) / 64;
+    uint64_t blocks = uint64_t(ctx->cu_count > 0 ? ctx->cu_count : 256) * uint64_t(per_cu);
+    if (blocks > waves) blocks = waves;
+    ctx->trav_ctr.reserve(kCounterSlots * 8);
+    unsigned long long *ctr =
+        ctx->trav_ctr.as<unsigned long long>() + (ctx->trav_ctr_next++ % kCounterSlots);
+    VXL_HIP(hipMemsetAsync(ctr, 0, 8, s));
+    hipLaunchKernelGGL(k_traverse_pl, dim3(uint32_t(blocks)), dim3(64), lds64, s, d_rays, R,
+                       tree->nodes, reinterpret_cast<const uint32_t *>(tree->leaves),
+                       tree->bounds, tree->n, tree->k, tree->leaf_count,
+                       reinterpret_cast<uint4 *>(d_hits), d_t, ctr, uint32_t(refill));
+    VXL_HIP(hipGetLastError());
+    return;
+  }
   const int tb = ctx->dda_block;
   const uint64_t blocks = (R + tb - 1) / tb;
   if (blocks > 0xffffffffull)

@@ -107,6 +107,8 @@ struct vxl_ctx {
   uint32_t sat_epoch = 0;  // setup + SAT calls enqueued (numbers the plan's parity slots)
   int sat_waves_per_cu = 28;  // resident SAT waves per CU (persistent, work-queue fed)
   int sat_units_per_wave = 1;  // work units per resident wave
+  int dda_refill = -1;  // > 0: persistent traversal lanes, refilled when that many are
+                        // idle; 0: one lane per ray; -1: chosen per tree by its size
   int dda_block = 64;  // one-wave workgroups: a finished wave frees its slot at once
   // last-call timing (ms), filled when timing is enabled
   bool timing = false;
@@ -116,6 +118,8 @@ struct vxl_ctx {
   // batch scratch: pooled leaf grids / brick bitmaps of a group of meshes (zero between
   // calls), per-group descriptors, pinned staging for their upload
   vxl::DevBuf pool_grid, pool_b1, batch_scratch, batch_l2, batch_desc, trav_desc;
+  vxl::DevBuf trav_ctr;         // work counters of persistent traversals (a ring of slots)
+  uint32_t trav_ctr_next = 0;
   bool pool_dirty = false;
   uint64_t pool_budget = 16ull << 30;  // leaf-grid bytes one batch group may use
   void *batch_host = nullptr, *batch_host_dev = nullptr;  // mapped: kernels may write it

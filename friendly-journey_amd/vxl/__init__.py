@@ -35,7 +35,7 @@ EXPORTS = [
     "vxl_voxelize_dense", "vxl_voxelize_dense_dev", "vxl_voxelize_tree", "vxl_voxelize_tree_dev",
     "vxl_tree_from_words", "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload",
     "vxl_tree_free", "vxl_traverse", "vxl_traverse_dev", "vxl_set_timing", "vxl_last_timing",
-    "vxl_set_tuning", "vxl_voxelize_batch", "vxl_voxelize_batch_dev", "vxl_traverse_batch",
+    "vxl_set_tuning", "vxl_set_dda_refill", "vxl_voxelize_batch", "vxl_voxelize_batch_dev", "vxl_traverse_batch",
     "vxl_traverse_batch_dev", "vxl_set_batch_budget", "vxl_autotune_batch",
     "vxl_server_create", "vxl_server_destroy", "vxl_server_hold", "vxl_server_submit_rays",
     "vxl_server_submit_voxelize", "vxl_server_poll", "vxl_server_wait", "vxl_server_stats_get",
@@ -124,6 +124,7 @@ def load(path: str = LIB_PATH):
     L.vxl_last_timing.argtypes = [_vp, C.POINTER(C.c_float)]
     L.vxl_last_timing.restype = None
     L.vxl_set_tuning.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
+    L.vxl_set_dda_refill.argtypes = [_vp, C.c_int]
     L.vxl_voxelize_batch.argtypes = [_vp, _vp, _vp, C.c_uint32, _vp, C.c_float, _vp, _vp]
     L.vxl_voxelize_batch_dev.argtypes = [_vp, _vp, _vp, C.c_uint32, _vp, C.c_float, _vp, _vp,
                                          _vp]
@@ -149,7 +150,7 @@ def load(path: str = LIB_PATH):
     for name in ("vxl_ctx_create", "vxl_voxelize_dense", "vxl_voxelize_dense_dev",
                  "vxl_voxelize_tree", "vxl_voxelize_tree_dev", "vxl_tree_from_words",
                  "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload", "vxl_traverse",
-                 "vxl_traverse_dev", "vxl_set_tuning", "vxl_voxelize_batch",
+                 "vxl_traverse_dev", "vxl_set_tuning", "vxl_set_dda_refill", "vxl_voxelize_batch",
                  "vxl_voxelize_batch_dev", "vxl_traverse_batch", "vxl_traverse_batch_dev",
                  "vxl_set_batch_budget", "vxl_autotune_batch"):
         getattr(L, name).restype = C.c_int
@@ -315,6 +316,12 @@ class Context:
 
     def set_tuning(self, sat_waves_per_cu=0, sat_units_per_wave=0, dda_block=0):
         _check(load().vxl_set_tuning(self.handle, sat_waves_per_cu, sat_units_per_wave, dda_block))
+
+    def set_dda_refill(self, refill_min: int):
+        """Lane schedule of traverse / traverse_dev: refill_min in [1, 64] runs persistent
+        waves whose idle lanes take new rays once refill_min are idle; 0 is one lane per
+        ray; -1 (the default) picks by tree size. The results do not depend on it."""
+        _check(load().vxl_set_dda_refill(self.handle, refill_min))
 
     # -- batches ---------------------------------------------------------------------------
     @staticmethod

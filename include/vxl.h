@@ -234,6 +234,12 @@ void vxl_last_timing(vxl_ctx *ctx, float out_ms[8]);
  * size. 0 keeps the current value. */
 vxl_status vxl_set_tuning(vxl_ctx *ctx, int sat_waves_per_cu, int sat_units_per_wave,
                           int dda_block);
+/* DDA lane schedule of vxl_traverse / vxl_traverse_dev: refill_min in [1, 64] runs a
+ * persistent grid whose waves hand a lane the next ray once refill_min of their lanes are
+ * idle; 0 launches one lane per ray; -1 (the default) picks per tree: persistent lanes
+ * (refill 8) for trees whose leaves, bounds and nodes exceed 256 MiB, else one lane per
+ * ray. Results are the same either way. */
+vxl_status vxl_set_dda_refill(vxl_ctx *ctx, int refill_min);
 
 #ifdef __cplusplus
 }

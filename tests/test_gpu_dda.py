@@ -121,3 +121,43 @@ def test_non_finite_and_degenerate_rays(ctx):
     want = ot.mirror(rays)
     bad = np.any(hits != want, axis=1)
     assert not bad.any(), f"{int(bad.sum())} mismatches, first rays {rays[np.nonzero(bad)[0][:3]]}"
+
+
+@pytest.mark.parametrize("refill", [1, 8, 64])
+def test_persistent_lanes_match_mirror(ctx, refill):
+    """The persistent lane schedule (idle lanes take new rays) walks every ray as the mirror
+    does: box rays, sphere camera + random rays, non-finite rays and ragged batch sizes
+    (1, 5, 63, 65, 4097 rays) -- a schedule changes which lane runs a ray, never its hit."""
+    ctx.set_dda_refill(refill)
+    try:
+        n = 128
+        words = O.fixture_words("dust", n, 4096)
+        ot = O.Tree.build(words, n)
+        gt = ctx.tree_from_words(words, n)
+        rays = W.box_rays(20000, n)
+        _check(ctx, gt, ot, rays, f"dust128 refill={refill}", min_hits=100)
+        for R in (1, 5, 63, 65, 4097):
+            _check(ctx, gt, ot, rays[:R], f"dust128 R={R} refill={refill}")
+        tris = W.uv_sphere()
+        n2, vs = W.grid_for(256)
+        w2, _, _ = O.voxelize_surface(tris, n2, (0, 0, 0), vs, 1e-4)
+        ot2 = O.Tree.build(w2, n2)
+        gt2 = ctx.voxelize_tree(tris, n2, (0, 0, 0), vs, 1e-4)
+        _check(ctx, gt2, ot2, W.camera_rays(256, 320, 180), f"sphere camera refill={refill}",
+               min_hits=1000)
+        _check(ctx, gt2, ot2, W.random_rays(50_000, 256), f"sphere random refill={refill}",
+               min_hits=1000)
+        odd = np.array([[5.0, 5.0, 5.0, np.nan, 1.0, 0.0], [-1.0, 3.5, 3.5, 1.0, 0.0, 0.0],
+                        [3.5, -1.0, 3.5, 0.0, np.inf, 0.0], [np.inf, 1.0, 1.0, -1.0, 0.0, 0.0]],
+                       np.float32)
+        hits, _ = ctx.traverse(gt, odd)
+        assert np.array_equal(hits, ot.mirror(odd))
+    finally:
+        ctx.set_dda_refill(-1)
+
+
+def test_dda_refill_argument_checked(ctx):
+    import vxl
+    for bad in (-2, 65):
+        with pytest.raises(vxl.VxlError):
+            ctx.set_dda_refill(bad)
