@@ -33,7 +33,12 @@ constexpr int kFrameWords = 5;
 
 __device__ __forceinline__ float axis_delta(float cs, float d, float rinv, bool big) {
   float q = cs * rinv;
-  if (big) q = cs / d;
+  if (__builtin_expect(big, 0)) {
+    // the division only for |d| >= 2^100; the empty volatile asm keeps the compiler from
+    // computing it on every lane and selecting
+    asm volatile("");
+    q = cs / d;
+  }
   return d > 0.0f ? q : (d < 0.0f ? -q : INFINITY);
 }
 
