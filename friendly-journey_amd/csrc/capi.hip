@@ -341,7 +341,8 @@ void vxl_ctx_destroy(vxl_ctx *ctx) {
                          &ctx->leafgrid, &ctx->bitmap,   &ctx->levelscratch, &ctx->scan_tmp,
                          &ctx->words_io, &ctx->rays_io,  &ctx->hits_io,  &ctx->t_io,
                          &ctx->pool_grid, &ctx->pool_b1, &ctx->batch_scratch, &ctx->batch_l2,
-                         &ctx->batch_desc, &ctx->trav_desc,  &ctx->trav_ctr};
+                         &ctx->batch_desc, &ctx->trav_desc,  &ctx->trav_ctr,
+                         &ctx->owner_io, &ctx->color_io, &ctx->owner_scratch};
   for (auto *b : bufs) b->release();
   // Trees outlive nothing they depend on: release their device memory now and detach
   // them; vxl_tree_free on a detached tree only frees the handle.
@@ -507,6 +508,92 @@ vxl_status vxl_voxelize_dense(vxl_ctx *ctx, const float *tris, uint64_t T, const
     vxl::grid_to_words(ctx, grid->n, ctx->words_io.as<uint32_t>());
     VXL_HIP(hipMemcpyAsync(occ_words, ctx->words_io.ptr, nwords * 4, hipMemcpyDeviceToHost,
                            ctx->stream));
+    VXL_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+// The dense voxelize with per-voxel owners and colours into device arrays: occupancy words,
+// owner (nullable) and color (nullable), n^3 u32 each; a colour without an owner array
+// takes its owners from ctx scratch.
+static void dense_owner_device(vxl_ctx *ctx, const float *d_tris, uint64_t T, const vxl_grid *grid,
+                               float epsilon, uint32_t *d_words, uint32_t *d_owner,
+                               uint32_t *d_color, uint64_t *pairs_out) {
+  const uint64_t nn = grid->n;
+  const uint64_t nv = nn * nn * nn;
+  const uint64_t nwords = (nv + 31) / 32;
+  uint32_t *own = d_owner;
+  if (!own && d_color) {
+    ctx->owner_scratch.reserve(nv * 4);
+    own = ctx->owner_scratch.as<uint32_t>();
+  }
+  if (own) VXL_HIP(hipMemsetAsync(own, 0xFF, nv * 4, ctx->stream));  // UINT32_MAX: no owner
+  if (T == 0) {
+    VXL_HIP(hipMemsetAsync(d_words, 0, nwords * 4, ctx->stream));
+    if (pairs_out) *pairs_out = 0;
+  } else {
+    vxl::prepare_grid(ctx, grid->n);
+    vxl::run_sat(ctx, d_tris, T, make_params(grid, epsilon), own);
+    if (pairs_out) *pairs_out = read_pairs(ctx);
+    vxl::grid_to_words(ctx, grid->n, d_words);
+  }
+  if (d_color) vxl::owner_colors(ctx, own, d_color, nv);
+}
+
+vxl_status vxl_voxelize_dense_owner_dev(vxl_ctx *ctx, const float *d_tris, uint64_t T,
+                                        const vxl_grid *grid, float epsilon,
+                                        uint32_t *d_occ_words, uint32_t *d_owner,
+                                        uint32_t *d_color, uint64_t *pairs_out, void *stream) {
+  return guarded(ctx, [&] {
+    uint32_t k = 0;
+    validate_grid(grid, &k);
+    if (T && !d_tris) throw Error(VXL_INVALID_ARG, "triangles pointer is null");
+    validate_count(T);
+    validate_eps(epsilon);
+    if (!d_occ_words) throw Error(VXL_INVALID_ARG, "occupancy output is null");
+    hipStream_t user = static_cast<hipStream_t>(stream);
+    if (user && user != ctx->stream) VXL_HIP(hipStreamSynchronize(user));
+    dense_owner_device(ctx, d_tris, T, grid, epsilon, d_occ_words, d_owner, d_color, pairs_out);
+    if (user && user != ctx->stream) VXL_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+vxl_status vxl_voxelize_dense_owner(vxl_ctx *ctx, const float *tris, uint64_t T,
+                                    const vxl_grid *grid, float epsilon, uint32_t *occ_words,
+                                    uint32_t *owner, uint32_t *color, uint64_t *pairs_out) {
+  return guarded(ctx, [&] {
+    uint32_t k = 0;
+    validate_grid(grid, &k);
+    validate_count(T);
+    validate_mesh_host(tris, T);
+    validate_eps(epsilon);
+    if (!occ_words) throw Error(VXL_INVALID_ARG, "occupancy output is null");
+    const uint64_t nn = grid->n;
+    const uint64_t nv = nn * nn * nn;
+    const uint64_t nwords = (nv + 31) / 32;
+    const float *d = nullptr;
+    if (T) {
+      ctx->tris_in.reserve(T * 36);
+      VXL_HIP(hipMemcpyAsync(ctx->tris_in.ptr, tris, T * 36, hipMemcpyHostToDevice, ctx->stream));
+      d = ctx->tris_in.as<float>();
+    }
+    ctx->words_io.reserve(nwords * 4);
+    uint32_t *d_owner = nullptr, *d_color = nullptr;
+    if (owner) {
+      ctx->owner_io.reserve(nv * 4);
+      d_owner = ctx->owner_io.as<uint32_t>();
+    }
+    if (color) {
+      ctx->color_io.reserve(nv * 4);
+      d_color = ctx->color_io.as<uint32_t>();
+    }
+    dense_owner_device(ctx, d, T, grid, epsilon, ctx->words_io.as<uint32_t>(), d_owner, d_color,
+                       pairs_out);
+    VXL_HIP(hipMemcpyAsync(occ_words, ctx->words_io.ptr, nwords * 4, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    if (owner)
+      VXL_HIP(hipMemcpyAsync(owner, d_owner, nv * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (color)
+      VXL_HIP(hipMemcpyAsync(color, d_color, nv * 4, hipMemcpyDeviceToHost, ctx->stream));
     VXL_HIP(hipStreamSynchronize(ctx->stream));
   });
 }

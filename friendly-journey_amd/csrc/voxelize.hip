@@ -2,7 +2,7 @@
 //
 // Stage 1 (k_tri_setup / k_tri_setup_batch): world->grid transform
 //   (voxelizer/src/core.rs:101-107), the epsilon-padded, clamped candidate box of
-//   reference_cpu.rs:135-150, the candidate-voxel count and a 64-byte record per triangle.
+//   reference_cpu.rs:135-150, the candidate-voxel count and an 80-byte record per triangle.
 // Stage 2: exclusive scan of the counts gives every (triangle, voxel) pair a global index;
 //   the total is the only value read back before the SAT launch.
 // Stage 3 (k_bstart): the triangle holding the first pair of each work unit.
@@ -17,6 +17,10 @@
 //   atomic ORs into the leaf grid and the brick bitmap (bit per 8^3 brick).
 // Many meshes share one launch: each record carries its mesh's leaf-grid base, so the
 //   pair loop is the same for one mesh or hundreds.
+// Owners (VoxelizeOpts::store_owner, reference_cpu.rs:163-169): with an owner array, each
+//   hit also lowers owner[x + n*(y + n*z)] to its triangle index by an atomic min (a
+//   relaxed read first: owners only fall, so a voxel already owned by a lower index
+//   needs no atomic). The minimum over all covering triangles is order-free.
 //
 // Bit-exactness notes:
 //  * predicate order is free: the result is a pure conjunction without side effects;
@@ -531,6 +535,15 @@ __device__ __forceinline__ void or_if_missing(unsigned long long *p, unsigned lo
   if ((cur & bits) != bits) atomicOr(p, bits);
 }
 
+// Lowers the owner of voxel (x, y, z) to triangle t (the reference keeps the lowest index).
+__device__ __forceinline__ void owner_min(uint32_t *__restrict__ owner, uint32_t n, uint32_t t,
+                                          uint32_t xy, uint32_t z) {
+  const uint64_t lin =
+      uint64_t(xy & 0xffffu) + uint64_t(n) * (uint64_t(xy >> 16) + uint64_t(n) * uint64_t(z));
+  const uint32_t cur = __hip_atomic_load(owner + lin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (t < cur) atomicMin(owner + lin, t);
+}
+
 // Writes the table to the leaf grid and the brick bitmap, then empties it.
 __device__ __forceinline__ void table_flush(unsigned long long *tk, unsigned long long *tv, int lane,
                                             unsigned long long *__restrict__ leafgrid,
@@ -563,13 +576,16 @@ __device__ __forceinline__ void table_flush(unsigned long long *tk, unsigned lon
 // windowed triangle run the build that carries no window decode. Both are held to 72
 // VGPRs (7 waves per SIMD): the few values spilled live in the per-unit loop, and the
 // extra wave hides more of the record loads than the spills cost.
-template <bool kWin>
+//
+// kOwner: also lower each hit voxel's owner to the triangle index (single-mesh calls, where
+// the record index is the triangle index).
+template <bool kWin, bool kOwner>
 __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(7))) void k_sat(
     const TriRec *__restrict__ rec, const WinRec *__restrict__ wrec,
     const uint64_t *__restrict__ coff,
     const uint32_t *__restrict__ ustart, uint64_t T, SatPlan *__restrict__ plan,
     unsigned long long *__restrict__ leafgrid, unsigned long long *__restrict__ brickbits,
-    uint32_t epoch) {
+    uint32_t epoch, uint32_t *__restrict__ owner, uint32_t own_n) {
   if ((plan->win_epoch == epoch) != kWin) return;  // uniform over the grid
   __shared__ uint32_t q_t[kWaves][kQueue], q_xy[kWaves][kQueue], q_z[kWaves][kQueue];
   __shared__ unsigned long long t_key[kWaves][kTable], t_val[kWaves][kTable];
@@ -743,6 +759,7 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(7))
       const uint32_t e = qn + lane;
       unsigned long long key, bit;
       const bool hit = finish_pair(rec, qt[e], qxy[e], qz[e], key, bit);
+      if (kOwner && hit) owner_min(owner, own_n, qt[e], qxy[e], qz[e]);
       used += table_insert(tk, tv, hit, key, bit);
       if (used >= kFlushAt) {
         table_flush(tk, tv, lane, leafgrid, brickbits);
@@ -759,6 +776,7 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(7))
     unsigned long long key = 0, bit = 0;
     bool hit = false;
     if (uint32_t(lane) < qn) hit = finish_pair(rec, qt[lane], qxy[lane], qz[lane], key, bit);
+    if (kOwner && hit) owner_min(owner, own_n, qt[lane], qxy[lane], qz[lane]);
     table_insert(tk, tv, hit, key, bit);
   }
   table_flush(tk, tv, lane, leafgrid, brickbits);
@@ -795,7 +813,8 @@ SatPlan *plan_area(vxl_ctx *ctx) {
 }
 
 void sat_after_setup(vxl_ctx *ctx, uint64_t Tp, unsigned long long *leafgrid,
-                     unsigned long long *brickbits, uint32_t epoch) {
+                     unsigned long long *brickbits, uint32_t epoch, uint32_t *owner = nullptr,
+                     uint32_t own_n = 0) {
   hipStream_t s = ctx->stream;
   const int tb = 256;
   uint64_t *count = ctx->tri_count.as<uint64_t>();
@@ -813,10 +832,17 @@ void sat_after_setup(vxl_ctx *ctx, uint64_t Tp, unsigned long long *leafgrid,
                        ustart);
   mark(ctx, 1);
   const uint64_t blocks = (max_waves + kWaves - 1) / kWaves;
-  hipLaunchKernelGGL(k_sat<false>, dim3(uint32_t(blocks)), dim3(kWaves * 64), 0, s, rec, wrec,
-                     coff, ustart, Tp, plan, leafgrid, brickbits, epoch);
-  hipLaunchKernelGGL(k_sat<true>, dim3(uint32_t(blocks)), dim3(kWaves * 64), 0, s, rec, wrec,
-                     coff, ustart, Tp, plan, leafgrid, brickbits, epoch);
+  if (owner) {
+    hipLaunchKernelGGL((k_sat<false, true>), dim3(uint32_t(blocks)), dim3(kWaves * 64), 0, s, rec,
+                       wrec, coff, ustart, Tp, plan, leafgrid, brickbits, epoch, owner, own_n);
+    hipLaunchKernelGGL((k_sat<true, true>), dim3(uint32_t(blocks)), dim3(kWaves * 64), 0, s, rec,
+                       wrec, coff, ustart, Tp, plan, leafgrid, brickbits, epoch, owner, own_n);
+  } else {
+    hipLaunchKernelGGL((k_sat<false, false>), dim3(uint32_t(blocks)), dim3(kWaves * 64), 0, s,
+                       rec, wrec, coff, ustart, Tp, plan, leafgrid, brickbits, epoch, nullptr, 0u);
+    hipLaunchKernelGGL((k_sat<true, false>), dim3(uint32_t(blocks)), dim3(kWaves * 64), 0, s, rec,
+                       wrec, coff, ustart, Tp, plan, leafgrid, brickbits, epoch, nullptr, 0u);
+  }
   mark(ctx, 2);
   VXL_HIP(hipGetLastError());
 }
@@ -829,7 +855,8 @@ void reserve_records(vxl_ctx *ctx, uint64_t Tp) {
 
 }  // namespace
 
-void run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizeParams &p) {
+void run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizeParams &p,
+             uint32_t *owner) {
   const int tb = 256;
   reserve_records(ctx, T);
   SatPlan *plan = plan_area(ctx);
@@ -838,7 +865,7 @@ void run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizeParams
                      p, ctx->tri_count.as<uint64_t>(), ctx->rec.as<TriRec>(), win_records(ctx, T),
                      &plan->skipped[e & 1], &plan->win_epoch, e);
   sat_after_setup(ctx, T, ctx->leafgrid.as<unsigned long long>(),
-                  ctx->bitmap.as<unsigned long long>(), e);
+                  ctx->bitmap.as<unsigned long long>(), e, owner, p.n);
 }
 
 const uint64_t *sat_pairs_dev(vxl_ctx *ctx) { return &ctx->bstart.as<SatPlan>()->P_ref; }
@@ -859,5 +886,37 @@ void run_sat_batch(vxl_ctx *ctx, const BatchMesh *d_mesh, const uint32_t *d_sb_b
 }
 
 uint64_t setup_slots_per_block() { return kSetupTB; }
+
+namespace {
+// hash_color (reference_cpu.rs:91-99): three steps of the 32-bit LCG x -> 1664525 x +
+// 1013904223 from the owner index, one byte each for r, g, b, and alpha 255.
+__device__ __forceinline__ uint32_t hash_color(uint32_t id) {
+  uint32_t x = id * 1664525u + 1013904223u;
+  const uint32_t r = x & 0xffu;
+  x = x * 1664525u + 1013904223u;
+  const uint32_t g = x & 0xffu;
+  x = x * 1664525u + 1013904223u;
+  const uint32_t b = x & 0xffu;
+  return r | (g << 8) | (b << 16) | (255u << 24);
+}
+
+// color[i] = hash_color(owner[i]) where a triangle owns voxel i, else 0
+// (reference_cpu.rs:173-186; voxelize.wgsl:264-281).
+__global__ void k_owner_color(const uint32_t *__restrict__ owner, uint32_t *__restrict__ color,
+                              uint64_t N) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const uint32_t o = owner[i];
+  color[i] = o != 0xFFFFFFFFu ? hash_color(o) : 0u;
+}
+}  // namespace
+
+void owner_colors(vxl_ctx *ctx, const uint32_t *owner, uint32_t *color, uint64_t N) {
+  if (N == 0) return;
+  const uint64_t blocks = (N + 255) / 256;
+  hipLaunchKernelGGL(k_owner_color, dim3(uint32_t(blocks)), dim3(256), 0, ctx->stream, owner,
+                     color, N);
+  VXL_HIP(hipGetLastError());
+}
 
 }  // namespace vxl

@@ -119,6 +119,7 @@ struct vxl_ctx {
   // calls), per-group descriptors, pinned staging for their upload
   vxl::DevBuf pool_grid, pool_b1, batch_scratch, batch_l2, batch_desc, trav_desc;
   vxl::DevBuf trav_ctr;         // work counters of persistent traversals (a ring of slots)
+  vxl::DevBuf owner_io, color_io, owner_scratch;  // per-voxel owners / colours (n^3 u32)
   uint32_t trav_ctr_next = 0;
   bool pool_dirty = false;
   uint64_t pool_budget = 16ull << 30;  // leaf-grid bytes one batch group may use
@@ -226,7 +227,11 @@ struct SatPlan {
 };
 // Enqueues setup + SAT into ctx->leafgrid / ctx->bitmap (level-1); no host round trip.
 // The reference's pair count stays on the device at sat_pairs_dev(ctx) until the next call.
-void run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizeParams &p);
+// owner (nullable): n^3 u32 preset to UINT32_MAX; each voxel's lowest covering triangle.
+void run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizeParams &p,
+             uint32_t *owner = nullptr);
+// color[i] = hash_color(owner[i]) for owned voxels, 0 elsewhere (N voxels).
+void owner_colors(vxl_ctx *ctx, const uint32_t *owner, uint32_t *color, uint64_t N);
 const uint64_t *sat_pairs_dev(vxl_ctx *ctx);
 // Setup + SAT of a batch group into the pooled grid; mesh g owns setup blocks
 // [sb_begin[g], sb_begin[g+1]) (setup_slots_per_block() record slots each).

@@ -32,7 +32,8 @@ STATUS = {
 # Every symbol include/vxl.h declares.
 EXPORTS = [
     "vxl_ctx_create", "vxl_ctx_destroy", "vxl_ctx_stream", "vxl_last_error", "vxl_status_str",
-    "vxl_voxelize_dense", "vxl_voxelize_dense_dev", "vxl_voxelize_tree", "vxl_voxelize_tree_dev",
+    "vxl_voxelize_dense", "vxl_voxelize_dense_dev", "vxl_voxelize_dense_owner",
+    "vxl_voxelize_dense_owner_dev", "vxl_voxelize_tree", "vxl_voxelize_tree_dev",
     "vxl_tree_from_words", "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload",
     "vxl_tree_free", "vxl_traverse", "vxl_traverse_dev", "vxl_set_timing", "vxl_last_timing",
     "vxl_set_tuning", "vxl_set_dda_refill", "vxl_voxelize_batch", "vxl_voxelize_batch_dev", "vxl_traverse_batch",
@@ -107,6 +108,10 @@ def load(path: str = LIB_PATH):
     L.vxl_voxelize_dense.argtypes = [_vp, _vp, _u64, C.POINTER(Grid), C.c_float, _vp, _pu64]
     L.vxl_voxelize_dense_dev.argtypes = [_vp, _vp, _u64, C.POINTER(Grid), C.c_float, _vp, _pu64,
                                          _vp]
+    L.vxl_voxelize_dense_owner.argtypes = [_vp, _vp, _u64, C.POINTER(Grid), C.c_float, _vp, _vp,
+                                           _vp, _pu64]
+    L.vxl_voxelize_dense_owner_dev.argtypes = [_vp, _vp, _u64, C.POINTER(Grid), C.c_float, _vp,
+                                               _vp, _vp, _pu64, _vp]
     L.vxl_voxelize_tree.argtypes = [_vp, _vp, _u64, C.POINTER(Grid), C.c_float, C.POINTER(_vp),
                                     _pu64]
     L.vxl_voxelize_tree_dev.argtypes = [_vp, _vp, _u64, C.POINTER(Grid), C.c_float,
@@ -148,6 +153,7 @@ def load(path: str = LIB_PATH):
                  "vxl_server_stats_get", "vxl_server_reset_stats"):
         getattr(L, name).restype = C.c_int
     for name in ("vxl_ctx_create", "vxl_voxelize_dense", "vxl_voxelize_dense_dev",
+                 "vxl_voxelize_dense_owner", "vxl_voxelize_dense_owner_dev",
                  "vxl_voxelize_tree", "vxl_voxelize_tree_dev", "vxl_tree_from_words",
                  "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload", "vxl_traverse",
                  "vxl_traverse_dev", "vxl_set_tuning", "vxl_set_dda_refill", "vxl_voxelize_batch",
@@ -253,6 +259,21 @@ class Context:
         _check(load().vxl_voxelize_dense(self.handle, _ptr(t), t.shape[0], C.byref(g), eps,
                                          _ptr(words), C.byref(pairs)))
         return words, int(pairs.value)
+
+    def voxelize_dense_owner(self, tris, n, origin=(0.0, 0.0, 0.0), voxel_size=1.0, eps=1e-4,
+                             owner=True, color=False):
+        """voxelize_surface with store_owner / store_color: (words, owner or None, color or
+        None, pairs); owner and color are n^3 u32, x-major."""
+        t = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+        words = np.zeros((n ** 3 + 31) // 32, np.uint32)
+        own = np.zeros(n ** 3, np.uint32) if owner else None
+        col = np.zeros(n ** 3, np.uint32) if color else None
+        pairs = C.c_uint64(0)
+        g = make_grid(n, origin, voxel_size)
+        _check(load().vxl_voxelize_dense_owner(
+            self.handle, _ptr(t), t.shape[0], C.byref(g), eps, _ptr(words),
+            _ptr(own) if owner else None, _ptr(col) if color else None, C.byref(pairs)))
+        return words, own, col, int(pairs.value)
 
     def voxelize_tree(self, tris, n, origin=(0.0, 0.0, 0.0), voxel_size=1.0, eps=1e-4):
         t = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)

@@ -91,6 +91,23 @@ vxl_status vxl_voxelize_dense_dev(vxl_ctx *ctx, const float *d_tris, uint64_t T,
                                   const vxl_grid *grid, float epsilon, uint32_t *d_occ_words,
                                   uint64_t *pairs_out, void *stream);
 
+/* ---- voxelize, dense output with per-voxel owners and colours: the store_owner /
+ *      store_color options of the same call (VoxelizeOpts, voxelizer/src/core.rs:465-501;
+ *      VoxelizationOutput.owner_id / color_rgba). owner (nullable): n^3 u32, x-major like
+ *      the words, the lowest index of a triangle covering the voxel, UINT32_MAX where none
+ *      does (reference_cpu.rs:118-122,163-169). color (nullable): n^3 u32,
+ *      hash_color(owner) (reference_cpu.rs:91-99) where a triangle covers the voxel, 0
+ *      elsewhere; with owner given this is reference_cpu.rs:173-186, and a colour alone
+ *      follows the GPU call, which hashes the owner without storing it
+ *      (voxelizer/shaders/voxelize.wgsl:264-281). */
+vxl_status vxl_voxelize_dense_owner(vxl_ctx *ctx, const float *tris, uint64_t T,
+                                    const vxl_grid *grid, float epsilon, uint32_t *occ_words,
+                                    uint32_t *owner, uint32_t *color, uint64_t *pairs_out);
+vxl_status vxl_voxelize_dense_owner_dev(vxl_ctx *ctx, const float *d_tris, uint64_t T,
+                                        const vxl_grid *grid, float epsilon,
+                                        uint32_t *d_occ_words, uint32_t *d_owner,
+                                        uint32_t *d_color, uint64_t *pairs_out, void *stream);
+
 /* ---- voxelize straight into a device-resident School-B tree: the composition
  *      voxelize_surface -> VoxelOccupancy::to_school_b (core.rs:224-233) =
  *      SparseTree::build (sparse.rs:241-305) + SchoolBBuffer::from_sparse

@@ -55,6 +55,10 @@ def lib():
         L.vo_voxelize_surface.restype = C.c_uint64
         L.vo_voxelize_surface.argtypes = [_f32p, C.c_uint64, C.c_uint32, _f32p, C.c_float,
                                           C.c_float, _u32p, _u32p]
+        L.vo_hash_color.restype = C.c_uint32
+        L.vo_hash_color.argtypes = [C.c_uint32]
+        L.vo_owner_colors.restype = None
+        L.vo_owner_colors.argtypes = [_u32p, C.c_uint64, _u32p]
         L.vo_count_candidates.restype = C.c_uint64
         L.vo_count_candidates.argtypes = [_f32p, C.c_uint64, C.c_uint32, _f32p, C.c_float,
                                           C.c_float]
@@ -167,6 +171,18 @@ def voxelize_surface(tris, n, origin=(0.0, 0.0, 0.0), voxel_size=1.0, eps=1e-4, 
                                       _p(words, _u32p),
                                       _p(own, _u32p) if owner else None)
     return words, own, int(pairs)
+
+
+def hash_color(i: int) -> int:
+    return int(lib().vo_hash_color(i))
+
+
+def owner_colors(owner: np.ndarray) -> np.ndarray:
+    """Colour per voxel from its owner (reference_cpu.rs:173-186), 0 where unowned."""
+    own = np.ascontiguousarray(owner, np.uint32)
+    col = np.zeros_like(own)
+    lib().vo_owner_colors(_p(own, _u32p), own.size, _p(col, _u32p))
+    return col
 
 
 def count_candidates(tris, n, origin=(0.0, 0.0, 0.0), voxel_size=1.0, eps=1e-4) -> int:

@@ -241,7 +241,24 @@ uint64_t vo_count_candidates(const float *tris, uint64_t T, uint32_t n, const fl
   return pairs;
 }
 
-/* reference_cpu.rs:110-200 (occupancy + owner; colour is out of scope) */
+/* reference_cpu.rs:91-99: three steps of the LCG x -> 1664525 x + 1013904223 (mod 2^32)
+ * from the id, the low byte of each as r, g, b; alpha 255; bytes little-endian. */
+uint32_t vo_hash_color(uint32_t id) {
+  uint32_t x = id * 1664525u + 1013904223u;
+  uint8_t r = (uint8_t)(x & 0xff);
+  x = x * 1664525u + 1013904223u;
+  uint8_t g = (uint8_t)(x & 0xff);
+  x = x * 1664525u + 1013904223u;
+  uint8_t b = (uint8_t)(x & 0xff);
+  return (uint32_t)r | ((uint32_t)g << 8) | ((uint32_t)b << 16) | (255u << 24);
+}
+
+/* reference_cpu.rs:173-186 with store_owner set: colour of each owned voxel, 0 elsewhere. */
+void vo_owner_colors(const uint32_t *owner, uint64_t N, uint32_t *color) {
+  for (uint64_t i = 0; i < N; i++) color[i] = owner[i] != UINT32_MAX ? vo_hash_color(owner[i]) : 0u;
+}
+
+/* reference_cpu.rs:110-200 (occupancy + owner; colours from owners: vo_owner_colors) */
 uint64_t vo_voxelize_surface(const float *tris, uint64_t T, uint32_t n, const float origin[3],
                              float vs, float eps, uint32_t *words, uint32_t *owner) {
   uint64_t nn = n;

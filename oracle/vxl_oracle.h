@@ -40,6 +40,9 @@ void vo_world_to_grid(const float p[3], const float origin[3], float voxel_size,
  * of (triangle, candidate voxel) pairs the epsilon-padded AABB scan enumerates. */
 uint64_t vo_voxelize_surface(const float *tris, uint64_t T, uint32_t n, const float origin[3],
                              float voxel_size, float eps, uint32_t *words, uint32_t *owner);
+/* hash_color (reference_cpu.rs:91-99) and the colour pass of reference_cpu.rs:173-186. */
+uint32_t vo_hash_color(uint32_t id);
+void vo_owner_colors(const uint32_t *owner, uint64_t N, uint32_t *color);
 /* Same candidate count without doing the SAT (for roofline accounting). */
 uint64_t vo_count_candidates(const float *tris, uint64_t T, uint32_t n, const float origin[3],
                              float voxel_size, float eps);

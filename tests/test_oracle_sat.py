@@ -80,6 +80,18 @@ def test_owner_is_lowest_triangle_index():
         assert owner[lin] == (0 if v in covered else 0xFFFFFFFF)
 
 
+def test_hash_color_lcg_known_answers():
+    """hash_color (reference_cpu.rs:91-99) takes the low bytes of three steps of the LCG
+    x -> 1664525 x + 1013904223 (mod 2^32). Known answers from that generator's published
+    sequence from seed 0 (Numerical Recipes in C, 2nd ed., sec. 7.1, "ranqd1"):
+    3C6EF35F, 47502932, D1CCF6E9, AAF95334, 6252E503."""
+    assert O.hash_color(0) == 0xFFE9325F            # bytes 5F, 32, E9, alpha FF
+    assert O.hash_color(0x3C6EF35F) == 0xFF34E932   # seed = first value: 32, E9, 34
+    assert O.hash_color(0x47502932) == 0xFF0334E9   # seed = second value: E9, 34, 03
+    own = np.array([0, 0xFFFFFFFF, 0x3C6EF35F], np.uint32)
+    assert list(O.owner_colors(own)) == [0xFFE9325F, 0, 0xFF34E932]
+
+
 def test_fully_outside_triangles_mark_nothing():
     for tri in M["outside_triangles"]:
         w, _, pairs = O.voxelize_surface(np.array([tri], np.float32), 8)
