@@ -311,8 +311,24 @@ __device__ __forceinline__ uint32_t walk_visit(const Ray &r, Walk &w,
   return kWalking;
 }
 
+// t of a hit at voxel v for the ray (o, d) widened to f64 (see the file comment).
+__device__ __forceinline__ double hit_t(const float o[3], const float d[3], uint4 v) {
+  const double vv[3] = {double(v.x), double(v.y), double(v.z)};
+  double tn = 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    if (d[a] != 0.0f) {
+      const double oa = double(o[a]);
+      const double inv = 1.0 / double(d[a]);
+      const double ta = (vv[a] - oa) * inv, tb = (vv[a] + 1.0 - oa) * inv;
+      tn = fmax(tn, fmin(ta, tb));
+    }
+  }
+  return tn;
+}
+
 // Writes ray ri's result: (x, y, z, 1) for a hit, (0, 0, 0, 0) for none, (0, 0, 0, ~0) for
-// a walk over the iteration bound; and t.
+// a walk over the iteration bound; and t when tout is given.
 __device__ __forceinline__ void store_result(uint64_t ri, const Ray &r, const Walk &w,
                                              uint32_t status, uint4 *__restrict__ hits,
                                              double *__restrict__ tout) {
@@ -320,24 +336,7 @@ __device__ __forceinline__ void store_result(uint64_t ri, const Ray &r, const Wa
   uint4 result = make_uint4(0u, 0u, 0u, status == kCapped ? 0xFFFFFFFFu : 0u);
   if (hit) result = make_uint4(w.ox + w.cx, w.oy + w.cy, w.oz + w.cz, 1u);
   hits[ri] = result;
-  if (tout) {
-    double t = 0.0;
-    if (hit) {
-      const double v[3] = {double(result.x), double(result.y), double(result.z)};
-      double tn = 0.0;
-#pragma unroll
-      for (int a = 0; a < 3; a++) {
-        if (r.d[a] != 0.0f) {
-          const double o = double(r.o[a]);
-          const double inv = 1.0 / double(r.d[a]);
-          const double ta = (v[a] - o) * inv, tb = (v[a] + 1.0 - o) * inv;
-          tn = fmax(tn, fmin(ta, tb));
-        }
-      }
-      t = tn;
-    }
-    tout[ri] = t;
-  }
+  if (tout) tout[ri] = hit ? hit_t(r.o, r.d, result) : 0.0;
 }
 
 __device__ __forceinline__ void traverse_ray(
@@ -369,17 +368,37 @@ __global__ __launch_bounds__(256) void k_traverse(
   traverse_ray(ri, rays, nodes, leafw, bounds, n, k, leaf_count, hits, tout);
 }
 
+// t of every ray from its hit record, as store_result computes it: the persistent walk
+// leaves t to this pass, where all 64 lanes of a wave work, instead of computing it for the
+// few lanes that finish in one visit.
+__global__ __launch_bounds__(256) void k_hit_t(const float *__restrict__ rays, uint64_t R,
+                                               const uint4 *__restrict__ hits,
+                                               double *__restrict__ tout) {
+  const uint64_t ri = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (ri >= R) return;
+  const uint4 h = hits[ri];
+  double t = 0.0;
+  if (h.w == 1u) {
+    const float2 a = *reinterpret_cast<const float2 *>(rays + 6 * ri);
+    const float2 b = *reinterpret_cast<const float2 *>(rays + 6 * ri + 2);
+    const float2 c = *reinterpret_cast<const float2 *>(rays + 6 * ri + 4);
+    const float o[3] = {a.x, a.y, b.x}, d[3] = {b.y, c.x, c.y};
+    t = hit_t(o, d, h);
+  }
+  tout[ri] = t;
+}
+
 // Persistent lanes: a grid sized to the device's resident one-wave blocks; each wave takes
 // runs of kRayRun consecutive rays from a counter (one atomic per run), and whenever at
 // least refill_min of its lanes are idle those lanes take the next rays of its run, so a
 // wave is not held by its longest walk. Every ray's walk is the same as in k_traverse;
-// only the lane that runs it differs.
+// only the lane that runs it differs. t is left to k_hit_t.
 constexpr uint32_t kRayRun = 256;
 
 __global__ __launch_bounds__(64) void k_traverse_pl(
     const float *__restrict__ rays, uint64_t R, const uint32_t *__restrict__ nodes,
     const uint32_t *__restrict__ leafw, const uint32_t *__restrict__ bounds, uint32_t n,
-    uint32_t k, uint64_t leaf_count, uint4 *__restrict__ hits, double *__restrict__ tout,
+    uint32_t k, uint64_t leaf_count, uint4 *__restrict__ hits,
     unsigned long long *__restrict__ counter, uint32_t refill_min) {
   extern __shared__ uint32_t frames_lds[];
   uint32_t *const frames = frames_lds + threadIdx.x;
@@ -392,9 +411,10 @@ __global__ __launch_bounds__(64) void k_traverse_pl(
   uint64_t run = 0, run_end = 0;  // the wave's rays not yet handed out (wave-uniform)
   bool active = false, exhausted = false;
   for (;;) {
+    // refill: the idle lanes take the next rays of the wave's run
     const unsigned long long idle = __ballot(!active);
     const uint32_t ni = __popcll(idle);
-    if (!exhausted && ni >= refill_min) {  // wave-uniform
+    if (!exhausted) {  // wave-uniform; ni >= refill_min here (or the wave just started)
       if (run == run_end) {
         unsigned long long base = 0;
         if (lane == 0) base = atomicAdd(counter, (unsigned long long)kRayRun);
@@ -416,7 +436,7 @@ __global__ __launch_bounds__(64) void k_traverse_pl(
             begin_walk(r, w, k, t0, nodes);
             active = true;
           } else {
-            store_result(ri, r, w, kLeft, hits, tout);
+            store_result(ri, r, w, kLeft, hits, nullptr);
           }
         }
       }
@@ -424,12 +444,18 @@ __global__ __launch_bounds__(64) void k_traverse_pl(
     } else if (ni == 64) {
       break;  // no rays left and every lane idle (wave-uniform)
     }
-    if (active) {
-      const uint32_t status = walk_visit(r, w, nodes, leafw, bounds, frames, 64);
-      if (status != kWalking) {
-        store_result(ri, r, w, status, hits, tout);
-        active = false;
+    // walk until refill_min lanes are idle (every lane, once the rays are exhausted); the
+    // refill stays outside this loop so the walk state is not merged with it every visit
+    const uint32_t until = exhausted ? 64u : refill_min;
+    for (;;) {
+      if (active) {
+        const uint32_t status = walk_visit(r, w, nodes, leafw, bounds, frames, 64);
+        if (status != kWalking) {
+          store_result(ri, r, w, status, hits, nullptr);
+          active = false;
+        }
       }
+      if (uint32_t(__popcll(__ballot(!active))) >= until) break;  // wave-uniform
     }
   }
 }
@@ -467,9 +493,9 @@ constexpr uint32_t kCounterSlots = 1024;
 
 // The automatic lane schedule: a tree whose walked arrays (64-byte leaves, 4-byte bounds,
 // 12-byte nodes) exceed the 256 MiB Infinity Cache is walked from HBM, where a wave's
-// lanes wait on leaf loads and refilling idle lanes pays (soup_2048: 3.44 -> 2.74 ms); a
+// lanes wait on leaf loads and refilling idle lanes pays (soup_2048: 3.18 -> 2.33 ms); a
 // cache-resident tree is bound by instruction issue, where the refills' partial-wave ray
-// setups cost more than the idle lanes (sphere_256: 0.98 vs 1.15 ms).
+// setups cost more than the idle lanes (sphere_256: 0.83 vs 0.95 ms).
 constexpr uint64_t kCacheBytes = 256ull << 20;
 constexpr int kAutoRefill = 8;
 
@@ -487,8 +513,8 @@ void launch_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, ui
   const size_t lds64 = frame_lds_bytes(tree->k, 64);
   const int refill = refill_for(ctx, tree);
   if (refill > 0) {
-    // persistent lanes: as many one-wave blocks as the device holds at once, each with a
-    // counter slot of its own launch (a ring, so launches on other streams keep theirs)
+    // persistent lanes: as many one-wave blocks as the device holds at once, each launch
+    // with a counter slot of its own (a ring, so launches on other streams keep theirs)
     int per_cu = 0;
     VXL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_traverse_pl, 64, lds64));
     if (per_cu < 1) per_cu = 1;
@@ -502,7 +528,12 @@ void launch_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, ui
     hipLaunchKernelGGL(k_traverse_pl, dim3(uint32_t(blocks)), dim3(64), lds64, s, d_rays, R,
                        tree->nodes, reinterpret_cast<const uint32_t *>(tree->leaves),
                        tree->bounds, tree->n, tree->k, tree->leaf_count,
-                       reinterpret_cast<uint4 *>(d_hits), d_t, ctr, uint32_t(refill));
+                       reinterpret_cast<uint4 *>(d_hits), ctr, uint32_t(refill));
+    if (d_t) {
+      const uint64_t tb = (R + 255) / 256;
+      hipLaunchKernelGGL(k_hit_t, dim3(uint32_t(tb)), dim3(256), 0, s, d_rays, R,
+                         reinterpret_cast<const uint4 *>(d_hits), d_t);
+    }
     VXL_HIP(hipGetLastError());
     return;
   }
