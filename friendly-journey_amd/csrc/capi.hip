@@ -342,7 +342,8 @@ void vxl_ctx_destroy(vxl_ctx *ctx) {
                          &ctx->words_io, &ctx->rays_io,  &ctx->hits_io,  &ctx->t_io,
                          &ctx->pool_grid, &ctx->pool_b1, &ctx->batch_scratch, &ctx->batch_l2,
                          &ctx->batch_desc, &ctx->trav_desc,  &ctx->trav_ctr,
-                         &ctx->owner_io, &ctx->color_io, &ctx->owner_scratch};
+                         &ctx->owner_io, &ctx->color_io, &ctx->owner_scratch,
+                         &ctx->leaf_map, &ctx->compact_io};
   for (auto *b : bufs) b->release();
   // Trees outlive nothing they depend on: release their device memory now and detach
   // them; vxl_tree_free on a detached tree only frees the handle.
@@ -595,6 +596,107 @@ vxl_status vxl_voxelize_dense_owner(vxl_ctx *ctx, const float *tris, uint64_t T,
     if (color)
       VXL_HIP(hipMemcpyAsync(color, d_color, nv * 4, hipMemcpyDeviceToHost, ctx->stream));
     VXL_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+vxl_status vxl_voxelize_tree_materials(vxl_ctx *ctx, const float *tris, uint64_t T,
+                                       const vxl_grid *grid, float epsilon,
+                                       const uint32_t *material_table, uint32_t table_len,
+                                       vxl_tree **out, uint64_t *pairs_out) {
+  return guarded(ctx, [&] {
+    if (!out) throw Error(VXL_INVALID_ARG, "out is null");
+    *out = nullptr;
+    uint32_t k = 0;
+    validate_grid(grid, &k);
+    validate_count(T);
+    validate_mesh_host(tris, T);
+    validate_eps(epsilon);
+    if (table_len && !material_table) throw Error(VXL_INVALID_ARG, "material table is null");
+    if (T) {
+      ctx->tris_in.reserve(T * 36);
+      VXL_HIP(hipMemcpyAsync(ctx->tris_in.ptr, tris, T * 36, hipMemcpyHostToDevice, ctx->stream));
+    }
+    const uint32_t *d_table = nullptr;
+    if (table_len) {
+      ctx->words_io.reserve(uint64_t(table_len) * 4);
+      VXL_HIP(hipMemcpyAsync(ctx->words_io.ptr, material_table, uint64_t(table_len) * 4,
+                             hipMemcpyHostToDevice, ctx->stream));
+      d_table = ctx->words_io.as<uint32_t>();
+    }
+    *out = vxl::voxelize_tree_materials(ctx, ctx->tris_in.as<float>(), T, grid->n, k,
+                                        make_params(grid, epsilon), d_table, table_len,
+                                        pairs_out);
+    ctx->trees.insert(*out);
+    VXL_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+vxl_status vxl_compact_surface(vxl_ctx *ctx, const float *tris, uint64_t T, const vxl_grid *grid,
+                               float epsilon, const uint32_t *material_table, uint32_t table_len,
+                               const int32_t g_origin[3], vxl_compact_voxel *out,
+                               uint64_t capacity, uint64_t *count) {
+  vxl_tree *tree = nullptr;
+  vxl_status st = vxl_voxelize_tree_materials(ctx, tris, T, grid, epsilon, material_table,
+                                              table_len, &tree, nullptr);
+  if (st != VXL_OK) return st;
+  st = guarded(ctx, [&] {
+    if (!count) throw Error(VXL_INVALID_ARG, "count is null");
+    const int32_t g0[3] = {g_origin ? g_origin[0] : 0, g_origin ? g_origin[1] : 0,
+                           g_origin ? g_origin[2] : 0};
+    // count first; stage and copy only when the caller's buffer holds them all
+    const uint64_t total = vxl::compact_tree(ctx, tree, g0, nullptr, 0);
+    *count = total;
+    if (out && total) {
+      if (capacity < total)
+        throw Error(VXL_STORAGE_EXCEEDED, "compact: " + std::to_string(total) +
+                                              " voxels exceed the output capacity");
+      ctx->compact_io.reserve(total * 16);
+      vxl::compact_tree(ctx, tree, g0, ctx->compact_io.as<int32_t>(), total);
+      VXL_HIP(hipMemcpyAsync(out, ctx->compact_io.ptr, total * 16, hipMemcpyDeviceToHost,
+                             ctx->stream));
+      VXL_HIP(hipStreamSynchronize(ctx->stream));
+    }
+  });
+  vxl_tree_free(tree);
+  return st;
+}
+
+vxl_status vxl_tree_from_compact(vxl_ctx *ctx, uint32_t n, const vxl_compact_voxel *voxels,
+                                 uint64_t count, vxl_tree **out, uint64_t *dropped) {
+  return guarded(ctx, [&] {
+    if (!out) throw Error(VXL_INVALID_ARG, "out is null");
+    *out = nullptr;
+    vxl_grid g{n, {0, 0, 0}, 1.0f};
+    uint32_t k = 0;
+    validate_grid(&g, &k);
+    if (count && !voxels) throw Error(VXL_INVALID_ARG, "voxels pointer is null");
+    if (count) {
+      ctx->compact_io.reserve(count * 16);
+      VXL_HIP(hipMemcpyAsync(ctx->compact_io.ptr, voxels, count * 16, hipMemcpyHostToDevice,
+                             ctx->stream));
+    }
+    *out = vxl::tree_from_compact(ctx, n, k, ctx->compact_io.as<int32_t>(), count, dropped);
+    ctx->trees.insert(*out);
+    VXL_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+vxl_status vxl_tree_materials_download(const vxl_tree *tree, uint16_t *materials) {
+  if (!tree) {
+    vxl::set_error("null tree");
+    return VXL_INVALID_ARG;
+  }
+  return guarded(tree->ctx, [&] {
+    if (!materials) throw Error(VXL_INVALID_ARG, "materials is null");
+    const size_t bytes = size_t(tree->leaf_count) * 1024;
+    if (!bytes) return;
+    if (tree->materials) {
+      VXL_HIP(hipMemcpyAsync(materials, tree->materials, bytes, hipMemcpyDeviceToHost,
+                             tree->ctx->stream));
+      VXL_HIP(hipStreamSynchronize(tree->ctx->stream));
+    } else {
+      std::memset(materials, 0, bytes);  // a tree built without materials: all MISSING
+    }
   });
 }
 

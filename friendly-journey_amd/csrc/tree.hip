@@ -676,10 +676,12 @@ void tree_free_device(vxl_tree *t) {
     if (t->bounds) (void)hipFree(t->bounds);
     if (t->codes) (void)hipFree(t->codes);
   }
+  if (t->materials) (void)hipFree(t->materials);
   t->nodes = nullptr;
   t->leaves = nullptr;
   t->bounds = nullptr;
   t->codes = nullptr;
+  t->materials = nullptr;
 }
 
 static vxl_tree *alloc_tree(vxl_ctx *ctx, uint32_t n, uint32_t k, uint64_t nodes,

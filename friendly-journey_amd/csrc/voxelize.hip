@@ -20,7 +20,10 @@
 // Owners (VoxelizeOpts::store_owner, reference_cpu.rs:163-169): with an owner array, each
 //   hit also lowers owner[x + n*(y + n*z)] to its triangle index by an atomic min (a
 //   relaxed read first: owners only fall, so a voxel already owned by a lower index
-//   needs no atomic). The minimum over all covering triangles is order-free.
+//   needs no atomic). The minimum over all covering triangles is order-free. A second
+//   mode keeps owners per leaf voxel (owner[leaf * 512 + intra-brick Morton index], the
+//   leaf found from a brick-code -> leaf map of an already built tree) and writes nothing
+//   else: the owner pass of the material path, where a dense n^3 array would not fit.
 //
 // Bit-exactness notes:
 //  * predicate order is free: the result is a pure conjunction without side effects;
@@ -535,11 +538,22 @@ __device__ __forceinline__ void or_if_missing(unsigned long long *p, unsigned lo
   if ((cur & bits) != bits) atomicOr(p, bits);
 }
 
+// Owner modes of k_sat: none, a dense n^3 array, or per leaf voxel of a built tree.
+constexpr int kOwnNone = 0, kOwnDense = 1, kOwnLeaf = 2;
+
 // Lowers the owner of voxel (x, y, z) to triangle t (the reference keeps the lowest index).
-__device__ __forceinline__ void owner_min(uint32_t *__restrict__ owner, uint32_t n, uint32_t t,
-                                          uint32_t xy, uint32_t z) {
-  const uint64_t lin =
-      uint64_t(xy & 0xffffu) + uint64_t(n) * (uint64_t(xy >> 16) + uint64_t(n) * uint64_t(z));
+template <int kOwn>
+__device__ __forceinline__ void owner_min(uint32_t *__restrict__ owner,
+                                          const uint32_t *__restrict__ leaf_map, uint32_t n,
+                                          uint32_t t, uint32_t xy, uint32_t z) {
+  const uint32_t x = xy & 0xffffu, y = xy >> 16;
+  uint64_t lin;
+  if (kOwn == kOwnDense) {
+    lin = uint64_t(x) + uint64_t(n) * (uint64_t(y) + uint64_t(n) * uint64_t(z));
+  } else {
+    const uint32_t leaf = leaf_map[dev::brick_code(x >> 3, y >> 3, z >> 3)];
+    lin = uint64_t(leaf) * 512 + dev::morton9(x, y, z);
+  }
   const uint32_t cur = __hip_atomic_load(owner + lin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (t < cur) atomicMin(owner + lin, t);
 }
@@ -577,15 +591,16 @@ __device__ __forceinline__ void table_flush(unsigned long long *tk, unsigned lon
 // VGPRs (7 waves per SIMD): the few values spilled live in the per-unit loop, and the
 // extra wave hides more of the record loads than the spills cost.
 //
-// kOwner: also lower each hit voxel's owner to the triangle index (single-mesh calls, where
-// the record index is the triangle index).
-template <bool kWin, bool kOwner>
+// kOwn: also lower each hit voxel's owner to the triangle index (single-mesh calls, where
+// the record index is the triangle index); kOwnLeaf writes owners only.
+template <bool kWin, int kOwn>
 __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(7))) void k_sat(
     const TriRec *__restrict__ rec, const WinRec *__restrict__ wrec,
     const uint64_t *__restrict__ coff,
     const uint32_t *__restrict__ ustart, uint64_t T, SatPlan *__restrict__ plan,
     unsigned long long *__restrict__ leafgrid, unsigned long long *__restrict__ brickbits,
-    uint32_t epoch, uint32_t *__restrict__ owner, uint32_t own_n) {
+    uint32_t epoch, uint32_t *__restrict__ owner, const uint32_t *__restrict__ leaf_map,
+    uint32_t own_n) {
   if ((plan->win_epoch == epoch) != kWin) return;  // uniform over the grid
   __shared__ uint32_t q_t[kWaves][kQueue], q_xy[kWaves][kQueue], q_z[kWaves][kQueue];
   __shared__ unsigned long long t_key[kWaves][kTable], t_val[kWaves][kTable];
@@ -759,11 +774,13 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(7))
       const uint32_t e = qn + lane;
       unsigned long long key, bit;
       const bool hit = finish_pair(rec, qt[e], qxy[e], qz[e], key, bit);
-      if (kOwner && hit) owner_min(owner, own_n, qt[e], qxy[e], qz[e]);
-      used += table_insert(tk, tv, hit, key, bit);
-      if (used >= kFlushAt) {
-        table_flush(tk, tv, lane, leafgrid, brickbits);
-        used = 0;
+      if (kOwn != kOwnNone && hit) owner_min<kOwn>(owner, leaf_map, own_n, qt[e], qxy[e], qz[e]);
+      if (kOwn != kOwnLeaf) {
+        used += table_insert(tk, tv, hit, key, bit);
+        if (used >= kFlushAt) {
+          table_flush(tk, tv, lane, leafgrid, brickbits);
+          used = 0;
+        }
       }
     }
     offn = offn_next;
@@ -776,10 +793,11 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(7))
     unsigned long long key = 0, bit = 0;
     bool hit = false;
     if (uint32_t(lane) < qn) hit = finish_pair(rec, qt[lane], qxy[lane], qz[lane], key, bit);
-    if (kOwner && hit) owner_min(owner, own_n, qt[lane], qxy[lane], qz[lane]);
-    table_insert(tk, tv, hit, key, bit);
+    if (kOwn != kOwnNone && hit)
+      owner_min<kOwn>(owner, leaf_map, own_n, qt[lane], qxy[lane], qz[lane]);
+    if (kOwn != kOwnLeaf) table_insert(tk, tv, hit, key, bit);
   }
-  table_flush(tk, tv, lane, leafgrid, brickbits);
+  if (kOwn != kOwnLeaf) table_flush(tk, tv, lane, leafgrid, brickbits);
 }
 
 }  // namespace
@@ -814,7 +832,7 @@ SatPlan *plan_area(vxl_ctx *ctx) {
 
 void sat_after_setup(vxl_ctx *ctx, uint64_t Tp, unsigned long long *leafgrid,
                      unsigned long long *brickbits, uint32_t epoch, uint32_t *owner = nullptr,
-                     uint32_t own_n = 0) {
+                     const uint32_t *leaf_map = nullptr, uint32_t own_n = 0) {
   hipStream_t s = ctx->stream;
   const int tb = 256;
   uint64_t *count = ctx->tri_count.as<uint64_t>();
@@ -832,16 +850,22 @@ void sat_after_setup(vxl_ctx *ctx, uint64_t Tp, unsigned long long *leafgrid,
                        ustart);
   mark(ctx, 1);
   const uint64_t blocks = (max_waves + kWaves - 1) / kWaves;
-  if (owner) {
-    hipLaunchKernelGGL((k_sat<false, true>), dim3(uint32_t(blocks)), dim3(kWaves * 64), 0, s, rec,
-                       wrec, coff, ustart, Tp, plan, leafgrid, brickbits, epoch, owner, own_n);
-    hipLaunchKernelGGL((k_sat<true, true>), dim3(uint32_t(blocks)), dim3(kWaves * 64), 0, s, rec,
-                       wrec, coff, ustart, Tp, plan, leafgrid, brickbits, epoch, owner, own_n);
+  const dim3 g(static_cast<uint32_t>(blocks), 1, 1), b(kWaves * 64, 1, 1);
+  if (owner && leaf_map) {
+    hipLaunchKernelGGL((k_sat<false, kOwnLeaf>), g, b, 0, s, rec, wrec, coff, ustart, Tp, plan,
+                       leafgrid, brickbits, epoch, owner, leaf_map, own_n);
+    hipLaunchKernelGGL((k_sat<true, kOwnLeaf>), g, b, 0, s, rec, wrec, coff, ustart, Tp, plan,
+                       leafgrid, brickbits, epoch, owner, leaf_map, own_n);
+  } else if (owner) {
+    hipLaunchKernelGGL((k_sat<false, kOwnDense>), g, b, 0, s, rec, wrec, coff, ustart, Tp, plan,
+                       leafgrid, brickbits, epoch, owner, nullptr, own_n);
+    hipLaunchKernelGGL((k_sat<true, kOwnDense>), g, b, 0, s, rec, wrec, coff, ustart, Tp, plan,
+                       leafgrid, brickbits, epoch, owner, nullptr, own_n);
   } else {
-    hipLaunchKernelGGL((k_sat<false, false>), dim3(uint32_t(blocks)), dim3(kWaves * 64), 0, s,
-                       rec, wrec, coff, ustart, Tp, plan, leafgrid, brickbits, epoch, nullptr, 0u);
-    hipLaunchKernelGGL((k_sat<true, false>), dim3(uint32_t(blocks)), dim3(kWaves * 64), 0, s, rec,
-                       wrec, coff, ustart, Tp, plan, leafgrid, brickbits, epoch, nullptr, 0u);
+    hipLaunchKernelGGL((k_sat<false, kOwnNone>), g, b, 0, s, rec, wrec, coff, ustart, Tp, plan,
+                       leafgrid, brickbits, epoch, nullptr, nullptr, 0u);
+    hipLaunchKernelGGL((k_sat<true, kOwnNone>), g, b, 0, s, rec, wrec, coff, ustart, Tp, plan,
+                       leafgrid, brickbits, epoch, nullptr, nullptr, 0u);
   }
   mark(ctx, 2);
   VXL_HIP(hipGetLastError());
@@ -856,7 +880,7 @@ void reserve_records(vxl_ctx *ctx, uint64_t Tp) {
 }  // namespace
 
 void run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizeParams &p,
-             uint32_t *owner) {
+             uint32_t *owner, const uint32_t *leaf_map) {
   const int tb = 256;
   reserve_records(ctx, T);
   SatPlan *plan = plan_area(ctx);
@@ -865,7 +889,7 @@ void run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizeParams
                      p, ctx->tri_count.as<uint64_t>(), ctx->rec.as<TriRec>(), win_records(ctx, T),
                      &plan->skipped[e & 1], &plan->win_epoch, e);
   sat_after_setup(ctx, T, ctx->leafgrid.as<unsigned long long>(),
-                  ctx->bitmap.as<unsigned long long>(), e, owner, p.n);
+                  ctx->bitmap.as<unsigned long long>(), e, owner, leaf_map, p.n);
 }
 
 const uint64_t *sat_pairs_dev(vxl_ctx *ctx) { return &ctx->bstart.as<SatPlan>()->P_ref; }

@@ -120,6 +120,8 @@ struct vxl_ctx {
   vxl::DevBuf pool_grid, pool_b1, batch_scratch, batch_l2, batch_desc, trav_desc;
   vxl::DevBuf trav_ctr;         // work counters of persistent traversals (a ring of slots)
   vxl::DevBuf owner_io, color_io, owner_scratch;  // per-voxel owners / colours (n^3 u32)
+  vxl::DevBuf leaf_map;          // brick code -> leaf index of the tree being given materials
+  vxl::DevBuf compact_io;        // compact voxels staged for the host
   uint32_t trav_ctr_next = 0;
   bool pool_dirty = false;
   uint64_t pool_budget = 16ull << 30;  // leaf-grid bytes one batch group may use
@@ -139,6 +141,7 @@ struct vxl_tree {
   uint64_t *leaves = nullptr;  // 8 x u64 per leaf (>= 1 entry allocated)
   uint32_t *bounds = nullptr;  // 1 x u32 per leaf (>= 1 entry allocated)
   uint64_t *codes = nullptr;   // 1 x u64 per leaf (>= 1 entry allocated)
+  uint16_t *materials = nullptr;  // 512 x u16 per leaf (intra-brick Morton order) or none
   uint64_t level_nodes[16] = {0};
   size_t bytes = 0;
   int slab = -1;  // arena slab holding the four arrays (-1: separate device allocations)
@@ -228,10 +231,23 @@ struct SatPlan {
 // Enqueues setup + SAT into ctx->leafgrid / ctx->bitmap (level-1); no host round trip.
 // The reference's pair count stays on the device at sat_pairs_dev(ctx) until the next call.
 // owner (nullable): n^3 u32 preset to UINT32_MAX; each voxel's lowest covering triangle.
+// With leaf_map (brick code -> leaf index of a tree built from the same call), owner is
+// leaf_count x 512 u32 indexed leaf * 512 + intra-brick Morton index, and the pass writes
+// nothing else (the leaf grid stays as it was).
 void run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizeParams &p,
-             uint32_t *owner = nullptr);
+             uint32_t *owner = nullptr, const uint32_t *leaf_map = nullptr);
 // color[i] = hash_color(owner[i]) for owned voxels, 0 elsewhere (N voxels).
 void owner_colors(vxl_ctx *ctx, const uint32_t *owner, uint32_t *color, uint64_t N);
+// materials.hip: the tree of a mesh with per-voxel materials from a per-triangle table (two
+// u16 per u32); the occupied voxels of a tree as (g + coord, material) int4 rows (returns
+// the count, writes when capacity allows); a tree with materials from such rows.
+vxl_tree *voxelize_tree_materials(vxl_ctx *ctx, const float *d_tris, uint64_t T, uint32_t n,
+                                  uint32_t k, const VoxelizeParams &p, const uint32_t *d_table,
+                                  uint32_t table_len, uint64_t *pairs);
+uint64_t compact_tree(vxl_ctx *ctx, const vxl_tree *t, const int32_t g[3], int32_t *d_out,
+                      uint64_t capacity);
+vxl_tree *tree_from_compact(vxl_ctx *ctx, uint32_t n, uint32_t k, const int32_t *d_vox,
+                            uint64_t count, uint64_t *dropped);
 const uint64_t *sat_pairs_dev(vxl_ctx *ctx);
 // Setup + SAT of a batch group into the pooled grid; mesh g owns setup blocks
 // [sb_begin[g], sb_begin[g+1]) (setup_slots_per_block() record slots each).

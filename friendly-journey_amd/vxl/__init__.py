@@ -34,6 +34,8 @@ EXPORTS = [
     "vxl_ctx_create", "vxl_ctx_destroy", "vxl_ctx_stream", "vxl_last_error", "vxl_status_str",
     "vxl_voxelize_dense", "vxl_voxelize_dense_dev", "vxl_voxelize_dense_owner",
     "vxl_voxelize_dense_owner_dev", "vxl_voxelize_tree", "vxl_voxelize_tree_dev",
+    "vxl_voxelize_tree_materials", "vxl_compact_surface", "vxl_tree_from_compact",
+    "vxl_tree_materials_download",
     "vxl_tree_from_words", "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload",
     "vxl_tree_free", "vxl_traverse", "vxl_traverse_dev", "vxl_set_timing", "vxl_last_timing",
     "vxl_set_tuning", "vxl_set_dda_refill", "vxl_voxelize_batch", "vxl_voxelize_batch_dev", "vxl_traverse_batch",
@@ -112,6 +114,12 @@ def load(path: str = LIB_PATH):
                                            _vp, _pu64]
     L.vxl_voxelize_dense_owner_dev.argtypes = [_vp, _vp, _u64, C.POINTER(Grid), C.c_float, _vp,
                                                _vp, _vp, _pu64, _vp]
+    L.vxl_voxelize_tree_materials.argtypes = [_vp, _vp, _u64, C.POINTER(Grid), C.c_float, _vp,
+                                              C.c_uint32, C.POINTER(_vp), _pu64]
+    L.vxl_compact_surface.argtypes = [_vp, _vp, _u64, C.POINTER(Grid), C.c_float, _vp,
+                                      C.c_uint32, _vp, _vp, _u64, _pu64]
+    L.vxl_tree_from_compact.argtypes = [_vp, C.c_uint32, _vp, _u64, C.POINTER(_vp), _pu64]
+    L.vxl_tree_materials_download.argtypes = [_vp, _vp]
     L.vxl_voxelize_tree.argtypes = [_vp, _vp, _u64, C.POINTER(Grid), C.c_float, C.POINTER(_vp),
                                     _pu64]
     L.vxl_voxelize_tree_dev.argtypes = [_vp, _vp, _u64, C.POINTER(Grid), C.c_float,
@@ -154,6 +162,8 @@ def load(path: str = LIB_PATH):
         getattr(L, name).restype = C.c_int
     for name in ("vxl_ctx_create", "vxl_voxelize_dense", "vxl_voxelize_dense_dev",
                  "vxl_voxelize_dense_owner", "vxl_voxelize_dense_owner_dev",
+                 "vxl_voxelize_tree_materials", "vxl_compact_surface", "vxl_tree_from_compact",
+                 "vxl_tree_materials_download",
                  "vxl_voxelize_tree", "vxl_voxelize_tree_dev", "vxl_tree_from_words",
                  "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload", "vxl_traverse",
                  "vxl_traverse_dev", "vxl_set_tuning", "vxl_set_dda_refill", "vxl_voxelize_batch",
@@ -210,6 +220,13 @@ class Tree:
         _check(load().vxl_tree_download(self.handle, _ptr(nodes), _ptr(leaves), _ptr(bounds),
                                         _ptr(codes)))
         return nodes, leaves, bounds, codes
+
+    def materials(self):
+        """[L, 512] u16 per-voxel materials in intra-brick Morton order (zeros for a tree
+        made without materials)."""
+        mats = np.zeros((self.leaf_count, 512), np.uint16)
+        _check(load().vxl_tree_materials_download(self.handle, _ptr(mats)))
+        return mats
 
     def free(self):
         if self._h:
@@ -296,6 +313,47 @@ class Context:
         tree = Tree(out.value, self)
         tree.pairs = int(pairs.value)
         return tree
+
+    def voxelize_tree_materials(self, tris, n, origin=(0.0, 0.0, 0.0), voxel_size=1.0,
+                                eps=1e-4, table=()):
+        """Voxelize into a tree carrying per-voxel materials from a per-triangle table of
+        two u16 per u32."""
+        t = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+        tb = np.ascontiguousarray(table, np.uint32)
+        out = _vp()
+        pairs = C.c_uint64(0)
+        g = make_grid(n, origin, voxel_size)
+        _check(load().vxl_voxelize_tree_materials(self.handle, _ptr(t), t.shape[0], C.byref(g),
+                                                  eps, _ptr(tb) if tb.size else None, tb.size,
+                                                  C.byref(out), C.byref(pairs)))
+        tree = Tree(out.value, self)
+        tree.pairs = int(pairs.value)
+        return tree
+
+    def compact_surface(self, tris, n, origin=(0.0, 0.0, 0.0), voxel_size=1.0, eps=1e-4,
+                        table=(), g_origin=(0, 0, 0)):
+        """compact_surface_sparse: [count, 4] i32 rows (vx, vy, vz, material)."""
+        t = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+        tb = np.ascontiguousarray(table, np.uint32)
+        go = np.ascontiguousarray(g_origin, np.int32)
+        g = make_grid(n, origin, voxel_size)
+        count = C.c_uint64(0)
+        args = (self.handle, _ptr(t), t.shape[0], C.byref(g), eps,
+                _ptr(tb) if tb.size else None, tb.size, _ptr(go))
+        _check(load().vxl_compact_surface(*args, None, 0, C.byref(count)))
+        out = np.zeros((int(count.value), 4), np.int32)
+        if out.shape[0]:
+            _check(load().vxl_compact_surface(*args, _ptr(out), out.shape[0], C.byref(count)))
+        return out
+
+    def tree_from_compact(self, voxels, n):
+        """tree_from_compact: (Tree with materials, number of voxels dropped)."""
+        v = np.ascontiguousarray(voxels, np.int32).reshape(-1, 4)
+        out = _vp()
+        dropped = C.c_uint64(0)
+        _check(load().vxl_tree_from_compact(self.handle, n, _ptr(v) if v.size else None,
+                                            v.shape[0], C.byref(out), C.byref(dropped)))
+        return Tree(out.value, self), int(dropped.value)
 
     def tree_from_words(self, words, n):
         w = np.ascontiguousarray(words, np.uint32)

@@ -108,6 +108,39 @@ vxl_status vxl_voxelize_dense_owner_dev(vxl_ctx *ctx, const float *d_tris, uint6
                                         uint32_t *d_occ_words, uint32_t *d_owner,
                                         uint32_t *d_color, uint64_t *pairs_out, void *stream);
 
+/* ---- per-voxel materials: the mesh path of the viewer (voxel-viewer/src/main.rs:984-1003),
+ *      GpuVoxelizer::compact_surface_sparse (voxelizer/src/gpu/sparse.rs:61-111,
+ *      shaders/compact_voxels.wgsl) then tree_from_compact (voxelizer/src/materials.rs:
+ *      153-192) over SparseTree::from_voxels (voxel-core/src/sparse.rs:324-366).
+ * material_table: two u16 per u32, triangle t's global material id in bits (t & 1) * 16
+ * of word t >> 1 (material_table_for_sparse, materials.rs:122-139); a voxel takes the id of
+ * its owner (lowest covering triangle, as store_owner), 0 (MISSING) when the owner is past
+ * the table. Trees made here carry 512 u16 per leaf in intra-brick Morton order
+ * (SparseTree::leaf_materials, sparse.rs:482-484). */
+typedef struct vxl_compact_voxel {
+  int32_t vx, vy, vz; /* global voxel coordinate: g_origin + grid coordinate */
+  uint32_t material;  /* global material id (CompactVoxel, voxelizer/src/core.rs:537-546) */
+} vxl_compact_voxel;
+/* voxelize + materials into a device tree (the two calls above fused; the tree is the one
+ * vxl_voxelize_tree returns, bit for bit). */
+vxl_status vxl_voxelize_tree_materials(vxl_ctx *ctx, const float *tris, uint64_t T,
+                                       const vxl_grid *grid, float epsilon,
+                                       const uint32_t *material_table, uint32_t table_len,
+                                       vxl_tree **out, uint64_t *pairs_out);
+/* compact_surface_sparse: every occupied voxel with its material, in leaf (brick Morton)
+ * order then intra-brick Morton order. *count receives the number of voxels; out may be
+ * NULL to ask for it; a non-NULL out with capacity < *count is VXL_STORAGE_EXCEEDED. */
+vxl_status vxl_compact_surface(vxl_ctx *ctx, const float *tris, uint64_t T, const vxl_grid *grid,
+                               float epsilon, const uint32_t *material_table, uint32_t table_len,
+                               const int32_t g_origin[3], vxl_compact_voxel *out,
+                               uint64_t capacity, uint64_t *count);
+/* tree_from_compact: voxels outside [0, n)^3 are dropped (counted in *dropped); the
+ * material is the low 16 bits; a repeated coordinate keeps the last entry's material. */
+vxl_status vxl_tree_from_compact(vxl_ctx *ctx, uint32_t n, const vxl_compact_voxel *voxels,
+                                 uint64_t count, vxl_tree **out, uint64_t *dropped);
+/* leaf_count x 512 u16 materials of a tree (zeros for a tree made without materials). */
+vxl_status vxl_tree_materials_download(const vxl_tree *tree, uint16_t *materials);
+
 /* ---- voxelize straight into a device-resident School-B tree: the composition
  *      voxelize_surface -> VoxelOccupancy::to_school_b (core.rs:224-233) =
  *      SparseTree::build (sparse.rs:241-305) + SchoolBBuffer::from_sparse

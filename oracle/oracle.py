@@ -19,6 +19,7 @@ _lib = None
 _u32p = C.POINTER(C.c_uint32)
 _u64p = C.POINTER(C.c_uint64)
 _f32p = C.POINTER(C.c_float)
+_i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)
 
 
@@ -59,6 +60,15 @@ def lib():
         L.vo_hash_color.argtypes = [C.c_uint32]
         L.vo_owner_colors.restype = None
         L.vo_owner_colors.argtypes = [_u32p, C.c_uint64, _u32p]
+        L.vo_resolve_material.restype = C.c_uint32
+        L.vo_resolve_material.argtypes = [C.c_uint32, _u32p, C.c_uint32]
+        L.vo_compact.restype = C.c_uint64
+        L.vo_compact.argtypes = [_u32p, _u32p, C.c_uint32, _u32p, C.c_uint32, _i32p, _i32p]
+        L.vo_compact_to_words.restype = C.c_uint64
+        L.vo_compact_to_words.argtypes = [_i32p, C.c_uint64, C.c_uint32, _u32p]
+        L.vo_compact_materials.restype = None
+        L.vo_compact_materials.argtypes = [_i32p, C.c_uint64, C.c_uint32, _u64p, C.c_uint64,
+                                           C.POINTER(C.c_uint16)]
         L.vo_count_candidates.restype = C.c_uint64
         L.vo_count_candidates.argtypes = [_f32p, C.c_uint64, C.c_uint32, _f32p, C.c_float,
                                           C.c_float]
@@ -183,6 +193,39 @@ def owner_colors(owner: np.ndarray) -> np.ndarray:
     col = np.zeros_like(own)
     lib().vo_owner_colors(_p(own, _u32p), own.size, _p(col, _u32p))
     return col
+
+
+def resolve_material(tri: int, table) -> int:
+    t = np.ascontiguousarray(table, np.uint32)
+    return int(lib().vo_resolve_material(tri, _p(t, _u32p), t.size))
+
+
+def compact(words, owner, n, table, g_origin=(0, 0, 0)) -> np.ndarray:
+    """compact_surface_sparse from dense outputs: (count, 4) i32 rows (vx, vy, vz, material),
+    x fastest."""
+    w = np.ascontiguousarray(words, np.uint32)
+    o = np.ascontiguousarray(owner, np.uint32)
+    t = np.ascontiguousarray(table, np.uint32)
+    g = np.ascontiguousarray(g_origin, np.int32)
+    c = lib().vo_compact(_p(w, _u32p), _p(o, _u32p), n, _p(t, _u32p), t.size, _p(g, _i32p), None)
+    out = np.zeros((c, 4), np.int32)
+    lib().vo_compact(_p(w, _u32p), _p(o, _u32p), n, _p(t, _u32p), t.size, _p(g, _i32p),
+                     _p(out, _i32p))
+    return out
+
+
+def tree_from_compact(vox, n, threads=1):
+    """tree_from_compact: (Tree over the in-range voxels, materials (L, 512) u16, dropped)."""
+    v = np.ascontiguousarray(vox, np.int32).reshape(-1, 4)
+    words = np.zeros((n ** 3 + 31) // 32, np.uint32)
+    dropped = lib().vo_compact_to_words(_p(v, _i32p), v.shape[0], n, _p(words, _u32p))
+    tree = Tree.build(words, n, threads=threads)
+    _, _, _, codes = tree.export()
+    codes = np.ascontiguousarray(codes, np.uint64)
+    mats = np.zeros((len(codes), 512), np.uint16)
+    lib().vo_compact_materials(_p(v, _i32p), v.shape[0], n, _p(codes, _u64p), len(codes),
+                               mats.ctypes.data_as(C.POINTER(C.c_uint16)))
+    return tree, mats, int(dropped)
 
 
 def count_candidates(tris, n, origin=(0.0, 0.0, 0.0), voxel_size=1.0, eps=1e-4) -> int:

@@ -258,6 +258,81 @@ void vo_owner_colors(const uint32_t *owner, uint64_t N, uint32_t *color) {
   for (uint64_t i = 0; i < N; i++) color[i] = owner[i] != UINT32_MAX ? vo_hash_color(owner[i]) : 0u;
 }
 
+/* compact_voxels.wgsl:46-66: the material of triangle `tri` from a table of two u16 per
+ * u32 (triangle t in bits (t & 1) * 16 of word t >> 1); 0 (MISSING) for no owner, an empty
+ * table or a triangle past its end. */
+uint32_t vo_resolve_material(uint32_t tri, const uint32_t *table, uint32_t table_len) {
+  if (tri == UINT32_MAX || table_len == 0) return 0;
+  const uint32_t w = tri >> 1;
+  if (w >= table_len) return 0;
+  return (table[w] >> ((tri & 1u) << 4)) & 0xFFFFu;
+}
+
+/* compact_surface_sparse (voxelizer/src/gpu/sparse.rs:61-111, compact_voxels.wgsl) restated
+ * from the dense outputs: every occupied voxel as (g_origin + (x, y, z), material of its
+ * owner), x fastest; returns the count. out (count x 4 i32) may be NULL to count only. */
+uint64_t vo_compact(const uint32_t *words, const uint32_t *owner, uint32_t n,
+                    const uint32_t *table, uint32_t table_len, const int32_t g_origin[3],
+                    int32_t *out) {
+  const uint64_t nn = n, nv = nn * nn * nn;
+  uint64_t c = 0;
+  for (uint64_t lin = 0; lin < nv; lin++) {
+    if (!((words[lin >> 5] >> (lin & 31)) & 1u)) continue;
+    if (out) {
+      const uint32_t x = (uint32_t)(lin % nn), y = (uint32_t)((lin / nn) % nn),
+                     z = (uint32_t)(lin / (nn * nn));
+      out[4 * c + 0] = g_origin[0] + (int32_t)x;
+      out[4 * c + 1] = g_origin[1] + (int32_t)y;
+      out[4 * c + 2] = g_origin[2] + (int32_t)z;
+      out[4 * c + 3] = (int32_t)vo_resolve_material(owner[lin], table, table_len);
+    }
+    c++;
+  }
+  return c;
+}
+
+/* tree_from_compact's filter (materials.rs:153-192): the occupancy words (n^3/32, zeroed
+ * here) of the voxels inside [0, n)^3; returns the number dropped. */
+uint64_t vo_compact_to_words(const int32_t *vox, uint64_t count, uint32_t n, uint32_t *words) {
+  const uint64_t nn = n;
+  memset(words, 0, (size_t)((nn * nn * nn + 31) / 32) * 4);
+  uint64_t dropped = 0;
+  for (uint64_t i = 0; i < count; i++) {
+    const int64_t x = vox[4 * i], y = vox[4 * i + 1], z = vox[4 * i + 2];
+    if (x < 0 || y < 0 || z < 0 || x >= (int64_t)n || y >= (int64_t)n || z >= (int64_t)n) {
+      dropped++;
+      continue;
+    }
+    const uint64_t lin = (uint64_t)x + nn * ((uint64_t)y + nn * (uint64_t)z);
+    words[lin >> 5] |= 1u << (lin & 31);
+  }
+  return dropped;
+}
+
+/* SparseTree::from_voxels materials (sparse.rs:324-366): per leaf (leaf i has brick code
+ * codes[i], ascending) 512 u16 in intra-brick Morton order (morton.rs:86-89), 0 where no
+ * voxel set one; voxels in input order, so a repeated coord keeps the last; the material
+ * is the low 16 bits (materials.rs:174-181). */
+void vo_compact_materials(const int32_t *vox, uint64_t count, uint32_t n, const uint64_t *codes,
+                          uint64_t L, uint16_t *mats) {
+  memset(mats, 0, (size_t)(L * 512) * 2);
+  for (uint64_t i = 0; i < count; i++) {
+    const int64_t x = vox[4 * i], y = vox[4 * i + 1], z = vox[4 * i + 2];
+    if (x < 0 || y < 0 || z < 0 || x >= (int64_t)n || y >= (int64_t)n || z >= (int64_t)n)
+      continue;
+    const uint64_t code = vo_morton_encode((uint32_t)x >> 3, (uint32_t)y >> 3, (uint32_t)z >> 3);
+    uint64_t lo = 0, hi = L;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) / 2;
+      if (codes[mid] < code) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo >= L || codes[lo] != code) continue; /* not a leaf of this tree */
+    const uint32_t m = vo_encode_brick((uint32_t)x & 7u, (uint32_t)y & 7u, (uint32_t)z & 7u);
+    mats[lo * 512 + m] = (uint16_t)((uint32_t)vox[4 * i + 3] & 0xFFFFu);
+  }
+}
+
 /* reference_cpu.rs:110-200 (occupancy + owner; colours from owners: vo_owner_colors) */
 uint64_t vo_voxelize_surface(const float *tris, uint64_t T, uint32_t n, const float origin[3],
                              float vs, float eps, uint32_t *words, uint32_t *owner) {

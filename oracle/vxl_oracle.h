@@ -43,6 +43,15 @@ uint64_t vo_voxelize_surface(const float *tris, uint64_t T, uint32_t n, const fl
 /* hash_color (reference_cpu.rs:91-99) and the colour pass of reference_cpu.rs:173-186. */
 uint32_t vo_hash_color(uint32_t id);
 void vo_owner_colors(const uint32_t *owner, uint64_t N, uint32_t *color);
+/* materials: compact_voxels.wgsl lookup, compact_surface_sparse from dense outputs, and
+ * tree_from_compact / SparseTree::from_voxels (see .c). */
+uint32_t vo_resolve_material(uint32_t tri, const uint32_t *table, uint32_t table_len);
+uint64_t vo_compact(const uint32_t *words, const uint32_t *owner, uint32_t n,
+                    const uint32_t *table, uint32_t table_len, const int32_t g_origin[3],
+                    int32_t *out);
+uint64_t vo_compact_to_words(const int32_t *vox, uint64_t count, uint32_t n, uint32_t *words);
+void vo_compact_materials(const int32_t *vox, uint64_t count, uint32_t n, const uint64_t *codes,
+                          uint64_t L, uint16_t *mats);
 /* Same candidate count without doing the SAT (for roofline accounting). */
 uint64_t vo_count_candidates(const float *tris, uint64_t T, uint32_t n, const float origin[3],
                              float voxel_size, float eps);
