@@ -97,6 +97,61 @@ def bench_voxelize(ctx, torch, W, steps, warmup):
                 setup_ms=mean(setup_ms), build_ms=mean(build_ms), **info), tris
 
 
+def bench_materials(ctx, torch, W, plain_ms, reps=5):
+    """The material path on terrain_1024 with a random per-triangle material table:
+    voxelize + materials into a tree (device input, HIP-event time of the whole call), its
+    compact voxel list, and tree_from_compact of that list (host arrays, PCIe-inclusive);
+    checks that the trees equal the plain voxelize tree and that the round trip keeps every
+    material."""
+    import numpy as np
+    tris = W.terrain_1024()
+    T = tris.shape[0]
+    n, vs = W.grid_for(1024)
+    table = np.random.default_rng(0x5EED_0009).integers(0, 2 ** 32, (T + 1) // 2,
+                                                         dtype=np.uint64).astype(np.uint32)
+    d = torch.from_numpy(tris).to("cuda")
+    dt = torch.from_numpy(table.view(np.int32)).to("cuda")
+    torch.cuda.synchronize()
+    log(f"materials: T={T} n={n}")
+    ctx.set_timing(True)
+    ctx.voxelize_tree_materials_dev(d.data_ptr(), T, n, (0, 0, 0), vs, 1e-4, dt.data_ptr(),
+                                    table.size).free()
+    ms = []
+    for _ in range(reps):
+        tr = ctx.voxelize_tree_materials_dev(d.data_ptr(), T, n, (0, 0, 0), vs, 1e-4,
+                                             dt.data_ptr(), table.size)
+        ms.append(ctx.last_timing()[6])
+        if _ < reps - 1:
+            tr.free()
+    ctx.set_timing(False)
+    plain = ctx.voxelize_tree_dev(d.data_ptr(), T, n, (0, 0, 0), vs)
+    same_tree = all(np.array_equal(a, b) for a, b in zip(tr.download(), plain.download()))
+    mats = tr.materials()
+    t0 = time.perf_counter()
+    cv = ctx.compact_surface(tris, n, (0, 0, 0), vs, 1e-4, table)
+    compact_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    back, dropped = ctx.tree_from_compact(cv, n)
+    tfc_s = time.perf_counter() - t0
+    round_trip = (dropped == 0 and np.array_equal(back.materials(), mats) and
+                  all(np.array_equal(a, b) for a, b in zip(back.download(), plain.download())))
+    mat_ms = sum(ms) / len(ms)
+    out = dict(workload="terrain_1024 with a random 16-bit material per triangle",
+               voxelize_materials_ms=round(mat_ms, 4),
+               voxelize_materials_mtris_per_s=round(T / mat_ms / 1e3, 1),
+               plain_voxelize_ms=round(plain_ms, 4),
+               overhead=round(mat_ms / plain_ms, 3), voxels=int(cv.shape[0]),
+               compact_host_ms=round(compact_s * 1e3, 2),
+               tree_from_compact_host_ms=round(tfc_s * 1e3, 2),
+               tree_from_compact_mvoxels_per_s=round(cv.shape[0] / tfc_s / 1e6, 1),
+               identical_tree=bool(same_tree), round_trip_identical=bool(round_trip),
+               note="the owner pass re-runs the SAT with owners kept per leaf voxel; host-array "
+                    "entry points include their PCIe copies")
+    for t in (tr, plain, back):
+        t.free()
+    return out
+
+
 def bench_dda(ctx, torch, W, reps=5):
     import numpy as np
     out = {}
@@ -393,6 +448,7 @@ def main():
     ap.add_argument("--no-dda", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-materials", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
@@ -451,6 +507,8 @@ def main():
                                   "every product, so an FMA would change results, and packed FP32 is not "
                                   "used; the 157.3 TFLOP/s FP32 vector figure counts a packed FMA as 4"},
     }
+    if not args.no_materials:
+        line["materials"] = bench_materials(ctx, torch, W, line["ms_per_step"])
     if not args.no_dda:
         dda = bench_dda(ctx, torch, W)
         for name, rec in dda.items():

@@ -631,6 +631,34 @@ vxl_status vxl_voxelize_tree_materials(vxl_ctx *ctx, const float *tris, uint64_t
   });
 }
 
+vxl_status vxl_voxelize_tree_materials_dev(vxl_ctx *ctx, const float *d_tris, uint64_t T,
+                                           const vxl_grid *grid, float epsilon,
+                                           const uint32_t *d_table, uint32_t table_len,
+                                           vxl_tree **out, uint64_t *pairs_out, void *stream) {
+  return guarded(ctx, [&] {
+    if (!out) throw Error(VXL_INVALID_ARG, "out is null");
+    *out = nullptr;
+    uint32_t k = 0;
+    validate_grid(grid, &k);
+    if (T && !d_tris) throw Error(VXL_INVALID_ARG, "triangles pointer is null");
+    validate_count(T);
+    validate_eps(epsilon);
+    if (table_len && !d_table) throw Error(VXL_INVALID_ARG, "material table is null");
+    hipStream_t user = static_cast<hipStream_t>(stream);
+    if (user && user != ctx->stream) VXL_HIP(hipStreamSynchronize(user));
+    if (ctx->timing) vxl::mark(ctx, 0);
+    *out = vxl::voxelize_tree_materials(ctx, d_tris, T, grid->n, k, make_params(grid, epsilon),
+                                        d_table, table_len, pairs_out);
+    ctx->trees.insert(*out);
+    if (ctx->timing) {
+      vxl::mark(ctx, 3);
+      VXL_HIP(hipEventSynchronize(ctx->ev[3]));
+      VXL_HIP(hipEventElapsedTime(&ctx->last_ms[6], ctx->ev[0], ctx->ev[3]));
+    }
+    if (user && user != ctx->stream) VXL_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
 vxl_status vxl_compact_surface(vxl_ctx *ctx, const float *tris, uint64_t T, const vxl_grid *grid,
                                float epsilon, const uint32_t *material_table, uint32_t table_len,
                                const int32_t g_origin[3], vxl_compact_voxel *out,

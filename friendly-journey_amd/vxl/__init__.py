@@ -34,8 +34,8 @@ EXPORTS = [
     "vxl_ctx_create", "vxl_ctx_destroy", "vxl_ctx_stream", "vxl_last_error", "vxl_status_str",
     "vxl_voxelize_dense", "vxl_voxelize_dense_dev", "vxl_voxelize_dense_owner",
     "vxl_voxelize_dense_owner_dev", "vxl_voxelize_tree", "vxl_voxelize_tree_dev",
-    "vxl_voxelize_tree_materials", "vxl_compact_surface", "vxl_tree_from_compact",
-    "vxl_tree_materials_download",
+    "vxl_voxelize_tree_materials", "vxl_voxelize_tree_materials_dev", "vxl_compact_surface",
+    "vxl_tree_from_compact", "vxl_tree_materials_download",
     "vxl_tree_from_words", "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload",
     "vxl_tree_free", "vxl_traverse", "vxl_traverse_dev", "vxl_set_timing", "vxl_last_timing",
     "vxl_set_tuning", "vxl_set_dda_refill", "vxl_voxelize_batch", "vxl_voxelize_batch_dev", "vxl_traverse_batch",
@@ -116,6 +116,8 @@ def load(path: str = LIB_PATH):
                                                _vp, _vp, _pu64, _vp]
     L.vxl_voxelize_tree_materials.argtypes = [_vp, _vp, _u64, C.POINTER(Grid), C.c_float, _vp,
                                               C.c_uint32, C.POINTER(_vp), _pu64]
+    L.vxl_voxelize_tree_materials_dev.argtypes = [_vp, _vp, _u64, C.POINTER(Grid), C.c_float,
+                                                  _vp, C.c_uint32, C.POINTER(_vp), _pu64, _vp]
     L.vxl_compact_surface.argtypes = [_vp, _vp, _u64, C.POINTER(Grid), C.c_float, _vp,
                                       C.c_uint32, _vp, _vp, _u64, _pu64]
     L.vxl_tree_from_compact.argtypes = [_vp, C.c_uint32, _vp, _u64, C.POINTER(_vp), _pu64]
@@ -162,8 +164,8 @@ def load(path: str = LIB_PATH):
         getattr(L, name).restype = C.c_int
     for name in ("vxl_ctx_create", "vxl_voxelize_dense", "vxl_voxelize_dense_dev",
                  "vxl_voxelize_dense_owner", "vxl_voxelize_dense_owner_dev",
-                 "vxl_voxelize_tree_materials", "vxl_compact_surface", "vxl_tree_from_compact",
-                 "vxl_tree_materials_download",
+                 "vxl_voxelize_tree_materials", "vxl_voxelize_tree_materials_dev",
+                 "vxl_compact_surface", "vxl_tree_from_compact", "vxl_tree_materials_download",
                  "vxl_voxelize_tree", "vxl_voxelize_tree_dev", "vxl_tree_from_words",
                  "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload", "vxl_traverse",
                  "vxl_traverse_dev", "vxl_set_tuning", "vxl_set_dda_refill", "vxl_voxelize_batch",
@@ -326,6 +328,19 @@ class Context:
         _check(load().vxl_voxelize_tree_materials(self.handle, _ptr(t), t.shape[0], C.byref(g),
                                                   eps, _ptr(tb) if tb.size else None, tb.size,
                                                   C.byref(out), C.byref(pairs)))
+        tree = Tree(out.value, self)
+        tree.pairs = int(pairs.value)
+        return tree
+
+    def voxelize_tree_materials_dev(self, d_tris: int, T: int, n, origin=(0.0, 0.0, 0.0),
+                                    voxel_size=1.0, eps=1e-4, d_table: int = 0,
+                                    table_len: int = 0, stream: int = 0):
+        out = _vp()
+        pairs = C.c_uint64(0)
+        g = make_grid(n, origin, voxel_size)
+        _check(load().vxl_voxelize_tree_materials_dev(
+            self.handle, _vp(d_tris), T, C.byref(g), eps, _vp(d_table) if d_table else None,
+            table_len, C.byref(out), C.byref(pairs), _vp(stream)))
         tree = Tree(out.value, self)
         tree.pairs = int(pairs.value)
         return tree

@@ -127,6 +127,11 @@ vxl_status vxl_voxelize_tree_materials(vxl_ctx *ctx, const float *tris, uint64_t
                                        const vxl_grid *grid, float epsilon,
                                        const uint32_t *material_table, uint32_t table_len,
                                        vxl_tree **out, uint64_t *pairs_out);
+/* the same with device triangles and device table; the tree is ready when it returns. */
+vxl_status vxl_voxelize_tree_materials_dev(vxl_ctx *ctx, const float *d_tris, uint64_t T,
+                                           const vxl_grid *grid, float epsilon,
+                                           const uint32_t *d_table, uint32_t table_len,
+                                           vxl_tree **out, uint64_t *pairs_out, void *stream);
 /* compact_surface_sparse: every occupied voxel with its material, in leaf (brick Morton)
  * order then intra-brick Morton order. *count receives the number of voxels; out may be
  * NULL to ask for it; a non-NULL out with capacity < *count is VXL_STORAGE_EXCEEDED. */
@@ -273,7 +278,7 @@ vxl_status vxl_server_reset_stats(vxl_server *server);
 /* ---- timing: when enabled, the last voxelize/traverse call records per-stage
  * device times (ms) measured with HIP events on the stream the kernels ran on.
  * stage 0 = setup+scan, 1 = SAT, 2 = tree build, 3 = traverse, 4 = whole voxelize
- * batch, 5 = traverse batch. */
+ * batch, 5 = traverse batch, 6 = whole vxl_voxelize_tree_materials_dev. */
 void vxl_set_timing(vxl_ctx *ctx, int enable);
 void vxl_last_timing(vxl_ctx *ctx, float out_ms[8]);
 
