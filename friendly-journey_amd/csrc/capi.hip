@@ -728,6 +728,56 @@ vxl_status vxl_tree_materials_download(const vxl_tree *tree, uint16_t *materials
   });
 }
 
+static void validate_camera(const vxl_tree *tree, const vxl_camera *cam) {
+  if (!tree) throw Error(VXL_INVALID_ARG, "tree is null");
+  if (!cam) throw Error(VXL_INVALID_ARG, "camera is null");
+  if (cam->n != float(tree->n))
+    throw Error(VXL_INVALID_ARG, "camera n must equal the tree's grid size");
+  if (cam->width > 65536 || cam->height > 65536)
+    throw Error(VXL_INVALID_ARG, "image larger than 65536 pixels on a side");
+}
+
+vxl_status vxl_render_dev(vxl_ctx *ctx, const vxl_tree *tree, const vxl_camera *camera,
+                          const uint32_t *d_colors, uint32_t ncolors, uint32_t *d_rgba,
+                          void *stream) {
+  return guarded(ctx, [&] {
+    validate_camera(tree, camera);
+    if (ncolors && !d_colors) throw Error(VXL_INVALID_ARG, "colour table is null");
+    if (uint64_t(camera->width) * camera->height && !d_rgba)
+      throw Error(VXL_INVALID_ARG, "image buffer is null");
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    if (ctx->timing) VXL_HIP(hipEventRecord(ctx->ev[4], s));
+    vxl::launch_render(ctx, tree, *camera, d_colors, ncolors, d_rgba, s);
+    if (ctx->timing) {
+      VXL_HIP(hipEventRecord(ctx->ev[5], s));
+      VXL_HIP(hipEventSynchronize(ctx->ev[5]));
+      VXL_HIP(hipEventElapsedTime(&ctx->last_ms[7], ctx->ev[4], ctx->ev[5]));
+    }
+  });
+}
+
+vxl_status vxl_render(vxl_ctx *ctx, const vxl_tree *tree, const vxl_camera *camera,
+                      const uint32_t *colors, uint32_t ncolors, uint32_t *rgba) {
+  return guarded(ctx, [&] {
+    validate_camera(tree, camera);
+    if (ncolors && !colors) throw Error(VXL_INVALID_ARG, "colour table is null");
+    const uint64_t P = uint64_t(camera->width) * camera->height;
+    if (P && !rgba) throw Error(VXL_INVALID_ARG, "image buffer is null");
+    if (!P) return;
+    ctx->hits_io.reserve(P * 4 + uint64_t(ncolors) * 4 + 256);
+    uint32_t *d_img = ctx->hits_io.as<uint32_t>();
+    uint32_t *d_col = nullptr;
+    if (ncolors) {
+      d_col = d_img + ((P + 63) / 64) * 64;
+      VXL_HIP(hipMemcpyAsync(d_col, colors, uint64_t(ncolors) * 4, hipMemcpyHostToDevice,
+                             ctx->stream));
+    }
+    vxl::launch_render(ctx, tree, *camera, d_col, ncolors, d_img, ctx->stream);
+    VXL_HIP(hipMemcpyAsync(rgba, d_img, P * 4, hipMemcpyDeviceToHost, ctx->stream));
+    VXL_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
 vxl_status vxl_tree_from_words(vxl_ctx *ctx, const uint32_t *words, uint32_t n, vxl_tree **out) {
   return guarded(ctx, [&] {
     if (!out) throw Error(VXL_INVALID_ARG, "out is null");

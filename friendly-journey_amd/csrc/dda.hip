@@ -460,6 +460,79 @@ __global__ __launch_bounds__(64) void k_traverse_pl(
   }
 }
 
+// ---- device ray generation + shading (render.wgsl:54-91) ----------------------------------
+
+__device__ __forceinline__ uint32_t unorm8(float v) {  // rgba8unorm: clamp, scale, round
+  v = v > 0.0f ? v : 0.0f;  // NaN -> 0
+  v = v < 1.0f ? v : 1.0f;
+  return uint32_t(rintf(v * 255.0f));
+}
+
+__device__ __forceinline__ uint32_t rgba8(float r, float g, float b, float a) {
+  return unorm8(r) | (unorm8(g) << 8) | (unorm8(b) << 16) | (unorm8(a) << 24);
+}
+
+// One wave per 8 x 8 pixel tile (neighbouring pixels walk the same cells). Per pixel the
+// camera ray of render.wgsl:66-70 in binary32 (normalize(v) as v / sqrt(v.v), the dot
+// product summed (x x + y y) + z z), the walk of k_traverse, and the shading of
+// render.wgsl:74-89: a hit takes its voxel's material (0 without materials), shaded by
+// position (voxel / n) for material 0 or by colors[material] (RGBA8, R in the low byte;
+// an id past the table reads 0); a miss takes the sky gradient. Output RGBA8 per pixel.
+__global__ __launch_bounds__(64) void k_render(vxl_camera cam, const uint32_t *__restrict__ nodes,
+                                               const uint32_t *__restrict__ leafw,
+                                               const uint32_t *__restrict__ bounds, uint32_t n,
+                                               uint32_t k, uint64_t leaf_count,
+                                               const uint16_t *__restrict__ mats,
+                                               const uint32_t *__restrict__ colors,
+                                               uint32_t ncolors, uint32_t *__restrict__ out) {
+  extern __shared__ uint32_t frames_lds[];
+  const uint32_t px = blockIdx.x * 8 + (threadIdx.x & 7), py = blockIdx.y * 8 + (threadIdx.x >> 3);
+  if (px >= cam.width || py >= cam.height) return;
+  const float w = float(cam.width), h = float(cam.height);
+  const float ndc_x = ((float(px) + 0.5f) / w * 2.0f - 1.0f) * cam.tan_half_fov * cam.aspect;
+  const float ndc_y = (1.0f - (float(py) + 0.5f) / h * 2.0f) * cam.tan_half_fov;
+  Ray r;
+  {
+    float v[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) v[a] = (cam.forward[a] + cam.right[a] * ndc_x) + cam.up[a] * ndc_y;
+    const float len = __fsqrt_rn((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      r.o[a] = cam.eye[a];
+      r.d[a] = v[a] / len;
+      r.rinv[a] = 1.0f / r.d[a];
+      r.big[a] = !(fabsf(r.d[a]) < 0x1p100f);
+    }
+  }
+  Walk wk;
+  wk.ox = wk.oy = wk.oz = wk.cx = wk.cy = wk.cz = 0;
+  uint32_t status = kLeft;
+  float t0;
+  if (clip_grid(r, n, leaf_count, t0)) {
+    begin_walk(r, wk, k, t0, nodes);
+    do {
+      status = walk_visit(r, wk, nodes, leafw, bounds, frames_lds + threadIdx.x, 64);
+    } while (status == kWalking);
+  }
+  uint32_t color;
+  if (status == kHit) {
+    const uint32_t mat = mats ? uint32_t(mats[uint64_t(wk.node) * 512 +
+                                              dev::morton9(wk.cx, wk.cy, wk.cz)])
+                              : 0u;
+    if (mat == 0) {
+      color = rgba8(float(wk.ox + wk.cx) / cam.n, float(wk.oy + wk.cy) / cam.n,
+                    float(wk.oz + wk.cz) / cam.n, 1.0f);
+    } else {
+      color = mat < ncolors ? colors[mat] : 0u;
+    }
+  } else {
+    const float t = float(py) / h;
+    color = rgba8(0.08f * (1.0f - t), 0.10f * (1.0f - t), 0.16f + 0.12f * t, 1.0f);
+  }
+  out[uint64_t(py) * cam.width + px] = color;
+}
+
 // One segment of a batched traversal, as the kernel sees it.
 struct TraverseSeg {
   const uint32_t *nodes, *leafw, *bounds;
@@ -545,6 +618,16 @@ void launch_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, ui
                      d_rays, R, tree->nodes,
                      reinterpret_cast<const uint32_t *>(tree->leaves), tree->bounds, tree->n,
                      tree->k, tree->leaf_count, reinterpret_cast<uint4 *>(d_hits), d_t);
+  VXL_HIP(hipGetLastError());
+}
+
+void launch_render(vxl_ctx *ctx, const vxl_tree *tree, const vxl_camera &cam,
+                   const uint32_t *d_colors, uint32_t ncolors, uint32_t *d_rgba, hipStream_t s) {
+  if (cam.width == 0 || cam.height == 0) return;
+  const dim3 grid((cam.width + 7) / 8, (cam.height + 7) / 8, 1);
+  hipLaunchKernelGGL(k_render, grid, dim3(64), frame_lds_bytes(tree->k, 64), s, cam, tree->nodes,
+                     reinterpret_cast<const uint32_t *>(tree->leaves), tree->bounds, tree->n,
+                     tree->k, tree->leaf_count, tree->materials, d_colors, ncolors, d_rgba);
   VXL_HIP(hipGetLastError());
 }
 

@@ -294,6 +294,9 @@ struct RaySegment {
 };
 // All segments in one kernel launch on stream s.
 void launch_traverse_batch(vxl_ctx *ctx, const RaySegment *segs, uint32_t S, hipStream_t s);
+// dda.hip: one RGBA8 image of the tree (render.wgsl); d_colors: the global colour table.
+void launch_render(vxl_ctx *ctx, const vxl_tree *tree, const vxl_camera &cam,
+                   const uint32_t *d_colors, uint32_t ncolors, uint32_t *d_rgba, hipStream_t s);
 
 // capi.hip: argument checks of a voxelize call (grid, triangle count, epsilon; returns k)
 // and the world-to-grid parameters.

@@ -35,7 +35,7 @@ EXPORTS = [
     "vxl_voxelize_dense", "vxl_voxelize_dense_dev", "vxl_voxelize_dense_owner",
     "vxl_voxelize_dense_owner_dev", "vxl_voxelize_tree", "vxl_voxelize_tree_dev",
     "vxl_voxelize_tree_materials", "vxl_voxelize_tree_materials_dev", "vxl_compact_surface",
-    "vxl_tree_from_compact", "vxl_tree_materials_download",
+    "vxl_tree_from_compact", "vxl_tree_materials_download", "vxl_render", "vxl_render_dev",
     "vxl_tree_from_words", "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload",
     "vxl_tree_free", "vxl_traverse", "vxl_traverse_dev", "vxl_set_timing", "vxl_last_timing",
     "vxl_set_tuning", "vxl_set_dda_refill", "vxl_voxelize_batch", "vxl_voxelize_batch_dev", "vxl_traverse_batch",
@@ -118,6 +118,8 @@ def load(path: str = LIB_PATH):
                                               C.c_uint32, C.POINTER(_vp), _pu64]
     L.vxl_voxelize_tree_materials_dev.argtypes = [_vp, _vp, _u64, C.POINTER(Grid), C.c_float,
                                                   _vp, C.c_uint32, C.POINTER(_vp), _pu64, _vp]
+    L.vxl_render.argtypes = [_vp, _vp, C.POINTER(Camera), _vp, C.c_uint32, _vp]
+    L.vxl_render_dev.argtypes = [_vp, _vp, C.POINTER(Camera), _vp, C.c_uint32, _vp, _vp]
     L.vxl_compact_surface.argtypes = [_vp, _vp, _u64, C.POINTER(Grid), C.c_float, _vp,
                                       C.c_uint32, _vp, _vp, _u64, _pu64]
     L.vxl_tree_from_compact.argtypes = [_vp, C.c_uint32, _vp, _u64, C.POINTER(_vp), _pu64]
@@ -166,6 +168,7 @@ def load(path: str = LIB_PATH):
                  "vxl_voxelize_dense_owner", "vxl_voxelize_dense_owner_dev",
                  "vxl_voxelize_tree_materials", "vxl_voxelize_tree_materials_dev",
                  "vxl_compact_surface", "vxl_tree_from_compact", "vxl_tree_materials_download",
+                 "vxl_render", "vxl_render_dev",
                  "vxl_voxelize_tree", "vxl_voxelize_tree_dev", "vxl_tree_from_words",
                  "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload", "vxl_traverse",
                  "vxl_traverse_dev", "vxl_set_tuning", "vxl_set_dda_refill", "vxl_voxelize_batch",
@@ -179,6 +182,30 @@ def load(path: str = LIB_PATH):
 def _check(rc: int):
     if rc != 0:
         raise VxlError(rc, (load().vxl_last_error() or b"").decode())
+
+
+class Camera(C.Structure):
+    """vxl_camera: render.wgsl's Camera (eye in grid units, tan of half the vertical field
+    of view, unit basis, aspect, grid size n) plus the image size."""
+    _fields_ = [("eye", C.c_float * 3), ("tan_half_fov", C.c_float), ("forward", C.c_float * 3),
+                ("aspect", C.c_float), ("right", C.c_float * 3), ("n", C.c_float),
+                ("up", C.c_float * 3), ("pad0", C.c_float), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("pad1", C.c_uint32), ("pad2", C.c_uint32)]
+
+
+def make_camera(cam16, width: int, height: int) -> Camera:
+    """From 16 f32 (eye, tan, forward, aspect, right, n, up, pad) and the image size."""
+    c = np.ascontiguousarray(cam16, np.float32)
+    cam = Camera()
+    cam.eye[:] = c[0:3].tolist()
+    cam.tan_half_fov = float(c[3])
+    cam.forward[:] = c[4:7].tolist()
+    cam.aspect = float(c[7])
+    cam.right[:] = c[8:11].tolist()
+    cam.n = float(c[11])
+    cam.up[:] = c[12:15].tolist()
+    cam.width, cam.height = width, height
+    return cam
 
 
 def _ptr(a: np.ndarray):
@@ -360,6 +387,22 @@ class Context:
         if out.shape[0]:
             _check(load().vxl_compact_surface(*args, _ptr(out), out.shape[0], C.byref(count)))
         return out
+
+    def render(self, tree: "Tree", cam16, width: int, height: int, colors=()):
+        """render_main: (height, width) u32 RGBA8 (R in the low byte)."""
+        cam = make_camera(cam16, width, height)
+        col = np.ascontiguousarray(colors, np.uint32)
+        img = np.zeros(height * width, np.uint32)
+        _check(load().vxl_render(self.handle, tree.handle, C.byref(cam),
+                                 _ptr(col) if col.size else None, col.size, _ptr(img)))
+        return img.reshape(height, width)
+
+    def render_dev(self, tree: "Tree", cam16, width: int, height: int, d_rgba: int,
+                   d_colors: int = 0, ncolors: int = 0, stream: int = 0):
+        cam = make_camera(cam16, width, height)
+        _check(load().vxl_render_dev(self.handle, tree.handle, C.byref(cam),
+                                     _vp(d_colors) if d_colors else None, ncolors, _vp(d_rgba),
+                                     _vp(stream)))
 
     def tree_from_compact(self, voxels, n):
         """tree_from_compact: (Tree with materials, number of voxels dropped)."""

@@ -184,6 +184,28 @@ vxl_status vxl_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *rays, u
 vxl_status vxl_traverse_dev(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, uint64_t R,
                             uint32_t *d_hits, double *d_t, void *stream);
 
+/* ---- device ray generation + shading: the render path (GpuRenderer,
+ *      voxel-gpu/src/render.rs:211-589, shaders/render.wgsl:54-91), one invocation per pixel
+ *      with no host rays and no readback of hits. Camera layout = render.wgsl's Camera: eye
+ *      in grid units, tan of half the vertical field of view, unit forward / right / up,
+ *      aspect = width / height, n = the tree's grid size. d_colors: the global colour table
+ *      (MaterialTable words, RGBA8 with R in the low byte), indexed by a voxel's material;
+ *      material 0 (or a tree without materials) is shaded by position. d_rgba: width x height
+ *      u32, row-major from the top row. */
+typedef struct vxl_camera {
+  float eye[3], tan_half_fov;
+  float forward[3], aspect;
+  float right[3], n;
+  float up[3], pad0;
+  uint32_t width, height, pad1, pad2;
+} vxl_camera;
+vxl_status vxl_render_dev(vxl_ctx *ctx, const vxl_tree *tree, const vxl_camera *camera,
+                          const uint32_t *d_colors, uint32_t ncolors, uint32_t *d_rgba,
+                          void *stream);
+/* the same with a host colour table and a host image (copied back; synchronous). */
+vxl_status vxl_render(vxl_ctx *ctx, const vxl_tree *tree, const vxl_camera *camera,
+                      const uint32_t *colors, uint32_t ncolors, uint32_t *rgba);
+
 /* ---- batches (no single reference counterpart: each replaces a host loop of the calls
  *      above, e.g. the per-mesh GpuVoxelizer::voxelize_surface -> to_school_b ->
  *      GpuTraverser::new/traverse sequence of voxel-cli/src/main.rs:1288-1300 and
@@ -278,7 +300,7 @@ vxl_status vxl_server_reset_stats(vxl_server *server);
 /* ---- timing: when enabled, the last voxelize/traverse call records per-stage
  * device times (ms) measured with HIP events on the stream the kernels ran on.
  * stage 0 = setup+scan, 1 = SAT, 2 = tree build, 3 = traverse, 4 = whole voxelize
- * batch, 5 = traverse batch, 6 = whole vxl_voxelize_tree_materials_dev. */
+ * batch, 5 = traverse batch, 6 = whole vxl_voxelize_tree_materials_dev, 7 = render. */
 void vxl_set_timing(vxl_ctx *ctx, int enable);
 void vxl_last_timing(vxl_ctx *ctx, float out_ms[8]);
 

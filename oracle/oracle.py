@@ -69,6 +69,11 @@ def lib():
         L.vo_compact_materials.restype = None
         L.vo_compact_materials.argtypes = [_i32p, C.c_uint64, C.c_uint32, _u64p, C.c_uint64,
                                            C.POINTER(C.c_uint16)]
+        L.vo_camera_ray.restype = None
+        L.vo_camera_ray.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _f32p]
+        L.vo_render.restype = None
+        L.vo_render.argtypes = [C.c_void_p, _f32p, C.c_uint32, C.c_uint32,
+                                C.POINTER(C.c_uint16), _u32p, C.c_uint32, _u32p, C.c_int]
         L.vo_count_candidates.restype = C.c_uint64
         L.vo_count_candidates.argtypes = [_f32p, C.c_uint64, C.c_uint32, _f32p, C.c_float,
                                           C.c_float]
@@ -226,6 +231,35 @@ def tree_from_compact(vox, n, threads=1):
     lib().vo_compact_materials(_p(v, _i32p), v.shape[0], n, _p(codes, _u64p), len(codes),
                                mats.ctypes.data_as(C.POINTER(C.c_uint16)))
     return tree, mats, int(dropped)
+
+
+def camera(eye, forward, right, up, tan, aspect, n) -> np.ndarray:
+    """render.wgsl's Camera as 16 f32: eye, tan, forward, aspect, right, n, up, pad."""
+    c = np.zeros(16, np.float32)
+    c[0:3], c[3], c[4:7], c[7], c[8:11], c[11], c[12:15] = eye, tan, forward, aspect, right, n, up
+    return c
+
+
+def camera_rays(cam, width, height) -> np.ndarray:
+    c = np.ascontiguousarray(cam, np.float32)
+    out = np.zeros((height * width, 6), np.float32)
+    for y in range(height):
+        for x in range(width):
+            lib().vo_camera_ray(_p(c, _f32p), width, height, x, y,
+                                _p(out[y * width + x], _f32p))
+    return out
+
+
+def render(tree, cam, width, height, mats=None, colors=None) -> np.ndarray:
+    """render_main per pixel: (height, width) u32 RGBA8 (R in the low byte)."""
+    c = np.ascontiguousarray(cam, np.float32)
+    out = np.zeros(height * width, np.uint32)
+    m = None if mats is None else np.ascontiguousarray(mats, np.uint16)
+    col = np.ascontiguousarray(colors if colors is not None else [], np.uint32)
+    lib().vo_render(tree._h, _p(c, _f32p), width, height,
+                    None if m is None else m.ctypes.data_as(C.POINTER(C.c_uint16)),
+                    _p(col, _u32p), col.size, _p(out, _u32p), _threads())
+    return out.reshape(height, width)
 
 
 def count_candidates(tris, n, origin=(0.0, 0.0, 0.0), voxel_size=1.0, eps=1e-4) -> int:

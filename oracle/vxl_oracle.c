@@ -917,6 +917,80 @@ void vo_mirror_traverse(const vo_tree *t, const float *rays, uint64_t R, uint32_
 }
 
 /* ------------------------------------------------------------------------- */
+/* render.wgsl:54-91 -- camera rays, traversal and shading per pixel            */
+/* ------------------------------------------------------------------------- */
+static uint32_t unorm8(float v) { /* rgba8unorm store: clamp, scale, round to nearest */
+  if (!(v > 0.0f)) v = 0.0f;
+  if (v > 1.0f) v = 1.0f;
+  return (uint32_t)rintf(v * 255.0f);
+}
+
+static uint32_t rgba8(float r, float g, float b, float a) {
+  return unorm8(r) | (unorm8(g) << 8) | (unorm8(b) << 16) | (unorm8(a) << 24);
+}
+
+/* The pixel's camera ray (render.wgsl:66-70) in binary32; normalize(v) is v / sqrt(v.v)
+ * with the dot product summed (x x + y y) + z z. */
+void vo_camera_ray(const float cam[16], uint32_t width, uint32_t height, uint32_t px,
+                   uint32_t py, float ray[6]) {
+  const float *eye = cam, tanf_ = cam[3], *fwd = cam + 4, aspect = cam[7], *right = cam + 8,
+              *up = cam + 12;
+  const float w = (float)width, h = (float)height;
+  const float ndc_x = (((float)px + 0.5f) / w * 2.0f - 1.0f) * tanf_ * aspect;
+  const float ndc_y = (1.0f - ((float)py + 0.5f) / h * 2.0f) * tanf_;
+  float v[3];
+  for (int a = 0; a < 3; a++) v[a] = (fwd[a] + right[a] * ndc_x) + up[a] * ndc_y;
+  const float len = sqrtf((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+  for (int a = 0; a < 3; a++) {
+    ray[a] = eye[a];
+    ray[3 + a] = v[a] / len;
+  }
+}
+
+/* render_main (render.wgsl:57-91): per pixel the camera ray walked as the mirror walks it;
+ * a hit takes its material (mats: L x 512 u16 in the leaf order of t, or NULL = none) and
+ * is shaded by position (material 0: voxel / n) or by the colour table (RGBA8, R in the
+ * low byte; an id past the table reads 0); a miss takes the sky gradient. */
+void vo_render(const vo_tree *t, const float cam[16], uint32_t width, uint32_t height,
+               const uint16_t *mats, const uint32_t *colors, uint32_t ncolors, uint32_t *out,
+               int threads) {
+  const uint64_t P = (uint64_t)width * height;
+  float *rays = (float *)malloc((size_t)P * 24);
+  uint32_t *hits = (uint32_t *)malloc((size_t)P * 16);
+  for (uint32_t y = 0; y < height; y++)
+    for (uint32_t x = 0; x < width; x++) vo_camera_ray(cam, width, height, x, y, rays + 6 * ((uint64_t)y * width + x));
+  vo_mirror_traverse(t, rays, P, hits, 1, NULL, threads);
+  const float camn = cam[11];
+  for (uint64_t i = 0; i < P; i++) {
+    const uint32_t y = (uint32_t)(i / width);
+    const uint32_t *hh = hits + 4 * i;
+    if (hh[3] == 1) {
+      uint32_t mat = 0;
+      if (mats && t->codes) {
+        const uint64_t code = vo_morton_encode(hh[0] >> 3, hh[1] >> 3, hh[2] >> 3);
+        uint64_t lo = 0, hi = t->nleaves;
+        while (lo < hi) {
+          const uint64_t mid = (lo + hi) / 2;
+          if (t->codes[mid] < code) lo = mid + 1;
+          else hi = mid;
+        }
+        if (lo < t->nleaves && t->codes[lo] == code)
+          mat = mats[lo * 512 + vo_encode_brick(hh[0] & 7u, hh[1] & 7u, hh[2] & 7u)];
+      }
+      if (mat == 0)
+        out[i] = rgba8((float)hh[0] / camn, (float)hh[1] / camn, (float)hh[2] / camn, 1.0f);
+      else
+        out[i] = mat < ncolors ? colors[mat] : 0u;
+    } else {
+      const float tt = (float)y / (float)height;
+      out[i] = rgba8(0.08f * (1.0f - tt), 0.10f * (1.0f - tt), 0.16f + 0.12f * tt, 1.0f);
+    }
+  }
+  free(rays);
+  free(hits);
+}
+
+/* ------------------------------------------------------------------------- */
 /* ray.rs:55-85, dda.rs:18-131, layout.rs:78-232 -- the f64 reference          */
 /* ------------------------------------------------------------------------- */
 static int ray_aabb(const double o[3], const double d[3], const double lo[3], const double hi[3],

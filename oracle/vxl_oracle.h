@@ -78,6 +78,13 @@ int vo_tree_is_occupied(const vo_tree *t, uint32_t x, uint32_t y, uint32_t z);
  * stats (nullable) R x 3 u32: leaf descents, node descents, cell steps. */
 void vo_mirror_traverse(const vo_tree *t, const float *rays, uint64_t R, uint32_t *hits,
                         int enable_skip, uint32_t *stats, int threads);
+/* render.wgsl:54-91: camera = {eye[3], tan, forward[3], aspect, right[3], n, up[3], pad}. */
+void vo_camera_ray(const float cam[16], uint32_t width, uint32_t height, uint32_t px,
+                   uint32_t py, float ray[6]);
+void vo_render(const vo_tree *t, const float cam[16], uint32_t width, uint32_t height,
+               const uint16_t *mats, const uint32_t *colors, uint32_t ncolors, uint32_t *out,
+               int threads);
+
 /* ---- layout.rs (f64) ---- rays R x 6 f64; school 0 = School-A arrays, 1 = School-B. */
 void vo_layout_traverse(const vo_tree *t, const double *rays, uint64_t R, uint32_t *hits,
                         double *t_enter, int school, int threads);
