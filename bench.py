@@ -184,6 +184,21 @@ def bench_dda(ctx, torch, W, reps=5):
                          lane_schedule=("persistent lanes, refill at 8 idle (tree above the "
                                         "256 MiB Infinity Cache)" if walked > 256 * 2 ** 20
                                         else "one lane per ray (cache-resident tree)"))
+        # render path: the same camera, rays made on the device, one RGBA8 per pixel
+        cam = W.render_camera(m, n)
+        img = torch.empty(1920 * 1080, dtype=torch.int32, device="cuda")
+        ctx.set_timing(True)
+        ctx.render_dev(tree, cam, 1920, 1080, img.data_ptr())
+        rms = []
+        for _ in range(reps):
+            ctx.render_dev(tree, cam, 1920, 1080, img.data_ptr())
+            rms.append(ctx.last_timing()[7])
+        ctx.set_timing(False)
+        r_ms = sum(rms) / len(rms)
+        out[name]["render_1920x1080"] = dict(kernel_ms=round(r_ms, 4),
+                                             mpixels_per_s=round(1920 * 1080 / r_ms / 1e3, 1),
+                                             note="device-generated camera rays, walk and shading; "
+                                                  "no ray or hit buffers")
         out[name]["_tree"] = tree
         out[name]["_rays"] = rays
     return out

@@ -172,11 +172,9 @@ def terrain_1024(seed=0x5EED_0002) -> np.ndarray:
     return np.concatenate([heightfield(1024, seed=seed), soup(100_000, seed=seed + 1)])
 
 
-def camera_rays(m: int, width=1920, height=1080, fov_deg=60.0, radius=1.5,
-                seed=0x5EED_0004) -> np.ndarray:
-    """Pinhole primary rays in voxel space: eye on a seeded point of the sphere of
-    radius 1.5 about the cube centre, looking at it; pixel directions follow the
-    reference's camera (voxel-gpu/shaders/render.wgsl:66-70)."""
+def camera_basis(fov_deg=60.0, radius=1.5, seed=0x5EED_0004):
+    """The bench camera in unit-cube space: eye on a seeded point of the sphere of radius
+    1.5 about the cube centre, looking at it; (eye, forward, right, up, tan(fov/2))."""
     rng = SplitMix64(seed)
     centre = np.array([0.5, 0.5, 0.5])
     eye = centre + radius * rng.sphere_dirs(1)[0]
@@ -186,7 +184,26 @@ def camera_rays(m: int, width=1920, height=1080, fov_deg=60.0, radius=1.5,
     right = np.cross(fwd, up_w)
     right /= np.linalg.norm(right)
     up = np.cross(right, fwd)
-    tan = math.tan(math.radians(fov_deg) / 2)
+    return eye, fwd, right, up, math.tan(math.radians(fov_deg) / 2)
+
+
+def render_camera(m: int, n: int, width=1920, height=1080, fov_deg=60.0, radius=1.5,
+                  seed=0x5EED_0004) -> np.ndarray:
+    """The bench camera as the render path's 16 f32 (eye in voxel units, tan, forward,
+    aspect, right, n, up, pad); the same eye and basis as camera_rays."""
+    eye, fwd, right, up, tan = camera_basis(fov_deg, radius, seed)
+    c = np.zeros(16, np.float32)
+    c[0:3], c[3], c[4:7], c[7] = eye * m, tan, fwd, width / height
+    c[8:11], c[11], c[12:15] = right, n, up
+    return c
+
+
+def camera_rays(m: int, width=1920, height=1080, fov_deg=60.0, radius=1.5,
+                seed=0x5EED_0004) -> np.ndarray:
+    """Pinhole primary rays in voxel space: eye on a seeded point of the sphere of
+    radius 1.5 about the cube centre, looking at it; pixel directions follow the
+    reference's camera (voxel-gpu/shaders/render.wgsl:66-70)."""
+    eye, fwd, right, up, tan = camera_basis(fov_deg, radius, seed)
     aspect = width / height
     px = ((np.arange(width) + 0.5) / width * 2.0 - 1.0) * tan * aspect
     py = (1.0 - (np.arange(height) + 0.5) / height * 2.0) * tan
