@@ -152,6 +152,61 @@ def bench_materials(ctx, torch, W, plain_ms, reps=5):
     return out
 
 
+def brush(center, radius):
+    """The voxels of a solid sphere (sparse.rs brush_voxels: dx^2 + dy^2 + dz^2 <= r^2)."""
+    import numpy as np
+    r = np.arange(-radius, radius + 1)
+    d = np.stack(np.meshgrid(r, r, r, indexing="ij"), -1).reshape(-1, 3)
+    d = d[(d ** 2).sum(1) <= radius * radius]
+    return d + np.asarray(center)
+
+
+def bench_edits(ctx, torch, W, stamps=20):
+    """Incremental edits on the terrain_1024 tree: brush stamps (radius 3, 123 voxels, set)
+    at random occupied bricks -- in-place leaf patches -- and in empty air, where bricks
+    appear and the tree is rebuilt. Host-array entry point: each stamp's time includes its
+    upload and the sync."""
+    import numpy as np
+    tris = W.terrain_1024()
+    n, vs = W.grid_for(1024)
+    d = torch.from_numpy(tris).to("cuda")
+    tree = ctx.voxelize_tree_dev(d.data_ptr(), tris.shape[0], n, (0, 0, 0), vs)
+    _, _, _, codes = tree.download()
+    rng = np.random.default_rng(0x5EED_000A)
+    log(f"edits: {stamps} in-place + {stamps} topology stamps on {tree.leaf_count} leaves")
+    inplace, topo = [], []
+    for c in rng.choice(codes, stamps, replace=False):
+        b = [int(v) for v in np.asarray(W_morton_decode(int(c)))]
+        e = brush([b[0] * 8 + 4, b[1] * 8 + 4, b[2] * 8 + 4], 3)
+        ed = np.concatenate([e, np.ones((len(e), 1), np.int64)], 1)
+        t0 = time.perf_counter()
+        _, tp = ctx.set_voxels(tree, ed)
+        (topo if tp else inplace).append(time.perf_counter() - t0)
+    for i in range(stamps):
+        e = brush([100 + 40 * i, 1000, 1900], 3)
+        ed = np.concatenate([e, np.ones((len(e), 1), np.int64)], 1)
+        t0 = time.perf_counter()
+        _, tp = ctx.set_voxels(tree, ed)
+        (topo if tp else inplace).append(time.perf_counter() - t0)
+    med = lambda v: round(float(np.median(v)) * 1e3, 3) if v else None
+    out = dict(workload="terrain_1024 tree, brush stamps of radius 3 (123 voxels)",
+               leaves=tree.leaf_count, inplace_stamps=len(inplace), inplace_ms_median=med(inplace),
+               topology_stamps=len(topo), topology_ms_median=med(topo),
+               note="in place = leaf words and bounds patched on the device; topology = the "
+                    "tree's arrays rebuilt from its leaves plus the new bricks")
+    tree.free()
+    return out
+
+
+def W_morton_decode(code):
+    """(bx, by, bz) of a brick Morton code (x in the low bit of each triple)."""
+    out = [0, 0, 0]
+    for b in range(21):
+        for a in range(3):
+            out[a] |= ((code >> (3 * b + a)) & 1) << b
+    return out
+
+
 def bench_dda(ctx, torch, W, reps=5):
     import numpy as np
     out = {}
@@ -524,6 +579,7 @@ def main():
     }
     if not args.no_materials:
         line["materials"] = bench_materials(ctx, torch, W, line["ms_per_step"])
+        line["edits"] = bench_edits(ctx, torch, W)
     if not args.no_dda:
         dda = bench_dda(ctx, torch, W)
         for name, rec in dda.items():

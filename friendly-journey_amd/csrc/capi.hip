@@ -1,5 +1,8 @@
 // extern "C" entry points of libvxl.so (declared in include/vxl.h).
+#include <algorithm>
 #include <cmath>
+#include <unordered_map>
+#include <vector>
 #include <cstdio>
 
 #include "vxl_internal.hpp"
@@ -343,7 +346,8 @@ void vxl_ctx_destroy(vxl_ctx *ctx) {
                          &ctx->pool_grid, &ctx->pool_b1, &ctx->batch_scratch, &ctx->batch_l2,
                          &ctx->batch_desc, &ctx->trav_desc,  &ctx->trav_ctr,
                          &ctx->owner_io, &ctx->color_io, &ctx->owner_scratch,
-                         &ctx->leaf_map, &ctx->compact_io};
+                         &ctx->leaf_map, &ctx->compact_io, &ctx->edit_io,
+                         &ctx->edit_scratch};
   for (auto *b : bufs) b->release();
   // Trees outlive nothing they depend on: release their device memory now and detach
   // them; vxl_tree_free on a detached tree only frees the handle.
@@ -725,6 +729,46 @@ vxl_status vxl_tree_materials_download(const vxl_tree *tree, uint16_t *materials
     } else {
       std::memset(materials, 0, bytes);  // a tree built without materials: all MISSING
     }
+  });
+}
+
+vxl_status vxl_tree_set_voxels(vxl_ctx *ctx, vxl_tree *tree, const vxl_voxel_edit *edits,
+                               uint64_t count, uint64_t *changed, int *topology) {
+  return guarded(ctx, [&] {
+    if (!tree) throw Error(VXL_INVALID_ARG, "tree is null");
+    if (tree->ctx != ctx) throw Error(VXL_INVALID_ARG, "tree belongs to another context");
+    if (count && !edits) throw Error(VXL_INVALID_ARG, "edits pointer is null");
+    if (changed) *changed = 0;
+    if (topology) *topology = 0;
+    // in order, last edit per voxel wins; out-of-bounds edits are no-ops (Edit::Unchanged)
+    std::unordered_map<uint64_t, uint64_t> last;
+    last.reserve(size_t(count));
+    const uint64_t nn = tree->n;
+    for (uint64_t i = 0; i < count; i++) {
+      const vxl_voxel_edit &e = edits[i];
+      if (e.x >= nn || e.y >= nn || e.z >= nn) continue;
+      last[uint64_t(e.x) + nn * (uint64_t(e.y) + nn * uint64_t(e.z))] = i;
+    }
+    if (last.empty()) return;
+    std::vector<uint64_t> order;
+    order.reserve(last.size());
+    for (const auto &kv : last) order.push_back(kv.second);
+    std::sort(order.begin(), order.end());
+    const uint64_t E = order.size();
+    vxl::staging_begin(ctx, E * sizeof(vxl_voxel_edit));
+    vxl_voxel_edit *h = static_cast<vxl_voxel_edit *>(vxl::staging_take(ctx, E * sizeof(vxl_voxel_edit)));
+    for (uint64_t i = 0; i < E; i++) {
+      h[i] = edits[order[i]];
+      h[i].occupied = h[i].occupied ? 1u : 0u;
+    }
+    ctx->edit_io.reserve(E * sizeof(vxl_voxel_edit));
+    VXL_HIP(hipMemcpyAsync(ctx->edit_io.ptr, h, E * sizeof(vxl_voxel_edit), hipMemcpyHostToDevice,
+                           ctx->stream));
+    vxl::staging_end(ctx, ctx->stream);
+    const vxl::EditResult r =
+        vxl::edit_tree(ctx, tree, ctx->edit_io.as<vxl_voxel_edit>(), E);
+    if (changed) *changed = r.changed;
+    if (topology) *topology = r.topology ? 1 : 0;
   });
 }
 

@@ -1088,4 +1088,224 @@ vxl_tree *build_tree_from_grid(vxl_ctx *ctx, uint32_t n, uint32_t k, const uint6
   return t;
 }
 
+// ---- incremental edits (SparseTree::set_voxel, voxel-core/src/sparse.rs:384-436) ---------
+namespace {
+
+// LeafBounds of one leaf from its 8 words (leaf.rs:185-213), by one thread: the same marks
+// and counts k_leaf_gather forms across 8 lanes.
+__device__ uint32_t leaf_bounds_of(const unsigned long long *__restrict__ words) {
+  uint32_t cnt = 0, occ = 0;
+#pragma unroll
+  for (int w = 0; w < 8; w++) {
+    const unsigned long long v = words[w];
+    cnt += __popcll(v);
+#pragma unroll
+    for (int a = 0; a < 3; a++) occ |= (word_coords(v, a) << ((w >> a) & 1) * 4) << (8 * a);
+  }
+  uint32_t mn[3], mx[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const uint32_t m = (occ >> (8 * a)) & 0xffu;
+    mn[a] = m ? uint32_t(__ffs(m) - 1) : 7u;
+    mx[a] = m ? 31u - uint32_t(__clz(m)) : 0u;
+  }
+  return (cnt > 64 || cnt == 0) ? kFull
+                                : (mn[0] | (mn[1] << 3) | (mn[2] << 6) | (mx[0] << 9) |
+                                   (mx[1] << 12) | (mx[2] << 15));
+}
+
+// Leaf index of brick `code` among the L ascending codes, or L when absent.
+__device__ __forceinline__ uint64_t find_leaf(const unsigned long long *__restrict__ codes,
+                                              uint64_t L, uint64_t code) {
+  uint64_t lo = 0, hi = L;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (codes[mid] < code) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < L && codes[lo] == code) ? lo : L;
+}
+
+struct EditFlags {
+  unsigned long long changed;  // voxels whose state changed
+  unsigned int topology;       // a brick appears or empties
+  unsigned int pad;
+};
+
+// One thread per (distinct, in-bounds) edit: an edit in an existing brick flips its bit in
+// place (a newly set voxel's material slot reads 0 again, sparse.rs:405-411); a set in a
+// missing brick is left for the rebuild (pending) and marks a topology change.
+__global__ void k_edit_patch(const vxl_voxel_edit *__restrict__ edits, uint64_t E,
+                             unsigned long long *__restrict__ leaves,
+                             const unsigned long long *__restrict__ codes, uint64_t L,
+                             uint16_t *__restrict__ mats, uint32_t *__restrict__ touched,
+                             EditFlags *__restrict__ flags) {
+  const uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  bool changed = false;
+  if (e < E) {
+    const vxl_voxel_edit ed = edits[e];
+    const uint64_t code = dev::brick_code(ed.x >> 3, ed.y >> 3, ed.z >> 3);
+    const uint32_t m = dev::morton9(ed.x, ed.y, ed.z);
+    const uint64_t leaf = find_leaf(codes, L, code);
+    uint32_t t = 0xFFFFFFFFu;
+    if (leaf < L) {
+      unsigned long long *w = leaves + leaf * 8 + (m >> 6);
+      const unsigned long long bit = 1ull << (m & 63);
+      if (ed.occupied) {
+        changed = !(atomicOr(w, bit) & bit);
+        if (changed && mats) mats[leaf * 512 + m] = 0;
+      } else {
+        changed = (atomicAnd(w, ~bit) & bit) != 0;
+      }
+      if (changed) t = uint32_t(leaf);
+    } else if (ed.occupied) {
+      changed = true;
+      t = 0xFFFFFFFEu;  // pending: a new brick
+      atomicOr(&flags->topology, 1u);
+    }
+    touched[e] = t;
+  }
+  const unsigned long long cm = __ballot(changed);
+  if ((threadIdx.x & 63) == 0 && cm) atomicAdd(&flags->changed, (unsigned long long)__popcll(cm));
+}
+
+// Bounds of every patched leaf (after all patches); an emptied leaf is a topology change.
+__global__ void k_edit_bounds(const uint32_t *__restrict__ touched, uint64_t E,
+                              const unsigned long long *__restrict__ leaves,
+                              uint32_t *__restrict__ bounds, EditFlags *__restrict__ flags) {
+  const uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const uint32_t t = touched[e];
+  if (t >= 0xFFFFFFFEu) return;
+  const unsigned long long *w = leaves + uint64_t(t) * 8;
+  bool empty = true;
+#pragma unroll
+  for (int i = 0; i < 8; i++) empty = empty && w[i] == 0;
+  if (empty) atomicOr(&flags->topology, 1u);
+  bounds[t] = leaf_bounds_of(w);
+}
+
+// The tree's (patched) leaves back into the dense leaf grid; a non-empty leaf sets its
+// brick's bitmap bit.
+__global__ void k_leaves_to_grid(const unsigned long long *__restrict__ leaves,
+                                 const unsigned long long *__restrict__ codes, uint64_t L,
+                                 unsigned long long *__restrict__ leafgrid,
+                                 unsigned long long *__restrict__ b1) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= L) return;
+  const uint64_t code = codes[i];
+  unsigned long long any = 0;
+#pragma unroll
+  for (int w = 0; w < 8; w++) {
+    const unsigned long long v = leaves[i * 8 + w];
+    leafgrid[code * 8 + w] = v;
+    any |= v;
+  }
+  if (any) atomicOr(&b1[code >> 6], 1ull << (code & 63));
+}
+
+__global__ void k_edit_new_bricks(const vxl_voxel_edit *__restrict__ edits,
+                                  const uint32_t *__restrict__ touched, uint64_t E,
+                                  unsigned long long *__restrict__ leafgrid,
+                                  unsigned long long *__restrict__ b1) {
+  const uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= E || touched[e] != 0xFFFFFFFEu) return;
+  const vxl_voxel_edit ed = edits[e];
+  const uint64_t code = dev::brick_code(ed.x >> 3, ed.y >> 3, ed.z >> 3);
+  const uint32_t m = dev::morton9(ed.x, ed.y, ed.z);
+  atomicOr(&leafgrid[code * 8 + (m >> 6)], 1ull << (m & 63));
+  atomicOr(&b1[code >> 6], 1ull << (code & 63));
+}
+
+// Materials of the rebuilt tree: a surviving brick keeps its 512 slots, a new one reads 0.
+__global__ void k_remap_materials(const unsigned long long *__restrict__ new_codes, uint64_t L2,
+                                  const unsigned long long *__restrict__ old_codes, uint64_t L1,
+                                  const uint16_t *__restrict__ old_mats,
+                                  uint16_t *__restrict__ new_mats) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= L2 * 512) return;
+  const uint64_t leaf = i >> 9;
+  const uint64_t j = find_leaf(old_codes, L1, new_codes[leaf]);
+  new_mats[i] = j < L1 ? old_mats[j * 512 + (i & 511)] : uint16_t(0);
+}
+
+}  // namespace
+
+EditResult edit_tree(vxl_ctx *ctx, vxl_tree *t, const vxl_voxel_edit *d_edits, uint64_t E) {
+  EditResult res{0, false};
+  if (E == 0) return res;
+  hipStream_t s = ctx->stream;
+  ctx->edit_scratch.reserve(E * 4 + 256);
+  uint32_t *touched = reinterpret_cast<uint32_t *>(ctx->edit_scratch.as<char>() + 256);
+  // flags in the mapped readback area (past what the tree build uses of it)
+  constexpr size_t kFlagSlot = 3328;
+  EditFlags *dflags =
+      reinterpret_cast<EditFlags *>(static_cast<char *>(ctx->host_pinned_dev) + kFlagSlot);
+  volatile EditFlags *hflags =
+      reinterpret_cast<volatile EditFlags *>(static_cast<char *>(ctx->host_pinned) + kFlagSlot);
+  hflags->changed = 0;
+  hflags->topology = 0;
+  const uint32_t blocks = uint32_t(div_up(E, kTB));
+  hipLaunchKernelGGL(k_edit_patch, dim3(blocks), dim3(kTB), 0, s, d_edits, E,
+                     reinterpret_cast<unsigned long long *>(t->leaves),
+                     reinterpret_cast<const unsigned long long *>(t->codes), t->leaf_count,
+                     t->materials, touched, dflags);
+  hipLaunchKernelGGL(k_edit_bounds, dim3(blocks), dim3(kTB), 0, s, touched, E,
+                     reinterpret_cast<const unsigned long long *>(t->leaves), t->bounds, dflags);
+  VXL_HIP(hipGetLastError());
+  wait_stream(ctx, s);
+  res.changed = hflags->changed;
+  res.topology = hflags->topology != 0;
+  if (!res.topology) return res;  // every change was in place (Edit::Leaf)
+  // a brick appears or empties (Edit::Topology): rebuild from the patched leaves plus the
+  // new bricks' voxels, as SparseTree rebuilds its levels from the spliced codes
+  prepare_grid(ctx, t->n);
+  unsigned long long *grid = ctx->leafgrid.as<unsigned long long>();
+  unsigned long long *b1 = ctx->bitmap.as<unsigned long long>();
+  if (t->leaf_count)
+    hipLaunchKernelGGL(k_leaves_to_grid, dim3(uint32_t(div_up(t->leaf_count, kTB))), dim3(kTB), 0,
+                       s, reinterpret_cast<const unsigned long long *>(t->leaves),
+                       reinterpret_cast<const unsigned long long *>(t->codes), t->leaf_count,
+                       grid, b1);
+  hipLaunchKernelGGL(k_edit_new_bricks, dim3(blocks), dim3(kTB), 0, s, d_edits, touched, E, grid,
+                     b1);
+  VXL_HIP(hipGetLastError());
+  vxl_tree *nt = build_tree_from_grid(ctx, t->n, t->k, nullptr, nullptr);
+  if (t->materials && nt->leaf_count) {
+    try {
+      void *p = nullptr;
+      if (hipMalloc(&p, size_t(nt->leaf_count) * 1024) != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(VXL_OOM, "edit: materials allocation failed");
+      }
+      nt->materials = static_cast<uint16_t *>(p);
+      hipLaunchKernelGGL(k_remap_materials, dim3(uint32_t(div_up(nt->leaf_count * 512, kTB))),
+                         dim3(kTB), 0, s, reinterpret_cast<const unsigned long long *>(nt->codes),
+                         nt->leaf_count, reinterpret_cast<const unsigned long long *>(t->codes),
+                         t->leaf_count, t->materials, nt->materials);
+      VXL_HIP(hipGetLastError());
+    } catch (...) {
+      tree_free_device(nt);
+      delete nt;
+      throw;
+    }
+  }
+  // the caller's handle takes the new arrays; the old ones are released with nt
+  std::swap(t->node_count, nt->node_count);
+  std::swap(t->leaf_count, nt->leaf_count);
+  std::swap(t->nodes, nt->nodes);
+  std::swap(t->leaves, nt->leaves);
+  std::swap(t->bounds, nt->bounds);
+  std::swap(t->codes, nt->codes);
+  std::swap(t->materials, nt->materials);
+  std::swap(t->level_nodes, nt->level_nodes);
+  std::swap(t->bytes, nt->bytes);
+  std::swap(t->slab, nt->slab);
+  std::swap(t->shared, nt->shared);
+  VXL_HIP(hipStreamSynchronize(s));  // the remap read the old arrays
+  tree_free_device(nt);
+  delete nt;
+  return res;
+}
+
 }  // namespace vxl

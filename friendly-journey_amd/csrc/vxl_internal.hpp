@@ -122,6 +122,7 @@ struct vxl_ctx {
   vxl::DevBuf owner_io, color_io, owner_scratch;  // per-voxel owners / colours (n^3 u32)
   vxl::DevBuf leaf_map;          // brick code -> leaf index of the tree being given materials
   vxl::DevBuf compact_io;        // compact voxels staged for the host
+  vxl::DevBuf edit_io, edit_scratch;  // voxel edits and their per-edit results
   uint32_t trav_ctr_next = 0;
   bool pool_dirty = false;
   uint64_t pool_budget = 16ull << 30;  // leaf-grid bytes one batch group may use
@@ -294,6 +295,14 @@ struct RaySegment {
 };
 // All segments in one kernel launch on stream s.
 void launch_traverse_batch(vxl_ctx *ctx, const RaySegment *segs, uint32_t S, hipStream_t s);
+// tree.hip: SparseTree::set_voxel over distinct in-bounds edits on the device; patches
+// leaves in place or, when a brick appears or empties, rebuilds the tree's arrays (the
+// handle stays the same).
+struct EditResult {
+  uint64_t changed;
+  bool topology;
+};
+EditResult edit_tree(vxl_ctx *ctx, vxl_tree *t, const vxl_voxel_edit *d_edits, uint64_t E);
 // dda.hip: one RGBA8 image of the tree (render.wgsl); d_colors: the global colour table.
 void launch_render(vxl_ctx *ctx, const vxl_tree *tree, const vxl_camera &cam,
                    const uint32_t *d_colors, uint32_t ncolors, uint32_t *d_rgba, hipStream_t s);

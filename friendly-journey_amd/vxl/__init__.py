@@ -36,6 +36,7 @@ EXPORTS = [
     "vxl_voxelize_dense_owner_dev", "vxl_voxelize_tree", "vxl_voxelize_tree_dev",
     "vxl_voxelize_tree_materials", "vxl_voxelize_tree_materials_dev", "vxl_compact_surface",
     "vxl_tree_from_compact", "vxl_tree_materials_download", "vxl_render", "vxl_render_dev",
+    "vxl_tree_set_voxels",
     "vxl_tree_from_words", "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload",
     "vxl_tree_free", "vxl_traverse", "vxl_traverse_dev", "vxl_set_timing", "vxl_last_timing",
     "vxl_set_tuning", "vxl_set_dda_refill", "vxl_voxelize_batch", "vxl_voxelize_batch_dev", "vxl_traverse_batch",
@@ -118,6 +119,7 @@ def load(path: str = LIB_PATH):
                                               C.c_uint32, C.POINTER(_vp), _pu64]
     L.vxl_voxelize_tree_materials_dev.argtypes = [_vp, _vp, _u64, C.POINTER(Grid), C.c_float,
                                                   _vp, C.c_uint32, C.POINTER(_vp), _pu64, _vp]
+    L.vxl_tree_set_voxels.argtypes = [_vp, _vp, _vp, _u64, _pu64, C.POINTER(C.c_int)]
     L.vxl_render.argtypes = [_vp, _vp, C.POINTER(Camera), _vp, C.c_uint32, _vp]
     L.vxl_render_dev.argtypes = [_vp, _vp, C.POINTER(Camera), _vp, C.c_uint32, _vp, _vp]
     L.vxl_compact_surface.argtypes = [_vp, _vp, _u64, C.POINTER(Grid), C.c_float, _vp,
@@ -168,7 +170,7 @@ def load(path: str = LIB_PATH):
                  "vxl_voxelize_dense_owner", "vxl_voxelize_dense_owner_dev",
                  "vxl_voxelize_tree_materials", "vxl_voxelize_tree_materials_dev",
                  "vxl_compact_surface", "vxl_tree_from_compact", "vxl_tree_materials_download",
-                 "vxl_render", "vxl_render_dev",
+                 "vxl_render", "vxl_render_dev", "vxl_tree_set_voxels",
                  "vxl_voxelize_tree", "vxl_voxelize_tree_dev", "vxl_tree_from_words",
                  "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload", "vxl_traverse",
                  "vxl_traverse_dev", "vxl_set_tuning", "vxl_set_dda_refill", "vxl_voxelize_batch",
@@ -227,6 +229,9 @@ class Tree:
     def __init__(self, handle: int, ctx: "Context"):
         self._h = handle
         self.ctx = ctx
+        self._refresh()
+
+    def _refresh(self):
         info = TreeInfo()
         _check(load().vxl_tree_info_get(_vp(self._h), C.byref(info)))
         self.n = info.n
@@ -387,6 +392,17 @@ class Context:
         if out.shape[0]:
             _check(load().vxl_compact_surface(*args, _ptr(out), out.shape[0], C.byref(count)))
         return out
+
+    def set_voxels(self, tree: "Tree", edits):
+        """SparseTree::set_voxel over rows (x, y, z, occupied), in order: (voxels changed,
+        whether bricks appeared or emptied and the tree was rebuilt). Updates tree's counts."""
+        e = np.ascontiguousarray(edits, np.uint32).reshape(-1, 4)
+        changed = C.c_uint64(0)
+        topo = C.c_int(0)
+        _check(load().vxl_tree_set_voxels(self.handle, tree.handle, _ptr(e) if e.size else None,
+                                          e.shape[0], C.byref(changed), C.byref(topo)))
+        tree._refresh()
+        return int(changed.value), bool(topo.value)
 
     def render(self, tree: "Tree", cam16, width: int, height: int, colors=()):
         """render_main: (height, width) u32 RGBA8 (R in the low byte)."""

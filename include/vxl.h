@@ -184,6 +184,22 @@ vxl_status vxl_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *rays, u
 vxl_status vxl_traverse_dev(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, uint64_t R,
                             uint32_t *d_hits, double *d_t, void *stream);
 
+/* ---- incremental edits on a resident tree: SparseTree::set_voxel (voxel-core/src/
+ *      sparse.rs:384-436) for a batch, applied in order (a voxel edited twice takes the
+ *      last edit); out-of-bounds edits change nothing. A voxel in an existing brick is
+ *      patched in place with its leaf's bounds (Edit::Leaf, the GpuRenderer::update_leaf /
+ *      SchoolBBuffer::patch_leaf case, school_b.rs:315-326); a newly set voxel's material
+ *      reads 0 again. When a brick appears or empties (Edit::Topology) the tree's arrays are
+ *      rebuilt, equal to a fresh build of the edited occupancy (sparse.rs:856-901); the
+ *      handle stays valid and surviving bricks keep their materials. *changed (nullable):
+ *      voxels whose state changed; *topology (nullable): 1 after a rebuild, else 0. */
+typedef struct vxl_voxel_edit {
+  uint32_t x, y, z;
+  uint32_t occupied; /* 1: set, 0: clear */
+} vxl_voxel_edit;
+vxl_status vxl_tree_set_voxels(vxl_ctx *ctx, vxl_tree *tree, const vxl_voxel_edit *edits,
+                               uint64_t count, uint64_t *changed, int *topology);
+
 /* ---- device ray generation + shading: the render path (GpuRenderer,
  *      voxel-gpu/src/render.rs:211-589, shaders/render.wgsl:54-91), one invocation per pixel
  *      with no host rays and no readback of hits. Camera layout = render.wgsl's Camera: eye
