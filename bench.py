@@ -176,7 +176,7 @@ def bench_edits(ctx, torch, W, stamps=20):
     log(f"edits: {stamps} in-place + {stamps} topology stamps on {tree.leaf_count} leaves")
     inplace, topo = [], []
     for c in rng.choice(codes, stamps, replace=False):
-        b = [int(v) for v in np.asarray(W_morton_decode(int(c)))]
+        b = [int(v) for v in np.asarray(brick_decode(int(c)))]
         e = brush([b[0] * 8 + 4, b[1] * 8 + 4, b[2] * 8 + 4], 3)
         ed = np.concatenate([e, np.ones((len(e), 1), np.int64)], 1)
         t0 = time.perf_counter()
@@ -198,7 +198,7 @@ def bench_edits(ctx, torch, W, stamps=20):
     return out
 
 
-def W_morton_decode(code):
+def brick_decode(code):
     """(bx, by, bz) of a brick Morton code (x in the low bit of each triple)."""
     out = [0, 0, 0]
     for b in range(21):
