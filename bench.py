@@ -152,6 +152,29 @@ def bench_materials(ctx, torch, W, plain_ms, reps=5):
     return out
 
 
+def bench_noise(ctx, torch, W):
+    """Noise fixtures at 2048^3 (the perlin and caves presets): the f64 field evaluated on
+    the device into the leaf grid and the tree built there."""
+    out = {}
+    for name, f in (("perlin", vxl_mod().NOISE_PERLIN), ("caves", vxl_mod().NOISE_CAVES)):
+        log(f"noise {name} at 2048^3")
+        ctx.generate_noise_tree(32, f).free()
+        t0 = time.perf_counter()
+        tr = ctx.generate_noise_tree(2048, f)
+        secs = time.perf_counter() - t0
+        out[name] = dict(ms=round(secs * 1e3, 2), leaves=tr.leaf_count,
+                         gvoxels_per_s=round(2048 ** 3 / secs / 1e9, 2))
+        tr.free()
+    out["note"] = ("wall time of the call (f64 field, leaf words by wave ballot, tree build, "
+                   "sync); the tree equals SparseTree::build of the CPU f64 field bit for bit")
+    return out
+
+
+def vxl_mod():
+    import vxl
+    return vxl
+
+
 def brush(center, radius):
     """The voxels of a solid sphere (sparse.rs brush_voxels: dx^2 + dy^2 + dz^2 <= r^2)."""
     import numpy as np
@@ -580,6 +603,7 @@ def main():
     if not args.no_materials:
         line["materials"] = bench_materials(ctx, torch, W, line["ms_per_step"])
         line["edits"] = bench_edits(ctx, torch, W)
+        line["noise_2048"] = bench_noise(ctx, torch, W)
     if not args.no_dda:
         dda = bench_dda(ctx, torch, W)
         for name, rec in dda.items():

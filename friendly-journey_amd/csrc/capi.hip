@@ -772,6 +772,21 @@ vxl_status vxl_tree_set_voxels(vxl_ctx *ctx, vxl_tree *tree, const vxl_voxel_edi
   });
 }
 
+vxl_status vxl_generate_noise_tree(vxl_ctx *ctx, uint32_t n, const vxl_noise *field,
+                                   vxl_tree **out) {
+  return guarded(ctx, [&] {
+    if (!out || !field) throw Error(VXL_INVALID_ARG, "null field or out");
+    *out = nullptr;
+    vxl_grid g{n, {0, 0, 0}, 1.0f};
+    uint32_t k = 0;
+    validate_grid(&g, &k);
+    if (field->octaves > 64) throw Error(VXL_INVALID_ARG, "more than 64 octaves");
+    *out = vxl::generate_noise_tree(ctx, n, k, *field);
+    ctx->trees.insert(*out);
+    VXL_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
 static void validate_camera(const vxl_tree *tree, const vxl_camera *cam) {
   if (!tree) throw Error(VXL_INVALID_ARG, "tree is null");
   if (!cam) throw Error(VXL_INVALID_ARG, "camera is null");

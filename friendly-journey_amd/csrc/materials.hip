@@ -47,12 +47,13 @@ __global__ void k_leaf_materials(const uint64_t *__restrict__ leaves,
                                  const uint32_t *__restrict__ owner, uint64_t N,
                                  const uint32_t *__restrict__ table, uint32_t len,
                                  uint16_t *__restrict__ mats) {
-  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const uint64_t leaf = i >> 9;
-  const uint32_t m = uint32_t(i & 511);
-  const bool occ = (leaves[leaf * 8 + (m >> 6)] >> (m & 63)) & 1ull;
-  mats[i] = occ ? uint16_t(resolve_material(owner[i], table, len)) : uint16_t(0);
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < N;
+       i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t leaf = i >> 9;
+    const uint32_t m = uint32_t(i & 511);
+    const bool occ = (leaves[leaf * 8 + (m >> 6)] >> (m & 63)) & 1ull;
+    mats[i] = occ ? uint16_t(resolve_material(owner[i], table, len)) : uint16_t(0);
+  }
 }
 
 __global__ void k_word_popc(const uint64_t *__restrict__ words, uint64_t W,
@@ -127,16 +128,26 @@ __global__ void k_compact_last(const int4 *__restrict__ vox, uint64_t count, uin
 
 __global__ void k_mats_from_last(const uint32_t *__restrict__ last, uint64_t N,
                                  const int4 *__restrict__ vox, uint16_t *__restrict__ mats) {
-  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const uint32_t l = last[i];
-  mats[i] = l ? uint16_t(uint32_t(vox[l - 1].w) & 0xFFFFu) : uint16_t(0);
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < N;
+       i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint32_t l = last[i];
+    mats[i] = l ? uint16_t(uint32_t(vox[l - 1].w) & 0xFFFFu) : uint16_t(0);
+  }
 }
 
+// Blocks for N threads, for kernels that take one element per thread.
 uint32_t grid_blocks(uint64_t N) {
   const uint64_t b = (N + kTB - 1) / kTB;
-  if (b > 0x7fffffffull) throw Error(VXL_WORKGROUPS_EXCEEDED, "materials: grid too large");
+  if (b * kTB > 0xffffffffull)
+    throw Error(VXL_WORKGROUPS_EXCEEDED, "materials: more than 2^32 threads in one launch");
   return uint32_t(b);
+}
+
+// Blocks for N elements of a grid-stride kernel: a launch holds at most 2^32 - 1 threads
+// per dimension, so the grid is capped and the kernel strides.
+uint32_t stride_blocks(uint64_t N) {
+  const uint64_t b = (N + kTB - 1) / kTB;
+  return uint32_t(b < (1ull << 22) ? b : (1ull << 22));
 }
 
 // The brick-code -> leaf index map of a tree, in ctx scratch ((n/8)^3 u32). Entries of
@@ -183,7 +194,7 @@ vxl_tree *voxelize_tree_materials(vxl_ctx *ctx, const float *d_tris, uint64_t T,
     VXL_HIP(hipMemsetAsync(owner, 0xFF, N * 4, ctx->stream));  // UINT32_MAX: no owner
     run_sat(ctx, d_tris, T, p, owner, map);
     t->materials = alloc_materials(t->leaf_count);
-    hipLaunchKernelGGL(k_leaf_materials, dim3(grid_blocks(N)), dim3(kTB), 0, ctx->stream,
+    hipLaunchKernelGGL(k_leaf_materials, dim3(stride_blocks(N)), dim3(kTB), 0, ctx->stream,
                        t->leaves, owner, N, d_table, table_len, t->materials);
     VXL_HIP(hipGetLastError());
   } catch (...) {
@@ -249,7 +260,7 @@ vxl_tree *tree_from_compact(vxl_ctx *ctx, uint32_t n, uint32_t k, const int32_t 
     hipLaunchKernelGGL(k_compact_last, dim3(grid_blocks(count)), dim3(kTB), 0, ctx->stream, vox,
                        count, n, map, last);
     t->materials = alloc_materials(t->leaf_count);
-    hipLaunchKernelGGL(k_mats_from_last, dim3(grid_blocks(N)), dim3(kTB), 0, ctx->stream, last, N,
+    hipLaunchKernelGGL(k_mats_from_last, dim3(stride_blocks(N)), dim3(kTB), 0, ctx->stream, last, N,
                        vox, t->materials);
     VXL_HIP(hipGetLastError());
   } catch (...) {

@@ -1222,11 +1222,12 @@ __global__ void k_remap_materials(const unsigned long long *__restrict__ new_cod
                                   const unsigned long long *__restrict__ old_codes, uint64_t L1,
                                   const uint16_t *__restrict__ old_mats,
                                   uint16_t *__restrict__ new_mats) {
-  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= L2 * 512) return;
-  const uint64_t leaf = i >> 9;
-  const uint64_t j = find_leaf(old_codes, L1, new_codes[leaf]);
-  new_mats[i] = j < L1 ? old_mats[j * 512 + (i & 511)] : uint16_t(0);
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < L2 * 512;
+       i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t leaf = i >> 9;
+    const uint64_t j = find_leaf(old_codes, L1, new_codes[leaf]);
+    new_mats[i] = j < L1 ? old_mats[j * 512 + (i & 511)] : uint16_t(0);
+  }
 }
 
 }  // namespace
@@ -1279,7 +1280,8 @@ EditResult edit_tree(vxl_ctx *ctx, vxl_tree *t, const vxl_voxel_edit *d_edits, u
         throw Error(VXL_OOM, "edit: materials allocation failed");
       }
       nt->materials = static_cast<uint16_t *>(p);
-      hipLaunchKernelGGL(k_remap_materials, dim3(uint32_t(div_up(nt->leaf_count * 512, kTB))),
+      const uint64_t rb = div_up(nt->leaf_count * 512, kTB);
+      hipLaunchKernelGGL(k_remap_materials, dim3(uint32_t(rb < (1ull << 22) ? rb : (1ull << 22))),
                          dim3(kTB), 0, s, reinterpret_cast<const unsigned long long *>(nt->codes),
                          nt->leaf_count, reinterpret_cast<const unsigned long long *>(t->codes),
                          t->leaf_count, t->materials, nt->materials);

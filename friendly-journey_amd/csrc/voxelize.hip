@@ -928,16 +928,18 @@ __device__ __forceinline__ uint32_t hash_color(uint32_t id) {
 // (reference_cpu.rs:173-186; voxelize.wgsl:264-281).
 __global__ void k_owner_color(const uint32_t *__restrict__ owner, uint32_t *__restrict__ color,
                               uint64_t N) {
-  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const uint32_t o = owner[i];
-  color[i] = o != 0xFFFFFFFFu ? hash_color(o) : 0u;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < N;
+       i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint32_t o = owner[i];
+    color[i] = o != 0xFFFFFFFFu ? hash_color(o) : 0u;
+  }
 }
 }  // namespace
 
 void owner_colors(vxl_ctx *ctx, const uint32_t *owner, uint32_t *color, uint64_t N) {
   if (N == 0) return;
-  const uint64_t blocks = (N + 255) / 256;
+  uint64_t blocks = (N + 255) / 256;  // grid-stride kernel: at most 2^22 blocks
+  if (blocks > (1ull << 22)) blocks = 1ull << 22;
   hipLaunchKernelGGL(k_owner_color, dim3(uint32_t(blocks)), dim3(256), 0, ctx->stream, owner,
                      color, N);
   VXL_HIP(hipGetLastError());

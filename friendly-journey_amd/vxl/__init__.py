@@ -36,7 +36,7 @@ EXPORTS = [
     "vxl_voxelize_dense_owner_dev", "vxl_voxelize_tree", "vxl_voxelize_tree_dev",
     "vxl_voxelize_tree_materials", "vxl_voxelize_tree_materials_dev", "vxl_compact_surface",
     "vxl_tree_from_compact", "vxl_tree_materials_download", "vxl_render", "vxl_render_dev",
-    "vxl_tree_set_voxels",
+    "vxl_tree_set_voxels", "vxl_generate_noise_tree",
     "vxl_tree_from_words", "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload",
     "vxl_tree_free", "vxl_traverse", "vxl_traverse_dev", "vxl_set_timing", "vxl_last_timing",
     "vxl_set_tuning", "vxl_set_dda_refill", "vxl_voxelize_batch", "vxl_voxelize_batch_dev", "vxl_traverse_batch",
@@ -119,6 +119,7 @@ def load(path: str = LIB_PATH):
                                               C.c_uint32, C.POINTER(_vp), _pu64]
     L.vxl_voxelize_tree_materials_dev.argtypes = [_vp, _vp, _u64, C.POINTER(Grid), C.c_float,
                                                   _vp, C.c_uint32, C.POINTER(_vp), _pu64, _vp]
+    L.vxl_generate_noise_tree.argtypes = [_vp, C.c_uint32, C.POINTER(Noise), C.POINTER(_vp)]
     L.vxl_tree_set_voxels.argtypes = [_vp, _vp, _vp, _u64, _pu64, C.POINTER(C.c_int)]
     L.vxl_render.argtypes = [_vp, _vp, C.POINTER(Camera), _vp, C.c_uint32, _vp]
     L.vxl_render_dev.argtypes = [_vp, _vp, C.POINTER(Camera), _vp, C.c_uint32, _vp, _vp]
@@ -171,6 +172,7 @@ def load(path: str = LIB_PATH):
                  "vxl_voxelize_tree_materials", "vxl_voxelize_tree_materials_dev",
                  "vxl_compact_surface", "vxl_tree_from_compact", "vxl_tree_materials_download",
                  "vxl_render", "vxl_render_dev", "vxl_tree_set_voxels",
+                 "vxl_generate_noise_tree",
                  "vxl_voxelize_tree", "vxl_voxelize_tree_dev", "vxl_tree_from_words",
                  "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload", "vxl_traverse",
                  "vxl_traverse_dev", "vxl_set_tuning", "vxl_set_dda_refill", "vxl_voxelize_batch",
@@ -184,6 +186,20 @@ def load(path: str = LIB_PATH):
 def _check(rc: int):
     if rc != 0:
         raise VxlError(rc, (load().vxl_last_error() or b"").decode())
+
+
+class Noise(C.Structure):
+    """vxl_noise: NoiseField's parameters (voxel-core fixtures.rs:259-279)."""
+    _fields_ = [("seed", C.c_uint32), ("octaves", C.c_uint32), ("ridged", C.c_uint32),
+                ("pad", C.c_uint32), ("frequency", C.c_double), ("lacunarity", C.c_double),
+                ("gain", C.c_double), ("warp", C.c_double), ("threshold", C.c_double)]
+
+
+# NoiseField presets (voxel-core fixtures.rs:284-314)
+NOISE_PERLIN = dict(seed=0x5EED1234, frequency=5.0, octaves=5, lacunarity=2.0, gain=0.5,
+                    ridged=False, warp=0.0, threshold=0.12)
+NOISE_CAVES = dict(seed=0xCA5E5EED, frequency=4.0, octaves=5, lacunarity=2.0, gain=0.5,
+                   ridged=True, warp=0.45, threshold=0.48)
 
 
 class Camera(C.Structure):
@@ -392,6 +408,15 @@ class Context:
         if out.shape[0]:
             _check(load().vxl_compact_surface(*args, _ptr(out), out.shape[0], C.byref(count)))
         return out
+
+    def generate_noise_tree(self, n: int, field: dict):
+        """The NoiseField fixture's tree (fields as NOISE_PERLIN / NOISE_CAVES)."""
+        f = Noise(seed=field["seed"], octaves=field["octaves"], ridged=1 if field["ridged"] else 0,
+                  frequency=field["frequency"], lacunarity=field["lacunarity"],
+                  gain=field["gain"], warp=field["warp"], threshold=field["threshold"])
+        out = _vp()
+        _check(load().vxl_generate_noise_tree(self.handle, n, C.byref(f), C.byref(out)))
+        return Tree(out.value, self)
 
     def set_voxels(self, tree: "Tree", edits):
         """SparseTree::set_voxel over rows (x, y, z, occupied), in order: (voxels changed,

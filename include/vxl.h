@@ -184,6 +184,20 @@ vxl_status vxl_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *rays, u
 vxl_status vxl_traverse_dev(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, uint64_t R,
                             uint32_t *d_hits, double *d_t, void *stream);
 
+/* ---- procedural noise fixtures: generate_noise_tree (voxel-gpu/src/generate.rs:99) over
+ *      NoiseField (voxel-core/src/fixtures.rs:259-346, noise.rs): fBm or ridged Perlin
+ *      noise, optionally domain-warped, thresholded; evaluated in f64 with the CPU field's
+ *      operations, so the tree is SparseTree::build of the CPU NoiseField bit for bit. Presets
+ *      (fixtures.rs:284-314): perlin = {0x5eed1234, 5 octaves, fBm, frequency 5, lacunarity 2,
+ *      gain 0.5, warp 0, threshold 0.12}; caves = {0xca5e5eed, 5, ridged, 4, 2, 0.5, 0.45,
+ *      0.48}. */
+typedef struct vxl_noise {
+  uint32_t seed, octaves, ridged, pad;
+  double frequency, lacunarity, gain, warp, threshold;
+} vxl_noise;
+vxl_status vxl_generate_noise_tree(vxl_ctx *ctx, uint32_t n, const vxl_noise *field,
+                                   vxl_tree **out);
+
 /* ---- incremental edits on a resident tree: SparseTree::set_voxel (voxel-core/src/
  *      sparse.rs:384-436) for a batch, applied in order (a voxel edited twice takes the
  *      last edit); out-of-bounds edits change nothing. A voxel in an existing brick is

@@ -74,6 +74,18 @@ def lib():
         L.vo_render.restype = None
         L.vo_render.argtypes = [C.c_void_p, _f32p, C.c_uint32, C.c_uint32,
                                 C.POINTER(C.c_uint16), _u32p, C.c_uint32, _u32p, C.c_int]
+        _f64p = C.POINTER(C.c_double)
+        L.vo_noise_grad.restype = C.c_double
+        L.vo_noise_grad.argtypes = [C.c_uint32, C.c_double, C.c_double, C.c_double]
+        L.vo_noise_fade.restype = C.c_double
+        L.vo_noise_fade.argtypes = [C.c_double]
+        L.vo_perlin3.restype = C.c_double
+        L.vo_perlin3.argtypes = [C.c_uint32, C.c_double, C.c_double, C.c_double]
+        L.vo_noise_value.restype = C.c_double
+        L.vo_noise_value.argtypes = [_f64p, _u32p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                     C.c_uint32]
+        L.vo_noise_words.restype = C.c_uint64
+        L.vo_noise_words.argtypes = [_f64p, _u32p, C.c_uint32, _u32p]
         L.vo_count_candidates.restype = C.c_uint64
         L.vo_count_candidates.argtypes = [_f32p, C.c_uint64, C.c_uint32, _f32p, C.c_float,
                                           C.c_float]
@@ -260,6 +272,47 @@ def render(tree, cam, width, height, mats=None, colors=None) -> np.ndarray:
                     None if m is None else m.ctypes.data_as(C.POINTER(C.c_uint16)),
                     _p(col, _u32p), col.size, _p(out, _u32p), _threads())
     return out.reshape(height, width)
+
+
+# NoiseField presets (voxel-core/src/fixtures.rs:284-314)
+NOISE_PERLIN = dict(seed=0x5EED1234, frequency=5.0, octaves=5, lacunarity=2.0, gain=0.5,
+                    ridged=False, warp=0.0, threshold=0.12)
+NOISE_CAVES = dict(seed=0xCA5E5EED, frequency=4.0, octaves=5, lacunarity=2.0, gain=0.5,
+                   ridged=True, warp=0.45, threshold=0.48)
+
+
+def _noise_args(field):
+    f = np.array([field["frequency"], field["lacunarity"], field["gain"], field["warp"],
+                  field["threshold"]], np.float64)
+    u = np.array([field["seed"], field["octaves"], 1 if field["ridged"] else 0], np.uint32)
+    return f, u
+
+
+def noise_grad(h, x, y, z) -> float:
+    return float(lib().vo_noise_grad(h, x, y, z))
+
+
+def noise_fade(t) -> float:
+    return float(lib().vo_noise_fade(t))
+
+
+def perlin3(seed, x, y, z) -> float:
+    return float(lib().vo_perlin3(seed, x, y, z))
+
+
+def noise_value(field, n, x, y, z) -> float:
+    f, u = _noise_args(field)
+    return float(lib().vo_noise_value(f.ctypes.data_as(C.POINTER(C.c_double)), _p(u, _u32p), n,
+                                      x, y, z))
+
+
+def noise_words(field, n):
+    """NoiseField::is_occupied as x-major words, and the occupied count."""
+    f, u = _noise_args(field)
+    words = np.zeros((n ** 3 + 31) // 32, np.uint32)
+    c = lib().vo_noise_words(f.ctypes.data_as(C.POINTER(C.c_double)), _p(u, _u32p), n,
+                             _p(words, _u32p))
+    return words, int(c)
 
 
 def count_candidates(tris, n, origin=(0.0, 0.0, 0.0), voxel_size=1.0, eps=1e-4) -> int:

@@ -917,6 +917,107 @@ void vo_mirror_traverse(const vo_tree *t, const float *rays, uint64_t R, uint32_
 }
 
 /* ------------------------------------------------------------------------- */
+/* noise.rs + fixtures.rs NoiseField -- the procedural noise fixtures, f64      */
+/* ------------------------------------------------------------------------- */
+static uint32_t noise_hash(uint32_t seed, int64_t ix, int64_t iy, int64_t iz) { /* noise.rs lattice_hash */
+  const uint32_t x = (uint32_t)ix, y = (uint32_t)iy, z = (uint32_t)iz;
+  uint32_t h = seed ^ (x * 0x8da6b343u) ^ (y * 0xd8163841u) ^ (z * 0xcb1ab31fu);
+  h ^= h >> 16;
+  h *= 0x7feb352du;
+  h ^= h >> 15;
+  return h;
+}
+
+double vo_noise_grad(uint32_t hash, double x, double y, double z) { /* noise.rs grad */
+  const uint32_t h = hash & 15u;
+  double u = h < 8 ? x : y;
+  double v = h < 4 ? y : ((h == 12 || h == 14) ? x : z);
+  if (h & 1u) u = -u;
+  if (h & 2u) v = -v;
+  return u + v;
+}
+
+double vo_noise_fade(double t) { return t * t * t * (t * (t * 6.0 - 15.0) + 10.0); }
+
+static double noise_lerp(double a, double b, double t) { return a + t * (b - a); }
+
+double vo_perlin3(uint32_t seed, double x, double y, double z) { /* noise.rs perlin3 */
+  const double fx = floor(x), fy = floor(y), fz = floor(z);
+  const int64_t ix = (int64_t)fx, iy = (int64_t)fy, iz = (int64_t)fz;
+  const double xf = x - fx, yf = y - fy, zf = z - fz;
+  const double u = vo_noise_fade(xf), v = vo_noise_fade(yf), w = vo_noise_fade(zf);
+#define G(dx, dy, dz)                                                                     \
+  vo_noise_grad(noise_hash(seed, ix + (dx), iy + (dy), iz + (dz)), xf - (double)(dx),      \
+                yf - (double)(dy), zf - (double)(dz))
+  const double x00 = noise_lerp(G(0, 0, 0), G(1, 0, 0), u);
+  const double x10 = noise_lerp(G(0, 1, 0), G(1, 1, 0), u);
+  const double x01 = noise_lerp(G(0, 0, 1), G(1, 0, 1), u);
+  const double x11 = noise_lerp(G(0, 1, 1), G(1, 1, 1), u);
+#undef G
+  const double y0 = noise_lerp(x00, x10, v), y1 = noise_lerp(x01, x11, v);
+  return noise_lerp(y0, y1, w);
+}
+
+static double noise_fractal(uint32_t seed, const double p[3], uint32_t octaves, double lac,
+                            double gain, int ridged) { /* noise.rs fractal */
+  double amp = 1.0, freq = 1.0, sum = 0.0, norm = 0.0;
+  for (uint32_t o = 0; o < octaves; o++) {
+    const double off = (double)o;
+    const double n = vo_perlin3(seed + o * 0x9e3779b9u, p[0] * freq + off * 17.13,
+                                p[1] * freq - off * 9.71, p[2] * freq + off * 5.37);
+    double contrib = n;
+    if (ridged) {
+      const double r = 1.0 - fabs(n);
+      contrib = r * r;
+    }
+    sum += amp * contrib;
+    norm += amp;
+    amp *= gain;
+    freq *= lac;
+  }
+  const double mean = norm > 0.0 ? sum / norm : 0.0;
+  return ridged ? mean * 2.0 - 1.0 : mean;
+}
+
+/* NoiseField::value (fixtures.rs:316-346): the sample point, the optional domain warp
+ * (noise.rs domain_warp, at most 3 octaves), the fractal. f = {frequency, lacunarity, gain,
+ * warp, threshold}, u = {seed, octaves, ridged}. */
+double vo_noise_value(const double f[5], const uint32_t u[3], uint32_t n, uint32_t x, uint32_t y,
+                      uint32_t z) {
+  const double s = f[0] / (double)n;
+  double p[3] = {((double)x + 0.5) * s, ((double)y + 0.5) * s, ((double)z + 0.5) * s};
+  if (f[3] > 0.0) {
+    const uint32_t wo = u[1] < 3 ? u[1] : 3;
+    const double a[3] = {p[0] + 1.7, p[1] + 9.2, p[2] + 3.3};
+    const double b[3] = {p[0] - 5.1, p[1] + 2.8, p[2] - 7.4};
+    const double c[3] = {p[0] + 8.6, p[1] - 4.5, p[2] + 1.9};
+    const double wx = noise_fractal(u[0] ^ 0x11111111u, a, wo, f[1], f[2], (int)u[2]);
+    const double wy = noise_fractal(u[0] ^ 0x22222222u, b, wo, f[1], f[2], (int)u[2]);
+    const double wz = noise_fractal(u[0] ^ 0x33333333u, c, wo, f[1], f[2], (int)u[2]);
+    p[0] = p[0] + f[3] * wx;
+    p[1] = p[1] + f[3] * wy;
+    p[2] = p[2] + f[3] * wz;
+  }
+  return noise_fractal(u[0], p, u[1], f[1], f[2], (int)u[2]);
+}
+
+/* NoiseField::is_occupied over the grid as x-major words (zeroed here); returns the count. */
+uint64_t vo_noise_words(const double f[5], const uint32_t u[3], uint32_t n, uint32_t *words) {
+  const uint64_t nn = n;
+  memset(words, 0, (size_t)((nn * nn * nn + 31) / 32) * 4);
+  uint64_t c = 0;
+  for (uint64_t z = 0; z < nn; z++)
+    for (uint64_t y = 0; y < nn; y++)
+      for (uint64_t x = 0; x < nn; x++)
+        if (vo_noise_value(f, u, n, (uint32_t)x, (uint32_t)y, (uint32_t)z) > f[4]) {
+          const uint64_t lin = x + nn * (y + nn * z);
+          words[lin >> 5] |= 1u << (lin & 31);
+          c++;
+        }
+  return c;
+}
+
+/* ------------------------------------------------------------------------- */
 /* render.wgsl:54-91 -- camera rays, traversal and shading per pixel            */
 /* ------------------------------------------------------------------------- */
 static uint32_t unorm8(float v) { /* rgba8unorm store: clamp, scale, round to nearest */

@@ -52,6 +52,14 @@ uint64_t vo_compact(const uint32_t *words, const uint32_t *owner, uint32_t n,
 uint64_t vo_compact_to_words(const int32_t *vox, uint64_t count, uint32_t n, uint32_t *words);
 void vo_compact_materials(const int32_t *vox, uint64_t count, uint32_t n, const uint64_t *codes,
                           uint64_t L, uint16_t *mats);
+/* noise.rs / fixtures.rs NoiseField in f64: f = {frequency, lacunarity, gain, warp,
+ * threshold}, u = {seed, octaves, ridged}. */
+double vo_noise_grad(uint32_t hash, double x, double y, double z);
+double vo_noise_fade(double t);
+double vo_perlin3(uint32_t seed, double x, double y, double z);
+double vo_noise_value(const double f[5], const uint32_t u[3], uint32_t n, uint32_t x, uint32_t y,
+                      uint32_t z);
+uint64_t vo_noise_words(const double f[5], const uint32_t u[3], uint32_t n, uint32_t *words);
 /* Same candidate count without doing the SAT (for roofline accounting). */
 uint64_t vo_count_candidates(const float *tris, uint64_t T, uint32_t n, const float origin[3],
                              float voxel_size, float eps);
