@@ -278,7 +278,8 @@ def bench_dda(ctx, torch, W, reps=5):
                                              note="device-generated camera rays, walk and shading; "
                                                   "no ray or hit buffers")
         out[name]["_tree"] = tree
-        out[name]["_rays"] = rays
+        out[name]["_drays"] = dr.data_ptr()
+        out[name]["_keep"] = dr
     return out
 
 
@@ -494,18 +495,13 @@ def bench_serve(ctx, torch, W, duration_s=2.0):
     return out
 
 
-def dda_walk_stats(tree, rays, sample=20000):
-    """Per-ray walk statistics counted by the oracle's mirror on a ray sample: leaf and node
-    descents and cell visits. Algorithmic HBM bytes per ray = 24 (ray) + 16 (hit) + 8 (t)
-    + 68 per leaf descent + 12 per node descent below the root."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np
-    import oracle as O
-    nodes, leaves, bounds, _ = tree.download()
-    ot = O.Tree.from_school_b(tree.n, nodes, leaves, bounds)
-    idx = np.linspace(0, len(rays) - 1, sample).astype(np.int64)
-    _, st = ot.mirror(rays[idx], stats=True)
-    d_leaf, d_node, visits = (float(v) for v in st.mean(0))
+def dda_walk_stats(ctx, tree, d_rays, R):
+    """Per-ray walk statistics of the whole batch, counted on the device by the walk itself
+    (vxl_traverse_stats): leaf and node descents and cell visits. Algorithmic HBM bytes per
+    ray = 24 (ray) + 16 (hit) + 8 (t) + 68 per leaf descent + 12 per node descent below the
+    root."""
+    ld, nd, visits = ctx.traverse_stats(tree, d_rays, R)
+    d_leaf, d_node, visits = ld / R, nd / R, visits / R
     return dict(bytes=24 + 16 + 8 + 68 * d_leaf + 12 * d_node, leaf_descents=d_leaf,
                 node_descents=d_node, visits=visits,
                 ops=DDA_OPS_RAY + DDA_OPS_VISIT * visits + DDA_OPS_DESCENT * (d_leaf + d_node))
@@ -607,10 +603,11 @@ def main():
     if not args.no_dda:
         dda = bench_dda(ctx, torch, W)
         for name, rec in dda.items():
-            ws = dda_walk_stats(rec.pop("_tree"), rec.pop("_rays"))
+            ws = dda_walk_stats(ctx, rec.pop("_tree"), rec.pop("_drays"), rec["rays"])
             secs = rec["kernel_ms"] / 1e3
             gbs = ws["bytes"] * rec["rays"] / secs / 1e9
             tops = ws["ops"] * rec["rays"] / secs / 1e12
+            rec.pop("_keep")
             rec["walk"] = {k: round(ws[k], 3) for k in ("visits", "leaf_descents", "node_descents")}
             rec["roofline"] = {"bound": "valu", "ops_per_ray": round(ws["ops"], 1),
                                "achieved": round(tops, 3), "peak": VALU_PEAK_TOPS, "unit": "TOP/s",

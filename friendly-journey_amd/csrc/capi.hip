@@ -787,6 +787,15 @@ vxl_status vxl_generate_noise_tree(vxl_ctx *ctx, uint32_t n, const vxl_noise *fi
   });
 }
 
+vxl_status vxl_traverse_stats(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, uint64_t R,
+                              uint64_t out[3]) {
+  return guarded(ctx, [&] {
+    if (!tree || !out) throw Error(VXL_INVALID_ARG, "null tree or out");
+    if (R && !d_rays) throw Error(VXL_INVALID_ARG, "null ray buffer");
+    vxl::traverse_stats(ctx, tree, d_rays, R, out);
+  });
+}
+
 static void validate_camera(const vxl_tree *tree, const vxl_camera *cam) {
   if (!tree) throw Error(VXL_INVALID_ARG, "tree is null");
   if (!cam) throw Error(VXL_INVALID_ARG, "camera is null");

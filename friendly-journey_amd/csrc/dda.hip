@@ -229,12 +229,16 @@ constexpr uint32_t kWalking = 0, kHit = 1, kLeft = 2, kCapped = 3;
 // leaf (one word load) or a child bit of the node record in registers; on a hit voxel
 // stop; on an occupied child whose leaf the ray reaches descend; else step, popping
 // parents until one steps to a new cell (mirror.rs:155-165). `frames` is this lane's
-// column of the LDS frame rows, `fstride` the row pitch.
+// column of the LDS frame rows, `fstride` the row pitch. kStats counts the descents into
+// leaves and into nodes (the mirror's walk statistics).
+template <bool kStats = false>
 __device__ __forceinline__ uint32_t walk_visit(const Ray &r, Walk &w,
                                                const uint32_t *__restrict__ nodes,
                                                const uint32_t *__restrict__ leafw,
                                                const uint32_t *__restrict__ bounds,
-                                               uint32_t *frames, uint32_t fstride) {
+                                               uint32_t *frames, uint32_t fstride,
+                                               uint32_t *leaf_desc = nullptr,
+                                               uint32_t *node_desc = nullptr) {
   if (++w.iter > kMaxIter) return kCapped;  // bounded: a legitimate walk never comes close
   uint32_t word, bitpos;
   const uint32_t mlo = w.mlo, mhi = w.mhi;
@@ -267,6 +271,10 @@ __device__ __forceinline__ uint32_t walk_visit(const Ray &r, Walk &w,
         f[4 * fstride] = __float_as_uint(w.tm2);
       }
       w.level -= 1;
+      if (kStats) {
+        if (w.level == 1) *leaf_desc += 1;
+        else *node_desc += 1;
+      }
       w.node = slot;
       w.ox = cox;
       w.oy = coy;
@@ -386,6 +394,43 @@ __global__ __launch_bounds__(256) void k_hit_t(const float *__restrict__ rays, u
     t = hit_t(o, d, h);
   }
   tout[ri] = t;
+}
+
+// The mirror's walk statistics over a ray batch (the reference's measure.rs counters):
+// sums of leaf descents, node descents and cell visits, one atomic per wave each.
+__global__ __launch_bounds__(64) void k_traverse_stats(
+    const float *__restrict__ rays, uint64_t R, const uint32_t *__restrict__ nodes,
+    const uint32_t *__restrict__ leafw, const uint32_t *__restrict__ bounds, uint32_t n,
+    uint32_t k, uint64_t leaf_count, unsigned long long *__restrict__ sums) {
+  extern __shared__ uint32_t frames_lds[];
+  const uint64_t ri = uint64_t(blockIdx.x) * 64 + threadIdx.x;
+  uint32_t ld = 0, nd = 0, visits = 0;
+  if (ri < R) {
+    Ray r;
+    load_ray(rays, ri, r);
+    Walk w;
+    w.ox = w.oy = w.oz = w.cx = w.cy = w.cz = 0;
+    float t0;
+    if (clip_grid(r, n, leaf_count, t0)) {
+      begin_walk(r, w, k, t0, nodes);
+      uint32_t st;
+      do {
+        visits++;
+        st = walk_visit<true>(r, w, nodes, leafw, bounds, frames_lds + threadIdx.x, 64, &ld, &nd);
+      } while (st == kWalking);
+    }
+  }
+  unsigned long long v[3] = {ld, nd, visits};
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) v[i] += (unsigned long long)__shfl_xor((long long)v[i], d, 64);
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(&sums[0], v[0]);
+    atomicAdd(&sums[1], v[1]);
+    atomicAdd(&sums[2], v[2]);
+  }
 }
 
 // Persistent lanes: a grid sized to the device's resident one-wave blocks; each wave takes
@@ -594,7 +639,7 @@ void launch_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, ui
     const uint64_t waves = (R + 63) / 64;
     uint64_t blocks = uint64_t(ctx->cu_count > 0 ? ctx->cu_count : 256) * uint64_t(per_cu);
     if (blocks > waves) blocks = waves;
-    ctx->trav_ctr.reserve(kCounterSlots * 8);
+    ctx->trav_ctr.reserve(kCounterSlots * 8 + 64);
     unsigned long long *ctr =
         ctx->trav_ctr.as<unsigned long long>() + (ctx->trav_ctr_next++ % kCounterSlots);
     VXL_HIP(hipMemsetAsync(ctr, 0, 8, s));
@@ -619,6 +664,28 @@ void launch_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, ui
                      reinterpret_cast<const uint32_t *>(tree->leaves), tree->bounds, tree->n,
                      tree->k, tree->leaf_count, reinterpret_cast<uint4 *>(d_hits), d_t);
   VXL_HIP(hipGetLastError());
+}
+
+void traverse_stats(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, uint64_t R,
+                    uint64_t out[3]) {
+  out[0] = out[1] = out[2] = 0;
+  if (R == 0) return;
+  // three sums past the ring of persistent-traversal counters (the call waits for them)
+  ctx->trav_ctr.reserve(kCounterSlots * 8 + 64);
+  unsigned long long *sums = ctx->trav_ctr.as<unsigned long long>() + kCounterSlots;
+  VXL_HIP(hipMemsetAsync(sums, 0, 24, ctx->stream));
+  const uint64_t blocks = (R + 63) / 64;
+  hipLaunchKernelGGL(k_traverse_stats, dim3(uint32_t(blocks)), dim3(64),
+                     frame_lds_bytes(tree->k, 64), ctx->stream, d_rays, R, tree->nodes,
+                     reinterpret_cast<const uint32_t *>(tree->leaves), tree->bounds, tree->n,
+                     tree->k, tree->leaf_count, sums);
+  VXL_HIP(hipGetLastError());
+  VXL_HIP(hipMemcpyAsync(ctx->host_pinned, sums, 24, hipMemcpyDeviceToHost, ctx->stream));
+  wait_stream(ctx, ctx->stream);
+  const uint64_t *h = static_cast<const uint64_t *>(ctx->host_pinned);
+  out[0] = h[0];
+  out[1] = h[1];
+  out[2] = h[2];
 }
 
 void launch_render(vxl_ctx *ctx, const vxl_tree *tree, const vxl_camera &cam,

@@ -36,7 +36,7 @@ EXPORTS = [
     "vxl_voxelize_dense_owner_dev", "vxl_voxelize_tree", "vxl_voxelize_tree_dev",
     "vxl_voxelize_tree_materials", "vxl_voxelize_tree_materials_dev", "vxl_compact_surface",
     "vxl_tree_from_compact", "vxl_tree_materials_download", "vxl_render", "vxl_render_dev",
-    "vxl_tree_set_voxels", "vxl_generate_noise_tree",
+    "vxl_tree_set_voxels", "vxl_generate_noise_tree", "vxl_traverse_stats",
     "vxl_tree_from_words", "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload",
     "vxl_tree_free", "vxl_traverse", "vxl_traverse_dev", "vxl_set_timing", "vxl_last_timing",
     "vxl_set_tuning", "vxl_set_dda_refill", "vxl_voxelize_batch", "vxl_voxelize_batch_dev", "vxl_traverse_batch",
@@ -119,6 +119,7 @@ def load(path: str = LIB_PATH):
                                               C.c_uint32, C.POINTER(_vp), _pu64]
     L.vxl_voxelize_tree_materials_dev.argtypes = [_vp, _vp, _u64, C.POINTER(Grid), C.c_float,
                                                   _vp, C.c_uint32, C.POINTER(_vp), _pu64, _vp]
+    L.vxl_traverse_stats.argtypes = [_vp, _vp, _vp, _u64, _pu64]
     L.vxl_generate_noise_tree.argtypes = [_vp, C.c_uint32, C.POINTER(Noise), C.POINTER(_vp)]
     L.vxl_tree_set_voxels.argtypes = [_vp, _vp, _vp, _u64, _pu64, C.POINTER(C.c_int)]
     L.vxl_render.argtypes = [_vp, _vp, C.POINTER(Camera), _vp, C.c_uint32, _vp]
@@ -172,7 +173,7 @@ def load(path: str = LIB_PATH):
                  "vxl_voxelize_tree_materials", "vxl_voxelize_tree_materials_dev",
                  "vxl_compact_surface", "vxl_tree_from_compact", "vxl_tree_materials_download",
                  "vxl_render", "vxl_render_dev", "vxl_tree_set_voxels",
-                 "vxl_generate_noise_tree",
+                 "vxl_generate_noise_tree", "vxl_traverse_stats",
                  "vxl_voxelize_tree", "vxl_voxelize_tree_dev", "vxl_tree_from_words",
                  "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload", "vxl_traverse",
                  "vxl_traverse_dev", "vxl_set_tuning", "vxl_set_dda_refill", "vxl_voxelize_batch",
@@ -408,6 +409,12 @@ class Context:
         if out.shape[0]:
             _check(load().vxl_compact_surface(*args, _ptr(out), out.shape[0], C.byref(count)))
         return out
+
+    def traverse_stats(self, tree: "Tree", d_rays: int, R: int):
+        """Sums over the rays of (leaf descents, node descents, cell visits) of the walk."""
+        out = (C.c_uint64 * 3)()
+        _check(load().vxl_traverse_stats(self.handle, tree.handle, _vp(d_rays), R, out))
+        return int(out[0]), int(out[1]), int(out[2])
 
     def generate_noise_tree(self, n: int, field: dict):
         """The NoiseField fixture's tree (fields as NOISE_PERLIN / NOISE_CAVES)."""

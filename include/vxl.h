@@ -214,6 +214,13 @@ typedef struct vxl_voxel_edit {
 vxl_status vxl_tree_set_voxels(vxl_ctx *ctx, vxl_tree *tree, const vxl_voxel_edit *edits,
                                uint64_t count, uint64_t *changed, int *topology);
 
+/* ---- walk statistics of a ray batch (the reference's measurement counters,
+ *      voxel-core/src/measure.rs:189-235, over the mirror's walk mirror.rs:263-326): sums
+ *      over the rays of descents into leaves, descents into nodes and cell visits, in
+ *      out[0..2]. Device rays, host result; for rooflines, not on the hot path. */
+vxl_status vxl_traverse_stats(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, uint64_t R,
+                              uint64_t out[3]);
+
 /* ---- device ray generation + shading: the render path (GpuRenderer,
  *      voxel-gpu/src/render.rs:211-589, shaders/render.wgsl:54-91), one invocation per pixel
  *      with no host rays and no readback of hits. Camera layout = render.wgsl's Camera: eye

@@ -161,3 +161,24 @@ def test_dda_refill_argument_checked(ctx):
     for bad in (-2, 65):
         with pytest.raises(vxl.VxlError):
             ctx.set_dda_refill(bad)
+
+
+def test_walk_statistics_match_the_mirror(ctx):
+    """vxl_traverse_stats counts the walk as the mirror counts it (mirror.rs:263-326 with
+    the oracle's {leaf descents, node descents, visits}): sums over the batch are equal."""
+    import torch
+    tris = W.uv_sphere()
+    n, vs = W.grid_for(256)
+    words, _, _ = O.voxelize_surface(tris, n, (0, 0, 0), vs, 1e-4)
+    ot = O.Tree.build(words, n)
+    gt = ctx.voxelize_tree(tris, n, (0, 0, 0), vs, 1e-4)
+    for label, rays in (("camera", W.camera_rays(256, 320, 180)),
+                        ("random", W.random_rays(30_000, 256)),
+                        ("box", W.box_rays(20_000, n))):
+        _say(f"walk statistics {label}")
+        rays = np.ascontiguousarray(rays, np.float32)
+        d = torch.from_numpy(rays).cuda()
+        got = ctx.traverse_stats(gt, d.data_ptr(), len(rays))
+        _, st = ot.mirror(rays, stats=True)
+        want = tuple(int(v) for v in st.astype(np.int64).sum(0))
+        assert got == want, f"{label}: {got} != {want}"
