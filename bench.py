@@ -364,6 +364,19 @@ def bench_batch(ctx, torch, W, reps=3):
             res[name + "_vox"].append(ms)
             for t in trees:
                 t.free()
+    # the plain per-mesh traversals captured once in a HIP graph and replayed
+    graph = ctx.traverse_graph(keep_p, rp0, roff, hp0, tpp0)
+    graph.launch()
+    torch.cuda.synchronize()
+    graph_ms = []
+    for _ in range(reps):
+        hp.fill_(-1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        graph.launch()
+        torch.cuda.synchronize()
+        graph_ms.append((time.perf_counter() - t0) * 1e3)
+    graph.free()
     same_hits = bool(torch.equal(hp, hb)) and bool(torch.equal(tp, tb))
     same_trees = True
     for i, (a, b) in enumerate(zip(keep_p, keep_b)):
@@ -386,6 +399,7 @@ def bench_batch(ctx, torch, W, reps=3):
                   "speedup": round(med["plain_vox"] / med["batched_vox"], 3),
                   "batched_mtris_per_s": round(T / med["batched_vox"] / 1e3, 1)},
         traverse={"plain_ms": round(med["plain_trav"], 2),
+                  "graph_ms": round(sorted(graph_ms)[len(graph_ms) // 2], 2),
                   "batched_ms": round(med["batched_trav"], 2),
                   "speedup": round(med["plain_trav"] / med["batched_trav"], 3),
                   "batched_mrays_per_s": round(R / med["batched_trav"] / 1e3, 1)},

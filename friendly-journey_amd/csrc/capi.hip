@@ -787,6 +787,99 @@ vxl_status vxl_generate_noise_tree(vxl_ctx *ctx, uint32_t n, const vxl_noise *fi
   });
 }
 
+struct vxl_graph {
+  vxl_ctx *ctx = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+vxl_status vxl_traverse_graph_create(vxl_ctx *ctx, const vxl_tree *const *trees, uint32_t count,
+                                     const float *d_rays, const uint64_t *ray_offsets,
+                                     uint32_t *d_hits, double *d_t, vxl_graph **out) {
+  return guarded(ctx, [&] {
+    if (!out) throw Error(VXL_INVALID_ARG, "out is null");
+    *out = nullptr;
+    traverse_batch_checks(trees, count, ray_offsets);
+    if (count && ray_offsets[count] && (!d_rays || !d_hits))
+      throw Error(VXL_INVALID_ARG, "null ray or hit buffer");
+    const auto segs = segments(trees, count, ray_offsets, d_rays, d_hits, d_t);
+    // The traversals are independent: the graph forks into kBranches chains (segments
+    // dealt by ray count to the least loaded chain) that join at the end, so its kernel
+    // nodes can run side by side instead of one after another.
+    constexpr int kBranches = 4;
+    hipStream_t cs = nullptr, side[kBranches] = {};
+    hipEvent_t fork = nullptr, joins[kBranches] = {};
+    auto *g = new vxl_graph();
+    g->ctx = ctx;
+    bool capturing = false;
+    auto release = [&] {
+      for (int b = 0; b < kBranches; b++) {
+        if (side[b]) (void)hipStreamDestroy(side[b]);
+        if (joins[b]) (void)hipEventDestroy(joins[b]);
+      }
+      if (fork) (void)hipEventDestroy(fork);
+      if (cs) (void)hipStreamDestroy(cs);
+    };
+    try {
+      VXL_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+      VXL_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+      for (int b = 0; b < kBranches; b++) {
+        VXL_HIP(hipStreamCreateWithFlags(&side[b], hipStreamNonBlocking));
+        VXL_HIP(hipEventCreateWithFlags(&joins[b], hipEventDisableTiming));
+      }
+      VXL_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      capturing = true;
+      VXL_HIP(hipEventRecord(fork, cs));
+      for (int b = 0; b < kBranches; b++) VXL_HIP(hipStreamWaitEvent(side[b], fork, 0));
+      uint64_t load[kBranches] = {};
+      for (const auto &sg : segs) {  // kernel launches only: no memset or copy nodes
+        int b = 0;
+        for (int i = 1; i < kBranches; i++)
+          if (load[i] < load[b]) b = i;
+        load[b] += sg.R;
+        vxl::launch_traverse_plain(ctx, sg.tree, sg.rays, sg.R, sg.hits, sg.t, side[b]);
+      }
+      for (int b = 0; b < kBranches; b++) {
+        VXL_HIP(hipEventRecord(joins[b], side[b]));
+        VXL_HIP(hipStreamWaitEvent(cs, joins[b], 0));
+      }
+      capturing = false;
+      VXL_HIP(hipStreamEndCapture(cs, &g->graph));
+      VXL_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    } catch (...) {
+      if (capturing) {
+        hipGraph_t dead = nullptr;
+        (void)hipStreamEndCapture(cs, &dead);
+        if (dead) (void)hipGraphDestroy(dead);
+      }
+      if (g->graph) (void)hipGraphDestroy(g->graph);
+      delete g;
+      release();
+      throw;
+    }
+    release();
+    *out = g;
+  });
+}
+
+vxl_status vxl_graph_launch(vxl_graph *graph, void *stream) {
+  if (!graph) {
+    vxl::set_error("null graph");
+    return VXL_INVALID_ARG;
+  }
+  return guarded(graph->ctx, [&] {
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : graph->ctx->stream;
+    VXL_HIP(hipGraphLaunch(graph->exec, s));
+  });
+}
+
+void vxl_graph_destroy(vxl_graph *graph) {
+  if (!graph) return;
+  if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+  if (graph->graph) (void)hipGraphDestroy(graph->graph);
+  delete graph;
+}
+
 vxl_status vxl_traverse_stats(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, uint64_t R,
                               uint64_t out[3]) {
   return guarded(ctx, [&] {

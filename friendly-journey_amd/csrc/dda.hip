@@ -655,9 +655,15 @@ void launch_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, ui
     VXL_HIP(hipGetLastError());
     return;
   }
+  launch_traverse_plain(ctx, tree, d_rays, R, d_hits, d_t, s);
+}
+
+void launch_traverse_plain(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, uint64_t R,
+                           uint32_t *d_hits, double *d_t, hipStream_t s) {
+  if (R == 0) return;
   const int tb = ctx->dda_block;
   const uint64_t blocks = (R + tb - 1) / tb;
-  if (blocks > 0xffffffffull)
+  if (blocks * uint64_t(tb) > 0xffffffffull)
     throw Error(VXL_WORKGROUPS_EXCEEDED, "traverse: ray batch exceeds the grid limit");
   hipLaunchKernelGGL(k_traverse, dim3(uint32_t(blocks)), dim3(tb), frame_lds_bytes(tree->k, tb), s,
                      d_rays, R, tree->nodes,

@@ -305,6 +305,9 @@ struct EditResult {
 EditResult edit_tree(vxl_ctx *ctx, vxl_tree *t, const vxl_voxel_edit *d_edits, uint64_t E);
 // noise.hip: the NoiseField fixture's tree, built on the device.
 vxl_tree *generate_noise_tree(vxl_ctx *ctx, uint32_t n, uint32_t k, const vxl_noise &f);
+// dda.hip: one lane per ray, nothing but the kernel launch (graph capture uses it).
+void launch_traverse_plain(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, uint64_t R,
+                           uint32_t *d_hits, double *d_t, hipStream_t s);
 // dda.hip: the walk statistics of a ray batch {leaf descents, node descents, visits}.
 void traverse_stats(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, uint64_t R,
                     uint64_t out[3]);

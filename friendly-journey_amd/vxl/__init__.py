@@ -37,6 +37,7 @@ EXPORTS = [
     "vxl_voxelize_tree_materials", "vxl_voxelize_tree_materials_dev", "vxl_compact_surface",
     "vxl_tree_from_compact", "vxl_tree_materials_download", "vxl_render", "vxl_render_dev",
     "vxl_tree_set_voxels", "vxl_generate_noise_tree", "vxl_traverse_stats",
+    "vxl_traverse_graph_create", "vxl_graph_launch", "vxl_graph_destroy",
     "vxl_tree_from_words", "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload",
     "vxl_tree_free", "vxl_traverse", "vxl_traverse_dev", "vxl_set_timing", "vxl_last_timing",
     "vxl_set_tuning", "vxl_set_dda_refill", "vxl_voxelize_batch", "vxl_voxelize_batch_dev", "vxl_traverse_batch",
@@ -120,6 +121,11 @@ def load(path: str = LIB_PATH):
     L.vxl_voxelize_tree_materials_dev.argtypes = [_vp, _vp, _u64, C.POINTER(Grid), C.c_float,
                                                   _vp, C.c_uint32, C.POINTER(_vp), _pu64, _vp]
     L.vxl_traverse_stats.argtypes = [_vp, _vp, _vp, _u64, _pu64]
+    L.vxl_traverse_graph_create.argtypes = [_vp, C.POINTER(_vp), C.c_uint32, _vp, _pu64, _vp,
+                                            _vp, C.POINTER(_vp)]
+    L.vxl_graph_launch.argtypes = [_vp, _vp]
+    L.vxl_graph_destroy.argtypes = [_vp]
+    L.vxl_graph_destroy.restype = None
     L.vxl_generate_noise_tree.argtypes = [_vp, C.c_uint32, C.POINTER(Noise), C.POINTER(_vp)]
     L.vxl_tree_set_voxels.argtypes = [_vp, _vp, _vp, _u64, _pu64, C.POINTER(C.c_int)]
     L.vxl_render.argtypes = [_vp, _vp, C.POINTER(Camera), _vp, C.c_uint32, _vp]
@@ -174,6 +180,7 @@ def load(path: str = LIB_PATH):
                  "vxl_compact_surface", "vxl_tree_from_compact", "vxl_tree_materials_download",
                  "vxl_render", "vxl_render_dev", "vxl_tree_set_voxels",
                  "vxl_generate_noise_tree", "vxl_traverse_stats",
+                 "vxl_traverse_graph_create", "vxl_graph_launch",
                  "vxl_voxelize_tree", "vxl_voxelize_tree_dev", "vxl_tree_from_words",
                  "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload", "vxl_traverse",
                  "vxl_traverse_dev", "vxl_set_tuning", "vxl_set_dda_refill", "vxl_voxelize_batch",
@@ -201,6 +208,28 @@ NOISE_PERLIN = dict(seed=0x5EED1234, frequency=5.0, octaves=5, lacunarity=2.0, g
                     ridged=False, warp=0.0, threshold=0.12)
 NOISE_CAVES = dict(seed=0xCA5E5EED, frequency=4.0, octaves=5, lacunarity=2.0, gain=0.5,
                    ridged=True, warp=0.45, threshold=0.48)
+
+
+class TraverseGraph:
+    """A replayable HIP graph of per-tree traversals (keeps its trees alive)."""
+
+    def __init__(self, handle: int, trees):
+        self._h = handle
+        self._trees = list(trees)
+
+    def launch(self, stream: int = 0):
+        _check(load().vxl_graph_launch(_vp(self._h), _vp(stream)))
+
+    def free(self):
+        if self._h:
+            load().vxl_graph_destroy(_vp(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Camera(C.Structure):
@@ -409,6 +438,17 @@ class Context:
         if out.shape[0]:
             _check(load().vxl_compact_surface(*args, _ptr(out), out.shape[0], C.byref(count)))
         return out
+
+    def traverse_graph(self, trees, d_rays: int, ray_offsets, d_hits: int, d_t: int = 0):
+        """The per-tree traversals captured in a HIP graph: returns a TraverseGraph whose
+        launch() replays them."""
+        arr = (_vp * len(trees))(*[t._h for t in trees])
+        off = np.ascontiguousarray(ray_offsets, np.uint64)
+        out = _vp()
+        _check(load().vxl_traverse_graph_create(self.handle, arr, len(trees), _vp(d_rays),
+                                                off.ctypes.data_as(_pu64), _vp(d_hits),
+                                                _vp(d_t) if d_t else None, C.byref(out)))
+        return TraverseGraph(out.value, trees)
 
     def traverse_stats(self, tree: "Tree", d_rays: int, R: int):
         """Sums over the rays of (leaf descents, node descents, cell visits) of the walk."""

@@ -214,6 +214,20 @@ typedef struct vxl_voxel_edit {
 vxl_status vxl_tree_set_voxels(vxl_ctx *ctx, vxl_tree *tree, const vxl_voxel_edit *edits,
                                uint64_t count, uint64_t *changed, int *topology);
 
+/* ---- the plain per-tree traversal launches captured in a HIP graph (C4's many ray batches
+ *      as one replayable submission; no reference counterpart: the reference submits each
+ *      traversal on its own). Captures, for g < count, the one-lane-per-ray traversal of
+ *      rays [ray_offsets[g], ray_offsets[g+1]) against trees[g] into d_hits / d_t, as four
+ *      independent chains that fork and join (so traversals overlap on the device); the
+ *      graph holds the trees' and buffers' device addresses, which must outlive it. Replays
+ *      give exactly vxl_traverse_dev's results. */
+typedef struct vxl_graph vxl_graph;
+vxl_status vxl_traverse_graph_create(vxl_ctx *ctx, const vxl_tree *const *trees, uint32_t count,
+                                     const float *d_rays, const uint64_t *ray_offsets,
+                                     uint32_t *d_hits, double *d_t, vxl_graph **out);
+vxl_status vxl_graph_launch(vxl_graph *graph, void *stream);
+void vxl_graph_destroy(vxl_graph *graph);
+
 /* ---- walk statistics of a ray batch (the reference's measurement counters,
  *      voxel-core/src/measure.rs:189-235, over the mirror's walk mirror.rs:263-326): sums
  *      over the rays of descents into leaves, descents into nodes and cell visits, in

@@ -214,3 +214,42 @@ def test_batch_rejects_bad_input(ctx):
     assert e.value.name == "VXL_INVALID_ARG"
     with pytest.raises(vxl.VxlError):
         ctx.set_batch_budget(1 << 20)
+
+
+def test_traverse_graph_replays_equal_eager(ctx):
+    """The per-tree traversals captured in a HIP graph: every replay (with eager work and
+    synchronisations between replays, outputs cleared before each) gives the eager hits."""
+    import torch
+    print("[case] traverse graph replays", flush=True)
+    trees, rays = [], []
+    for i, (m, kind) in enumerate(((32, "sphere"), (128, "soup"), (256, "sphere"), (64, "soup"),
+                                   (512, "soup"), (128, "sphere"))):
+        n, vs = W.grid_for(m)
+        tris = W.uv_sphere(24 + 8 * i, 12 + 4 * i) if kind == "sphere" else \
+            W.soup(2000 + 500 * i, seed=40 + i, mu=np.log(4 / m))
+        trees.append(ctx.voxelize_tree(tris, n, (0, 0, 0), vs))
+        rays.append(W.random_rays(1000 + 3000 * i, m, seed=50 + i))
+    off = np.zeros(len(rays) + 1, np.uint64)
+    off[1:] = np.cumsum([len(r) for r in rays])
+    R = int(off[-1])
+    d_rays = torch.from_numpy(np.concatenate(rays)).cuda()
+    h_eager = torch.zeros((R, 4), dtype=torch.int32, device="cuda")
+    t_eager = torch.zeros(R, dtype=torch.float64, device="cuda")
+    for g, tr in enumerate(trees):
+        a, z = int(off[g]), int(off[g + 1])
+        ctx.traverse_dev(tr, d_rays.data_ptr() + 24 * a, z - a, h_eager.data_ptr() + 16 * a,
+                         t_eager.data_ptr() + 8 * a)
+    torch.cuda.synchronize()
+    h = torch.zeros_like(h_eager)
+    t = torch.zeros_like(t_eager)
+    graph = ctx.traverse_graph(trees, d_rays.data_ptr(), off, h.data_ptr(), t.data_ptr())
+    for rep in range(3):
+        h.fill_(-1)
+        t.fill_(-1.0)
+        torch.cuda.synchronize()
+        graph.launch()
+        ctx.traverse_dev(trees[0], d_rays.data_ptr(), int(off[1]), h_eager.data_ptr(),
+                         t_eager.data_ptr())  # eager work between replays
+        torch.cuda.synchronize()
+        assert torch.equal(h, h_eager) and torch.equal(t, t_eager), f"replay {rep}"
+    graph.free()
