@@ -219,15 +219,17 @@ __device__ __forceinline__ uint64_t setup_one(const float *__restrict__ tri,
 }
 
 // The cells the wave's windows skip (0 for a lane without a windowed triangle), added once
-// to *acc; a wave with any windowed triangle also stamps *win with the call's epoch.
+// to acc[slot]; a wave with any windowed triangle also stamps *win with the call's epoch.
 __device__ __forceinline__ void note_windows(unsigned long long *acc, unsigned int *win,
-                                             uint32_t epoch, uint64_t v) {
+                                             uint32_t epoch, uint64_t v, uint32_t slot = 0) {
   if (__ballot(v != 0) == 0) return;  // wave-uniform: the common case
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) v += (uint64_t)__shfl_xor((long long)v, d, 64);
   if ((threadIdx.x & 63) == 0) {
-    atomicAdd(acc, (unsigned long long)v);
-    __hip_atomic_store(win, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    atomicAdd(acc + slot, (unsigned long long)v);
+    // one store is enough: later waves see it and leave the line alone
+    if (__hip_atomic_load(win, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch)
+      __hip_atomic_store(win, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -238,7 +240,7 @@ __global__ void k_tri_setup(const float *__restrict__ tris, uint64_t T, Voxelize
   const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   uint64_t skipped = 0;
   if (t < T) skipped = setup_one(tris + 9 * t, p, 0u, count + t, rec + t, wrec + t);
-  note_windows(skipped_acc, win, epoch, skipped);
+  note_windows(skipped_acc, win, epoch, skipped, blockIdx.x % kSkipSlots);
 }
 
 // Many meshes: mesh g owns setup blocks [sb_begin[g], sb_begin[g+1]) and record slots
@@ -273,8 +275,12 @@ __global__ void k_sat_plan(const uint64_t *__restrict__ total, uint64_t want,
   uint64_t cpu = (nchunks + want - 1) / want;
   if (cpu < kChunksPerUnit) cpu = kChunksPerUnit;
   plan->P = P;
-  plan->P_ref = P + plan->skipped[epoch & 1];
-  plan->skipped[(epoch + 1) & 1] = 0;
+  unsigned long long skip = 0;
+  for (int i = 0; i < kSkipSlots; i++) {
+    skip += plan->skipped[epoch & 1][i];
+    plan->skipped[(epoch + 1) & 1][i] = 0;
+  }
+  plan->P_ref = P + skip;
   plan->cpu = cpu;
   plan->nunits = (nchunks + cpu - 1) / cpu;
   plan->counter = 0;
@@ -817,13 +823,15 @@ uint64_t plan_want(vxl_ctx *ctx, uint64_t *max_waves) {
   return *max_waves * uint64_t(ctx->sat_units_per_wave);
 }
 
+static_assert(sizeof(SatPlan) <= kPlanBytes, "the unit start table follows the plan");
+
 // The plan area (SatPlan, then the unit starts), sized before setup writes into it; a new
 // area starts with both skipped-cell slots clear.
 SatPlan *plan_area(vxl_ctx *ctx) {
   uint64_t mw = 0;
   const uint64_t want = plan_want(ctx, &mw);
   const size_t had = ctx->bstart.bytes;
-  ctx->bstart.reserve(256 + (want + 64) * 4);
+  ctx->bstart.reserve(kPlanBytes + (want + 64) * 4);
   SatPlan *plan = ctx->bstart.as<SatPlan>();
   if (ctx->bstart.bytes != had)
     VXL_HIP(hipMemsetAsync(plan->skipped, 0, sizeof(plan->skipped), ctx->stream));
@@ -843,7 +851,7 @@ void sat_after_setup(vxl_ctx *ctx, uint64_t Tp, unsigned long long *leafgrid,
   uint64_t max_waves = 0;
   const uint64_t want = plan_want(ctx, &max_waves);
   SatPlan *plan = plan_area(ctx);
-  uint32_t *ustart = reinterpret_cast<uint32_t *>(ctx->bstart.as<char>() + 256);
+  uint32_t *ustart = reinterpret_cast<uint32_t *>(ctx->bstart.as<char>() + kPlanBytes);
   hipLaunchKernelGGL(k_sat_plan, dim3(1), dim3(1), 0, s, coff + Tp, want, plan, epoch);
   if (Tp)
     hipLaunchKernelGGL(k_bstart, dim3((Tp + tb - 1) / tb), dim3(tb), 0, s, coff, count, Tp, plan,
@@ -887,7 +895,7 @@ void run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizeParams
   const uint32_t e = ++ctx->sat_epoch;
   hipLaunchKernelGGL(k_tri_setup, dim3((T + tb - 1) / tb), dim3(tb), 0, ctx->stream, d_tris, T,
                      p, ctx->tri_count.as<uint64_t>(), ctx->rec.as<TriRec>(), win_records(ctx, T),
-                     &plan->skipped[e & 1], &plan->win_epoch, e);
+                     plan->skipped[e & 1], &plan->win_epoch, e);
   sat_after_setup(ctx, T, ctx->leafgrid.as<unsigned long long>(),
                   ctx->bitmap.as<unsigned long long>(), e, owner, leaf_map, p.n);
 }

@@ -218,14 +218,17 @@ inline size_t batch_staging_bytes(uint32_t G) {
          up(G * sizeof(BatchTotals)) + up(G * 8) + 1024;
 }
 // The SAT launch plan the device derives from the pair total (voxelize.hip).
+constexpr int kSkipSlots = 64;
+constexpr size_t kPlanBytes = 2048;  // SatPlan, then the unit start table
 struct SatPlan {
   uint64_t P;       // decoded pairs (what the SAT kernel walks)
   uint64_t cpu;     // chunks of 64 pairs per work unit
   uint64_t nunits;  // work units
   uint64_t P_ref;   // the reference's candidate pairs (bounding-box cells), reported
-  // cells the windows of call e skip accumulate in skipped[e & 1]; the plan of call e
-  // reads it and clears skipped[(e + 1) & 1] for the next call on the stream
-  unsigned long long skipped[2];
+  // cells the windows of call e skip accumulate in skipped[e & 1] (spread over
+  // kSkipSlots slots by setup block, so the atomics do not meet on one address); the plan
+  // of call e sums them and clears skipped[(e + 1) & 1] for the next call on the stream
+  unsigned long long skipped[2][kSkipSlots];
   unsigned int counter;    // next unit to hand out
   unsigned int win_epoch;  // == e when call e holds a windowed triangle
 };
