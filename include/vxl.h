@@ -5,7 +5,7 @@
  * point names the reference interface it replaces (paths relative to the reference
  * repository root). Plain pointers and sizes only; no C++ or torch types.
  *
- * Conventions (mirroring the reference boundary, SURVEY-independent summary):
+ * Conventions (mirroring the reference boundary):
  *  - Inputs are borrowed; outputs go to caller-owned buffers or library-owned
  *    opaque handles released with an explicit *_free / *_destroy.
  *  - Entry points without a `_dev` suffix take HOST pointers and are synchronous,
