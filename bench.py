@@ -278,6 +278,7 @@ def bench_dda(ctx, torch, W, reps=5):
                                              note="device-generated camera rays, walk and shading; "
                                                   "no ray or hit buffers")
         out[name]["_tree"] = tree
+        out[name]["_rays"] = rays
         out[name]["_drays"] = dr.data_ptr()
         out[name]["_keep"] = dr
     return out
@@ -542,6 +543,31 @@ def cpu_baseline(tris, W):
                         f"threads, {leaves} leaves"))
 
 
+def cpu_dda_baseline(tree, rays):
+    """The oracle's f32 mirror traversal (the restatement of mirror.rs:254, the reference's
+    CPU twin of the device walk, which its bench command times on one thread,
+    voxel-cli/src/main.rs:846-853) over the same tree (downloaded from the device) and the
+    same rays: every fourth ray on one thread, then all of them on every core."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    nodes, leaves, bounds, _ = tree.download()
+    otree = O.Tree.from_school_b(tree.n, nodes, leaves, bounds)
+    del nodes, leaves, bounds
+    cores = min(os.cpu_count() or 1, 64)
+    out = {}
+    for label, threads, sample in (("one_thread", 1, rays[::4]), ("all_cores", cores, rays)):
+        sample = np.ascontiguousarray(sample)
+        t0 = time.perf_counter()
+        otree.mirror(sample, threads=threads)
+        dt = time.perf_counter() - t0
+        out[label] = dict(value=round(sample.shape[0] / dt / 1e6, 3), unit="Mrays/s",
+                          cores=threads, seconds=round(dt, 3), rays=int(sample.shape[0]))
+    out["kind"] = "port"
+    out["sample"] = "the bench rays against the same tree; one thread on every fourth ray"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -610,6 +636,16 @@ def main():
                                   "every product, so an FMA would change results, and packed FP32 is not "
                                   "used; the 157.3 TFLOP/s FP32 vector figure counts a packed FMA as 4"},
     }
+    # tree build: HBM-bound by its algorithmic bytes (brick-flag scan at 1 bit per brick
+    # code, 64 B leaf + 4 B bounds per leaf, 12 B per node) against the stage's event time
+    bricks = (vox["n"] // 8) ** 3
+    build_bytes = bricks // 8 + 68 * vox["leaves"] + 12 * vox["nodes"]
+    build_gbs = build_bytes / (vox["build_ms"] / 1e3) / 1e9
+    line["tree_build_roofline"] = {
+        "bound": "hbm", "achieved": round(build_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(build_gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes": build_bytes,
+        "note": "a chain of short kernels and one read-back of the level sizes: launch latency, "
+                "not bandwidth, sets its time"}
     if not args.no_materials:
         line["materials"] = bench_materials(ctx, torch, W, line["ms_per_step"])
         line["edits"] = bench_edits(ctx, torch, W)
@@ -617,7 +653,12 @@ def main():
     if not args.no_dda:
         dda = bench_dda(ctx, torch, W)
         for name, rec in dda.items():
-            ws = dda_walk_stats(ctx, rec.pop("_tree"), rec.pop("_drays"), rec["rays"])
+            tree, rays = rec.pop("_tree"), rec.pop("_rays")
+            ws = dda_walk_stats(ctx, tree, rec.pop("_drays"), rec["rays"])
+            if not args.no_cpu:
+                log(f"dda {name}: cpu baseline (oracle mirror traversal)")
+                rec["cpu_baseline"] = cpu_dda_baseline(tree, rays)
+            del tree, rays
             secs = rec["kernel_ms"] / 1e3
             gbs = ws["bytes"] * rec["rays"] / secs / 1e9
             tops = ws["ops"] * rec["rays"] / secs / 1e12

@@ -481,10 +481,17 @@ __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
   return (uint64_t(hi) << 32) | lo;
 }
 
-// quotient of a < 2^22 by b, from a float reciprocal, corrected to the exact value
+// Low 32 bits of the product of the low 24 bits of a and b (one full-rate multiply), as
+// an unsigned value: HIP's __umul24 returns int, which would make shifts of it arithmetic.
+__device__ __forceinline__ uint32_t mul24(uint32_t a, uint32_t b) {
+  return static_cast<uint32_t>(__umul24(a, b));
+}
+
+// quotient of a < 2^22 by b, from a float reciprocal, corrected to the exact value; every
+// product is below 2^24, so the 24-bit multiply is exact
 __device__ __forceinline__ uint32_t udiv_narrow(uint32_t a, uint32_t b, float rb) {
   uint32_t q = uint32_t(float(a) * rb);
-  const int32_t r = int32_t(a - q * b);
+  const int32_t r = int32_t(a - mul24(q, b));
   if (r < 0) q -= 1;
   else if (uint32_t(r) >= b) q += 1;
   return q;
@@ -492,12 +499,38 @@ __device__ __forceinline__ uint32_t udiv_narrow(uint32_t a, uint32_t b, float rb
 
 constexpr int kWaves = 4;    // waves per block
 constexpr int kQueue = 128;  // survivor queue entries per wave
-constexpr int kTable = 256;  // leaf-word merge table slots per wave
-constexpr int kFlushAt = 128;  // flush the table once this many slots are in use
+constexpr int kTable = 128;  // leaf-word merge table slots per wave
+constexpr int kTableBits = 7;
+constexpr int kFlushAt = 64;  // flush the table once this many slots are in use
 constexpr unsigned long long kEmpty = ~0ull;
+static_assert(kTable == 1 << kTableBits, "the slot hash yields kTableBits bits");
 
-// Phase B on one survivor: box faces + edge axes. On a hit, returns the leaf-grid word
-// index (brick code * 8 + sub-brick word) and the bit.
+// Merge-table key of a hit voxel: its mesh's grid base and its 4^3 sub-brick (one leaf-grid
+// word) as packed coordinates, sx | sy << 11 | sz << 22 | grid_base << 33. Sub-brick
+// coordinates are below 2048 (n <= 8192). Packing costs a few bit operations per hit;
+// the Morton spreading to the leaf-grid word index (key_word) runs once per table entry
+// at flush time instead.
+__device__ __forceinline__ unsigned long long pack_key(uint32_t grid_base, uint32_t x, uint32_t y,
+                                                       uint32_t z) {
+  const uint32_t sx = x >> 2, sy = y >> 2, sz = z >> 2;
+  const uint32_t lo = sx | (sy << 11) | (sz << 22);
+  const uint32_t hi = (grid_base << 1) | (sz >> 10);
+  return (uint64_t(hi) << 32) | lo;
+}
+
+// Leaf-grid word index of a packed key: the mesh's first word (grid_base * 512) + the
+// brick's Morton code * 8 + the sub-brick's octant in the brick.
+__device__ __forceinline__ unsigned long long key_word(unsigned long long key) {
+  const uint32_t lo = uint32_t(key), hi = uint32_t(key >> 32);
+  const uint32_t sx = lo & 0x7ffu, sy = (lo >> 11) & 0x7ffu, sz = (lo >> 22) | ((hi & 1u) << 10);
+  const uint32_t code = dev::brick_code(sx >> 1, sy >> 1, sz >> 1);
+  const uint32_t oct = (sx & 1u) | ((sy & 1u) << 1) | ((sz & 1u) << 2);
+  return (uint64_t(hi >> 1) << 9) + uint64_t(code) * 8 + oct;
+}
+
+// Phase B on one survivor: box faces + edge axes. On a hit, returns the packed merge-table
+// key of the voxel's leaf-grid word and the voxel's bit in that word (its 6-bit Morton
+// index inside the 4^3 sub-brick).
 __device__ __forceinline__ bool finish_pair(const TriRec *__restrict__ rec, uint32_t t, uint32_t xy,
                                             uint32_t z, unsigned long long &key,
                                             unsigned long long &bit) {
@@ -505,10 +538,8 @@ __device__ __forceinline__ bool finish_pair(const TriRec *__restrict__ rec, uint
   const uint32_t x = xy & 0xffffu, y = xy >> 16;
   const Rel v = rel_to(r.g, float(x) + 0.5f, float(y) + 0.5f, float(z) + 0.5f);
   const bool hit = (r.flags & kLiteral) ? rest_passes<true>(v) : rest_passes_finite(v);
-  const uint32_t code = dev::brick_code(x >> 3, y >> 3, z >> 3);
-  const uint32_t m = dev::morton9(x, y, z);
-  key = (uint64_t(r.grid_base) << 9) + uint64_t(code) * 8 + (m >> 6);
-  bit = 1ull << (m & 63);
+  key = pack_key(r.grid_base, x, y, z);
+  bit = 1ull << dev::child_bit(x, y, z);
   return hit;
 }
 
@@ -519,7 +550,10 @@ __device__ __forceinline__ uint32_t table_insert(unsigned long long *tk, unsigne
                                                  unsigned long long bit) {
   bool fresh = false;
   if (hit) {
-    uint32_t slot = (uint32_t(key) * 0x9E3779B1u ^ uint32_t(key >> 32)) >> 24;
+    // Fibonacci hash of the key folded to 24 bits: the top kTableBits of
+    // (h * floor(2^24 / golden ratio)) mod 2^24, from one full-rate 24-bit multiply
+    const uint32_t lo = uint32_t(key), h = lo ^ (lo >> 15) ^ uint32_t(key >> 32);
+    uint32_t slot = (mul24(h, 0x9E3779u) >> (24 - kTableBits)) & (kTable - 1);
     for (;;) {
       const unsigned long long old = atomicCAS(&tk[slot], kEmpty, key);
       if (old == kEmpty || old == key) {
@@ -574,8 +608,9 @@ __device__ __forceinline__ void table_flush(unsigned long long *tk, unsigned lon
     const int slot = i * 64 + lane;
     const unsigned long long key = tk[slot];
     if (key != kEmpty) {
-      or_if_missing(&leafgrid[key], tv[slot]);
-      const uint64_t code = key >> 3;
+      const unsigned long long word = key_word(key);
+      or_if_missing(&leafgrid[word], tv[slot]);
+      const uint64_t code = word >> 3;
       or_if_missing(&brickbits[code >> 6], 1ull << (code & 63));
       tk[slot] = kEmpty;
       tv[slot] = 0ull;
@@ -593,14 +628,15 @@ __device__ __forceinline__ void table_flush(unsigned long long *tk, unsigned lon
 //
 // Two builds: kWin decodes windowed triangles; both are launched and the one that does not
 // match the call (windowed triangles present or not) leaves at once, so calls without a
-// windowed triangle run the build that carries no window decode. Both are held to 72
-// VGPRs (7 waves per SIMD): the few values spilled live in the per-unit loop, and the
-// extra wave hides more of the record loads than the spills cost.
+// windowed triangle run the build that carries no window decode. Both are held to 64
+// VGPRs (8 waves per SIMD, the most a SIMD holds; the LDS of 8 blocks, 8 x 14 KiB, fits
+// a CU). Unit-level values (unit, chunk range, first triangle) stay wave-uniform in
+// scalar registers, which is what keeps the per-lane state within 64 VGPRs.
 //
 // kOwn: also lower each hit voxel's owner to the triangle index (single-mesh calls, where
 // the record index is the triangle index); kOwnLeaf writes owners only.
 template <bool kWin, int kOwn>
-__global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(7))) void k_sat(
+__global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))) void k_sat(
     const TriRec *__restrict__ rec, const WinRec *__restrict__ wrec,
     const uint64_t *__restrict__ coff,
     const uint32_t *__restrict__ ustart, uint64_t T, SatPlan *__restrict__ plan,
@@ -620,7 +656,7 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(7))
   // none at its first try leaves before touching LDS (the grid is sized for the device)
   uint32_t unit = 0;
   if (lane == 0) unit = atomicAdd(next_unit, 1u);
-  unit = __shfl(unit, 0, 64);
+  unit = __builtin_amdgcn_readfirstlane(unit);  // lane 0's value, in a scalar register
   if (uint64_t(unit) >= nunits) return;  // wave-uniform
 #pragma unroll
   for (int i = 0; i < kTable / 64; i++) {
@@ -634,7 +670,7 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(7))
   for (bool first = true;; first = false) {
   if (!first) {
     if (lane == 0) unit = atomicAdd(next_unit, 1u);
-    unit = __shfl(unit, 0, 64);
+    unit = __builtin_amdgcn_readfirstlane(unit);
     if (uint64_t(unit) >= nunits) break;  // wave-uniform exit: every wave ends here
   }
   const uint64_t c0 = uint64_t(unit) * chunks_per_unit;
@@ -716,13 +752,16 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(7))
       const TriRec r = rec[t];  // one copy: the record's loads issue together
       const uint64_t li = p - start;
       const uint32_t ex = r.mnz_ex >> 16;
-      if (!(r.flags & kWide)) {
-        const uint32_t l32 = uint32_t(li);
+      {
+        // the narrow decode runs on every lane (li < 2^22 unless kWide), so the record's
+        // reciprocals are loaded with the rest of it; wide triangles decode again below
+        const uint32_t l32 = uint32_t(li) & 0x3fffffu;
         z = udiv_narrow(l32, r.exy, r.rcp_exy);
-        const uint32_t rem = l32 - z * r.exy;
+        const uint32_t rem = l32 - mul24(z, r.exy);
         y = udiv_narrow(rem, ex, r.rcp_ex);
-        x = rem - y * ex;
-      } else {
+        x = rem - mul24(y, ex);
+      }
+      if (__builtin_expect((r.flags & kWide) != 0, 0)) {
         const uint64_t zz = li / r.exy, rem = li - zz * r.exy;
         z = uint32_t(zz);
         y = uint32_t(rem / ex);

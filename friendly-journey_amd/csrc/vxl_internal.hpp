@@ -105,7 +105,7 @@ struct vxl_ctx {
   std::vector<vxl::Slab> slabs;     // tree arena
   // tuning
   uint32_t sat_epoch = 0;  // setup + SAT calls enqueued (numbers the plan's parity slots)
-  int sat_waves_per_cu = 28;  // resident SAT waves per CU (persistent, work-queue fed)
+  int sat_waves_per_cu = 32;  // resident SAT waves per CU (8 per SIMD; persistent, work-queue fed)
   int sat_units_per_wave = 1;  // work units per resident wave
   int dda_refill = -1;  // > 0: persistent traversal lanes, refilled when that many are
                         // idle; 0: one lane per ray; -1: chosen per tree by its size
