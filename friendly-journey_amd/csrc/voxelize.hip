@@ -8,10 +8,11 @@
 // Stage 3 (k_bstart): the triangle holding the first pair of each work unit.
 // Stage 4 (k_sat): a persistent grid of waves takes work units from an atomic counter and
 //   walks them 64 pairs at a time, one pair per lane. The lane's triangle comes from the
-//   starts of the next 64 triangles, held one per lane: a DPP OR-reduce marks the chunk
-//   positions where triangles start, and the lane's triangle is the number of marks at
-//   or below it. The pair decodes to its voxel in the reference's scan order (x fastest,
-//   reference_cpu.rs:152-154), the plane test runs first, survivors queue in LDS, and
+//   starts of the next 64 triangles, held one per lane: a forward lane permute and a
+//   ballot mark the chunk positions where triangles start, and the lane's triangle is the
+//   number of marks at or below it. The pair decodes to its voxel in the reference's scan
+//   order (x fastest, reference_cpu.rs:152-154), the plane test runs first, survivors
+//   queue in LDS, and
 //   every 64 queued survivors run the remaining 12 axes as a full wave. Each hit merges
 //   into a per-wave LDS table keyed by leaf-grid word; the table is flushed with 64-bit
 //   atomic ORs into the leaf grid and the brick bitmap (bit per 8^3 brick).
@@ -233,19 +234,57 @@ __device__ __forceinline__ void note_windows(unsigned long long *acc, unsigned i
   }
 }
 
-__global__ void k_tri_setup(const float *__restrict__ tris, uint64_t T, VoxelizeParams p,
-                            uint64_t *__restrict__ count, TriRec *__restrict__ rec,
-                            WinRec *__restrict__ wrec, unsigned long long *__restrict__ skipped_acc,
-                            unsigned int *__restrict__ win, uint32_t epoch) {
-  const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+constexpr int kSetupTB = 256;  // setup threads per block (one triangle each)
+
+// Setup of one block's triangles: `tris`, `count`, `rec`, `wrec` point at the block's first
+// triangle, count, record and window record; nt <= kSetupTB triangles are valid. The
+// triangles come in through LDS with coalesced loads (lane k reads float k), and the
+// finished 80-byte records go out through the same LDS so that the block stores them as
+// consecutive 16-byte lanes instead of one 80-byte stride per lane. Returns the lane's
+// skipped-cell count (setup_one).
+__device__ __forceinline__ uint64_t setup_block(const float *__restrict__ tris, uint32_t nt,
+                                                const VoxelizeParams &p, uint32_t grid_base,
+                                                uint64_t *__restrict__ count,
+                                                TriRec *__restrict__ rec,
+                                                WinRec *__restrict__ wrec) {
+  constexpr uint32_t kStage = kSetupTB * sizeof(TriRec) / 16;
+  static_assert(kSetupTB * 9 * 4 <= kStage * 16, "the triangles fit the record stage");
+  __shared__ uint4 stage[kStage];
+  float *tin = reinterpret_cast<float *>(stage);
+  const uint32_t i = threadIdx.x;
+  for (uint32_t k = i; k < nt * 9; k += kSetupTB) tin[k] = tris[k];
+  __syncthreads();
+  float tri[9];
+  if (i < nt) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) tri[k] = tin[9 * i + k];
+  }
+  __syncthreads();
   uint64_t skipped = 0;
-  if (t < T) skipped = setup_one(tris + 9 * t, p, 0u, count + t, rec + t, wrec + t);
+  // a triangle without candidates leaves its slot unwritten: its record is never read
+  if (i < nt)
+    skipped = setup_one(tri, p, grid_base, count + i, reinterpret_cast<TriRec *>(stage) + i,
+                        wrec + i);
+  __syncthreads();
+  uint4 *dst = reinterpret_cast<uint4 *>(rec);
+  for (uint32_t k = i; k < nt * (sizeof(TriRec) / 16); k += kSetupTB) dst[k] = stage[k];
+  return skipped;
+}
+
+__global__ __launch_bounds__(kSetupTB) void k_tri_setup(
+    const float *__restrict__ tris, uint64_t T, VoxelizeParams p, uint64_t *__restrict__ count,
+    TriRec *__restrict__ rec, WinRec *__restrict__ wrec,
+    unsigned long long *__restrict__ skipped_acc, unsigned int *__restrict__ win,
+    uint32_t epoch) {
+  const uint64_t t0 = uint64_t(blockIdx.x) * kSetupTB;
+  const uint32_t nt = uint32_t(min(uint64_t(kSetupTB), T - t0));
+  const uint64_t skipped =
+      setup_block(tris + 9 * t0, nt, p, 0u, count + t0, rec + t0, wrec + t0);
   note_windows(skipped_acc, win, epoch, skipped, blockIdx.x % kSkipSlots);
 }
 
 // Many meshes: mesh g owns setup blocks [sb_begin[g], sb_begin[g+1]) and record slots
 // from sb_begin[g] * kSetupTB on; the slots past its triangle count hold count 0.
-constexpr int kSetupTB = 256;
 __global__ __launch_bounds__(kSetupTB) void k_tri_setup_batch(
     const BatchMesh *__restrict__ mesh,
     const uint32_t *__restrict__ sb_begin, uint32_t G, uint64_t *__restrict__ count,
@@ -253,13 +292,12 @@ __global__ __launch_bounds__(kSetupTB) void k_tri_setup_batch(
     unsigned long long *__restrict__ ref_pairs, unsigned int *__restrict__ win, uint32_t epoch) {
   const uint32_t g = dev::upper_index(sb_begin, G, blockIdx.x);
   const BatchMesh &m = mesh[g];
-  const uint64_t tl = uint64_t(blockIdx.x - sb_begin[g]) * kSetupTB + threadIdx.x;
-  const uint64_t slot = uint64_t(blockIdx.x) * kSetupTB + threadIdx.x;
-  uint64_t skipped = 0;
-  if (tl >= m.T) count[slot] = 0;
-  else
-    skipped = setup_one(m.tris + 9 * tl, m.p, uint32_t(m.b1_begin), count + slot, rec + slot,
-                        wrec + slot);
+  const uint64_t tl0 = uint64_t(blockIdx.x - sb_begin[g]) * kSetupTB;
+  const uint64_t slot0 = uint64_t(blockIdx.x) * kSetupTB;
+  const uint32_t nt = tl0 < m.T ? uint32_t(min(uint64_t(kSetupTB), m.T - tl0)) : 0u;
+  if (threadIdx.x >= nt) count[slot0 + threadIdx.x] = 0;
+  const uint64_t skipped = setup_block(m.tris + 9 * tl0, nt, m.p, uint32_t(m.b1_begin),
+                                       count + slot0, rec + slot0, wrec + slot0);
   note_windows(ref_pairs + g, win, epoch, skipped);  // a block belongs to one mesh
 }
 
@@ -439,34 +477,6 @@ __device__ __forceinline__ bool rest_passes(const Rel &v) {
 #undef VXL_DOT
 #undef VXL_AXIS
   return true;
-}
-
-__device__ __forceinline__ unsigned long long wave_or64(unsigned long long v) {
-  uint32_t lo = uint32_t(v), hi = uint32_t(v >> 32);
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    lo |= (uint32_t)__shfl_xor((int)lo, d, 64);
-    hi |= (uint32_t)__shfl_xor((int)hi, d, 64);
-  }
-  return (uint64_t(hi) << 32) | lo;
-}
-
-// OR over the 64 lanes of the wave, returned uniform: Hillis-Steele within each row of 16
-// lanes (row_shr 1, 2, 4, 8), then row_bcast 15 and 31 carry the row totals up to lane 63.
-__device__ __forceinline__ uint32_t wave_or_u32_dpp(uint32_t v) {
-  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);
-  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);
-  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);
-  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);
-  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, true);
-  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, true);
-  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
-}
-
-__device__ __forceinline__ unsigned long long wave_or64_dpp(unsigned long long v) {
-  const uint32_t lo = wave_or_u32_dpp(uint32_t(v));
-  const uint32_t hi = wave_or_u32_dpp(uint32_t(v >> 32));
-  return (uint64_t(hi) << 32) | lo;
 }
 
 __device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int lane) {
@@ -687,8 +697,15 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))
     const uint64_t p = base + uint64_t(lane);
     const bool valid = p < P;
     const uint64_t rpos = offn - base;  // >= 1
-    const unsigned long long marks = wave_or64_dpp(rpos < 64 ? (1ull << rpos) : 0ull);
-    const uint32_t starts_in = __popcll(__ballot(rpos < 64));
+    // Chunk positions where a triangle starts: every lane whose start falls in the chunk
+    // sends a flag to the lane of that position by a forward permute (the others send to
+    // lane 0, which no start occupies), and a ballot of the received flags is the mark
+    // mask. Lanes no one sends to receive 0. Two starts at one position (a triangle with
+    // no candidates between them) leave fewer marks than starts: the search below.
+    const bool in_chunk = rpos < 64;
+    const int flag = __builtin_amdgcn_ds_permute(in_chunk ? int(rpos) * 4 : 0, 1);
+    const unsigned long long marks = __ballot(flag != 0) & ~1ull;
+    const uint32_t starts_in = __popcll(__ballot(in_chunk));
     uint32_t t;
     uint64_t start;
     if (__popcll(marks) == starts_in) {
@@ -928,13 +945,13 @@ void reserve_records(vxl_ctx *ctx, uint64_t Tp) {
 
 void run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizeParams &p,
              uint32_t *owner, const uint32_t *leaf_map) {
-  const int tb = 256;
   reserve_records(ctx, T);
   SatPlan *plan = plan_area(ctx);
   const uint32_t e = ++ctx->sat_epoch;
-  hipLaunchKernelGGL(k_tri_setup, dim3((T + tb - 1) / tb), dim3(tb), 0, ctx->stream, d_tris, T,
-                     p, ctx->tri_count.as<uint64_t>(), ctx->rec.as<TriRec>(), win_records(ctx, T),
-                     plan->skipped[e & 1], &plan->win_epoch, e);
+  hipLaunchKernelGGL(k_tri_setup, dim3((T + kSetupTB - 1) / kSetupTB), dim3(kSetupTB), 0,
+                     ctx->stream, d_tris, T, p, ctx->tri_count.as<uint64_t>(),
+                     ctx->rec.as<TriRec>(), win_records(ctx, T), plan->skipped[e & 1],
+                     &plan->win_epoch, e);
   sat_after_setup(ctx, T, ctx->leafgrid.as<unsigned long long>(),
                   ctx->bitmap.as<unsigned long long>(), e, owner, leaf_map, p.n);
 }
