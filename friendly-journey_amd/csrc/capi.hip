@@ -347,7 +347,7 @@ void vxl_ctx_destroy(vxl_ctx *ctx) {
                          &ctx->batch_desc, &ctx->trav_desc,  &ctx->trav_ctr,
                          &ctx->owner_io, &ctx->color_io, &ctx->owner_scratch,
                          &ctx->leaf_map, &ctx->compact_io, &ctx->edit_io,
-                         &ctx->edit_scratch};
+                         &ctx->edit_scratch, &ctx->setup_sums};
   for (auto *b : bufs) b->release();
   // Trees outlive nothing they depend on: release their device memory now and detach
   // them; vxl_tree_free on a detached tree only frees the handle.

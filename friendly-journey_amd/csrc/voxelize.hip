@@ -3,9 +3,12 @@
 // Stage 1 (k_tri_setup / k_tri_setup_batch): world->grid transform
 //   (voxelizer/src/core.rs:101-107), the epsilon-padded, clamped candidate box of
 //   reference_cpu.rs:135-150, the candidate-voxel count and an 80-byte record per triangle.
-// Stage 2: exclusive scan of the counts gives every (triangle, voxel) pair a global index;
-//   the total is the only value read back before the SAT launch.
-// Stage 3 (k_bstart): the triangle holding the first pair of each work unit.
+// Stage 2: exclusive scan of the counts gives every (triangle, voxel) pair a global index.
+//   Setup also leaves each block's total; one workgroup scans those and derives the SAT
+//   plan from the grand total (k_setup_plan), all on the device with no host read-back.
+// Stage 3 (k_offsets_units): per setup block, the pair offsets and the triangle holding the
+//   first pair of each work unit. (Calls with over 16.7M triangles take a device-wide scan,
+//   k_sat_plan and k_bstart instead.)
 // Stage 4 (k_sat): a persistent grid of waves takes work units from an atomic counter and
 //   walks them 64 pairs at a time, one pair per lane. The lane's triangle comes from the
 //   starts of the next 64 triangles, held one per lane: a forward lane permute and a
@@ -105,7 +108,7 @@ __device__ __forceinline__ void tri_grid(const float *__restrict__ tri, const Vo
 // reference_cpu.rs:139-154 enumerates): 0 unless the triangle is windowed.
 __device__ __forceinline__ uint64_t setup_one(const float *__restrict__ tri,
                                               const VoxelizeParams &p, uint32_t grid_base,
-                                              uint64_t *__restrict__ count,
+                                              uint64_t *__restrict__ count, uint64_t &decoded_out,
                                               TriRec *__restrict__ rec,
                                               WinRec *__restrict__ wrec) {
   TriRec r;
@@ -128,6 +131,7 @@ __device__ __forceinline__ uint64_t setup_one(const float *__restrict__ tri,
     ext[a] = mx < mn ? 1u : uint32_t(mx - mn + 1);
   }
   *count = c;
+  decoded_out = c;
   if (c == 0) return 0;  // never read
   r.mnxy = uint32_t(r_mn[0]) | (uint32_t(r_mn[1]) << 16);
   r.mnz_ex = uint32_t(r_mn[2]) | (ext[0] << 16);
@@ -212,6 +216,7 @@ __device__ __forceinline__ uint64_t setup_one(const float *__restrict__ tri,
   // narrow decode: every local index < 2^22, so the float quotient is off by at most 1
   if (decoded >> 22) r.flags |= kWide;
   *count = decoded;
+  decoded_out = decoded;
   r.rcp_ex = 1.0f / float(r.mnz_ex >> 16);
   r.rcp_exy = 1.0f / float(r.exy);
   r.grid_base = grid_base;
@@ -236,17 +241,24 @@ __device__ __forceinline__ void note_windows(unsigned long long *acc, unsigned i
 
 constexpr int kSetupTB = 256;  // setup threads per block (one triangle each)
 
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) v += uint64_t(__shfl_xor((long long)v, d, 64));
+  return v;
+}
+
 // Setup of one block's triangles: `tris`, `count`, `rec`, `wrec` point at the block's first
 // triangle, count, record and window record; nt <= kSetupTB triangles are valid. The
 // triangles come in through LDS with coalesced loads (lane k reads float k), and the
 // finished 80-byte records go out through the same LDS so that the block stores them as
-// consecutive 16-byte lanes instead of one 80-byte stride per lane. Returns the lane's
-// skipped-cell count (setup_one).
+// consecutive 16-byte lanes instead of one 80-byte stride per lane. The block's candidate
+// total goes to *block_sum. Returns the lane's skipped-cell count (setup_one).
 __device__ __forceinline__ uint64_t setup_block(const float *__restrict__ tris, uint32_t nt,
                                                 const VoxelizeParams &p, uint32_t grid_base,
                                                 uint64_t *__restrict__ count,
                                                 TriRec *__restrict__ rec,
-                                                WinRec *__restrict__ wrec) {
+                                                WinRec *__restrict__ wrec,
+                                                uint64_t *__restrict__ block_sum) {
   constexpr uint32_t kStage = kSetupTB * sizeof(TriRec) / 16;
   static_assert(kSetupTB * 9 * 4 <= kStage * 16, "the triangles fit the record stage");
   __shared__ uint4 stage[kStage];
@@ -260,12 +272,21 @@ __device__ __forceinline__ uint64_t setup_block(const float *__restrict__ tris, 
     for (int k = 0; k < 9; k++) tri[k] = tin[9 * i + k];
   }
   __syncthreads();
-  uint64_t skipped = 0;
+  uint64_t skipped = 0, decoded = 0;
   // a triangle without candidates leaves its slot unwritten: its record is never read
   if (i < nt)
-    skipped = setup_one(tri, p, grid_base, count + i, reinterpret_cast<TriRec *>(stage) + i,
-                        wrec + i);
+    skipped = setup_one(tri, p, grid_base, count + i, decoded,
+                        reinterpret_cast<TriRec *>(stage) + i, wrec + i);
+  __shared__ uint64_t wave_sums[kSetupTB / 64];
+  decoded = wave_sum_u64(decoded);
+  if ((i & 63) == 0) wave_sums[i >> 6] = decoded;
   __syncthreads();
+  if (i == 0) {
+    uint64_t b = 0;
+#pragma unroll
+    for (int w = 0; w < kSetupTB / 64; w++) b += wave_sums[w];
+    *block_sum = b;
+  }
   uint4 *dst = reinterpret_cast<uint4 *>(rec);
   for (uint32_t k = i; k < nt * (sizeof(TriRec) / 16); k += kSetupTB) dst[k] = stage[k];
   return skipped;
@@ -273,13 +294,13 @@ __device__ __forceinline__ uint64_t setup_block(const float *__restrict__ tris, 
 
 __global__ __launch_bounds__(kSetupTB) void k_tri_setup(
     const float *__restrict__ tris, uint64_t T, VoxelizeParams p, uint64_t *__restrict__ count,
-    TriRec *__restrict__ rec, WinRec *__restrict__ wrec,
+    TriRec *__restrict__ rec, WinRec *__restrict__ wrec, uint64_t *__restrict__ block_sums,
     unsigned long long *__restrict__ skipped_acc, unsigned int *__restrict__ win,
     uint32_t epoch) {
   const uint64_t t0 = uint64_t(blockIdx.x) * kSetupTB;
   const uint32_t nt = uint32_t(min(uint64_t(kSetupTB), T - t0));
-  const uint64_t skipped =
-      setup_block(tris + 9 * t0, nt, p, 0u, count + t0, rec + t0, wrec + t0);
+  const uint64_t skipped = setup_block(tris + 9 * t0, nt, p, 0u, count + t0, rec + t0,
+                                       wrec + t0, block_sums + blockIdx.x);
   note_windows(skipped_acc, win, epoch, skipped, blockIdx.x % kSkipSlots);
 }
 
@@ -288,7 +309,7 @@ __global__ __launch_bounds__(kSetupTB) void k_tri_setup(
 __global__ __launch_bounds__(kSetupTB) void k_tri_setup_batch(
     const BatchMesh *__restrict__ mesh,
     const uint32_t *__restrict__ sb_begin, uint32_t G, uint64_t *__restrict__ count,
-    TriRec *__restrict__ rec, WinRec *__restrict__ wrec,
+    TriRec *__restrict__ rec, WinRec *__restrict__ wrec, uint64_t *__restrict__ block_sums,
     unsigned long long *__restrict__ ref_pairs, unsigned int *__restrict__ win, uint32_t epoch) {
   const uint32_t g = dev::upper_index(sb_begin, G, blockIdx.x);
   const BatchMesh &m = mesh[g];
@@ -297,7 +318,8 @@ __global__ __launch_bounds__(kSetupTB) void k_tri_setup_batch(
   const uint32_t nt = tl0 < m.T ? uint32_t(min(uint64_t(kSetupTB), m.T - tl0)) : 0u;
   if (threadIdx.x >= nt) count[slot0 + threadIdx.x] = 0;
   const uint64_t skipped = setup_block(m.tris + 9 * tl0, nt, m.p, uint32_t(m.b1_begin),
-                                       count + slot0, rec + slot0, wrec + slot0);
+                                       count + slot0, rec + slot0, wrec + slot0,
+                                       block_sums + blockIdx.x);
   note_windows(ref_pairs + g, win, epoch, skipped);  // a block belongs to one mesh
 }
 
@@ -335,6 +357,95 @@ __global__ void k_bstart(const uint64_t *__restrict__ coff, const uint64_t *__re
   uint64_t lo = coff[t], hi = lo + ccount[t];
   uint64_t b = (lo + chunk - 1) / chunk;
   for (; b * chunk < hi && b < nunits; b++) ustart[b] = uint32_t(t);
+}
+
+// ---- pair offsets and the plan from the setup blocks' sums -----------------------------
+// The setup kernels leave each block's candidate total in block_sums. When there are at
+// most kFusedBlocks of them, one workgroup scans those totals and derives the plan from
+// the grand total (k_setup_plan), and one pass over the counts writes the pair offsets and
+// the work-unit starts (k_offsets_units): three launches where a device-wide scan, the
+// plan and the unit-start pass take six.
+constexpr uint32_t kPlanThreads = 1024;
+constexpr uint64_t kFusedBlocks = 65536;
+
+// Exclusive scan of one value per thread over a block of NT threads; returns the total.
+template <int NT>
+__device__ __forceinline__ uint64_t block_excl_scan_u64(uint64_t v, uint64_t *excl) {
+  __shared__ uint64_t wave_tot[NT / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint64_t o = uint64_t(__shfl_up((long long)inc, d, 64));
+    if (lane >= d) inc += o;
+  }
+  if (lane == 63) wave_tot[wave] = inc;
+  __syncthreads();
+  uint64_t before = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; w++) {
+    const uint64_t t = wave_tot[w];
+    if (w < wave) before += t;
+    total += t;
+  }
+  *excl = before + inc - v;
+  return total;
+}
+
+// One workgroup: block_sums[0..nb) -> their exclusive prefix sums in place, the grand total
+// P to *total (= the offset past the last triangle), and the plan (k_sat_plan's fields).
+__global__ __launch_bounds__(kPlanThreads) void k_setup_plan(uint64_t *__restrict__ block_sums,
+                                                             uint64_t nb, uint64_t want,
+                                                             SatPlan *__restrict__ plan,
+                                                             uint32_t epoch,
+                                                             uint64_t *__restrict__ total) {
+  const uint64_t per = (nb + kPlanThreads - 1) / kPlanThreads;
+  const uint64_t b0 = min(uint64_t(threadIdx.x) * per, nb), b1 = min(b0 + per, nb);
+  uint64_t s = 0;
+  for (uint64_t b = b0; b < b1; b++) s += block_sums[b];
+  uint64_t run = 0;
+  const uint64_t P = block_excl_scan_u64<kPlanThreads>(s, &run);
+  for (uint64_t b = b0; b < b1; b++) {
+    const uint64_t v = block_sums[b];
+    block_sums[b] = run;
+    run += v;
+  }
+  if (threadIdx.x == 0) {
+    *total = P;
+    const uint64_t nchunks = (P + 63) / 64;
+    uint64_t cpu = (nchunks + want - 1) / want;
+    if (cpu < kChunksPerUnit) cpu = kChunksPerUnit;
+    plan->P = P;
+    unsigned long long skip = 0;
+    for (int i = 0; i < kSkipSlots; i++) {
+      skip += plan->skipped[epoch & 1][i];
+      plan->skipped[(epoch + 1) & 1][i] = 0;
+    }
+    plan->P_ref = P + skip;
+    plan->cpu = cpu;
+    plan->nunits = (nchunks + cpu - 1) / cpu;
+    plan->counter = 0;
+  }
+}
+
+// One setup block's counts per workgroup: the block-local exclusive scan plus the block's
+// offset is each triangle's first pair (coff); a triangle whose pairs hold the first pair of
+// work units is their start triangle (k_bstart's rule).
+__global__ __launch_bounds__(kSetupTB) void k_offsets_units(
+    const uint64_t *__restrict__ count, uint64_t *__restrict__ coff,
+    const uint64_t *__restrict__ block_offsets, uint64_t Tc, const SatPlan *__restrict__ plan,
+    uint32_t *__restrict__ ustart) {
+  const uint64_t t = uint64_t(blockIdx.x) * kSetupTB + threadIdx.x;
+  const uint64_t c = t < Tc ? count[t] : 0;
+  uint64_t excl = 0;
+  block_excl_scan_u64<kSetupTB>(c, &excl);
+  if (t >= Tc) return;
+  const uint64_t lo = block_offsets[blockIdx.x] + excl, hi = lo + c;
+  coff[t] = lo;
+  if (c == 0) return;
+  const uint64_t chunk = plan->cpu * 64, nunits = plan->nunits;
+  for (uint64_t b = (lo + chunk - 1) / chunk; b * chunk < hi && b < nunits; b++)
+    ustart[b] = uint32_t(t);
 }
 
 // ---- the SAT predicate, reference_cpu.rs:27-85 -----------------------------------------
@@ -903,15 +1014,24 @@ void sat_after_setup(vxl_ctx *ctx, uint64_t Tp, unsigned long long *leafgrid,
   TriRec *rec = ctx->rec.as<TriRec>();
   const WinRec *wrec = win_records(ctx, Tp);
   uint64_t *coff = ctx->coff.as<uint64_t>();
-  scan_exclusive_u64(count, coff, Tp, coff + Tp, ctx->scan_tmp, s);
   uint64_t max_waves = 0;
   const uint64_t want = plan_want(ctx, &max_waves);
   SatPlan *plan = plan_area(ctx);
   uint32_t *ustart = reinterpret_cast<uint32_t *>(ctx->bstart.as<char>() + kPlanBytes);
-  hipLaunchKernelGGL(k_sat_plan, dim3(1), dim3(1), 0, s, coff + Tp, want, plan, epoch);
-  if (Tp)
+  const uint64_t nb = (Tp + kSetupTB - 1) / kSetupTB;
+  if (nb <= kFusedBlocks) {
+    uint64_t *sums = ctx->setup_sums.as<uint64_t>();
+    hipLaunchKernelGGL(k_setup_plan, dim3(1), dim3(kPlanThreads), 0, s, sums, nb, want, plan,
+                       epoch, coff + Tp);
+    if (nb)
+      hipLaunchKernelGGL(k_offsets_units, dim3(uint32_t(nb)), dim3(kSetupTB), 0, s, count, coff,
+                         sums, Tp, plan, ustart);
+  } else {
+    scan_exclusive_u64(count, coff, Tp, coff + Tp, ctx->scan_tmp, s);
+    hipLaunchKernelGGL(k_sat_plan, dim3(1), dim3(1), 0, s, coff + Tp, want, plan, epoch);
     hipLaunchKernelGGL(k_bstart, dim3((Tp + tb - 1) / tb), dim3(tb), 0, s, coff, count, Tp, plan,
                        ustart);
+  }
   mark(ctx, 1);
   const uint64_t blocks = (max_waves + kWaves - 1) / kWaves;
   const dim3 g(static_cast<uint32_t>(blocks), 1, 1), b(kWaves * 64, 1, 1);
@@ -937,6 +1057,7 @@ void sat_after_setup(vxl_ctx *ctx, uint64_t Tp, unsigned long long *leafgrid,
 
 void reserve_records(vxl_ctx *ctx, uint64_t Tp) {
   ctx->tri_count.reserve(Tp * 8);
+  ctx->setup_sums.reserve(((Tp + kSetupTB - 1) / kSetupTB + 1) * 8);
   ctx->rec.reserve(Tp * (sizeof(TriRec) + sizeof(WinRec)));
   ctx->coff.reserve(Tp * 8 + 8);
 }
@@ -950,8 +1071,8 @@ void run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizeParams
   const uint32_t e = ++ctx->sat_epoch;
   hipLaunchKernelGGL(k_tri_setup, dim3((T + kSetupTB - 1) / kSetupTB), dim3(kSetupTB), 0,
                      ctx->stream, d_tris, T, p, ctx->tri_count.as<uint64_t>(),
-                     ctx->rec.as<TriRec>(), win_records(ctx, T), plan->skipped[e & 1],
-                     &plan->win_epoch, e);
+                     ctx->rec.as<TriRec>(), win_records(ctx, T), ctx->setup_sums.as<uint64_t>(),
+                     plan->skipped[e & 1], &plan->win_epoch, e);
   sat_after_setup(ctx, T, ctx->leafgrid.as<unsigned long long>(),
                   ctx->bitmap.as<unsigned long long>(), e, owner, leaf_map, p.n);
 }
@@ -968,8 +1089,8 @@ void run_sat_batch(vxl_ctx *ctx, const BatchMesh *d_mesh, const uint32_t *d_sb_b
   if (setup_blocks)
     hipLaunchKernelGGL(k_tri_setup_batch, dim3(setup_blocks), dim3(kSetupTB), 0, ctx->stream,
                        d_mesh, d_sb_begin, G, ctx->tri_count.as<uint64_t>(),
-                       ctx->rec.as<TriRec>(), win_records(ctx, Tp), d_ref_pairs,
-                       &plan->win_epoch, e);
+                       ctx->rec.as<TriRec>(), win_records(ctx, Tp),
+                       ctx->setup_sums.as<uint64_t>(), d_ref_pairs, &plan->win_epoch, e);
   sat_after_setup(ctx, Tp, leafgrid, brickbits, e);
 }
 

@@ -92,6 +92,7 @@ struct vxl_ctx {
   int cu_count = 0;
   // voxelize scratch
   vxl::DevBuf tris_in, tri_count, tri_flag, tri_cidx, rec, ccount, coff, bstart;
+  vxl::DevBuf setup_sums;  // per-setup-block candidate sums, scanned in place
   vxl::DevBuf leafgrid;  // zero between calls
   uint32_t leafgrid_bricks = 0;  // (n/8)^3 the grid is sized for
   bool leafgrid_dirty = false;
