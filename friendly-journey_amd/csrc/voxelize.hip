@@ -225,18 +225,13 @@ __device__ __forceinline__ uint64_t setup_one(const float *__restrict__ tri,
 }
 
 // The cells the wave's windows skip (0 for a lane without a windowed triangle), added once
-// to acc[slot]; a wave with any windowed triangle also stamps *win with the call's epoch.
-__device__ __forceinline__ void note_windows(unsigned long long *acc, unsigned int *win,
-                                             uint32_t epoch, uint64_t v, uint32_t slot = 0) {
+// to acc[slot].
+__device__ __forceinline__ void note_windows(unsigned long long *acc, uint64_t v,
+                                             uint32_t slot = 0) {
   if (__ballot(v != 0) == 0) return;  // wave-uniform: the common case
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) v += (uint64_t)__shfl_xor((long long)v, d, 64);
-  if ((threadIdx.x & 63) == 0) {
-    atomicAdd(acc + slot, (unsigned long long)v);
-    // one store is enough: later waves see it and leave the line alone
-    if (__hip_atomic_load(win, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch)
-      __hip_atomic_store(win, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  if ((threadIdx.x & 63) == 0) atomicAdd(acc + slot, (unsigned long long)v);
 }
 
 constexpr int kSetupTB = 256;  // setup threads per block (one triangle each)
@@ -295,13 +290,12 @@ __device__ __forceinline__ uint64_t setup_block(const float *__restrict__ tris, 
 __global__ __launch_bounds__(kSetupTB) void k_tri_setup(
     const float *__restrict__ tris, uint64_t T, VoxelizeParams p, uint64_t *__restrict__ count,
     TriRec *__restrict__ rec, WinRec *__restrict__ wrec, uint64_t *__restrict__ block_sums,
-    unsigned long long *__restrict__ skipped_acc, unsigned int *__restrict__ win,
-    uint32_t epoch) {
+    unsigned long long *__restrict__ skipped_acc) {
   const uint64_t t0 = uint64_t(blockIdx.x) * kSetupTB;
   const uint32_t nt = uint32_t(min(uint64_t(kSetupTB), T - t0));
   const uint64_t skipped = setup_block(tris + 9 * t0, nt, p, 0u, count + t0, rec + t0,
                                        wrec + t0, block_sums + blockIdx.x);
-  note_windows(skipped_acc, win, epoch, skipped, blockIdx.x % kSkipSlots);
+  note_windows(skipped_acc, skipped, blockIdx.x % kSkipSlots);
 }
 
 // Many meshes: mesh g owns setup blocks [sb_begin[g], sb_begin[g+1]) and record slots
@@ -310,7 +304,7 @@ __global__ __launch_bounds__(kSetupTB) void k_tri_setup_batch(
     const BatchMesh *__restrict__ mesh,
     const uint32_t *__restrict__ sb_begin, uint32_t G, uint64_t *__restrict__ count,
     TriRec *__restrict__ rec, WinRec *__restrict__ wrec, uint64_t *__restrict__ block_sums,
-    unsigned long long *__restrict__ ref_pairs, unsigned int *__restrict__ win, uint32_t epoch) {
+    unsigned long long *__restrict__ ref_pairs) {
   const uint32_t g = dev::upper_index(sb_begin, G, blockIdx.x);
   const BatchMesh &m = mesh[g];
   const uint64_t tl0 = uint64_t(blockIdx.x - sb_begin[g]) * kSetupTB;
@@ -320,7 +314,7 @@ __global__ __launch_bounds__(kSetupTB) void k_tri_setup_batch(
   const uint64_t skipped = setup_block(m.tris + 9 * tl0, nt, m.p, uint32_t(m.b1_begin),
                                        count + slot0, rec + slot0, wrec + slot0,
                                        block_sums + blockIdx.x);
-  note_windows(ref_pairs + g, win, epoch, skipped);  // a block belongs to one mesh
+  note_windows(ref_pairs + g, skipped);  // a block belongs to one mesh
 }
 
 constexpr uint64_t kChunksPerUnit = 16;  // at least 1024 pairs per work unit
@@ -410,42 +404,66 @@ __global__ __launch_bounds__(kPlanThreads) void k_setup_plan(uint64_t *__restric
     block_sums[b] = run;
     run += v;
   }
-  if (threadIdx.x == 0) {
-    *total = P;
-    const uint64_t nchunks = (P + 63) / 64;
-    uint64_t cpu = (nchunks + want - 1) / want;
-    if (cpu < kChunksPerUnit) cpu = kChunksPerUnit;
-    plan->P = P;
-    unsigned long long skip = 0;
-    for (int i = 0; i < kSkipSlots; i++) {
-      skip += plan->skipped[epoch & 1][i];
-      plan->skipped[(epoch + 1) & 1][i] = 0;
+  // the first wave: the skipped cells of this call's windows (one slot per lane), and the
+  // other epoch's slots cleared for the next call
+  static_assert(kSkipSlots == 64, "one skipped-cell slot per lane of the first wave");
+  if (threadIdx.x < 64) {
+    const uint64_t skip = wave_sum_u64(plan->skipped[epoch & 1][threadIdx.x]);
+    plan->skipped[(epoch + 1) & 1][threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+      *total = P;
+      const uint64_t nchunks = (P + 63) / 64;
+      uint64_t cpu = (nchunks + want - 1) / want;
+      if (cpu < kChunksPerUnit) cpu = kChunksPerUnit;
+      plan->P = P;
+      plan->P_ref = P + skip;
+      plan->cpu = cpu;
+      plan->nunits = (nchunks + cpu - 1) / cpu;
+      plan->counter = 0;
     }
-    plan->P_ref = P + skip;
-    plan->cpu = cpu;
-    plan->nunits = (nchunks + cpu - 1) / cpu;
-    plan->counter = 0;
   }
 }
 
-// One setup block's counts per workgroup: the block-local exclusive scan plus the block's
-// offset is each triangle's first pair (coff); a triangle whose pairs hold the first pair of
-// work units is their start triangle (k_bstart's rule).
-__global__ __launch_bounds__(kSetupTB) void k_offsets_units(
+// kOffsetBlocks setup blocks' counts per workgroup, one setup block per wave: the
+// wave-local exclusive scan of each lane's kOffsetItems consecutive counts plus the setup
+// block's offset is each triangle's first pair (coff); a triangle whose pairs hold the
+// first pair of work units is their start triangle (k_bstart's rule).
+constexpr int kOffsetItems = kSetupTB / 64;  // counts per lane: one setup block per wave
+constexpr int kOffsetBlocks = 4;              // setup blocks (waves) per workgroup
+__global__ __launch_bounds__(64 * kOffsetBlocks) void k_offsets_units(
     const uint64_t *__restrict__ count, uint64_t *__restrict__ coff,
-    const uint64_t *__restrict__ block_offsets, uint64_t Tc, const SatPlan *__restrict__ plan,
-    uint32_t *__restrict__ ustart) {
-  const uint64_t t = uint64_t(blockIdx.x) * kSetupTB + threadIdx.x;
-  const uint64_t c = t < Tc ? count[t] : 0;
-  uint64_t excl = 0;
-  block_excl_scan_u64<kSetupTB>(c, &excl);
-  if (t >= Tc) return;
-  const uint64_t lo = block_offsets[blockIdx.x] + excl, hi = lo + c;
-  coff[t] = lo;
-  if (c == 0) return;
+    const uint64_t *__restrict__ block_offsets, uint64_t nb, uint64_t Tc,
+    const SatPlan *__restrict__ plan, uint32_t *__restrict__ ustart) {
+  const uint64_t sb = uint64_t(blockIdx.x) * kOffsetBlocks + (threadIdx.x >> 6);
+  if (sb >= nb) return;  // wave-uniform
+  const int lane = threadIdx.x & 63;
+  const uint64_t t0 = sb * kSetupTB + uint64_t(lane) * kOffsetItems;
+  uint64_t c[kOffsetItems], s = 0;
+#pragma unroll
+  for (int k = 0; k < kOffsetItems; k++) {
+    c[k] = t0 + k < Tc ? count[t0 + k] : 0;
+    s += c[k];
+  }
+  uint64_t inc = s;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint64_t o = uint64_t(__shfl_up((long long)inc, d, 64));
+    if (lane >= d) inc += o;
+  }
+  uint64_t lo = block_offsets[sb] + inc - s;
   const uint64_t chunk = plan->cpu * 64, nunits = plan->nunits;
-  for (uint64_t b = (lo + chunk - 1) / chunk; b * chunk < hi && b < nunits; b++)
-    ustart[b] = uint32_t(t);
+#pragma unroll
+  for (int k = 0; k < kOffsetItems; k++) {
+    const uint64_t t = t0 + k;
+    if (t < Tc) {
+      coff[t] = lo;
+      const uint64_t hi = lo + c[k];
+      if (c[k])
+        for (uint64_t b = (lo + chunk - 1) / chunk; b * chunk < hi && b < nunits; b++)
+          ustart[b] = uint32_t(t);
+    }
+    lo += c[k];
+  }
 }
 
 // ---- the SAT predicate, reference_cpu.rs:27-85 -----------------------------------------
@@ -747,24 +765,21 @@ __device__ __forceinline__ void table_flush(unsigned long long *tk, unsigned lon
 // queued survivors run phase B as a full wave. Hits are merged per leaf word in an LDS
 // table and flushed to HBM in bulk.
 //
-// Two builds: kWin decodes windowed triangles; both are launched and the one that does not
-// match the call (windowed triangles present or not) leaves at once, so calls without a
-// windowed triangle run the build that carries no window decode. Both are held to 64
-// VGPRs (8 waves per SIMD, the most a SIMD holds; the LDS of 8 blocks, 8 x 14 KiB, fits
-// a CU). Unit-level values (unit, chunk range, first triangle) stay wave-uniform in
-// scalar registers, which is what keeps the per-lane state within 64 VGPRs.
+// The kernel is held to 64 VGPRs (8 waves per SIMD, the most a SIMD holds; the LDS of 8
+// blocks, 8 x 14 KiB, fits a CU). Unit-level values (unit, chunk range, first triangle)
+// stay wave-uniform in scalar registers, which is what keeps the per-lane state within 64
+// VGPRs. The window decode is a branch that calls without windowed triangles never take.
 //
 // kOwn: also lower each hit voxel's owner to the triangle index (single-mesh calls, where
 // the record index is the triangle index); kOwnLeaf writes owners only.
-template <bool kWin, int kOwn>
+template <int kOwn>
 __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))) void k_sat(
     const TriRec *__restrict__ rec, const WinRec *__restrict__ wrec,
     const uint64_t *__restrict__ coff,
     const uint32_t *__restrict__ ustart, uint64_t T, SatPlan *__restrict__ plan,
     unsigned long long *__restrict__ leafgrid, unsigned long long *__restrict__ brickbits,
-    uint32_t epoch, uint32_t *__restrict__ owner, const uint32_t *__restrict__ leaf_map,
+    uint32_t *__restrict__ owner, const uint32_t *__restrict__ leaf_map,
     uint32_t own_n) {
-  if ((plan->win_epoch == epoch) != kWin) return;  // uniform over the grid
   __shared__ uint32_t q_t[kWaves][kQueue], q_xy[kWaves][kQueue], q_z[kWaves][kQueue];
   __shared__ unsigned long long t_key[kWaves][kTable], t_val[kWaves][kTable];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -896,7 +911,7 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))
         x = uint32_t(rem - uint64_t(y) * ex);
       }
       bool inside = true;
-      if (kWin && __builtin_expect((r.flags & kWindow) != 0, 0)) {
+      if (__builtin_expect((r.flags & kWindow) != 0, 0)) {
         // (w, la, lb) -> the cell w of the window of column (a, b) along c
         const WinRec wf = wrec[t];
         const uint32_t cinfo = wf.cinfo;
@@ -1024,8 +1039,8 @@ void sat_after_setup(vxl_ctx *ctx, uint64_t Tp, unsigned long long *leafgrid,
     hipLaunchKernelGGL(k_setup_plan, dim3(1), dim3(kPlanThreads), 0, s, sums, nb, want, plan,
                        epoch, coff + Tp);
     if (nb)
-      hipLaunchKernelGGL(k_offsets_units, dim3(uint32_t(nb)), dim3(kSetupTB), 0, s, count, coff,
-                         sums, Tp, plan, ustart);
+      hipLaunchKernelGGL(k_offsets_units, dim3(uint32_t((nb + kOffsetBlocks - 1) / kOffsetBlocks)),
+                         dim3(64 * kOffsetBlocks), 0, s, count, coff, sums, nb, Tp, plan, ustart);
   } else {
     scan_exclusive_u64(count, coff, Tp, coff + Tp, ctx->scan_tmp, s);
     hipLaunchKernelGGL(k_sat_plan, dim3(1), dim3(1), 0, s, coff + Tp, want, plan, epoch);
@@ -1035,22 +1050,15 @@ void sat_after_setup(vxl_ctx *ctx, uint64_t Tp, unsigned long long *leafgrid,
   mark(ctx, 1);
   const uint64_t blocks = (max_waves + kWaves - 1) / kWaves;
   const dim3 g(static_cast<uint32_t>(blocks), 1, 1), b(kWaves * 64, 1, 1);
-  if (owner && leaf_map) {
-    hipLaunchKernelGGL((k_sat<false, kOwnLeaf>), g, b, 0, s, rec, wrec, coff, ustart, Tp, plan,
-                       leafgrid, brickbits, epoch, owner, leaf_map, own_n);
-    hipLaunchKernelGGL((k_sat<true, kOwnLeaf>), g, b, 0, s, rec, wrec, coff, ustart, Tp, plan,
-                       leafgrid, brickbits, epoch, owner, leaf_map, own_n);
-  } else if (owner) {
-    hipLaunchKernelGGL((k_sat<false, kOwnDense>), g, b, 0, s, rec, wrec, coff, ustart, Tp, plan,
-                       leafgrid, brickbits, epoch, owner, nullptr, own_n);
-    hipLaunchKernelGGL((k_sat<true, kOwnDense>), g, b, 0, s, rec, wrec, coff, ustart, Tp, plan,
-                       leafgrid, brickbits, epoch, owner, nullptr, own_n);
-  } else {
-    hipLaunchKernelGGL((k_sat<false, kOwnNone>), g, b, 0, s, rec, wrec, coff, ustart, Tp, plan,
-                       leafgrid, brickbits, epoch, nullptr, nullptr, 0u);
-    hipLaunchKernelGGL((k_sat<true, kOwnNone>), g, b, 0, s, rec, wrec, coff, ustart, Tp, plan,
-                       leafgrid, brickbits, epoch, nullptr, nullptr, 0u);
-  }
+  if (owner && leaf_map)
+    hipLaunchKernelGGL((k_sat<kOwnLeaf>), g, b, 0, s, rec, wrec, coff, ustart, Tp, plan, leafgrid,
+                       brickbits, owner, leaf_map, own_n);
+  else if (owner)
+    hipLaunchKernelGGL((k_sat<kOwnDense>), g, b, 0, s, rec, wrec, coff, ustart, Tp, plan, leafgrid,
+                       brickbits, owner, nullptr, own_n);
+  else
+    hipLaunchKernelGGL((k_sat<kOwnNone>), g, b, 0, s, rec, wrec, coff, ustart, Tp, plan, leafgrid,
+                       brickbits, nullptr, nullptr, 0u);
   mark(ctx, 2);
   VXL_HIP(hipGetLastError());
 }
@@ -1072,7 +1080,7 @@ void run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizeParams
   hipLaunchKernelGGL(k_tri_setup, dim3((T + kSetupTB - 1) / kSetupTB), dim3(kSetupTB), 0,
                      ctx->stream, d_tris, T, p, ctx->tri_count.as<uint64_t>(),
                      ctx->rec.as<TriRec>(), win_records(ctx, T), ctx->setup_sums.as<uint64_t>(),
-                     plan->skipped[e & 1], &plan->win_epoch, e);
+                     plan->skipped[e & 1]);
   sat_after_setup(ctx, T, ctx->leafgrid.as<unsigned long long>(),
                   ctx->bitmap.as<unsigned long long>(), e, owner, leaf_map, p.n);
 }
@@ -1090,7 +1098,7 @@ void run_sat_batch(vxl_ctx *ctx, const BatchMesh *d_mesh, const uint32_t *d_sb_b
     hipLaunchKernelGGL(k_tri_setup_batch, dim3(setup_blocks), dim3(kSetupTB), 0, ctx->stream,
                        d_mesh, d_sb_begin, G, ctx->tri_count.as<uint64_t>(),
                        ctx->rec.as<TriRec>(), win_records(ctx, Tp),
-                       ctx->setup_sums.as<uint64_t>(), d_ref_pairs, &plan->win_epoch, e);
+                       ctx->setup_sums.as<uint64_t>(), d_ref_pairs);
   sat_after_setup(ctx, Tp, leafgrid, brickbits, e);
 }
 

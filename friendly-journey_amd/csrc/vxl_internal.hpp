@@ -231,7 +231,6 @@ struct SatPlan {
   // of call e sums them and clears skipped[(e + 1) & 1] for the next call on the stream
   unsigned long long skipped[2][kSkipSlots];
   unsigned int counter;    // next unit to hand out
-  unsigned int win_epoch;  // == e when call e holds a windowed triangle
 };
 // Enqueues setup + SAT into ctx->leafgrid / ctx->bitmap (level-1); no host round trip.
 // The reference's pair count stays on the device at sat_pairs_dev(ctx) until the next call.
