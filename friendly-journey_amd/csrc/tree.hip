@@ -719,15 +719,89 @@ struct BuildSizes {
   uint32_t tot[32];
 };
 
-__global__ void k_build_sizes(const uint64_t *__restrict__ pn2_total,
-                              const uint32_t *__restrict__ tot, int top,
-                              const uint64_t *__restrict__ pairs, BuildSizes *__restrict__ out) {
-  const int i = threadIdx.x;
-  if (i == 0) {
-    out->pn2_total = *pn2_total;
+// ---- single-tree level 2 without a device-wide scan ------------------------------------
+// k_level2_sums leaves each block's sum of the packed (popcount | nonzero << 32) values of
+// its B1 words (a block's popcounts sum to at most 256 * 64, so the halves never carry);
+// k_small_levels_sizes scans those sums in its one workgroup before the upper levels and
+// then writes the sizes the host reads; k_emit_a2_local rebuilds each word's packed offset
+// from a block-local scan plus the block's offset.
+__device__ __forceinline__ uint64_t packed_word(unsigned long long w) {
+  return uint64_t(__popcll(w)) | (uint64_t(w != 0ull) << 32);
+}
+
+// Exclusive scan of one u64 per thread over a block of NT threads; returns the total.
+template <int NT>
+__device__ __forceinline__ uint64_t block_scan_u64(uint64_t v, uint64_t *excl) {
+  __shared__ uint64_t wave_tot[NT / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint64_t o = uint64_t(__shfl_up((long long)inc, d, 64));
+    if (lane >= d) inc += o;
+  }
+  if (lane == 63) wave_tot[wave] = inc;
+  __syncthreads();
+  uint64_t before = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; w++) {
+    const uint64_t t = wave_tot[w];
+    if (w < wave) before += t;
+    total += t;
+  }
+  *excl = before + inc - v;
+  __syncthreads();
+  return total;
+}
+
+__global__ __launch_bounds__(kTB) void k_level2_sums(const unsigned long long *__restrict__ bin,
+                                                       uint64_t win,
+                                                       uint64_t *__restrict__ block_sums,
+                                                       unsigned long long *__restrict__ bout) {
+  const uint64_t j = uint64_t(blockIdx.x) * kTB + threadIdx.x;
+  const unsigned long long w = j < win ? bin[j] : 0ull;
+  const unsigned long long b = __ballot(w != 0ull);
+  if (bout && (threadIdx.x & 63) == 0 && j < win) bout[j >> 6] = b;
+  uint64_t excl;
+  const uint64_t total = block_scan_u64<kTB>(packed_word(w), &excl);
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+// One workgroup: the level-2 block sums (at most kSmallThreads of them) to their exclusive
+// prefix sums in place, the upper levels 3..top, and the sizes (packed level-2 total, SAT
+// pairs, per-level totals) into the host-mapped BuildSizes.
+__global__ __launch_bounds__(kSmallThreads) void k_small_levels_sizes(
+    LevelPtrs lp, int top, const unsigned long long *__restrict__ b2, uint32_t *__restrict__ tot,
+    uint64_t *__restrict__ block_sums, uint32_t nb2, const uint64_t *__restrict__ pairs,
+    BuildSizes *__restrict__ out) {
+  const uint32_t tid = threadIdx.x;
+  const uint64_t v = tid < nb2 ? block_sums[tid] : 0;
+  uint64_t excl;
+  const uint64_t pn2_total = block_scan_u64<kSmallThreads>(v, &excl);
+  if (tid < nb2) block_sums[tid] = excl;
+  if (top >= 3) small_levels(lp, 3, top, b2, tot);
+  __syncthreads();  // tot[] of every level is written
+  if (tid == 0) {
+    out->pn2_total = pn2_total;
     out->pairs = pairs ? *pairs : 0;
   }
-  if (i < 32) out->tot[i] = (top >= 3 && i >= 6 && i < 2 * top + 2) ? tot[i] : 0u;
+  if (tid < 32) out->tot[tid] = (top >= 3 && tid >= 6 && int(tid) < 2 * top + 2) ? tot[tid] : 0u;
+}
+
+__global__ __launch_bounds__(kTB) void k_emit_a2_local(
+    const unsigned long long *__restrict__ bin, uint64_t win,
+    const uint64_t *__restrict__ block_offsets, unsigned long long *__restrict__ a_mask,
+    uint32_t *__restrict__ a_base, uint32_t *__restrict__ a_code) {
+  const uint64_t j = uint64_t(blockIdx.x) * kTB + threadIdx.x;
+  const unsigned long long w = j < win ? bin[j] : 0ull;
+  uint64_t excl;
+  block_scan_u64<kTB>(packed_word(w), &excl);
+  if (!w) return;
+  const uint64_t v = block_offsets[blockIdx.x] + excl;
+  const uint32_t i = uint32_t(v >> 32);
+  a_mask[i] = w;
+  a_base[i] = uint32_t(v);
+  a_code[i] = uint32_t(j);
 }
 
 // n <= 2048 (k <= 4): every level above 2 has at most 4096 input words.
@@ -748,7 +822,10 @@ static vxl_tree *build_tree_fast(vxl_ctx *ctx, uint32_t n, uint32_t k, const uin
   ctx->levelscratch.reserve(scratch);
   char *p = static_cast<char *>(ctx->levelscratch.ptr);
   uint32_t *tot = carve<uint32_t>(p, 64);
-  uint64_t *pn2 = carve<uint64_t>(p, w1 + 1);
+  const uint32_t nb2 = uint32_t(div_up(w1, kTB));  // level-2 blocks
+  if (nb2 > uint32_t(kSmallThreads))  // n <= 2048: (256^3 / 64) / 256 = 1024 blocks at most
+    throw Error(VXL_INVALID_ARG, "tree build: level-2 block count exceeds one workgroup");
+  uint64_t *l2sums = carve<uint64_t>(p, nb2 + 1);
   unsigned long long *b2 = top >= 3 ? carve<unsigned long long>(p, lp.win[3]) : nullptr;
   for (int L = 2; L <= top; L++) {
     const uint64_t e = lp.win[L] + 1;
@@ -762,16 +839,13 @@ static vxl_tree *build_tree_fast(vxl_ctx *ctx, uint32_t n, uint32_t k, const uin
     lp.bs[L] = carve<uint32_t>(p, e);
   }
   unsigned long long *b1 = ctx->bitmap.as<unsigned long long>();
-  // 1. level 2: packed popcount/flag scan over B1, B2 by ballot
-  hipLaunchKernelGGL(k_level2, dim3(div_up(w1, kTB)), dim3(kTB), 0, s, b1, w1, pn2, b2);
-  scan_exclusive_u64(pn2, pn2, w1, pn2 + w1, ctx->scan_tmp, s);
-  // 2. upper levels in one workgroup
-  if (top >= 3)
-    hipLaunchKernelGGL(k_small_levels, dim3(1), dim3(kSmallThreads), 0, s, lp, 3, top, b2, tot);
-  // 3. totals (the one host round trip of the call)
+  // 1. level 2: per-block sums of the packed popcount/flag values of B1, B2 by ballot
+  hipLaunchKernelGGL(k_level2_sums, dim3(nb2), dim3(kTB), 0, s, b1, w1, l2sums, b2);
+  // 2. the level-2 block offsets, the upper levels and the sizes, in one workgroup
   BuildSizes *hs = static_cast<BuildSizes *>(ctx->host_pinned);
-  hipLaunchKernelGGL(k_build_sizes, dim3(1), dim3(64), 0, s, pn2 + w1, tot, top, pairs_dev,
-                     static_cast<BuildSizes *>(ctx->host_pinned_dev));
+  hipLaunchKernelGGL(k_small_levels_sizes, dim3(1), dim3(kSmallThreads), 0, s, lp, top, b2, tot,
+                     l2sums, nb2, pairs_dev, static_cast<BuildSizes *>(ctx->host_pinned_dev));
+  // 3. the sizes (the one host round trip of the call)
   wait_stream(ctx, s);
   const uint64_t leaves = uint32_t(hs->pn2_total);
   const uint64_t n2 = hs->pn2_total >> 32;
@@ -787,8 +861,8 @@ static vxl_tree *build_tree_fast(vxl_ctx *ctx, uint32_t n, uint32_t k, const uin
     for (int L = 2; L <= int(k) + 1; L++) t->level_nodes[L] = level_nodes[L];
   if (leaves == 0) return t;
   // 4. level-2 School-A arrays, leaf gather (clears the grid and B1)
-  hipLaunchKernelGGL(k_emit_a2, dim3(div_up(w1, kTB)), dim3(kTB), 0, s, b1, w1, pn2,
-                     lp.a_mask[2], lp.a_base[2], lp.a_code[2]);
+  hipLaunchKernelGGL(k_emit_a2_local, dim3(nb2), dim3(kTB), 0, s, b1, w1, l2sums, lp.a_mask[2],
+                     lp.a_base[2], lp.a_code[2]);
   hipLaunchKernelGGL(k_leaf_gather, dim3(div_up(n2 * 64, kTB)), dim3(kTB), 0, s, n2,
                      lp.a_mask[2], lp.a_base[2], lp.a_code[2],
                      ctx->leafgrid.as<unsigned long long>(), b1,
