@@ -253,3 +253,16 @@ def test_traverse_graph_replays_equal_eager(ctx):
         torch.cuda.synchronize()
         assert torch.equal(h, h_eager) and torch.equal(t, t_eager), f"replay {rep}"
     graph.free()
+
+
+def test_batch_over_16m_triangles_equals_single(ctx):
+    """A batch whose meshes need more than 65,536 setup blocks scans the candidate counts
+    device-wide; its trees still equal the per-mesh calls."""
+    base = W.uv_sphere(48, 24)
+    reps = -(-(65536 * 256 + 1) // len(base))
+    mix = [("tiled_sphere_128", np.tile(base, (reps, 1)), 128, 1 / 128),
+           ("soup_512", W.soup(3000, seed=31, mu=np.log(2.0 / 256)), 512, 1 / 256)]
+    _say(f"batch with {sum(len(t) for _, t, _, _ in mix)} triangles vs single calls")
+    batch = ctx.voxelize_batch([t for _, t, _, _ in mix], [(n, (0, 0, 0), vs) for _, _, n, vs in mix])
+    for (name, tris, n, vs), bt in zip(mix, batch):
+        _assert_same_tree(bt, ctx.voxelize_tree(tris, n, (0, 0, 0), vs, 1e-4), name)

@@ -142,3 +142,18 @@ def test_tree_from_words_fixtures(ctx):
         _say(f"from_words {kind} n={n}")
         words = O.fixture_words(kind, n, *params)
         _tree_equal(ctx.tree_from_words(words, n), O.Tree.build(words, n))
+
+
+def test_over_16m_triangles_take_the_device_wide_scan(ctx):
+    """Calls with more than 65,536 setup blocks (16,777,216 triangles) scan the candidate
+    counts device-wide instead of in one workgroup. A small sphere tiled past that size
+    voxelizes to the sphere's own tree, with the sphere's pair count times the tile count."""
+    base = W.uv_sphere(48, 24)
+    n, vs = 128, 1.0 / 128
+    reps = -(-(65536 * 256 + 1) // len(base))
+    tris = np.tile(base, (reps, 1))
+    _say(f"tiled sphere: {len(tris)} triangles ({reps} copies of {len(base)}) at n={n}")
+    words, _, pairs = O.voxelize_surface(base, n, (0, 0, 0), vs, 1e-4)
+    tree = ctx.voxelize_tree(tris, n, (0, 0, 0), vs, 1e-4)
+    assert tree.pairs == reps * pairs
+    _tree_equal(tree, O.Tree.build(words, n))
