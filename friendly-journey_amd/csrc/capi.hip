@@ -1,6 +1,7 @@
 // extern "C" entry points of libvxl.so (declared in include/vxl.h).
 #include <algorithm>
 #include <cmath>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 #include <cstdio>
@@ -114,12 +115,77 @@ void validate_grid(const vxl_grid *g, uint32_t *k) {
     if (!std::isfinite(g->origin[a])) throw Error(VXL_INVALID_ARG, "origin_world must be finite");
 }
 
-// MeshInput::validate (core.rs:427-459): every vertex finite.
-void validate_mesh_host(const float *tris, uint64_t T) {
-  if (T && !tris) throw Error(VXL_INVALID_ARG, "triangles pointer is null");
-  for (uint64_t i = 0; i < T * 9; i++)
-    if (!std::isfinite(tris[i])) throw Error(VXL_INVALID_ARG, "triangle contains a non-finite vertex");
+// Whether any of n binary32 values is not finite (all 8 exponent bits set): branch-free,
+// so the compiler vectorises it.
+bool any_non_finite(const uint32_t *src, uint64_t n) {
+  uint32_t bad = 0;
+  for (uint64_t i = 0; i < n; i++) bad |= uint32_t((src[i] & 0x7f800000u) == 0x7f800000u);
+  return bad != 0;
 }
+
+// The same check while copying the values from src to dst.
+bool copy_check_finite(const uint32_t *src, uint32_t *dst, uint64_t n) {
+  uint32_t bad = 0;
+  for (uint64_t i = 0; i < n; i++) {
+    const uint32_t v = src[i];
+    dst[i] = v;
+    bad |= uint32_t((v & 0x7f800000u) == 0x7f800000u);
+  }
+  return bad != 0;
+}
+
+// MeshInput::validate (core.rs:427-459: every vertex finite) fused with the upload of the
+// host triangles into ctx->tris_in on ctx->stream. Large meshes go in chunks of 16M values
+// through two pinned buffers: up to 8 threads copy and check a chunk, then its DMA runs
+// while the next chunk is copied into the other buffer (a buffer is reused once its last
+// DMA has completed). A non-finite value fails the call with VXL_INVALID_ARG before any
+// kernel runs, as the reference validates the mesh before any device work.
+void upload_mesh_host(vxl_ctx *ctx, const float *tris, uint64_t T) {
+  if (T && !tris) throw Error(VXL_INVALID_ARG, "triangles pointer is null");
+  if (T == 0) return;
+  const uint64_t nf = T * 9;
+  ctx->tris_in.reserve(nf * 4);
+  uint32_t *dst = ctx->tris_in.as<uint32_t>();
+  const uint32_t *src = reinterpret_cast<const uint32_t *>(tris);
+  const Error bad_vertex(VXL_INVALID_ARG, "triangle contains a non-finite vertex");
+  constexpr uint64_t kSmall = 1u << 18;  // values: below this, one thread and a direct copy
+  if (nf < kSmall) {
+    if (any_non_finite(src, nf)) throw bad_vertex;
+    VXL_HIP(hipMemcpyAsync(dst, src, nf * 4, hipMemcpyHostToDevice, ctx->stream));
+    return;
+  }
+  constexpr uint64_t kChunk = 16u << 20;  // values per chunk (64 MiB)
+  for (int b = 0; b < 2; b++) {
+    if (!ctx->up_host[b]) VXL_HIP(hipHostMalloc(&ctx->up_host[b], kChunk * 4, hipHostMallocDefault));
+    if (!ctx->up_ev[b]) VXL_HIP(hipEventCreateWithFlags(&ctx->up_ev[b], hipEventDisableTiming));
+  }
+  const unsigned hw = std::thread::hardware_concurrency();
+  const unsigned nth = std::max(1u, std::min(8u, hw ? hw : 1u));
+  for (uint64_t off = 0, c = 0; off < nf; off += kChunk, c++) {
+    const uint64_t len = std::min(kChunk, nf - off);
+    const int b = int(c & 1);
+    if (ctx->up_pending[b]) {
+      VXL_HIP(hipEventSynchronize(ctx->up_ev[b]));
+      ctx->up_pending[b] = false;
+    }
+    uint32_t *stage = static_cast<uint32_t *>(ctx->up_host[b]);
+    const uint64_t slice = (len + nth - 1) / nth;
+    std::vector<char> bad(nth, 0);
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < nth; t++) {
+      const uint64_t lo = std::min(len, t * slice), hi = std::min(len, lo + slice);
+      pool.emplace_back([=, &bad] { bad[t] = copy_check_finite(src + off + lo, stage + lo, hi - lo); });
+    }
+    bad[0] = copy_check_finite(src + off, stage, std::min(len, slice));
+    for (auto &th : pool) th.join();
+    for (char x : bad)
+      if (x) throw bad_vertex;
+    VXL_HIP(hipMemcpyAsync(dst + off, stage, len * 4, hipMemcpyHostToDevice, ctx->stream));
+    VXL_HIP(hipEventRecord(ctx->up_ev[b], ctx->stream));
+    ctx->up_pending[b] = true;
+  }
+}
+
 
 void validate_count(uint64_t T) {
   if (T > 0xffffffffull) throw Error(VXL_INVALID_ARG, "triangle count must be < 2^32");
@@ -361,6 +427,10 @@ void vxl_ctx_destroy(vxl_ctx *ctx) {
     if (e) (void)hipEventDestroy(e);
   if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
   if (ctx->batch_host) (void)hipHostFree(ctx->batch_host);
+  for (int b = 0; b < 2; b++) {
+    if (ctx->up_host[b]) (void)hipHostFree(ctx->up_host[b]);
+    if (ctx->up_ev[b]) (void)hipEventDestroy(ctx->up_ev[b]);
+  }
   if (ctx->staging_free) (void)hipEventDestroy(ctx->staging_free);
   if (ctx->trav_done) (void)hipEventDestroy(ctx->trav_done);
   if (ctx->ev_wait) (void)hipEventDestroy(ctx->ev_wait);
@@ -447,14 +517,10 @@ vxl_status vxl_voxelize_tree(vxl_ctx *ctx, const float *tris, uint64_t T, const 
     uint32_t k = 0;
     validate_grid(grid, &k);
     validate_count(T);
-    validate_mesh_host(tris, T);
+    upload_mesh_host(ctx, tris, T);  // validates, then copies
     validate_eps(epsilon);
     const float *d = nullptr;
-    if (T) {
-      ctx->tris_in.reserve(T * 36);
-      VXL_HIP(hipMemcpyAsync(ctx->tris_in.ptr, tris, T * 36, hipMemcpyHostToDevice, ctx->stream));
-      d = ctx->tris_in.as<float>();
-    }
+    if (T) d = ctx->tris_in.as<float>();
     *out = voxelize_tree_impl(ctx, d, T, grid, k, epsilon, pairs_out);
     ctx->trees.insert(*out);
     VXL_HIP(hipStreamSynchronize(ctx->stream));
@@ -494,7 +560,7 @@ vxl_status vxl_voxelize_dense(vxl_ctx *ctx, const float *tris, uint64_t T, const
     uint32_t k = 0;
     validate_grid(grid, &k);
     validate_count(T);
-    validate_mesh_host(tris, T);
+    upload_mesh_host(ctx, tris, T);  // validates, then copies
     validate_eps(epsilon);
     if (!occ_words) throw Error(VXL_INVALID_ARG, "occupancy output is null");
     const uint64_t nn = grid->n;
@@ -504,9 +570,7 @@ vxl_status vxl_voxelize_dense(vxl_ctx *ctx, const float *tris, uint64_t T, const
       if (pairs_out) *pairs_out = 0;
       return;
     }
-    ctx->tris_in.reserve(T * 36);
     ctx->words_io.reserve(nwords * 4);
-    VXL_HIP(hipMemcpyAsync(ctx->tris_in.ptr, tris, T * 36, hipMemcpyHostToDevice, ctx->stream));
     vxl::prepare_grid(ctx, grid->n);
     vxl::run_sat(ctx, ctx->tris_in.as<float>(), T, make_params(grid, epsilon));
     if (pairs_out) *pairs_out = read_pairs(ctx);
@@ -569,18 +633,14 @@ vxl_status vxl_voxelize_dense_owner(vxl_ctx *ctx, const float *tris, uint64_t T,
     uint32_t k = 0;
     validate_grid(grid, &k);
     validate_count(T);
-    validate_mesh_host(tris, T);
+    upload_mesh_host(ctx, tris, T);  // validates, then copies
     validate_eps(epsilon);
     if (!occ_words) throw Error(VXL_INVALID_ARG, "occupancy output is null");
     const uint64_t nn = grid->n;
     const uint64_t nv = nn * nn * nn;
     const uint64_t nwords = (nv + 31) / 32;
     const float *d = nullptr;
-    if (T) {
-      ctx->tris_in.reserve(T * 36);
-      VXL_HIP(hipMemcpyAsync(ctx->tris_in.ptr, tris, T * 36, hipMemcpyHostToDevice, ctx->stream));
-      d = ctx->tris_in.as<float>();
-    }
+    if (T) d = ctx->tris_in.as<float>();
     ctx->words_io.reserve(nwords * 4);
     uint32_t *d_owner = nullptr, *d_color = nullptr;
     if (owner) {
@@ -613,13 +673,10 @@ vxl_status vxl_voxelize_tree_materials(vxl_ctx *ctx, const float *tris, uint64_t
     uint32_t k = 0;
     validate_grid(grid, &k);
     validate_count(T);
-    validate_mesh_host(tris, T);
+    upload_mesh_host(ctx, tris, T);  // validates, then copies
     validate_eps(epsilon);
     if (table_len && !material_table) throw Error(VXL_INVALID_ARG, "material table is null");
-    if (T) {
-      ctx->tris_in.reserve(T * 36);
-      VXL_HIP(hipMemcpyAsync(ctx->tris_in.ptr, tris, T * 36, hipMemcpyHostToDevice, ctx->stream));
-    }
+
     const uint32_t *d_table = nullptr;
     if (table_len) {
       ctx->words_io.reserve(uint64_t(table_len) * 4);
@@ -1094,13 +1151,9 @@ vxl_status vxl_voxelize_batch(vxl_ctx *ctx, const float *tris, const uint64_t *t
       throw Error(VXL_INVALID_ARG, "null offsets, grids or out");
     const uint64_t T = count ? tri_offsets[count] - tri_offsets[0] : 0;
     if (count && tri_offsets[0] != 0) throw Error(VXL_INVALID_ARG, "tri_offsets[0] must be 0");
-    validate_mesh_host(tris, T);
+    upload_mesh_host(ctx, tris, T);  // validates, then copies
     const float *d = nullptr;
-    if (T) {
-      ctx->tris_in.reserve(T * 36);
-      VXL_HIP(hipMemcpyAsync(ctx->tris_in.ptr, tris, T * 36, hipMemcpyHostToDevice, ctx->stream));
-      d = ctx->tris_in.as<float>();
-    }
+    if (T) d = ctx->tris_in.as<float>();
     voxelize_batch_impl(ctx, d, tri_offsets, count, grids, epsilon, out, pairs_out);
     VXL_HIP(hipStreamSynchronize(ctx->stream));
   });
