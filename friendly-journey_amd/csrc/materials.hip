@@ -43,16 +43,39 @@ __global__ void k_leaf_map(const uint64_t *__restrict__ codes, uint64_t L,
   if (i < L) map[codes[i]] = uint32_t(i);
 }
 
-__global__ void k_leaf_materials(const uint64_t *__restrict__ leaves,
-                                 const uint32_t *__restrict__ owner, uint64_t N,
-                                 const uint32_t *__restrict__ table, uint32_t len,
-                                 uint16_t *__restrict__ mats) {
-  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < N;
+// Owners start at UINT32_MAX (no owner) at occupied leaf voxels only: the owner pass hits
+// no other voxel, and the materials pass reads no other owner. One thread per leaf word.
+__global__ void k_owner_init(const uint64_t *__restrict__ leaves, uint64_t W,
+                             uint32_t *__restrict__ owner) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < W;
        i += uint64_t(gridDim.x) * blockDim.x) {
-    const uint64_t leaf = i >> 9;
-    const uint32_t m = uint32_t(i & 511);
-    const bool occ = (leaves[leaf * 8 + (m >> 6)] >> (m & 63)) & 1ull;
-    mats[i] = occ ? uint16_t(resolve_material(owner[i], table, len)) : uint16_t(0);
+    unsigned long long bits = leaves[i];
+    while (bits) {
+      const int b = __ffsll(bits) - 1;
+      bits &= bits - 1ull;
+      owner[i * 64 + uint64_t(b)] = 0xFFFFFFFFu;
+    }
+  }
+}
+
+// One thread per 8 consecutive leaf voxels (one 16-byte store of 8 materials): an occupied
+// voxel's material is its owner's table entry, an empty voxel's 0. Owners are read for
+// occupied voxels only.
+__global__ void k_leaf_materials(const uint64_t *__restrict__ leaves,
+                                 const uint32_t *__restrict__ owner, uint64_t N8,
+                                 const uint32_t *__restrict__ table, uint32_t len,
+                                 uint4 *__restrict__ mats8) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < N8;
+       i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint32_t occ = uint32_t(leaves[i >> 3] >> ((i & 7) * 8)) & 0xffu;
+    uint32_t m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (occ) {
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if ((occ >> k) & 1u) m[k] = resolve_material(owner[i * 8 + k], table, len);
+    }
+    mats8[i] = make_uint4(m[0] | (m[1] << 16), m[2] | (m[3] << 16), m[4] | (m[5] << 16),
+                          m[6] | (m[7] << 16));
   }
 }
 
@@ -191,11 +214,15 @@ vxl_tree *voxelize_tree_materials(vxl_ctx *ctx, const float *d_tris, uint64_t T,
     uint32_t *map = leaf_map(ctx, t);
     ctx->owner_scratch.reserve(N * 4);
     uint32_t *owner = ctx->owner_scratch.as<uint32_t>();
-    VXL_HIP(hipMemsetAsync(owner, 0xFF, N * 4, ctx->stream));  // UINT32_MAX: no owner
-    run_sat(ctx, d_tris, T, p, owner, map);
+    const uint64_t W = t->leaf_count * 8;
+    hipLaunchKernelGGL(k_owner_init, dim3(stride_blocks(W)), dim3(kTB), 0, ctx->stream,
+                       reinterpret_cast<const uint64_t *>(t->leaves), W, owner);
+    // the owner pass walks the same pairs: it reuses the records, offsets and plan above
+    sat_owner_pass(ctx, T, owner, map, n);
     t->materials = alloc_materials(t->leaf_count);
-    hipLaunchKernelGGL(k_leaf_materials, dim3(stride_blocks(N)), dim3(kTB), 0, ctx->stream,
-                       t->leaves, owner, N, d_table, table_len, t->materials);
+    hipLaunchKernelGGL(k_leaf_materials, dim3(stride_blocks(N / 8)), dim3(kTB), 0, ctx->stream,
+                       reinterpret_cast<const uint64_t *>(t->leaves), owner, N / 8, d_table,
+                       table_len, reinterpret_cast<uint4 *>(t->materials));
     VXL_HIP(hipGetLastError());
   } catch (...) {
     tree_free_device(t);

@@ -1087,6 +1087,26 @@ void run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizeParams
 
 const uint64_t *sat_pairs_dev(vxl_ctx *ctx) { return &ctx->bstart.as<SatPlan>()->P_ref; }
 
+namespace {
+__global__ void k_plan_rearm(SatPlan *__restrict__ plan) { plan->counter = 0; }
+}  // namespace
+
+void sat_owner_pass(vxl_ctx *ctx, uint64_t T, uint32_t *owner, const uint32_t *leaf_map,
+                    uint32_t own_n) {
+  hipStream_t s = ctx->stream;
+  SatPlan *plan = ctx->bstart.as<SatPlan>();
+  uint32_t *ustart = reinterpret_cast<uint32_t *>(ctx->bstart.as<char>() + kPlanBytes);
+  hipLaunchKernelGGL(k_plan_rearm, dim3(1), dim3(1), 0, s, plan);
+  uint64_t max_waves = 0;
+  (void)plan_want(ctx, &max_waves);
+  const dim3 g(static_cast<uint32_t>((max_waves + kWaves - 1) / kWaves), 1, 1), b(kWaves * 64, 1, 1);
+  hipLaunchKernelGGL((k_sat<kOwnLeaf>), g, b, 0, s, ctx->rec.as<TriRec>(), win_records(ctx, T),
+                     ctx->coff.as<uint64_t>(), ustart, T, plan,
+                     ctx->leafgrid.as<unsigned long long>(), ctx->bitmap.as<unsigned long long>(),
+                     owner, leaf_map, own_n);
+  VXL_HIP(hipGetLastError());
+}
+
 void run_sat_batch(vxl_ctx *ctx, const BatchMesh *d_mesh, const uint32_t *d_sb_begin,
                    uint32_t G, uint32_t setup_blocks, unsigned long long *leafgrid,
                    unsigned long long *brickbits, unsigned long long *d_ref_pairs) {

@@ -256,6 +256,10 @@ uint64_t compact_tree(vxl_ctx *ctx, const vxl_tree *t, const int32_t g[3], int32
                       uint64_t capacity);
 vxl_tree *tree_from_compact(vxl_ctx *ctx, uint32_t n, uint32_t k, const int32_t *d_vox,
                             uint64_t count, uint64_t *dropped);
+// The owner pass of the call run_sat just enqueued (same triangles): k_sat in leaf-owner
+// mode over the records, offsets and plan that call left, with the work counter re-armed.
+void sat_owner_pass(vxl_ctx *ctx, uint64_t T, uint32_t *owner, const uint32_t *leaf_map,
+                    uint32_t own_n);
 const uint64_t *sat_pairs_dev(vxl_ctx *ctx);
 // Setup + SAT of a batch group into the pooled grid; mesh g owns setup blocks
 // [sb_begin[g], sb_begin[g+1]) (setup_slots_per_block() record slots each).
