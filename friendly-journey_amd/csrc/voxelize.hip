@@ -163,19 +163,21 @@ __device__ __forceinline__ uint64_t setup_one(const float *__restrict__ tri,
   uint64_t decoded = c;
   // Window (hoisted triangles whose box is at least twice the window): with c the axis of
   // the normal's largest component, a cell passes the plane test only if its centre lies
-  // within hw = R / |n_c| of the plane along c, where R bounds the test's radius plus the
+  // within R / |n_c| of the plane along c, where R bounds the test's radius plus the
   // rounding of |d| (relative 1e-4 of rp and of sum |n_i| |v0_i| over the box, orders of
-  // magnitude above the few-ulp errors of the binary32 sums), and hw carries one more cell of
-  // slack for the binary32 evaluation of the window start (values below 2^15, so its
-  // rounding stays below 0.01 of a cell). So the cells floor(c0 - hw - 1/2) .. + W - 1 of
-  // each column, W = ceil(2 hw) + 2, hold every candidate that can pass; the rest of the box
-  // is never enumerated. Cells outside the bounding box are skipped when decoded. A window
-  // never has fewer than 4 cells (hw > 1), so boxes shorter than 8 along c keep the box.
+  // magnitude above the few-ulp errors of the binary32 sums). The window of a column starts
+  // at lo = floor(c0 - hw - 1/2), c0 the plane's c at the column centre as binary32 gives
+  // it (error e) and hw = R / |n_c| + s, slack s > e; then lo is at or below the lowest
+  // passing cell, and W >= 2 hw - s + e + 2 cells reach the highest one: W = ceil(2 hw +
+  // 1.95) with e < 1/16 when s = 1/8. The start k0 + ka a + kb b takes about ten roundings
+  // of values below mag (the sum of the magnitudes it adds), so e < mag 2^-20: s = 1/8 for
+  // mag < 2^16, s = 1 for mag < 2^19, and no window beyond. Cells outside the bounding box
+  // are skipped when decoded; the rest of the box is never enumerated.
   // (axis picks are selects: a runtime index into r or ext would put them in scratch)
   const float an0 = fabsf(r.nrm[0]), an1 = fabsf(r.nrm[1]), an2 = fabsf(r.nrm[2]);
   const bool c1 = an1 > an0 && an1 >= an2, c2 = !c1 && an2 > an0 && an2 > an1;
   const uint32_t ext_c = c1 ? ext[1] : (c2 ? ext[2] : ext[0]);
-  if (hoist && ext_c >= 8) {
+  if (hoist && ext_c >= 6) {
     const float nc = c1 ? r.nrm[1] : (c2 ? r.nrm[2] : r.nrm[0]);
     if (nc != 0.0f) {
       float S = 0.0f;
@@ -186,23 +188,29 @@ __device__ __forceinline__ uint64_t setup_one(const float *__restrict__ tri,
         S += fabsf(r.nrm[i]) * fmaxf(fabsf(lo), fabsf(hi));
       }
       const float R = r.rp * (1.0f + 1e-4f) + 1e-4f * S;
-      const float hw = R / fabsf(nc) + 1.0f;
-      const float wd = ceilf(2.0f * hw) + 2.0f;
       // the other two axes in increasing order: a = first, b = second
       const bool c0 = !c1 && !c2;
       const uint32_t ext_a = c0 ? ext[1] : ext[0], ext_b = c2 ? ext[1] : ext[2];
-      if (wd <= 15.0f) {
+      const float na = c0 ? r.nrm[1] : r.nrm[0], nb = c2 ? r.nrm[1] : r.nrm[2];
+      const float ga = c0 ? r.g[1] : r.g[0], gb = c2 ? r.g[1] : r.g[2];
+      const float gc = c1 ? r.g[1] : (c2 ? r.g[2] : r.g[0]);
+      const int32_t ma = c0 ? r_mn[1] : r_mn[0], mb = c2 ? r_mn[1] : r_mn[2];
+      const float ka = -na / nc, kb = -nb / nc;
+      const float tc = (na * (0.5f - ga) + nb * (0.5f - gb)) / nc;
+      const float mag = fabsf(gc) + fabsf(tc) + fabsf(ka) * float(ma + int32_t(ext_a)) +
+                        fabsf(kb) * float(mb + int32_t(ext_b)) + R / fabsf(nc) + 2.0f;
+      const float slack = mag < 65536.0f ? 0.125f : 1.0f;
+      const float hw = R / fabsf(nc) + slack;
+      const float wd = ceilf(2.0f * hw + 1.95f);
+      if (wd <= 15.0f && mag < 524288.0f) {
         const uint32_t W = uint32_t(wd);
         const uint64_t decw = uint64_t(W) * ext_a * ext_b;
         if (2 * decw <= c) {
-          const float na = c0 ? r.nrm[1] : r.nrm[0], nb = c2 ? r.nrm[1] : r.nrm[2];
-          const float ga = c0 ? r.g[1] : r.g[0], gb = c2 ? r.g[1] : r.g[2];
-          const float gc = c1 ? r.g[1] : (c2 ? r.g[2] : r.g[0]);
           const uint32_t ax = c1 ? 1u : (c2 ? 2u : 0u);
           WinRec w;
-          w.ka = -na / nc;
-          w.kb = -nb / nc;
-          w.k0 = gc - (na * (0.5f - ga) + nb * (0.5f - gb)) / nc - hw - 0.5f;
+          w.ka = ka;
+          w.kb = kb;
+          w.k0 = gc - tc - hw - 0.5f;
           w.cinfo = ax | (W << 2) | (ext_c << 6);
           *wrec = w;
           r.mnz_ex = uint32_t(r_mn[2]) | (W << 16);
