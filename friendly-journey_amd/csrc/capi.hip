@@ -197,6 +197,42 @@ void download_host(vxl_ctx *ctx, void *dst, const void *src, size_t bytes) {
   copy_out(prev, prev_off, prev_len);
 }
 
+// Host -> device copy of `bytes` from src (the caller's host memory) to dst (device),
+// enqueued on ctx->stream; src may be reused on return. Large copies go in 64 MiB chunks:
+// up to 8 threads copy a chunk into a pinned buffer, whose DMA then overlaps the copy of
+// the next chunk into the other buffer.
+void upload_host(vxl_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (!bytes) return;
+  if (bytes < (size_t(4) << 20)) {
+    VXL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return;
+  }
+  stage_buffers(ctx);
+  const unsigned nth = stage_threads();
+  char *out = static_cast<char *>(dst);
+  const char *in = static_cast<const char *>(src);
+  for (size_t off = 0, c = 0; off < bytes; off += kStageBytes, c++) {
+    const size_t len = std::min(kStageBytes, bytes - off);
+    const int b = int(c & 1);
+    if (ctx->up_pending[b]) {
+      VXL_HIP(hipEventSynchronize(ctx->up_ev[b]));
+      ctx->up_pending[b] = false;
+    }
+    char *stage = static_cast<char *>(ctx->up_host[b]);
+    const size_t slice = (len + nth - 1) / nth;
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < nth; t++) {
+      const size_t lo = std::min(len, t * slice), hi = std::min(len, lo + slice);
+      if (hi > lo) pool.emplace_back([=] { std::memcpy(stage + lo, in + off + lo, hi - lo); });
+    }
+    std::memcpy(stage, in + off, std::min(len, slice));
+    for (auto &th : pool) th.join();
+    VXL_HIP(hipMemcpyAsync(out + off, stage, len, hipMemcpyHostToDevice, ctx->stream));
+    VXL_HIP(hipEventRecord(ctx->up_ev[b], ctx->stream));
+    ctx->up_pending[b] = true;
+  }
+}
+
 // MeshInput::validate (core.rs:427-459: every vertex finite) fused with the upload of the
 // host triangles into ctx->tris_in on ctx->stream. Large meshes go in chunks of 16M values
 // through two pinned buffers: up to 8 threads copy and check a chunk, then its DMA runs
@@ -817,8 +853,7 @@ vxl_status vxl_tree_from_compact(vxl_ctx *ctx, uint32_t n, const vxl_compact_vox
     if (count && !voxels) throw Error(VXL_INVALID_ARG, "voxels pointer is null");
     if (count) {
       ctx->compact_io.reserve(count * 16);
-      VXL_HIP(hipMemcpyAsync(ctx->compact_io.ptr, voxels, count * 16, hipMemcpyHostToDevice,
-                             ctx->stream));
+      upload_host(ctx, ctx->compact_io.ptr, voxels, count * 16);
     }
     *out = vxl::tree_from_compact(ctx, n, k, ctx->compact_io.as<int32_t>(), count, dropped);
     ctx->trees.insert(*out);
@@ -1062,8 +1097,7 @@ vxl_status vxl_tree_from_words(vxl_ctx *ctx, const uint32_t *words, uint32_t n, 
     const uint64_t nn = n;
     const uint64_t nwords = (nn * nn * nn + 31) / 32;
     ctx->words_io.reserve(nwords * 4);
-    VXL_HIP(hipMemcpyAsync(ctx->words_io.ptr, words, nwords * 4, hipMemcpyHostToDevice,
-                           ctx->stream));
+    upload_host(ctx, ctx->words_io.ptr, words, nwords * 4);
     vxl::prepare_grid(ctx, n);
     vxl::words_to_grid(ctx, n, ctx->words_io.as<uint32_t>());
     *out = vxl::build_tree_from_grid(ctx, n, k);
@@ -1131,10 +1165,10 @@ vxl_status vxl_tree_upload(vxl_ctx *ctx, uint32_t n, const vxl_node *nodes, uint
       VXL_HIP(hipMalloc(&t->codes, leaf_count ? leaf_count * 8 : 16));
       hipStream_t s = ctx->stream;
       if (node_count)
-        VXL_HIP(hipMemcpyAsync(t->nodes, nodes, node_count * 12, hipMemcpyHostToDevice, s));
+        upload_host(ctx, t->nodes, nodes, node_count * 12);
       if (leaf_count) {
-        VXL_HIP(hipMemcpyAsync(t->leaves, leaf_words, leaf_count * 64, hipMemcpyHostToDevice, s));
-        VXL_HIP(hipMemcpyAsync(t->bounds, bounds, leaf_count * 4, hipMemcpyHostToDevice, s));
+        upload_host(ctx, t->leaves, leaf_words, leaf_count * 64);
+        upload_host(ctx, t->bounds, bounds, leaf_count * 4);
         VXL_HIP(hipMemsetAsync(t->codes, 0, leaf_count * 8, s));
       }
       VXL_HIP(hipStreamSynchronize(s));
@@ -1244,7 +1278,7 @@ vxl_status vxl_traverse_batch(vxl_ctx *ctx, const vxl_tree *const *trees, uint32
     ctx->rays_io.reserve(R * 24);
     ctx->hits_io.reserve(R * 16);
     if (t) ctx->t_io.reserve(R * 8);
-    VXL_HIP(hipMemcpyAsync(ctx->rays_io.ptr, rays, R * 24, hipMemcpyHostToDevice, s));
+    upload_host(ctx, ctx->rays_io.ptr, rays, R * 24);
     const auto segs = segments(trees, count, ray_offsets, ctx->rays_io.as<float>(),
                                ctx->hits_io.as<uint32_t>(), t ? ctx->t_io.as<double>() : nullptr);
     vxl::launch_traverse_batch(ctx, segs.data(), count, s);
@@ -1374,7 +1408,7 @@ vxl_status vxl_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *rays, u
     ctx->rays_io.reserve(R * 24);
     ctx->hits_io.reserve(R * 16);
     if (t) ctx->t_io.reserve(R * 8);
-    VXL_HIP(hipMemcpyAsync(ctx->rays_io.ptr, rays, R * 24, hipMemcpyHostToDevice, s));
+    upload_host(ctx, ctx->rays_io.ptr, rays, R * 24);
     vxl::launch_traverse(ctx, tree, ctx->rays_io.as<float>(), R, ctx->hits_io.as<uint32_t>(),
                          t ? ctx->t_io.as<double>() : nullptr, s);
     download_host(ctx, hits, ctx->hits_io.ptr, R * 16);
