@@ -226,6 +226,39 @@ __global__ void k_place(uint64_t nl, int is_level2, const unsigned long long *__
   nodes_b[3ull * mypos + 2] = is_level2 ? a_base[c] : mybs;
 }
 
+// Node entries of one tree once its node array exists: threads [0, n2) place the level-2
+// nodes under their level-3 parents (k_place's rule, child_base = the School-A leaf base);
+// the rest write the entries of levels 3..k+1, whose positions upper_place already
+// computed. Upper node u belongs to level L when ubeg[L] <= u < ubeg[L + 1].
+struct UpperNodes {
+  uint32_t ubeg[kLevels + 1];
+};
+
+__global__ void k_place_tree(uint64_t n2, uint64_t total, LevelPtrs lp, int k, UpperNodes un,
+                             uint32_t *__restrict__ nodes_b) {
+  const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= total) return;
+  if (c < n2) {
+    const uint32_t q = lp.a_code[2][c];
+    const uint32_t p = lp.ni[3][q >> 6];
+    const uint32_t mypos = lp.bs[3][p] + (uint32_t(c) - lp.a_base[3][p]);
+    const unsigned long long m = lp.a_mask[2][c];
+    nodes_b[3ull * mypos + 0] = uint32_t(m);
+    nodes_b[3ull * mypos + 1] = uint32_t(m >> 32);
+    nodes_b[3ull * mypos + 2] = lp.a_base[2][c];
+    return;
+  }
+  const uint32_t u = uint32_t(c - n2);
+  int L = 3;
+  while (L < k + 1 && u >= un.ubeg[L + 1]) L++;
+  const uint32_t i = u - un.ubeg[L];
+  const uint32_t mypos = lp.pos[L][i];
+  const unsigned long long m = lp.a_mask[L][i];
+  nodes_b[3ull * mypos + 0] = uint32_t(m);
+  nodes_b[3ull * mypos + 1] = uint32_t(m >> 32);
+  nodes_b[3ull * mypos + 2] = lp.bs[L][i];
+}
+
 // x-major dense words from the leaf grid: word i covers x0..x0+31 of row (y, z).
 __global__ void k_grid_to_words(uint32_t n, uint64_t nwords,
                                 const unsigned long long *__restrict__ leafgrid,
@@ -393,7 +426,8 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_levels_batch(
 }
 
 // Subtree sizes (levels 3..k+1), the root, and School-B placement of levels k..3, in one
-// workgroup; the level-2 placement follows as a multi-block pass.
+// workgroup; the level-2 placement follows as a multi-block pass. With nodes_b null only the
+// positions (pos, bs) are computed; the node entries are written later (k_place_tree).
 __device__ __forceinline__ void upper_place(const LevelPtrs &lp, int k,
                                             const uint32_t *__restrict__ tot,
                                             uint32_t *__restrict__ nodes_b) {
@@ -425,10 +459,12 @@ __device__ __forceinline__ void upper_place(const LevelPtrs &lp, int k,
     __syncthreads();
   }
   if (tid == 0) {
-    const unsigned long long m = lp.a_mask[k + 1][0];
-    nodes_b[0] = uint32_t(m);
-    nodes_b[1] = uint32_t(m >> 32);
-    nodes_b[2] = 1u;
+    if (nodes_b) {
+      const unsigned long long m = lp.a_mask[k + 1][0];
+      nodes_b[0] = uint32_t(m);
+      nodes_b[1] = uint32_t(m >> 32);
+      nodes_b[2] = 1u;
+    }
     lp.pos[k + 1][0] = 0;
     lp.bs[k + 1][0] = 1;
   }
@@ -445,19 +481,15 @@ __device__ __forceinline__ void upper_place(const LevelPtrs &lp, int k,
       const uint32_t mybs = pbs + mp + (lp.desc[L][c] - lp.desc[L][pb]);
       lp.pos[L][c] = mypos;
       lp.bs[L][c] = mybs;
-      const unsigned long long m = lp.a_mask[L][c];
-      nodes_b[3ull * mypos + 0] = uint32_t(m);
-      nodes_b[3ull * mypos + 1] = uint32_t(m >> 32);
-      nodes_b[3ull * mypos + 2] = mybs;
+      if (nodes_b) {
+        const unsigned long long m = lp.a_mask[L][c];
+        nodes_b[3ull * mypos + 0] = uint32_t(m);
+        nodes_b[3ull * mypos + 1] = uint32_t(m >> 32);
+        nodes_b[3ull * mypos + 2] = mybs;
+      }
     }
     __syncthreads();
   }
-}
-
-__global__ __launch_bounds__(kSmallThreads) void k_upper_place(LevelPtrs lp, int k,
-                                                                 const uint32_t *__restrict__ tot,
-                                                                 uint32_t *__restrict__ nodes_b) {
-  upper_place(lp, k, tot, nodes_b);
 }
 
 // ---- batch groups -------------------------------------------------------------------
@@ -768,8 +800,10 @@ __global__ __launch_bounds__(kTB) void k_level2_sums(const unsigned long long *_
 }
 
 // One workgroup: the level-2 block sums (at most kSmallThreads of them) to their exclusive
-// prefix sums in place, the upper levels 3..top, and the sizes (packed level-2 total, SAT
-// pairs, per-level totals) into the host-mapped BuildSizes.
+// prefix sums in place, the upper levels 3..top with their School-B positions (k = top - 1
+// >= 2), and the sizes (packed level-2 total, SAT pairs, per-level totals) into the
+// host-mapped BuildSizes. The positions need no output array, so they are done here, before
+// the host reads the sizes and allocates the tree.
 __global__ __launch_bounds__(kSmallThreads) void k_small_levels_sizes(
     LevelPtrs lp, int top, const unsigned long long *__restrict__ b2, uint32_t *__restrict__ tot,
     uint64_t *__restrict__ block_sums, uint32_t nb2, const uint64_t *__restrict__ pairs,
@@ -779,7 +813,11 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_levels_sizes(
   uint64_t excl;
   const uint64_t pn2_total = block_scan_u64<kSmallThreads>(v, &excl);
   if (tid < nb2) block_sums[tid] = excl;
-  if (top >= 3) small_levels(lp, 3, top, b2, tot);
+  if (top >= 3) {
+    small_levels(lp, 3, top, b2, tot);
+    __syncthreads();  // tot[] of every level is written
+    upper_place(lp, top - 1, tot, nullptr);
+  }
   __syncthreads();  // tot[] of every level is written
   if (tid == 0) {
     out->pn2_total = pn2_total;
@@ -845,7 +883,10 @@ static vxl_tree *build_tree_fast(vxl_ctx *ctx, uint32_t n, uint32_t k, const uin
   BuildSizes *hs = static_cast<BuildSizes *>(ctx->host_pinned);
   hipLaunchKernelGGL(k_small_levels_sizes, dim3(1), dim3(kSmallThreads), 0, s, lp, top, b2, tot,
                      l2sums, nb2, pairs_dev, static_cast<BuildSizes *>(ctx->host_pinned_dev));
-  // 3. the sizes (the one host round trip of the call)
+  // 3. level-2 School-A arrays (no output array needed: queued before the host waits)
+  hipLaunchKernelGGL(k_emit_a2_local, dim3(nb2), dim3(kTB), 0, s, b1, w1, l2sums, lp.a_mask[2],
+                     lp.a_base[2], lp.a_code[2]);
+  // 4. the sizes (the one host round trip of the call)
   wait_stream(ctx, s);
   const uint64_t leaves = uint32_t(hs->pn2_total);
   const uint64_t n2 = hs->pn2_total >> 32;
@@ -860,24 +901,27 @@ static vxl_tree *build_tree_fast(vxl_ctx *ctx, uint32_t n, uint32_t k, const uin
   if (k >= 1 && leaves)
     for (int L = 2; L <= int(k) + 1; L++) t->level_nodes[L] = level_nodes[L];
   if (leaves == 0) return t;
-  // 4. level-2 School-A arrays, leaf gather (clears the grid and B1)
-  hipLaunchKernelGGL(k_emit_a2_local, dim3(nb2), dim3(kTB), 0, s, b1, w1, l2sums, lp.a_mask[2],
-                     lp.a_base[2], lp.a_code[2]);
+  // 5. leaf gather (clears the grid and B1)
   hipLaunchKernelGGL(k_leaf_gather, dim3(div_up(n2 * 64, kTB)), dim3(kTB), 0, s, n2,
                      lp.a_mask[2], lp.a_base[2], lp.a_code[2],
                      ctx->leafgrid.as<unsigned long long>(), b1,
                      reinterpret_cast<unsigned long long *>(t->leaves), t->bounds,
                      reinterpret_cast<unsigned long long *>(t->codes), nullptr, 0u, nullptr);
-  // 5. School-B placement
+  // 6. School-B node entries (upper positions came with the sizes)
   if (k == 1) {
     hipLaunchKernelGGL(k_root, dim3(1), dim3(1), 0, s, lp.a_mask[2], lp.a_base[2], 1, t->nodes,
                        lp.pos[2], lp.bs[2]);
   } else if (k >= 2) {
-    hipLaunchKernelGGL(k_upper_place, dim3(1), dim3(kSmallThreads), 0, s, lp, int(k), tot,
-                       t->nodes);
-    hipLaunchKernelGGL(k_place, dim3(div_up(n2, kTB)), dim3(kTB), 0, s, n2, 1, lp.a_mask[2],
-                       lp.a_base[2], lp.a_code[2], nullptr, lp.ni[3], lp.a_mask[3], lp.a_base[3],
-                       lp.bs[3], lp.pos[2], lp.bs[2], t->nodes);
+    UpperNodes un;
+    std::memset(&un, 0, sizeof(un));
+    uint64_t u = 0;
+    for (int L = 3; L <= int(k) + 1; L++) {
+      un.ubeg[L] = uint32_t(u);
+      u += level_nodes[L];
+    }
+    un.ubeg[k + 2] = uint32_t(u);
+    hipLaunchKernelGGL(k_place_tree, dim3(div_up(n2 + u, kTB)), dim3(kTB), 0, s, n2, n2 + u, lp,
+                       int(k), un, t->nodes);
   }
   VXL_HIP(hipGetLastError());
   return t;

@@ -401,16 +401,32 @@ __global__ __launch_bounds__(kPlanThreads) void k_setup_plan(uint64_t *__restric
                                                              SatPlan *__restrict__ plan,
                                                              uint32_t epoch,
                                                              uint64_t *__restrict__ total) {
+  // each thread owns `per` consecutive sums, loaded kHeld at a time as independent loads
+  // (one memory round trip per kHeld sums); up to kHeld of them stay in registers between
+  // the two passes
+  constexpr int kHeld = 16;
   const uint64_t per = (nb + kPlanThreads - 1) / kPlanThreads;
   const uint64_t b0 = min(uint64_t(threadIdx.x) * per, nb), b1 = min(b0 + per, nb);
+  uint64_t held[kHeld];
   uint64_t s = 0;
-  for (uint64_t b = b0; b < b1; b++) s += block_sums[b];
+  for (uint64_t b = b0; b < b1; b += kHeld) {
+#pragma unroll
+    for (int k = 0; k < kHeld; k++) held[k] = b + k < b1 ? block_sums[b + k] : 0;
+#pragma unroll
+    for (int k = 0; k < kHeld; k++) s += held[k];
+  }
   uint64_t run = 0;
   const uint64_t P = block_excl_scan_u64<kPlanThreads>(s, &run);
-  for (uint64_t b = b0; b < b1; b++) {
-    const uint64_t v = block_sums[b];
-    block_sums[b] = run;
-    run += v;
+  for (uint64_t b = b0; b < b1; b += kHeld) {
+    if (per > kHeld) {  // more than one batch: reload (the held values are the last batch)
+#pragma unroll
+      for (int k = 0; k < kHeld; k++) held[k] = b + k < b1 ? block_sums[b + k] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kHeld; k++) {
+      if (b + k < b1) block_sums[b + k] = run;
+      run += held[k];
+    }
   }
   // the first wave: the skipped cells of this call's windows (one slot per lane), and the
   // other epoch's slots cleared for the next call
@@ -460,15 +476,16 @@ __global__ __launch_bounds__(64 * kOffsetBlocks) void k_offsets_units(
   }
   uint64_t lo = block_offsets[sb] + inc - s;
   const uint64_t chunk = plan->cpu * 64, nunits = plan->nunits;
+  // the first unit starting at or after the lane's first pair (one division per lane; the
+  // lane's triangles are consecutive, so b only moves forward)
+  uint64_t b = (lo + chunk - 1) / chunk, bstart = b * chunk;
 #pragma unroll
   for (int k = 0; k < kOffsetItems; k++) {
     const uint64_t t = t0 + k;
     if (t < Tc) {
       coff[t] = lo;
       const uint64_t hi = lo + c[k];
-      if (c[k])
-        for (uint64_t b = (lo + chunk - 1) / chunk; b * chunk < hi && b < nunits; b++)
-          ustart[b] = uint32_t(t);
+      for (; bstart < hi && b < nunits; b++, bstart += chunk) ustart[b] = uint32_t(t);
     }
     lo += c[k];
   }
