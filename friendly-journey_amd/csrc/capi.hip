@@ -149,6 +149,35 @@ unsigned stage_threads() {
   return std::max(1u, std::min(8u, hw ? hw : 1u));
 }
 
+// fn(t, lo, hi) over [0, len) in nth slices: slice 0 and any slice whose helper thread
+// cannot be started run on the calling thread, the others on helper threads; returns once
+// every slice is done. fn must not throw.
+template <class F>
+void run_slices(size_t len, unsigned nth, const F &fn) {
+  const size_t slice = (len + nth - 1) / nth;
+  std::vector<std::thread> pool;
+  std::vector<unsigned> here;
+  pool.reserve(nth);  // no allocation once a thread runs
+  here.reserve(nth);
+  here.push_back(0u);
+  for (unsigned t = 1; t < nth; t++) {
+    if (std::min(len, t * slice) >= len) break;
+    try {
+      pool.emplace_back([&fn, t, slice, len] {
+        const size_t lo = t * slice;
+        fn(t, lo, std::min(len, lo + slice));
+      });
+    } catch (...) {
+      here.push_back(t);
+    }
+  }
+  for (unsigned t : here) {
+    const size_t lo = std::min(len, t * slice);
+    fn(t, lo, std::min(len, lo + slice));
+  }
+  for (auto &th : pool) th.join();
+}
+
 // Device -> host copy of `bytes` from src (device) into dst (the caller's host memory), after
 // the work already enqueued on ctx->stream; complete on return. Large copies go in 64 MiB
 // chunks through the two pinned buffers: a chunk's DMA runs while up to 8 threads copy the
@@ -168,14 +197,9 @@ void download_host(vxl_ctx *ctx, void *dst, const void *src, size_t bytes) {
     VXL_HIP(hipEventSynchronize(ctx->up_ev[b]));
     ctx->up_pending[b] = false;
     const char *stage = static_cast<const char *>(ctx->up_host[b]);
-    const size_t slice = (len + nth - 1) / nth;
-    std::vector<std::thread> pool;
-    for (unsigned t = 1; t < nth; t++) {
-      const size_t lo = std::min(len, t * slice), hi = std::min(len, lo + slice);
-      if (hi > lo) pool.emplace_back([=] { std::memcpy(out + off + lo, stage + lo, hi - lo); });
-    }
-    std::memcpy(out + off, stage, std::min(len, slice));
-    for (auto &th : pool) th.join();
+    run_slices(len, nth, [=](unsigned, size_t lo, size_t hi) {
+      std::memcpy(out + off + lo, stage + lo, hi - lo);
+    });
   };
   int prev = -1;
   size_t prev_off = 0, prev_len = 0;
@@ -219,14 +243,9 @@ void upload_host(vxl_ctx *ctx, void *dst, const void *src, size_t bytes) {
       ctx->up_pending[b] = false;
     }
     char *stage = static_cast<char *>(ctx->up_host[b]);
-    const size_t slice = (len + nth - 1) / nth;
-    std::vector<std::thread> pool;
-    for (unsigned t = 1; t < nth; t++) {
-      const size_t lo = std::min(len, t * slice), hi = std::min(len, lo + slice);
-      if (hi > lo) pool.emplace_back([=] { std::memcpy(stage + lo, in + off + lo, hi - lo); });
-    }
-    std::memcpy(stage, in + off, std::min(len, slice));
-    for (auto &th : pool) th.join();
+    run_slices(len, nth, [=](unsigned, size_t lo, size_t hi) {
+      std::memcpy(stage + lo, in + off + lo, hi - lo);
+    });
     VXL_HIP(hipMemcpyAsync(out + off, stage, len, hipMemcpyHostToDevice, ctx->stream));
     VXL_HIP(hipEventRecord(ctx->up_ev[b], ctx->stream));
     ctx->up_pending[b] = true;
@@ -264,15 +283,11 @@ void upload_mesh_host(vxl_ctx *ctx, const float *tris, uint64_t T) {
       ctx->up_pending[b] = false;
     }
     uint32_t *stage = static_cast<uint32_t *>(ctx->up_host[b]);
-    const uint64_t slice = (len + nth - 1) / nth;
     std::vector<char> bad(nth, 0);
-    std::vector<std::thread> pool;
-    for (unsigned t = 1; t < nth; t++) {
-      const uint64_t lo = std::min(len, t * slice), hi = std::min(len, lo + slice);
-      pool.emplace_back([=, &bad] { bad[t] = copy_check_finite(src + off + lo, stage + lo, hi - lo); });
-    }
-    bad[0] = copy_check_finite(src + off, stage, std::min(len, slice));
-    for (auto &th : pool) th.join();
+    char *badp = bad.data();
+    run_slices(len, nth, [=](unsigned t, size_t lo, size_t hi) {
+      badp[t] = copy_check_finite(src + off + lo, stage + lo, hi - lo);
+    });
     for (char x : bad)
       if (x) throw bad_vertex;
     VXL_HIP(hipMemcpyAsync(dst + off, stage, len * 4, hipMemcpyHostToDevice, ctx->stream));
