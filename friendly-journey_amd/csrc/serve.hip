@@ -51,6 +51,7 @@ struct Request {
   vxl_status status = VXL_OK;
   std::string error;
   bool done = false;
+  bool claimed = false;  // a vxl_server_wait call owns the request (it deletes it)
 };
 
 struct InFlight {
@@ -425,6 +426,8 @@ vxl_status vxl_server_wait(vxl_server *S, uint64_t id, vxl_tree **tree_out, uint
   const vxl_status st = vxl::server_guarded(S, [&] {
     std::unique_lock<std::mutex> lk(S->mu);
     r = vxl::find(S, id);
+    if (r->claimed) throw Error(VXL_INVALID_ARG, "request is already being waited on");
+    r->claimed = true;
     S->cv_done.wait(lk, [&] { return r->done; });
     S->live.erase(id);
   });

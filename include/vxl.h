@@ -342,7 +342,8 @@ vxl_status vxl_server_submit_voxelize(vxl_server *server, const float *d_tris, u
                                       const vxl_grid *grid, float epsilon, uint64_t *id);
 vxl_status vxl_server_poll(vxl_server *server, uint64_t id, int *done);
 /* Blocks until request id completes and retires it: its status is returned; a voxelize
- * request's tree goes to *tree_out (freed when tree_out is NULL). */
+ * request's tree goes to *tree_out (freed when tree_out is NULL). One wait per request: a
+ * second wait on an id already being waited on (or retired) returns VXL_INVALID_ARG. */
 vxl_status vxl_server_wait(vxl_server *server, uint64_t id, vxl_tree **tree_out,
                            uint64_t *pairs_out, double *latency_ms);
 vxl_status vxl_server_stats_get(vxl_server *server, vxl_server_stats *out);

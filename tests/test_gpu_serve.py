@@ -154,3 +154,36 @@ def test_submit_validation(fx):
     with pytest.raises(vxl.VxlError):
         srv.wait(987654321)
     srv.close()
+
+
+def test_one_wait_per_request(fx):
+    """Two threads waiting on one request: one collects it, the other is refused with
+    VXL_INVALID_ARG (the request is claimed, or already retired), and a wait after
+    retirement is refused the same way."""
+    import threading
+    srv = vxl.Server(0, batching=True)
+    R = 200_000
+    hits = torch.zeros((R, 4), dtype=torch.int32, device="cuda")
+    rays = fx.d_rays[3][:R] if fx.d_rays[3].shape[0] >= R else fx.d_rays[3]
+    R = rays.shape[0]
+    _say(f"double wait, {R} rays")
+    rid = srv.submit_rays(fx.trees[3], rays.data_ptr(), R, hits.data_ptr(), 0)
+    results = []
+
+    def waiter():
+        try:
+            srv.wait(rid, keep_tree=False)
+            results.append("ok")
+        except vxl.VxlError as e:
+            results.append(e.name)
+
+    ths = [threading.Thread(target=waiter) for _ in range(2)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    assert sorted(results) == ["VXL_INVALID_ARG", "ok"], results
+    with pytest.raises(vxl.VxlError) as e:
+        srv.wait(rid)
+    assert e.value.name == "VXL_INVALID_ARG"
+    srv.close()
