@@ -171,6 +171,9 @@ vxl_status vxl_tree_download(const vxl_tree *tree, vxl_node *nodes, uint64_t *le
 vxl_status vxl_tree_upload(vxl_ctx *ctx, uint32_t n, const vxl_node *nodes, uint64_t node_count,
                            const uint64_t *leaf_words, const uint32_t *bounds,
                            uint64_t leaf_count, vxl_tree **out);
+/* Frees a tree once the context's stream has drained; its device memory goes back to the
+ * context for the next trees. Work the caller enqueued on its own streams that reads the
+ * tree must be complete before the call (synchronise those streams first). */
 void vxl_tree_free(vxl_tree *tree);
 
 /* ---- traversal: replaces GpuTraverser::traverse (voxel-gpu/src/traverse.rs:154-278),
