@@ -27,6 +27,13 @@ def _cases():
         # every triangle twice, second copy later: owners are the first copies' indices
         ("soup_dup_128", np.concatenate([soup[:500], soup[:500]]), 128, 1 / 128),
         ("big_random_128", rng.uniform(0.0, 1.0, (30, 9)).astype(np.float32), 128, 1 / 128),
+        # two triangles, one with a 12.4M-cell unhoisted box (64-bit decode), overlapping a
+        # small one listed after it: owners of the shared voxels are the wide triangle's
+        ("wide_unhoisted_512", np.array([[0.1234567, 0.7654321, 0.3141592, 250.3, 10.7, 200.1,
+                                          30.2, 245.9, 120.6],
+                                         [95.5, 95.5, 106.25, 110.5, 98.5, 128.25, 97.5, 112.5,
+                                          121.25]],
+                                        np.float32), 512, 1.0),
     ]
 
 

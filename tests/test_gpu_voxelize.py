@@ -40,6 +40,16 @@ def _cases():
         ("big_steep_128", np.array([[0.1, 0.1, 0.05, 0.9, 0.2, 0.95, 0.3, 0.9, 0.5],
                                     [0.05, 0.9, 0.1, 0.95, 0.1, 0.2, 0.5, 0.5, 0.99]], np.float32),
          128, (0, 0, 0), 1 / 128, 1e-4),
+        # more than 2^22 candidates in one triangle's decoded box (64-bit decode): a vertex
+        # with fine fractional bits makes g - c inexact for far centres, so the plane is not
+        # hoisted and the whole 12.4M-cell box is enumerated
+        ("wide_unhoisted_512", np.array([[0.1234567, 0.7654321, 0.3141592, 250.3, 10.7, 200.1,
+                                          30.2, 245.9, 120.6]], np.float32), 512, (0, 0, 0), 1.0,
+         1e-4),
+        # a finite world vertex whose grid coordinate overflows to +inf (literal products,
+        # 13M candidates in the clamped box)
+        ("wide_literal_512", np.array([[3e38, 50.0, 50.0, 150.0, 150.0, 200.0, 150.25, 100.0,
+                                        120.0]], np.float32), 512, (0, 0, 0), 0.5, 1e-4),
     ]
     return out
 
