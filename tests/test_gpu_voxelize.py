@@ -50,6 +50,15 @@ def _cases():
         # 13M candidates in the clamped box)
         ("wide_literal_512", np.array([[3e38, 50.0, 50.0, 150.0, 150.0, 200.0, 150.25, 100.0,
                                         120.0]], np.float32), 512, (0, 0, 0), 0.5, 1e-4),
+        # a plane crossing the grid from vertices millions of cells away (exact integers, so
+        # the plane is hoisted with normal components near 1e11): the radius bound's relative
+        # term makes the window wider than 15 cells, so the whole clamped box is enumerated
+        ("far_vertices_128", np.array([[-1.4e6, -1.0e6, -99950.0, 1.4e6, -1.0e6, 40050.0,
+                                        0.0, 1.4e6, 42050.0]], np.float32), 128, (0, 0, 0), 1.0,
+         1e-4),
+        ("farther_vertices_128", np.array([[-5.6e6, -4.0e6, -399950.0, 5.6e6, -4.0e6, 160050.0,
+                                            0.0, 5.6e6, 168050.0]], np.float32), 128, (0, 0, 0),
+         1.0, 1e-4),
     ]
     return out
 
