@@ -63,24 +63,6 @@ __global__ __launch_bounds__(kTB) void k_emit_a(const unsigned long long *__rest
   a_code[i] = uint32_t(j);
 }
 
-__device__ __forceinline__ uint32_t select_bit(unsigned long long m, uint32_t s) {
-  // position of the s-th (0-based) set bit of m
-  uint32_t pos = 0;
-#pragma unroll
-  for (int width = 32; width >= 1; width >>= 1) {
-    unsigned long long low = m & ((1ull << width) - 1ull);
-    uint32_t c = __popcll(low);
-    if (s >= c) {
-      s -= c;
-      m >>= width;
-      pos += width;
-    } else {
-      m = low;
-    }
-  }
-  return pos;
-}
-
 // Coordinates 0..3 along `axis` occupied in a 4^3 sub-brick word, as a 4-bit mask
 // (bit = x0|y0<<1|z0<<2|x1<<3|y1<<4|z1<<5).
 __device__ __forceinline__ uint32_t word_coords(unsigned long long w, int axis) {
@@ -132,11 +114,18 @@ __global__ __launch_bounds__(kTB) void k_leaf_gather(
   const uint32_t j = a_code[node];
   const uint64_t code_bias = n2_beg ? mesh[dev::upper_index(n2_beg, G, node)].b1_begin << 6 : 0;
   const uint32_t nch = __popcll(mask);
+  // the position of the mask's s-th set bit, held by lane s: lane j whose bit is set sends j
+  // to the lane of its rank (the set bits below it); the other lanes send to lane 63, which
+  // is no rank's lane unless all 64 bits are set (and then every lane sends its own rank)
+  const bool own = (mask >> lane) & 1ull;
+  const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+  const int bitpos = __builtin_amdgcn_ds_permute(own ? int(rank) * 4 : 63 * 4, lane);
   const int sub = lane >> 3, w = lane & 7;
   for (uint32_t g = 0; g < nch; g += 8) {
     const uint32_t s = g + sub;
     const bool act = s < nch;
-    const uint32_t bit = act ? select_bit(mask, s) : 0;
+    const uint32_t bit = uint32_t(__shfl(bitpos, act ? int(s) : 0, 64));
     const uint64_t code = (uint64_t(j) << 6) | bit;
     unsigned long long v = 0;
     if (act) {
