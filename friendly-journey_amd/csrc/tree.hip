@@ -481,6 +481,73 @@ __device__ __forceinline__ void upper_place(const LevelPtrs &lp, int k,
   }
 }
 
+// The positions upper_place computes (pos, bs, desc of levels 3..k+1), for one tree with
+// 2 <= k <= 4 (at most kSmallWords level-3 nodes), with fewer dependent memory round trips:
+// the level sizes are read once, each node's subtree size stays in a register between the
+// scan and the store, and the root's children (level k, in code order) take positions
+// 1 + c without a parent lookup. Node entries are written later (k_place_tree).
+__device__ __forceinline__ void upper_positions(const LevelPtrs &lp, int k,
+                                                const uint32_t *__restrict__ tot) {
+  __shared__ uint32_t sh[kSmallThreads / 64];
+  constexpr int kIpt = int(kSmallWords / kSmallThreads);
+  const uint32_t tid = threadIdx.x;
+  const uint32_t n3 = tot[2 * 3 + 1], n4 = tot[2 * 4 + 1], n5 = tot[2 * 5 + 1];
+  for (int L = 3; L <= k + 1; L++) {
+    const uint32_t nl = L == 3 ? n3 : (L == 4 ? n4 : n5);
+    uint32_t d[kIpt];
+    uint32_t local = 0;
+#pragma unroll
+    for (int i = 0; i < kIpt; i++) {
+      const uint32_t c = tid * kIpt + i;
+      d[i] = 0;
+      if (c < nl) {
+        const uint32_t m = __popcll(lp.a_mask[L][c]);
+        const uint32_t b = lp.a_base[L][c];
+        d[i] = m + (L > 3 ? lp.desc[L - 1][b + m] - lp.desc[L - 1][b] : 0u);
+      }
+      local += d[i];
+    }
+    uint32_t excl;
+    const uint32_t total = block_scan_u32(local, &excl, sh);
+#pragma unroll
+    for (int i = 0; i < kIpt; i++) {
+      const uint32_t c = tid * kIpt + i;
+      if (c < nl) lp.desc[L][c] = excl;  // E_L: exclusive scan of desc
+      excl += d[i];
+    }
+    if (tid == 0) lp.desc[L][nl] = total;
+    __syncthreads();
+  }
+  // the root (level k + 1) at position 0 with its block at 1; its children are level k's
+  // nodes 0..nk-1 in order, so child c sits at 1 + c and its block follows the root's
+  // block and the subtrees of the children before it (E_k[0] = 0)
+  if (tid == 0) {
+    lp.pos[k + 1][0] = 0;
+    lp.bs[k + 1][0] = 1;
+  }
+  if (k >= 3) {
+    const uint32_t nl = k == 3 ? n3 : n4;  // level-k nodes
+    const uint32_t mp = __popcll(lp.a_mask[k + 1][0]);
+    for (uint32_t c = tid; c < nl; c += kSmallThreads) {
+      lp.pos[k][c] = 1 + c;
+      lp.bs[k][c] = 1 + mp + lp.desc[k][c];
+    }
+  }
+  __syncthreads();
+  if (k == 4) {  // level 3 under its level-4 parents
+    for (uint32_t c = tid; c < n3; c += kSmallThreads) {
+      const uint32_t q = lp.a_code[3][c];
+      const uint32_t p = lp.ni[4][q >> 6];
+      const uint32_t pb = lp.a_base[4][p];
+      const uint32_t mp = __popcll(lp.a_mask[4][p]);
+      const uint32_t pbs = lp.bs[4][p];
+      lp.pos[3][c] = pbs + (c - pb);
+      lp.bs[3][c] = pbs + mp + (lp.desc[3][c] - lp.desc[3][pb]);
+    }
+  }
+  __syncthreads();
+}
+
 // ---- batch groups -------------------------------------------------------------------
 // Group-wide arrays: pn2 is the packed scan over every mesh's B1 region in order, so a
 // level-2 node's global index and leaf base are group-wide; per mesh they start at
@@ -805,7 +872,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_levels_sizes(
   if (top >= 3) {
     small_levels(lp, 3, top, b2, tot);
     __syncthreads();  // tot[] of every level is written
-    upper_place(lp, top - 1, tot, nullptr);
+    upper_positions(lp, top - 1, tot);
   }
   __syncthreads();  // tot[] of every level is written
   if (tid == 0) {
