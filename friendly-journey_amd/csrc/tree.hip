@@ -21,6 +21,8 @@
 //  * The leaf gather copies each occupied brick's 8 words out of the leaf grid into the
 //    Morton-sorted leaf array, computes the packed LeafBounds (leaf.rs:185-213), and
 //    writes zeros back, so the grid and B1 are all-zero again when the build ends.
+#include <atomic>
+
 #include "vxl_internal.hpp"
 
 namespace vxl {
@@ -805,6 +807,8 @@ struct BuildSizes {
   uint64_t pn2_total;
   uint64_t pairs;
   uint32_t tot[32];
+  uint32_t seq;  // written last: the call's sequence number once the fields above are in
+  uint32_t pad;
 };
 
 // ---- single-tree level 2 without a device-wide scan ------------------------------------
@@ -863,7 +867,7 @@ __global__ __launch_bounds__(kTB) void k_level2_sums(const unsigned long long *_
 __global__ __launch_bounds__(kSmallThreads) void k_small_levels_sizes(
     LevelPtrs lp, int top, const unsigned long long *__restrict__ b2, uint32_t *__restrict__ tot,
     uint64_t *__restrict__ block_sums, uint32_t nb2, const uint64_t *__restrict__ pairs,
-    BuildSizes *__restrict__ out) {
+    BuildSizes *__restrict__ out, uint32_t seq) {
   const uint32_t tid = threadIdx.x;
   const uint64_t v = tid < nb2 ? block_sums[tid] : 0;
   uint64_t excl;
@@ -874,12 +878,38 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_levels_sizes(
     __syncthreads();  // tot[] of every level is written
     upper_positions(lp, top - 1, tot);
   }
-  __syncthreads();  // tot[] of every level is written
+  // one thread writes every size (tot[] holds its own earlier writes), makes them visible
+  // to the host (system-scope fence), then writes the sequence number the host waits for
   if (tid == 0) {
     out->pn2_total = pn2_total;
     out->pairs = pairs ? *pairs : 0;
+    for (int i = 0; i < 32; i++) out->tot[i] = (top >= 3 && i >= 6 && i < 2 * top + 2) ? tot[i] : 0u;
+    __threadfence_system();
+    *reinterpret_cast<volatile uint32_t *>(&out->seq) = seq;
   }
-  if (tid < 32) out->tot[tid] = (top >= 3 && tid >= 6 && int(tid) < 2 * top + 2) ? tot[tid] : 0u;
+}
+
+// Waits until the size kernel has written `seq` to the mapped BuildSizes, so the host goes
+// on while the kernel queued after it runs. Every 256 polls the event recorded after those
+// kernels is checked too: a failed launch ends the wait with its error, and finished
+// kernels whose number is still missing are an error rather than an endless wait.
+static void wait_sizes(const BuildSizes *hs, uint32_t seq, hipEvent_t ev) {
+  const volatile uint32_t *flag = &hs->seq;
+  for (uint32_t i = 1;; i++) {
+    if (*flag == seq) break;
+    if ((i & 255u) == 0) {
+      const hipError_t e = hipEventQuery(ev);
+      if (e == hipSuccess) {
+        if (*flag == seq) break;
+        throw Error(VXL_DEVICE_ERROR, "tree build: the size kernel finished without its sizes");
+      }
+      if (e != hipErrorNotReady) {
+        (void)hipGetLastError();
+        throw Error(VXL_DEVICE_ERROR, std::string("tree build failed: ") + hipGetErrorString(e));
+      }
+    }
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
 }
 
 __global__ __launch_bounds__(kTB) void k_emit_a2_local(
@@ -937,13 +967,18 @@ static vxl_tree *build_tree_fast(vxl_ctx *ctx, uint32_t n, uint32_t k, const uin
   hipLaunchKernelGGL(k_level2_sums, dim3(nb2), dim3(kTB), 0, s, b1, w1, l2sums, b2);
   // 2. the level-2 block offsets, the upper levels and the sizes, in one workgroup
   BuildSizes *hs = static_cast<BuildSizes *>(ctx->host_pinned);
+  const uint32_t seq = ++ctx->size_seq;
   hipLaunchKernelGGL(k_small_levels_sizes, dim3(1), dim3(kSmallThreads), 0, s, lp, top, b2, tot,
-                     l2sums, nb2, pairs_dev, static_cast<BuildSizes *>(ctx->host_pinned_dev));
+                     l2sums, nb2, pairs_dev, static_cast<BuildSizes *>(ctx->host_pinned_dev),
+                     seq);
   // 3. level-2 School-A arrays (no output array needed: queued before the host waits)
   hipLaunchKernelGGL(k_emit_a2_local, dim3(nb2), dim3(kTB), 0, s, b1, w1, l2sums, lp.a_mask[2],
                      lp.a_base[2], lp.a_code[2]);
+  VXL_HIP(hipGetLastError());
+  if (!ctx->ev_wait) VXL_HIP(hipEventCreateWithFlags(&ctx->ev_wait, hipEventDisableTiming));
+  VXL_HIP(hipEventRecord(ctx->ev_wait, s));  // after both kernels: no marker between them
   // 4. the sizes (the one host round trip of the call)
-  wait_stream(ctx, s);
+  wait_sizes(hs, seq, ctx->ev_wait);
   const uint64_t leaves = uint32_t(hs->pn2_total);
   const uint64_t n2 = hs->pn2_total >> 32;
   if (pairs) *pairs = hs->pairs;
