@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes one k_sat launch moves on terrain_1024, from two rocprofv3 --pmc passes
 # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB per dispatch, factors from a 1 GiB copy kernel);
 # profiles/k_sat__terrain_1024.md holds the commands and the raw counter values.
-K_SAT_TRAFFIC_BYTES = 347_345_085
+K_SAT_TRAFFIC_BYTES = 347_315_342
 # Algorithmic VALU lane-ops of one ray walk, counted on the reference's f32 walker: a cell
 # visit (bit test + Walker::step) ~22, a descent (Walker::enter on 3 axes + the frame save)
 # ~50, ray setup and clip ~60, the f64 t of a hit ~20.
