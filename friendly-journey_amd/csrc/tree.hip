@@ -520,8 +520,8 @@ __device__ __forceinline__ void upper_positions(const LevelPtrs &lp, int k,
     if (tid == 0) lp.desc[L][nl] = total;
     __syncthreads();
   }
-  // the root (level k + 1) at position 0 with its block at 1; its children are level k's
-  // nodes 0..nk-1 in order, so child c sits at 1 + c and its block follows the root's
+  // the root (level k + 1) at position 0 with its block at 1; its children are all of
+  // level k's nodes in order, so child c sits at 1 + c and its block follows the root's
   // block and the subtrees of the children before it (E_k[0] = 0)
   if (tid == 0) {
     lp.pos[k + 1][0] = 0;
