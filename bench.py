@@ -78,6 +78,7 @@ def bench_voxelize(ctx, torch, W, steps, warmup):
     elapsed = time.perf_counter() - t0
     info = dict(pairs=trees[0].pairs, leaves=trees[0].leaf_count, nodes=trees[0].node_count,
                 tree_bytes=trees[0].device_bytes)
+    info["tree_arrays"] = trees[-1].download()  # checked against the oracle's tree below
     for tr in trees:
         tr.free()
     # stage times (HIP events on the library's stream around setup, SAT and build) from a
@@ -286,6 +287,7 @@ def bench_dda(ctx, torch, W, reps=5):
                                                   "no ray or hit buffers")
         out[name]["_tree"] = tree
         out[name]["_rays"] = rays
+        out[name]["_hits"] = dh.cpu().numpy().view(np.uint32)
         out[name]["_drays"] = dr.data_ptr()
         out[name]["_keep"] = dr
     return out
@@ -529,9 +531,12 @@ def dda_walk_stats(ctx, tree, d_rays, R):
                 ops=DDA_OPS_RAY + DDA_OPS_VISIT * visits + DDA_OPS_DESCENT * (d_leaf + d_node))
 
 
-def cpu_baseline(tris, W):
+def cpu_baseline(tris, W, gpu_tree):
     """The oracle on the full workload: serial voxelize_surface_cpu, then SparseTree::build
-    with one thread per core plus School-B serialisation (as the reference threads them)."""
+    with one thread per core plus School-B serialisation (as the reference threads them).
+    The oracle's tree is then compared, array for array, with the GPU's tree of the same
+    workload (gpu_tree: nodes, leaves, bounds, codes as downloaded)."""
+    import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     n, vs = W.grid_for(1024)
@@ -542,15 +547,16 @@ def cpu_baseline(tris, W):
     tree = O.Tree.build(words, n, threads=cores)
     t2 = time.perf_counter()
     leaves = tree.leaf_count
+    same = all(np.array_equal(a, b) for a, b in zip(tree.export(), gpu_tree))
     del tree, words
     return dict(value=round(tris.shape[0] / (t2 - t0) / 1e6, 4), unit="Mtris/s", cores=cores,
                 kind="port",
                 sample=(f"full workload ({tris.shape[0]} triangles, n={n}): voxelize "
                         f"{t1 - t0:.2f} s on 1 thread + tree build {t2 - t1:.2f} s on {cores} "
-                        f"threads, {leaves} leaves"))
+                        f"threads, {leaves} leaves")), bool(same)
 
 
-def cpu_dda_baseline(tree, rays):
+def cpu_dda_baseline(tree, rays, gpu_hits):
     """The oracle's f32 mirror traversal (the restatement of mirror.rs:254, the reference's
     CPU twin of the device walk, which its bench command times on one thread,
     voxel-cli/src/main.rs:846-853) over the same tree (downloaded from the device) and the
@@ -563,16 +569,19 @@ def cpu_dda_baseline(tree, rays):
     del nodes, leaves, bounds
     cores = min(os.cpu_count() or 1, 64)
     out = {}
+    same = None
     for label, threads, sample in (("one_thread", 1, rays[::4]), ("all_cores", cores, rays)):
         sample = np.ascontiguousarray(sample)
         t0 = time.perf_counter()
-        otree.mirror(sample, threads=threads)
+        hits = otree.mirror(sample, threads=threads)
         dt = time.perf_counter() - t0
+        if label == "all_cores":  # every ray of the batch: the GPU's hits must be the mirror's
+            same = bool(np.array_equal(hits, gpu_hits))
         out[label] = dict(value=round(sample.shape[0] / dt / 1e6, 3), unit="Mrays/s",
                           cores=threads, seconds=round(dt, 3), rays=int(sample.shape[0]))
     out["kind"] = "port"
     out["sample"] = "the bench rays against the same tree; one thread on every fourth ray"
-    return out
+    return out, same
 
 
 def main():
@@ -607,6 +616,7 @@ def main():
         print(json.dumps({"copy_probe_bytes": src.numel() * 4, "copies": 3}), flush=True)
         return
     ctx = vxl.Context(0)
+    parity = {}  # oracle comparisons of what the run computed (with the CPU baselines)
     vox, tris = bench_voxelize(ctx, torch, W, args.steps, args.warmup)
     ms_per_step = vox["elapsed"] / vox["steps"] * 1e3
     value = vox["T"] / (ms_per_step / 1e3) / 1e6
@@ -662,10 +672,12 @@ def main():
         for name, rec in dda.items():
             tree, rays = rec.pop("_tree"), rec.pop("_rays")
             ws = dda_walk_stats(ctx, tree, rec.pop("_drays"), rec["rays"])
+            gpu_hits = rec.pop("_hits")
             if not args.no_cpu:
                 log(f"dda {name}: cpu baseline (oracle mirror traversal)")
-                rec["cpu_baseline"] = cpu_dda_baseline(tree, rays)
-            del tree, rays
+                rec["cpu_baseline"], parity[f"dda_{name}_hits_identical"] = cpu_dda_baseline(
+                    tree, rays, gpu_hits)
+            del tree, rays, gpu_hits
             secs = rec["kernel_ms"] / 1e3
             gbs = ws["bytes"] * rec["rays"] / secs / 1e9
             tops = ws["ops"] * rec["rays"] / secs / 1e12
@@ -688,11 +700,16 @@ def main():
     if not args.no_cpu:
         log("cpu baseline (oracle) on the full workload")
         model, _ = host_cpu()
-        cb = cpu_baseline(tris, W)
+        cb, parity["terrain_1024_tree_identical"] = cpu_baseline(tris, W, vox["tree_arrays"])
         cb["host"] = model
         line["cpu_baseline"] = cb
     ctx.close()
+    line["parity"] = parity
     print(json.dumps(line), flush=True)
+    failed = [k for k, v in parity.items() if v is not True]
+    if failed:
+        print(f"bench: results differ from the oracle: {failed}", file=sys.stderr, flush=True)
+        sys.exit(1)
 
 
 if __name__ == "__main__":

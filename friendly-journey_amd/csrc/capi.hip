@@ -428,8 +428,47 @@ std::vector<vxl::RaySegment> segments(const vxl_tree *const *trees, uint32_t cou
 void traverse_batch_checks(const vxl_tree *const *trees, uint32_t count, const uint64_t *off) {
   if (count && (!trees || !off)) throw Error(VXL_INVALID_ARG, "null trees or ray offsets");
   for (uint32_t g = 0; g < count; g++) {
-    if (!trees[g]) throw Error(VXL_INVALID_ARG, "null tree in batch");
+    vxl::check_tree(trees[g]);
     if (off[g + 1] < off[g]) throw Error(VXL_INVALID_ARG, "ray_offsets must be non-decreasing");
+  }
+}
+
+// A caller-supplied School-B structure, checked once on the host before upload: the walk
+// indexes nodes and leaves by child_base + popcount rank without bounds checks (the
+// reference's WGSL relies on clamped storage reads instead), so every node reachable from
+// the root must address children inside the arrays -- nodes for levels >= 3, leaves for
+// level-2 nodes (school_b.rs:68-98, node.rs:64-72) -- and no node may be reached at two
+// levels. k = 0 trees have no nodes: the root is leaf 0.
+void validate_school_b(uint32_t k, const vxl_node *nodes, uint64_t node_count,
+                       uint64_t leaf_count) {
+  if (k == 0) return;
+  std::vector<uint8_t> level(node_count, 0);  // level each node was reached at (0: not yet)
+  std::vector<uint64_t> stack{0};
+  level[0] = uint8_t(k + 1);
+  while (!stack.empty()) {
+    const uint64_t i = stack.back();
+    stack.pop_back();
+    const vxl_node &nd = nodes[i];
+    const uint64_t kids = uint64_t(__builtin_popcount(nd.mask_lo)) + __builtin_popcount(nd.mask_hi);
+    const uint32_t L = level[i];
+    const uint64_t end = uint64_t(nd.child_base) + kids;
+    if (L == 2) {
+      if (end > leaf_count)
+        throw Error(VXL_INVALID_ARG, "node " + std::to_string(i) +
+                                         " addresses leaves past leaf_count");
+      continue;
+    }
+    if (end > node_count)
+      throw Error(VXL_INVALID_ARG, "node " + std::to_string(i) +
+                                       " addresses nodes past node_count");
+    for (uint64_t c = nd.child_base; c < end; c++) {
+      if (level[c] == 0) {
+        level[c] = uint8_t(L - 1);
+        stack.push_back(c);
+      } else if (level[c] != L - 1) {
+        throw Error(VXL_INVALID_ARG, "node " + std::to_string(c) + " is reached at two levels");
+      }
+    }
   }
 }
 
@@ -497,6 +536,9 @@ vxl_status vxl_ctx_create(int device, vxl_ctx **out) {
     VXL_HIP(hipSetDevice(device));
     VXL_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     VXL_HIP(hipHostMalloc(&ctx->host_pinned, 4096, hipHostMallocDefault));
+    // pinned memory can come back from an earlier owner (another context) with its contents:
+    // the tree build's size read-back waits for a sequence number here, so start from zero
+    std::memset(ctx->host_pinned, 0, 4096);
     VXL_HIP(hipHostGetDevicePointer(&ctx->host_pinned_dev, ctx->host_pinned, 0));
     hipDeviceProp_t prop;
     VXL_HIP(hipGetDeviceProperties(&prop, device));
@@ -530,6 +572,8 @@ void vxl_ctx_destroy(vxl_ctx *ctx) {
   for (vxl_tree *t : ctx->trees) {
     vxl::tree_free_device(t);
     t->ctx = nullptr;
+    t->leaf_count = t->node_count = 0;
+    t->bytes = 0;
   }
   ctx->trees.clear();
   vxl::arena_destroy(ctx);
@@ -897,7 +941,7 @@ vxl_status vxl_tree_materials_download(const vxl_tree *tree, uint16_t *materials
 vxl_status vxl_tree_set_voxels(vxl_ctx *ctx, vxl_tree *tree, const vxl_voxel_edit *edits,
                                uint64_t count, uint64_t *changed, int *topology) {
   return guarded(ctx, [&] {
-    if (!tree) throw Error(VXL_INVALID_ARG, "tree is null");
+    vxl::check_tree(tree);
     if (tree->ctx != ctx) throw Error(VXL_INVALID_ARG, "tree belongs to another context");
     if (count && !edits) throw Error(VXL_INVALID_ARG, "edits pointer is null");
     if (changed) *changed = 0;
@@ -1045,14 +1089,15 @@ void vxl_graph_destroy(vxl_graph *graph) {
 vxl_status vxl_traverse_stats(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, uint64_t R,
                               uint64_t out[3]) {
   return guarded(ctx, [&] {
-    if (!tree || !out) throw Error(VXL_INVALID_ARG, "null tree or out");
+    vxl::check_tree(tree);
+    if (!out) throw Error(VXL_INVALID_ARG, "out is null");
     if (R && !d_rays) throw Error(VXL_INVALID_ARG, "null ray buffer");
     vxl::traverse_stats(ctx, tree, d_rays, R, out);
   });
 }
 
 static void validate_camera(const vxl_tree *tree, const vxl_camera *cam) {
-  if (!tree) throw Error(VXL_INVALID_ARG, "tree is null");
+  vxl::check_tree(tree);
   if (!cam) throw Error(VXL_INVALID_ARG, "camera is null");
   if (cam->n != float(tree->n))
     throw Error(VXL_INVALID_ARG, "camera n must equal the tree's grid size");
@@ -1167,6 +1212,7 @@ vxl_status vxl_tree_upload(vxl_ctx *ctx, uint32_t n, const vxl_node *nodes, uint
       throw Error(VXL_INVALID_ARG, "null structure array");
     if (leaf_count && k >= 1 && node_count == 0)
       throw Error(VXL_INVALID_ARG, "a non-empty tree with k >= 1 needs nodes");
+    if (leaf_count) validate_school_b(k, nodes, node_count, leaf_count);
     vxl_tree *t = new vxl_tree();
     t->ctx = ctx;
     t->n = n;
@@ -1213,7 +1259,7 @@ void vxl_tree_free(vxl_tree *tree) {
 vxl_status vxl_traverse_dev(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, uint64_t R,
                             uint32_t *d_hits, double *d_t, void *stream) {
   return guarded(ctx, [&] {
-    if (!tree) throw Error(VXL_INVALID_ARG, "tree is null");
+    vxl::check_tree(tree);
     if (R && (!d_rays || !d_hits)) throw Error(VXL_INVALID_ARG, "null ray or hit buffer");
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
     if (ctx->timing) VXL_HIP(hipEventRecord(ctx->ev[4], s));
@@ -1416,7 +1462,7 @@ vxl_status vxl_autotune_batch(vxl_ctx *ctx, const float *d_tris, const uint64_t 
 vxl_status vxl_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *rays, uint64_t R,
                         uint32_t *hits, double *t) {
   return guarded(ctx, [&] {
-    if (!tree) throw Error(VXL_INVALID_ARG, "tree is null");
+    vxl::check_tree(tree);
     if (R == 0) return;
     if (!rays || !hits) throw Error(VXL_INVALID_ARG, "null ray or hit buffer");
     hipStream_t s = ctx->stream;

@@ -609,6 +609,10 @@ size_t frame_lds_bytes(uint32_t k, uint64_t lanes) {
 // Work counters of persistent traversals in flight at once on one context (at most).
 constexpr uint32_t kCounterSlots = 1024;
 
+// Zeroes a persistent traversal's work counter on the launch's stream: a kernel rather than
+// a memset, so the pair stays correct when a caller captures the call in a HIP graph.
+__global__ void k_clear_counter(unsigned long long *ctr) { *ctr = 0ull; }
+
 // The automatic lane schedule: a tree whose walked arrays (64-byte leaves, 4-byte bounds,
 // 12-byte nodes) exceed the 256 MiB Infinity Cache is walked from HBM, where a wave's
 // lanes wait on leaf loads and refilling idle lanes pays (soup_2048: 3.18 -> 2.33 ms); a
@@ -642,7 +646,7 @@ void launch_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, ui
     ctx->trav_ctr.reserve(kCounterSlots * 8 + 64);
     unsigned long long *ctr =
         ctx->trav_ctr.as<unsigned long long>() + (ctx->trav_ctr_next++ % kCounterSlots);
-    VXL_HIP(hipMemsetAsync(ctr, 0, 8, s));
+    hipLaunchKernelGGL(k_clear_counter, dim3(1), dim3(1), 0, s, ctr);
     hipLaunchKernelGGL(k_traverse_pl, dim3(uint32_t(blocks)), dim3(64), lds64, s, d_rays, R,
                        tree->nodes, reinterpret_cast<const uint32_t *>(tree->leaves),
                        tree->bounds, tree->n, tree->k, tree->leaf_count,

@@ -379,7 +379,8 @@ vxl_status vxl_server_hold(vxl_server *S, int64_t delta_bytes) {
 vxl_status vxl_server_submit_rays(vxl_server *S, const vxl_tree *tree, const float *d_rays,
                                   uint64_t R, uint32_t *d_hits, double *d_t, uint64_t *id) {
   return vxl::server_guarded(S, [&] {
-    if (!tree || !id) throw Error(VXL_INVALID_ARG, "null tree or id");
+    vxl::check_tree(tree);
+    if (!id) throw Error(VXL_INVALID_ARG, "id is null");
     if (R && (!d_rays || !d_hits)) throw Error(VXL_INVALID_ARG, "null ray or hit buffer");
     auto *r = new vxl::Request();
     r->kind = vxl::kRays;

@@ -968,6 +968,9 @@ static vxl_tree *build_tree_fast(vxl_ctx *ctx, uint32_t n, uint32_t k, const uin
   // 2. the level-2 block offsets, the upper levels and the sizes, in one workgroup
   BuildSizes *hs = static_cast<BuildSizes *>(ctx->host_pinned);
   const uint32_t seq = ++ctx->size_seq;
+  // the number this build waits for is not in the mapped area before its size kernel writes
+  // it: earlier builds of this context wrote smaller numbers, and the area starts zeroed
+  *reinterpret_cast<volatile uint32_t *>(&hs->seq) = 0u;
   hipLaunchKernelGGL(k_small_levels_sizes, dim3(1), dim3(kSmallThreads), 0, s, lp, top, b2, tot,
                      l2sums, nb2, pairs_dev, static_cast<BuildSizes *>(ctx->host_pinned_dev),
                      seq);
@@ -986,6 +989,12 @@ static vxl_tree *build_tree_fast(vxl_ctx *ctx, uint32_t n, uint32_t k, const uin
   uint64_t level_nodes[16] = {0};
   level_nodes[2] = n2;
   for (int L = 3; L <= top; L++) level_nodes[L] = hs->tot[2 * L + 1];
+  // the kernels that follow index per-level scratch and the new tree by these sizes: refuse
+  // any that the grid cannot hold rather than launch them out of bounds
+  const uint64_t bricks = uint64_t(n / 8) * (n / 8) * (n / 8);
+  bool sane = leaves <= bricks && n2 <= w1 && n2 <= leaves && (leaves == 0) == (n2 == 0);
+  for (int L = 3; L <= top; L++) sane = sane && level_nodes[L] <= lp.win[L] && level_nodes[L] <= n2;
+  if (!sane) throw Error(VXL_DEVICE_ERROR, "tree build: the sizes read back are inconsistent");
   if (k >= 1 && leaves)
     for (int L = 2; L <= int(k) + 1; L++) nodes += level_nodes[L];
   vxl_tree *t = alloc_tree(ctx, n, k, leaves ? nodes : 0, leaves);

@@ -157,6 +157,16 @@ struct vxl_tree {
 
 namespace vxl {
 
+// A tree the device path may read: non-null and still attached to a live context.
+// vxl_ctx_destroy releases the device memory of the context's trees and detaches them
+// (ctx = nullptr, counts zeroed); every entry point that reads a tree refuses such a
+// handle on the host instead of launching a kernel over freed memory.
+inline void check_tree(const vxl_tree *t) {
+  if (!t) throw Error(VXL_INVALID_ARG, "tree is null");
+  if (!t->ctx)
+    throw Error(VXL_INVALID_ARG, "tree is detached: its context (or server) was destroyed");
+}
+
 // Records timing event i on the context stream when timing is enabled.
 inline void mark(vxl_ctx *ctx, int i) {
   if (ctx->timing) (void)hipEventRecord(ctx->ev[i], ctx->stream);

@@ -167,13 +167,22 @@ vxl_status vxl_tree_info_get(const vxl_tree *tree, vxl_tree_info *info);
  * pointer may be NULL. leaf_words: 8 u64 per leaf; codes: brick Morton code per leaf. */
 vxl_status vxl_tree_download(const vxl_tree *tree, vxl_node *nodes, uint64_t *leaf_words,
                              uint32_t *bounds, uint64_t *codes);
-/* Uploads a host School-B structure (GpuTraverser::new's upload, traverse.rs:70-150). */
+/* Uploads a host School-B structure (GpuTraverser::new's upload, traverse.rs:70-150).
+ * The structure is checked on the host first: every node reachable from the root must
+ * address its children inside the arrays (child_base + popcount(mask) <= node_count for
+ * nodes above level 2, <= leaf_count for level-2 nodes) and be reached at one level only;
+ * otherwise VXL_INVALID_ARG and nothing is uploaded. */
 vxl_status vxl_tree_upload(vxl_ctx *ctx, uint32_t n, const vxl_node *nodes, uint64_t node_count,
                            const uint64_t *leaf_words, const uint32_t *bounds,
                            uint64_t leaf_count, vxl_tree **out);
 /* Frees a tree once the context's stream has drained; its device memory goes back to the
  * context for the next trees. Work the caller enqueued on its own streams that reads the
- * tree must be complete before the call (synchronise those streams first). */
+ * tree must be complete before the call (synchronise those streams first).
+ * A tree lives no longer than the context that made it: vxl_ctx_destroy releases the device
+ * memory of its live trees and detaches their handles, and every call given a detached tree
+ * returns VXL_INVALID_ARG without device work; vxl_tree_free still releases the handle.
+ * Trees collected with vxl_server_wait belong to the server's lane contexts and are detached
+ * by vxl_server_destroy in the same way. */
 void vxl_tree_free(vxl_tree *tree);
 
 /* ---- traversal: replaces GpuTraverser::traverse (voxel-gpu/src/traverse.rs:154-278),
@@ -333,7 +342,7 @@ typedef struct vxl_server_stats {
 } vxl_server_stats;
 vxl_status vxl_server_create(int device, const vxl_server_config *cfg, vxl_server **out);
 /* Serves every pending request, then stops. Voxelize results not yet collected with
- * vxl_server_wait are released. */
+ * vxl_server_wait are released; collected ones are detached (see vxl_tree_free). */
 void vxl_server_destroy(vxl_server *server);
 /* Adds (or, negative, releases) bytes held resident, e.g. the trees queries run against. */
 vxl_status vxl_server_hold(vxl_server *server, int64_t delta_bytes);

@@ -56,6 +56,9 @@ def lib():
         L.vo_voxelize_surface.restype = C.c_uint64
         L.vo_voxelize_surface.argtypes = [_f32p, C.c_uint64, C.c_uint32, _f32p, C.c_float,
                                           C.c_float, _u32p, _u32p]
+        L.vo_voxelize_surface_mt.restype = C.c_uint64
+        L.vo_voxelize_surface_mt.argtypes = [_f32p, C.c_uint64, C.c_uint32, _f32p, C.c_float,
+                                             C.c_float, _u32p, _u32p, C.c_int]
         L.vo_hash_color.restype = C.c_uint32
         L.vo_hash_color.argtypes = [C.c_uint32]
         L.vo_owner_colors.restype = None
@@ -91,6 +94,8 @@ def lib():
                                           C.c_float]
         L.vo_tree_build.restype = C.c_void_p
         L.vo_tree_build.argtypes = [_u32p, C.c_uint32, C.c_int]
+        L.vo_tree_build_rows.restype = C.c_void_p
+        L.vo_tree_build_rows.argtypes = [_u32p, C.c_uint32, C.c_int]
         L.vo_tree_from_school_b.restype = C.c_void_p
         L.vo_tree_from_school_b.argtypes = [C.c_uint32, _u32p, C.c_uint64, _u64p, C.c_uint64,
                                             _u32p]
@@ -197,6 +202,22 @@ def voxelize_surface(tris, n, origin=(0.0, 0.0, 0.0), voxel_size=1.0, eps=1e-4, 
     pairs = lib().vo_voxelize_surface(_p(t, _f32p), t.shape[0], n, _p(o, _f32p), voxel_size, eps,
                                       _p(words, _u32p),
                                       _p(own, _u32p) if owner else None)
+    return words, own, int(pairs)
+
+
+def voxelize_surface_mt(tris, n, origin=(0.0, 0.0, 0.0), voxel_size=1.0, eps=1e-4,
+                        owner=False, threads=None):
+    """voxelize_surface with the scan split over threads by z (same words, owners and pair
+    count as the serial restatement; for checks at the large grids)."""
+    t = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    o = np.ascontiguousarray(origin, np.float32)
+    nv = n ** 3
+    words = np.zeros((nv + 31) // 32, np.uint32)
+    own = np.zeros(nv, np.uint32) if owner else None
+    pairs = lib().vo_voxelize_surface_mt(_p(t, _f32p), t.shape[0], n, _p(o, _f32p), voxel_size,
+                                         eps, _p(words, _u32p),
+                                         _p(own, _u32p) if owner else None,
+                                         threads or _threads())
     return words, own, int(pairs)
 
 
@@ -358,9 +379,12 @@ class Tree:
         self.level_counts = {L: int(c[2 + L]) for L in range(16) if c[2 + L]}
 
     @classmethod
-    def build(cls, words, n, threads=None):
+    def build(cls, words, n, threads=None, rows=False):
+        """SparseTree::build + from_sparse; rows=True reads each 8-voxel brick row as one
+        byte of the words (the same tree, faster at 2048^3: for checks, not the baseline)."""
         w = np.ascontiguousarray(words, np.uint32)
-        h = lib().vo_tree_build(_p(w, _u32p), n, threads or _threads())
+        fn = lib().vo_tree_build_rows if rows else lib().vo_tree_build
+        h = fn(_p(w, _u32p), n, threads or _threads())
         if not h:
             raise ValueError(f"resolution {n} is not 8*4^k")
         return cls(h, n)

@@ -40,6 +40,11 @@ void vo_world_to_grid(const float p[3], const float origin[3], float voxel_size,
  * of (triangle, candidate voxel) pairs the epsilon-padded AABB scan enumerates. */
 uint64_t vo_voxelize_surface(const float *tris, uint64_t T, uint32_t n, const float origin[3],
                              float voxel_size, float eps, uint32_t *words, uint32_t *owner);
+/* vo_voxelize_surface with the scan split over threads by z-plane rows: the same words,
+ * owners and pair count (occupancy is an OR, the owner a minimum); for fast checks only. */
+uint64_t vo_voxelize_surface_mt(const float *tris, uint64_t T, uint32_t n, const float origin[3],
+                                float voxel_size, float eps, uint32_t *words, uint32_t *owner,
+                                int threads);
 /* hash_color (reference_cpu.rs:91-99) and the colour pass of reference_cpu.rs:173-186. */
 uint32_t vo_hash_color(uint32_t id);
 void vo_owner_colors(const uint32_t *owner, uint64_t N, uint32_t *color);
@@ -68,6 +73,8 @@ uint64_t vo_count_candidates(const float *tris, uint64_t T, uint32_t n, const fl
 typedef struct vo_tree vo_tree;
 /* SparseTree::build over an x-major word field, then SchoolBBuffer::from_sparse. */
 vo_tree *vo_tree_build(const uint32_t *words, uint32_t n, int threads);
+/* The same tree, reading each 8-voxel brick row of the words as one byte (for checks). */
+vo_tree *vo_tree_build_rows(const uint32_t *words, uint32_t n, int threads);
 /* Wrap School-B arrays produced elsewhere (e.g. by the GPU) for traversal. */
 vo_tree *vo_tree_from_school_b(uint32_t n, const uint32_t *nodes, uint64_t node_count,
                                const uint64_t *leaves, uint64_t leaf_count,

@@ -164,3 +164,100 @@ def brute_force_first_hit(voxels, ray):
             if best is None or t < best[1]:
                 best = (tuple(v), t)
     return best
+
+
+# ---- School-B positions by closed form --------------------------------------------------------
+def _popcount64(a):
+    a = np.asarray(a, np.uint64)
+    c = np.zeros(a.shape, np.int64)
+    for b in range(64):
+        c += ((a >> np.uint64(b)) & np.uint64(1)).astype(np.int64)
+    return c
+
+
+def school_b_from_levels(k, levels):
+    """The School-B node array of School-A levels (levels[L]: N_L x 3 u32 rows mask_lo,
+    mask_hi, child_base for L = 2..k+1), by arithmetic instead of a recursive walk.
+
+    Parent-first depth-first order (school_b.rs:68-98) as closed forms: below(v), the node
+    entries a node's subtree adds under it, bottom-up (0 at level 2, else its child count plus
+    its children's below); then top-down, the root at 0 with its child block at 1, a node's
+    j-th child at block(v) + j, and that child's own block after v's whole block and the
+    subtrees of its earlier siblings: block(c_j) = block(v) + m(v) + sum_{i<j} below(c_i).
+    Level-2 nodes keep their School-A leaf base."""
+    top = k + 1
+    mask = {L: levels[L][:, 0].astype(np.uint64) | (levels[L][:, 1].astype(np.uint64) << np.uint64(32))
+            for L in range(2, top + 1)}
+    m = {L: _popcount64(mask[L]) for L in mask}
+    cb = {L: levels[L][:, 2].astype(np.int64) for L in mask}
+    below = {2: np.zeros(len(levels[2]), np.int64)}
+    prefix = {}
+    for L in range(3, top + 1):
+        prefix[L - 1] = np.concatenate([[0], np.cumsum(below[L - 1])])
+        below[L] = m[L] + prefix[L - 1][cb[L] + m[L]] - prefix[L - 1][cb[L]]
+    assert len(levels[top]) == 1
+    pos = {top: np.array([0])}
+    block = {top: np.array([1])}
+    for L in range(top, 2, -1):
+        nchild = len(levels[L - 1])
+        c = np.arange(nchild)
+        parent = np.searchsorted(cb[L], c, side="right") - 1
+        j = c - cb[L][parent]
+        pos[L - 1] = block[L][parent] + j
+        block[L - 1] = (block[L][parent] + m[L][parent] +
+                        prefix[L - 1][c] - prefix[L - 1][cb[L][parent]])
+    total = sum(len(levels[L]) for L in range(2, top + 1))
+    out = np.zeros((total, 3), np.uint32)
+    for L in range(2, top + 1):
+        out[pos[L], 0] = levels[L][:, 0]
+        out[pos[L], 1] = levels[L][:, 1]
+        out[pos[L], 2] = levels[L][:, 2] if L == 2 else block[L]
+    return out
+
+
+def voxels_to_words(vox, n):
+    """x-major u32 occupancy words of a voxel array (N x 3), vectorised."""
+    v = np.asarray(vox, np.uint64)
+    lin = v[:, 0] + np.uint64(n) * (v[:, 1] + np.uint64(n) * v[:, 2])
+    words = np.zeros((n ** 3 + 31) // 32, np.uint32)
+    np.bitwise_or.at(words, (lin >> np.uint64(5)).astype(np.int64),
+                     (np.uint32(1) << (lin & np.uint64(31)).astype(np.uint32)))
+    return words
+
+
+# ---- exact ties in the walker's step: expected hits derived by hand ----------------------------
+# Walker::step (mirror.rs:87-116) takes the axis of the smallest t_max, the LOWEST axis on a tie,
+# and a frame whose chosen axis leaves it is exhausted (the parent steps next, with the same
+# rule). Rays start at voxel centres with directions of 0/+-1 components, so every t_max is a
+# half-integer computed exactly in binary32 and binary64 and ties are exact. Each case: grid n,
+# occupied voxels, ray (origin + direction), expected hit voxel (None: no hit). Derivations:
+#  * (6.5,6.5,2.5) + (1,1,0): cells (6,6,2) (7,6,2) (7,7,2); at t = 1.5 the leaf frame (x first)
+#    and the root frame (x first again, cells of 8) both tie, so the ray enters brick (1,0,0) at
+#    (8,8,2.5), clamped into it as (8,7,2); (7,8,2) is never visited. A highest-axis rule would
+#    enter brick (0,1,0) at (7,8,2) instead.
+#  * (2.5,6.5,6.5) + (0,1,1): y before z: (2,7,6) (2,7,7) (2,8,7); (2,7,8) never.
+#  * (6.5,6.5,6.5) + (1,1,1): (7,6,6) (7,7,6) (7,7,7) (8,7,7); (6,7,6) (6,6,7) (7,8,7) (7,7,8) never.
+#  * (9.5,6.5,2.5) + (-1,1,0): (8,6,2) (8,7,2); at t = 1.5 x first leaves brick (1,0,0) for
+#    (0,0,0), entered at (8,8) clamped to (7,7,2); then y leaves that brick for (0,1,0) at
+#    (7,8,2); (8,8,2) never.
+#  * n = 128, (30.5,30.5,2.5) + (1,1,0): the tie at (32,32) is resolved x first in the leaf, the
+#    level-2 frame (cells of 8) and the root (cells of 32): (31,31,2) then (32,31,2); (31,32,2)
+#    never.
+TIE_CASES = [
+    (32, [(8, 7, 2)], (6.5, 6.5, 2.5, 1, 1, 0), (8, 7, 2)),
+    (32, [(7, 8, 2)], (6.5, 6.5, 2.5, 1, 1, 0), None),
+    (32, [(7, 8, 2), (8, 7, 2)], (6.5, 6.5, 2.5, 1, 1, 0), (8, 7, 2)),
+    (32, [(7, 6, 2)], (6.5, 6.5, 2.5, 1, 1, 0), (7, 6, 2)),
+    (32, [(6, 7, 2)], (6.5, 6.5, 2.5, 1, 1, 0), None),
+    (32, [(2, 8, 7)], (2.5, 6.5, 6.5, 0, 1, 1), (2, 8, 7)),
+    (32, [(2, 7, 8)], (2.5, 6.5, 6.5, 0, 1, 1), None),
+    (32, [(7, 6, 6)], (6.5, 6.5, 6.5, 1, 1, 1), (7, 6, 6)),
+    (32, [(6, 7, 6), (6, 6, 7)], (6.5, 6.5, 6.5, 1, 1, 1), None),
+    (32, [(8, 7, 7)], (6.5, 6.5, 6.5, 1, 1, 1), (8, 7, 7)),
+    (32, [(7, 8, 7), (7, 7, 8)], (6.5, 6.5, 6.5, 1, 1, 1), None),
+    (32, [(7, 7, 2)], (9.5, 6.5, 2.5, -1, 1, 0), (7, 7, 2)),
+    (32, [(8, 8, 2)], (9.5, 6.5, 2.5, -1, 1, 0), None),
+    (32, [(7, 8, 2), (8, 8, 2)], (9.5, 6.5, 2.5, -1, 1, 0), (7, 8, 2)),
+    (128, [(32, 31, 2)], (30.5, 30.5, 2.5, 1, 1, 0), (32, 31, 2)),
+    (128, [(31, 32, 2)], (30.5, 30.5, 2.5, 1, 1, 0), None),
+]

@@ -3,7 +3,8 @@ tier-A oracle (oracle.rs) and the f32 mirror (mirror.rs)."""
 import numpy as np
 import oracle as O
 import pytest
-from helpers import brute_force_first_hit, kats, single_voxel_school_b, words_to_set
+from helpers import (TIE_CASES, brute_force_first_hit, kats, single_voxel_school_b,
+                     voxels_to_words, words_to_set)
 
 D = kats()["dda"]
 
@@ -130,3 +131,20 @@ def test_mirror_matches_f64_within_grazing_bound():
         total += len(rays)
         assert np.any(hd != hd32, axis=1).sum() < len(rays) * 0.01
     assert mism / total < 0.01
+
+
+@pytest.mark.parametrize("case", range(len(TIE_CASES)))
+def test_walker_ties_take_the_lowest_axis(case):
+    """Exact t_max ties (helpers.TIE_CASES, derived by hand from mirror.rs:87-116): the mirror,
+    the f64 layout traversal (dda.rs, the same rule) and the dense oracle (oracle.rs) give the
+    hand-derived voxel."""
+    n, occ, ray, want = TIE_CASES[case]
+    words = voxels_to_words(np.array(occ), n)
+    t = O.Tree.build(words, n)
+    expect = [0, 0, 0, 0] if want is None else list(want) + [1]
+    assert t.mirror(np.array([ray], np.float32))[0].tolist() == expect
+    for school in (0, 1):
+        h, _ = t.layout(np.array([ray], np.float64), school=school)
+        assert h[0].tolist() == expect
+    h, _ = O.first_hit_dense(words, n, np.array([ray], np.float64))
+    assert h[0].tolist() == expect

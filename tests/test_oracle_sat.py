@@ -162,3 +162,20 @@ def test_voxelize_matches_bruteforce_on_random_soup(n):
                     if O.triangle_box_overlap((x + .5, y + .5, z + .5), HALF, g[0], g[1], g[2]):
                         want.add((x, y, z))
     assert got == want
+
+
+@pytest.mark.parametrize("threads", [2, 3, 8])
+def test_threaded_scan_equals_serial(threads):
+    """The z-split scan used by the large parity tests gives the serial scan's words, owners
+    and pair count (a z-row dropped or taken twice would change one of them)."""
+    from vxl import workloads as W
+    cases = [(W.uv_sphere(32, 16), 128, 1 / 128), (W.soup(3000, seed=9, mu=np.log(5 / 128)), 128,
+                                                   1 / 128),
+             (W.heightfield(24), 64, 1 / 64)]
+    for tris, n, vs in cases:
+        w1, o1, p1 = O.voxelize_surface(tris, n, (0, 0, 0), vs, 1e-4, owner=True)
+        w2, o2, p2 = O.voxelize_surface_mt(tris, n, (0, 0, 0), vs, 1e-4, owner=True,
+                                           threads=threads)
+        assert p1 == p2
+        assert np.array_equal(w1, w2)
+        assert np.array_equal(o1, o2)

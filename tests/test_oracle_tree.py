@@ -3,7 +3,8 @@
 import numpy as np
 import oracle as O
 import pytest
-from helpers import school_b_occupied, single_voxel_school_b, words_to_set
+from helpers import (school_b_from_levels, school_b_occupied, single_voxel_school_b,
+                     voxels_to_words, words_to_set)
 
 FIXTURES = [
     ("checkerboard", 32, ()),
@@ -86,3 +87,63 @@ def test_single_voxel_school_b_matches_hand_built(n, v):
     nodes, leaves, bounds, _ = t.export()
     hn, hl, hb = single_voxel_school_b(n, v)
     assert np.array_equal(nodes, hn) and np.array_equal(leaves, hl) and np.array_equal(bounds, hb)
+
+
+def test_row_read_build_equals_probe_build():
+    """The byte-per-row brick read used by the 2048^3 checks builds the probe build's tree
+    (a row taken from the wrong byte or bit would move voxels)."""
+    for kind, n, params in [("dust", 128, (4096,)), ("octant", 32, (0b0110_1001,)),
+                            ("wire", 128, (16, 1)), ("checkerboard", 8, ()), ("empty", 32, ()),
+                            ("single", 512, (509, 3, 260))]:
+        w = O.fixture_words(kind, n, *params)
+        a = O.Tree.build(w, n).export()
+        b = O.Tree.build(w, n, rows=True).export()
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y), (kind, n)
+
+
+def _levels(t, k):
+    return {L: t.level(L) for L in range(2, k + 2)}
+
+
+@pytest.mark.parametrize("kind,n,params,k", [("dust", 512, (20000,), 3), ("wire", 512, (64, 2), 3),
+                                              ("octant", 512, (0b0110_1001,), 3)])
+def test_school_b_positions_closed_form_k3(kind, n, params, k):
+    """Every School-B entry equals its closed-form position (helpers.school_b_from_levels)."""
+    t = O.Tree.build(O.fixture_words(kind, n, *params), n)
+    nodes, _, _, _ = t.export()
+    assert np.array_equal(nodes, school_b_from_levels(k, _levels(t, k)))
+
+
+def test_school_b_positions_closed_form_k4():
+    """At k = 4 (Resolution(2048)) parent-first DFS and breadth-first order differ: the
+    level-3 block of the root's first child comes before the level-4 siblings' subtrees. 30,000
+    random voxels in a few clusters, so subtrees differ in size."""
+    n, k = 2048, 4
+    rng = np.random.default_rng(0x5EED)
+    centres = rng.integers(0, n, (6, 3))
+    vox = np.concatenate([np.clip(c + rng.normal(0, 90, (5000, 3)), 0, n - 1) for c in centres])
+    t = O.Tree.build(voxels_to_words(vox.astype(np.int64), n), n, rows=True)
+    nodes, leaves, _, _ = t.export()
+    levels = _levels(t, k)
+    assert all(len(levels[L]) > 1 for L in (2, 3, 4))
+    assert np.array_equal(nodes, school_b_from_levels(k, levels))
+    bfs = np.concatenate([levels[L] for L in range(k + 1, 1, -1)])  # level order, for contrast
+    assert not np.array_equal(nodes[:, :2], bfs[:, :2])
+
+
+def test_school_b_k4_hand_built_dfs():
+    """Three voxels at Resolution(2048): root children A (cell 0) and B (cell 1); A has level-3
+    children A1, A2; B has B1; each has one level-2 node and one leaf. By school_b.rs:68-98
+    (root at 0; a node's children contiguously at the current end, then each child's subtree)
+    the entries are root, A, B, A1, A2, L2(A1), L2(A2), B1, L2(B1), with child_base 1, 3, 7, 5,
+    6, leaf 0, leaf 1, 8, leaf 2 -- breadth-first order would put B1 before the level-2 nodes."""
+    n = 2048
+    vox = np.array([[0, 0, 0], [128, 0, 0], [512, 0, 0]])
+    t = O.Tree.build(voxels_to_words(vox, n), n, rows=True)
+    nodes, leaves, bounds, codes = t.export()
+    want = np.array([[3, 0, 1], [3, 0, 3], [1, 0, 7], [1, 0, 5], [1, 0, 6], [1, 0, 0], [1, 0, 1],
+                     [1, 0, 8], [1, 0, 2]], np.uint32)
+    assert np.array_equal(nodes, want)
+    assert [O.morton_decode(int(c)) for c in codes] == [(0, 0, 0), (16, 0, 0), (64, 0, 0)]
+    assert np.all(leaves[:, 0] == 1) and not leaves[:, 1:].any()
