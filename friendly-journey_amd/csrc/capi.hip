@@ -69,6 +69,8 @@ using vxl::Error;
 namespace {
 
 constexpr uint32_t kMaxK = 5;  // the traversal's frame stack holds 6 levels (mirror.rs:22)
+// Rays one allocation-free traversal takes (its limit; the work counter is 64-bit).
+constexpr uint64_t kNoAllocMaxRays = 1ull << 40;
 
 template <class F>
 vxl_status guarded(vxl_ctx *ctx, F &&f) {
@@ -544,6 +546,7 @@ vxl_status vxl_ctx_create(int device, vxl_ctx **out) {
     VXL_HIP(hipGetDeviceProperties(&prop, device));
     ctx->cu_count = prop.multiProcessorCount;
     for (auto &e : ctx->ev) VXL_HIP(hipEventCreate(&e));
+    vxl::noalloc_setup(device);  // once per device: the allocation-free path's scratch
   } catch (const Error &e) {
     vxl::set_error(e.msg);
     delete ctx;
@@ -1269,6 +1272,20 @@ vxl_status vxl_traverse_dev(vxl_ctx *ctx, const vxl_tree *tree, const float *d_r
       VXL_HIP(hipEventSynchronize(ctx->ev[5]));
       VXL_HIP(hipEventElapsedTime(&ctx->last_ms[3], ctx->ev[4], ctx->ev[5]));
     }
+  });
+}
+
+vxl_status vxl_traverse_dev_noalloc(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays,
+                                    uint64_t R, uint32_t *d_hits, double *d_t, void *stream) {
+  return guarded(ctx, [&] {
+    vxl::check_tree(tree);
+    if (R && (!d_rays || !d_hits)) throw Error(VXL_INVALID_ARG, "null ray or hit buffer");
+    if (R > kNoAllocMaxRays)
+      throw Error(VXL_STORAGE_EXCEEDED, "allocation-free traversal takes at most 2^40 rays");
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    // kernels only: the persistent schedule's counter is a slot of the per-device scratch
+    vxl::launch_traverse(ctx, tree, d_rays, R, d_hits, d_t, s,
+                         R ? vxl::noalloc_counter(ctx->device) : nullptr);
   });
 }
 

@@ -19,6 +19,16 @@
 // voxel (voxel-core/src/layout.rs:176-183).
 #include "vxl_internal.hpp"
 
+#include <atomic>
+
+// Work counters of vxl_traverse_dev_noalloc (the entry point for callers that cannot
+// allocate): kNoAllocSlots 8-byte counters, one per persistent traversal in flight, used in
+// turn. Each GPU the library runs on holds its own copy until the process ends. Host code
+// never passes this name to a kernel: it is a host-side stand-in, so each device's address is
+// looked up once with hipGetSymbolAddress (noalloc_setup) and only that address is used.
+constexpr uint32_t kNoAllocSlots = 64;
+static __device__ __align__(256) unsigned long long scratch[kNoAllocSlots];
+
 namespace vxl {
 namespace {
 
@@ -630,7 +640,7 @@ int refill_for(const vxl_ctx *ctx, const vxl_tree *tree) {
 }  // namespace
 
 void launch_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, uint64_t R,
-                     uint32_t *d_hits, double *d_t, hipStream_t s) {
+                     uint32_t *d_hits, double *d_t, hipStream_t s, unsigned long long *counter) {
   if (R == 0) return;
   const size_t lds64 = frame_lds_bytes(tree->k, 64);
   const int refill = refill_for(ctx, tree);
@@ -643,9 +653,11 @@ void launch_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, ui
     const uint64_t waves = (R + 63) / 64;
     uint64_t blocks = uint64_t(ctx->cu_count > 0 ? ctx->cu_count : 256) * uint64_t(per_cu);
     if (blocks > waves) blocks = waves;
-    ctx->trav_ctr.reserve(kCounterSlots * 8 + 64);
-    unsigned long long *ctr =
-        ctx->trav_ctr.as<unsigned long long>() + (ctx->trav_ctr_next++ % kCounterSlots);
+    unsigned long long *ctr = counter;
+    if (!ctr) {
+      ctx->trav_ctr.reserve(kCounterSlots * 8 + 64);
+      ctr = ctx->trav_ctr.as<unsigned long long>() + (ctx->trav_ctr_next++ % kCounterSlots);
+    }
     hipLaunchKernelGGL(k_clear_counter, dim3(1), dim3(1), 0, s, ctr);
     hipLaunchKernelGGL(k_traverse_pl, dim3(uint32_t(blocks)), dim3(64), lds64, s, d_rays, R,
                        tree->nodes, reinterpret_cast<const uint32_t *>(tree->leaves),
@@ -746,6 +758,39 @@ void launch_traverse_batch(vxl_ctx *ctx, const RaySegment *segs, uint32_t S, hip
                      reinterpret_cast<const uint32_t *>(d + seg_bytes), S);
   VXL_HIP(hipGetLastError());
   VXL_HIP(hipEventRecord(ctx->trav_done, s));
+}
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+unsigned long long *g_noalloc_ptr[kMaxDevices] = {};  // scratch's address on each device
+std::atomic<uint32_t> g_noalloc_next[kMaxDevices];    // next slot per device
+}  // namespace
+
+void noalloc_setup(int device) {
+  if (device < 0 || device >= kMaxDevices)
+    throw Error(VXL_NO_DEVICE, "device index beyond the allocation-free path's table");
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (g_noalloc_ptr[device]) return;
+  // the caller made `device` current; the symbol's address differs per device
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    (void)hipGetLastError();
+    throw Error(VXL_DEVICE_ERROR, std::string("hipGetSymbolAddress(scratch) failed: ") +
+                                      hipGetErrorString(e));
+  }
+  g_noalloc_ptr[device] = static_cast<unsigned long long *>(p);
+}
+
+unsigned long long *noalloc_counter(int device) {
+  unsigned long long *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (device >= 0 && device < kMaxDevices) base = g_noalloc_ptr[device];
+  }
+  if (!base) throw Error(VXL_DEVICE_ERROR, "allocation-free scratch not set up for this device");
+  return base + (g_noalloc_next[device].fetch_add(1, std::memory_order_relaxed) % kNoAllocSlots);
 }
 
 }  // namespace vxl

@@ -304,8 +304,15 @@ void build_trees_batch(vxl_ctx *ctx, const std::vector<BatchMesh> &meshes,
                        const unsigned long long *d_ref_pairs);
 
 // dda.hip
+// counter (nullable): the persistent schedule's work counter; null takes a slot of the
+// context's ring (allocated on first use).
 void launch_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, uint64_t R,
-                     uint32_t *d_hits, double *d_t, hipStream_t s);
+                     uint32_t *d_hits, double *d_t, hipStream_t s,
+                     unsigned long long *counter = nullptr);
+// dda.hip: the allocation-free traversal's counter slots (a per-device static array): its
+// address found once per device at context creation (hipGetSymbolAddress), one slot per call.
+void noalloc_setup(int device);
+unsigned long long *noalloc_counter(int device);
 // One segment of a batched traversal: R rays at `rays` (6 f32 each) against `tree`,
 // results to hits (4 u32 each) and t (nullable).
 struct RaySegment {

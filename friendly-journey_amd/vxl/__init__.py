@@ -39,7 +39,8 @@ EXPORTS = [
     "vxl_tree_set_voxels", "vxl_generate_noise_tree", "vxl_traverse_stats",
     "vxl_traverse_graph_create", "vxl_graph_launch", "vxl_graph_destroy",
     "vxl_tree_from_words", "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload",
-    "vxl_tree_free", "vxl_traverse", "vxl_traverse_dev", "vxl_set_timing", "vxl_last_timing",
+    "vxl_tree_free", "vxl_traverse", "vxl_traverse_dev", "vxl_traverse_dev_noalloc",
+    "vxl_set_timing", "vxl_last_timing",
     "vxl_set_tuning", "vxl_set_dda_refill", "vxl_voxelize_batch", "vxl_voxelize_batch_dev", "vxl_traverse_batch",
     "vxl_traverse_batch_dev", "vxl_set_batch_budget", "vxl_autotune_batch",
     "vxl_server_create", "vxl_server_destroy", "vxl_server_hold", "vxl_server_submit_rays",
@@ -146,6 +147,7 @@ def load(path: str = LIB_PATH):
     L.vxl_tree_free.restype = None
     L.vxl_traverse.argtypes = [_vp, _vp, _vp, _u64, _vp, _vp]
     L.vxl_traverse_dev.argtypes = [_vp, _vp, _vp, _u64, _vp, _vp, _vp]
+    L.vxl_traverse_dev_noalloc.argtypes = [_vp, _vp, _vp, _u64, _vp, _vp, _vp]
     L.vxl_set_timing.argtypes = [_vp, C.c_int]
     L.vxl_set_timing.restype = None
     L.vxl_last_timing.argtypes = [_vp, C.POINTER(C.c_float)]
@@ -183,7 +185,7 @@ def load(path: str = LIB_PATH):
                  "vxl_traverse_graph_create", "vxl_graph_launch",
                  "vxl_voxelize_tree", "vxl_voxelize_tree_dev", "vxl_tree_from_words",
                  "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload", "vxl_traverse",
-                 "vxl_traverse_dev", "vxl_set_tuning", "vxl_set_dda_refill", "vxl_voxelize_batch",
+                 "vxl_traverse_dev", "vxl_traverse_dev_noalloc", "vxl_set_tuning", "vxl_set_dda_refill", "vxl_voxelize_batch",
                  "vxl_voxelize_batch_dev", "vxl_traverse_batch", "vxl_traverse_batch_dev",
                  "vxl_set_batch_budget", "vxl_autotune_batch"):
         getattr(L, name).restype = C.c_int
@@ -529,6 +531,13 @@ class Context:
                      stream: int = 0):
         _check(load().vxl_traverse_dev(self.handle, tree.handle, _vp(d_rays), R, _vp(d_hits),
                                        _vp(d_t) if d_t else None, _vp(stream)))
+
+    def traverse_dev_noalloc(self, tree: Tree, d_rays: int, R: int, d_hits: int, d_t: int = 0,
+                             stream: int = 0):
+        """vxl_traverse_dev_noalloc: the same traversal with no device allocation."""
+        _check(load().vxl_traverse_dev_noalloc(self.handle, tree.handle, _vp(d_rays), R,
+                                               _vp(d_hits), _vp(d_t) if d_t else None,
+                                               _vp(stream)))
 
     # -- timing / tuning -----------------------------------------------------------------
     def set_timing(self, on: bool):

@@ -196,6 +196,20 @@ vxl_status vxl_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *rays, u
 vxl_status vxl_traverse_dev(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, uint64_t R,
                             uint32_t *d_hits, double *d_t, void *stream);
 
+/* ---- allocation-free traversal. No counterpart in the reference: it is for callers that
+ *      cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ *      harnesses that forbid hipMalloc). The same results and lane schedule as
+ *      vxl_traverse_dev; the call allocates nothing (no hipMalloc, hipFree or pinned host
+ *      memory, no growth of context scratch) and enqueues kernels only. The persistent
+ *      schedule's work counter lives in a fixed per-GPU array inside the library (64 counters,
+ *      512 bytes, found at vxl_ctx_create and held until the process ends), one counter per
+ *      call in turn, zeroed by a kernel on the call's stream. More than 64 of these calls in
+ *      flight at once on one GPU race on a counter; replays of a captured call reuse its
+ *      counter, so replays must not overlap each other. R above 2^40 rays is
+ *      VXL_STORAGE_EXCEEDED. */
+vxl_status vxl_traverse_dev_noalloc(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays,
+                                    uint64_t R, uint32_t *d_hits, double *d_t, void *stream);
+
 /* ---- procedural noise fixtures: generate_noise_tree (voxel-gpu/src/generate.rs:99) over
  *      NoiseField (voxel-core/src/fixtures.rs:259-346, noise.rs): fBm or ridged Perlin
  *      noise, optionally domain-warped, thresholded; evaluated in f64 with the CPU field's
