@@ -584,6 +584,35 @@ def cpu_dda_baseline(tree, rays, gpu_hits):
     return out, same
 
 
+def valu_probe(torch):
+    """The measured VALU roof: independent chains of binary32 multiply + add (and of u32 add +
+    xor), two VALU instructions per link, never fused, on 8 resident waves per SIMD; lane-ops
+    per second from HIP events over a known instruction count (libvxl_probe.so)."""
+    import ctypes as C
+    lib = C.CDLL(os.path.join(ROOT, "friendly-journey_amd", "libvxl_probe.so"))
+    lib.vxl_probe_valu.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_uint64)]
+    props = torch.cuda.get_device_properties(0)
+    cus = props.multi_processor_count
+    blocks = cus * 8  # 8 blocks of 4 waves per CU: 8 waves per SIMD
+    out = {"cus": cus, "blocks": blocks, "threads_per_block": 256}
+    for kind, name in ((0, "f32_mul_add"), (1, "u32_add_xor")):
+        best = None
+        for iters in (4000, 16000):
+            ms, ops = C.c_double(0), C.c_uint64(0)
+            rc = lib.vxl_probe_valu(kind, blocks, iters, C.byref(ms), C.byref(ops))
+            if rc != 0:
+                raise RuntimeError(f"valu probe failed: HIP error {rc}")
+            rate = ops.value / (ms.value / 1e3)
+            if best is None or rate > best[0]:
+                best = (rate, ms.value, ops.value, iters)
+        rate, ms, ops, iters = best
+        out[name] = {"lane_ops_per_s": rate, "tops": round(rate / 1e12, 2), "ms": round(ms, 4),
+                     "lane_ops": ops, "iters": iters,
+                     "wave_instr_per_cu_cycle_at_2400mhz": round(rate / 64 / cus / 2.4e9, 3)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -596,6 +625,8 @@ def main():
     ap.add_argument("--no-materials", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (counter calibration)")
+    ap.add_argument("--valu-probe", action="store_true",
+                    help="only measure the VALU issue rate (non-packed, non-FMA lane-ops/s)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this path has no multi-GPU form here "
@@ -614,6 +645,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": src.numel() * 4, "copies": 3}), flush=True)
+        return
+    if args.valu_probe:
+        print(json.dumps({"valu_probe": valu_probe(torch)}), flush=True)
         return
     ctx = vxl.Context(0)
     parity = {}  # oracle comparisons of what the run computed (with the CPU baselines)

@@ -42,6 +42,24 @@ def _compile(src: str) -> str:
     return obj
 
 
+PROBE_SRC = os.path.join(HERE, "probe", "valu_probe.hip")
+PROBE_LIB = os.path.join(HERE, "libvxl_probe.so")
+
+
+def build_probe(verbose: bool = False) -> str:
+    """libvxl_probe.so: the VALU issue-rate probe bench.py --valu-probe runs (measurement
+    infrastructure, not part of the libvxl C ABI)."""
+    tmp = PROBE_LIB + ".tmp"
+    r = subprocess.run([HIPCC, *CFLAGS, "-shared", "-o", tmp, PROBE_SRC], capture_output=True,
+                       text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {PROBE_SRC}:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, PROBE_LIB)
+    if verbose:
+        print(f"built {PROBE_LIB}")
+    return PROBE_LIB
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     srcs = sources()
@@ -55,6 +73,7 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_probe(verbose)
     return LIB
 
 

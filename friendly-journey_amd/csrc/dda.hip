@@ -9,6 +9,12 @@
 //    power of two commutes with rounding unless 1/d is subnormal, which that bound
 //    excludes. Otherwise the division is done as written.
 //
+// A frame's cells are tested against a 64-bit mask held in registers: a node frame's child
+// mask, or, in a leaf frame, the leaf's u64 word of the 4^3 sub-brick the cell is in (the 6 low
+// bits of the intra-brick Morton index are the child_bit formula of the cell's low two
+// coordinate bits; the 3 high bits pick the word), reloaded only when a step crosses into
+// another sub-brick. So a leaf is read once per sub-brick visited, not once per cell.
+//
 // Frame storage: the active frame lives in registers. Each ancestor frame is kept in an
 // LDS row named by its level (the stack index of a frame is root_level - level, so "the
 // frame at level L" is unique at any time), one column per lane.
@@ -33,7 +39,10 @@ namespace vxl {
 namespace {
 
 constexpr uint32_t kFull = (7u << 9) | (7u << 12) | (7u << 15);
-constexpr uint32_t kMaxIter = 1u << 22;
+// Descents one walk may make (a safety bound a legitimate walk never comes close to: a
+// frame steps at most 3 (dim - 1) times and is entered only by a descent, so this bounds
+// every walk's visits; checked at descents, not at every visit).
+constexpr uint32_t kMaxDescents = 1u << 18;
 
 // A suspended ancestor frame lives in LDS, one row per (level, field), one column per
 // thread: node, packed cell, t_max x 3. Its t_delta is not kept (a function of the frame's
@@ -89,6 +98,8 @@ __device__ __forceinline__ uint32_t frame_cell_size(uint32_t level) {
 }
 
 // leaf_reaches_f32 (mirror.rs:202-249), with 1/d precomputed (same value per call).
+// Each axis is evaluated branch-free (selects instead of the d == 0 branch), with the same
+// values: for d == 0 the products are computed and discarded.
 __device__ __forceinline__ bool leaf_reaches(uint32_t packed, const Ray &r, uint32_t ox,
                                              uint32_t oy, uint32_t oz, float t_enter) {
   if (packed == kFull) return true;
@@ -101,27 +112,23 @@ __device__ __forceinline__ bool leaf_reaches(uint32_t packed, const Ray &r, uint
   for (int a = 0; a < 3; a++) {
     const float lo = float(org[a] + mn[a]) - 1.0f;
     const float hi = float(org[a] + mx[a] + 1u) + 1.0f;
-    if (r.d[a] == 0.0f) {
-      if (r.o[a] < lo || r.o[a] > hi) miss = true;
-    } else {
-      float t1 = (lo - r.o[a]) * r.rinv[a];
-      float t2 = (hi - r.o[a]) * r.rinv[a];
-      if (t1 > t2) {
-        const float s = t1;
-        t1 = t2;
-        t2 = s;
-      }
-      t_near = fmaxf(t_near, t1);
-      t_far = fminf(t_far, t2);
-    }
+    const bool flat = r.d[a] == 0.0f;
+    miss = miss || (flat && (r.o[a] < lo || r.o[a] > hi));
+    const float t1 = (lo - r.o[a]) * r.rinv[a];
+    const float t2 = (hi - r.o[a]) * r.rinv[a];
+    const bool sw = t1 > t2;
+    const float tn = sw ? t2 : t1, tf = sw ? t1 : t2;
+    t_near = flat ? t_near : fmaxf(t_near, tn);
+    t_far = flat ? t_far : fminf(t_far, tf);
   }
   if (miss || t_near > t_far) return false;
   return t_far >= t_enter;
 }
 
 // The walk of one lane: the active frame (level, node, its origin, cell, t_max, t_delta,
-// t_entry) and, for a node frame, the node record held in registers. Ancestor frames are
-// in LDS (above).
+// t_entry) and its 64-bit scan mask: for a node frame the node's child mask with child_base
+// in `cbase`, for a leaf frame the leaf word of the current 4^3 sub-brick with that
+// sub-brick's index in `cbase`. Ancestor frames are in LDS (above).
 struct Walk {
   uint32_t level, node, ox, oy, oz, cx, cy, cz;
   float tm0, tm1, tm2, td0, td1, td2, te;
@@ -143,7 +150,9 @@ __device__ __forceinline__ void load_ray(const float *__restrict__ rays, uint64_
 }
 
 // Grid clip, ray_aabb_f32 (mirror.rs:169-192) + the t1 < 0 check (:273-277); false when
-// the ray misses the grid or the tree is empty, else t0 is the entry parameter.
+// the ray misses the grid or the tree is empty, else t0 is the entry parameter. Branch-free:
+// the slab interval only narrows, so "empty after some axis" (the early return) is "empty
+// after all three"; an axis with d == 0 tests the origin instead and leaves the interval.
 __device__ __forceinline__ bool clip_grid(const Ray &r, uint32_t n, uint64_t leaf_count,
                                           float &t0_out) {
   const float nf = float(n);
@@ -151,23 +160,15 @@ __device__ __forceinline__ bool clip_grid(const Ray &r, uint32_t n, uint64_t lea
   bool missed = leaf_count == 0;
 #pragma unroll
   for (int a = 0; a < 3; a++) {
-    if (missed) break;
-    if (r.d[a] == 0.0f) {
-      if (r.o[a] < 0.0f || r.o[a] > nf) missed = true;
-    } else {
-      float ta = (0.0f - r.o[a]) * r.rinv[a];
-      float tb = (nf - r.o[a]) * r.rinv[a];
-      if (ta > tb) {
-        const float s = ta;
-        ta = tb;
-        tb = s;
-      }
-      t0 = fmaxf(t0, ta);
-      t1 = fminf(t1, tb);
-      if (t0 > t1) missed = true;
-    }
+    const bool flat = r.d[a] == 0.0f;
+    missed = missed || (flat && (r.o[a] < 0.0f || r.o[a] > nf));
+    const float ta = (0.0f - r.o[a]) * r.rinv[a];
+    const float tb = (nf - r.o[a]) * r.rinv[a];
+    const bool sw = ta > tb;
+    t0 = flat ? t0 : fmaxf(t0, sw ? tb : ta);
+    t1 = flat ? t1 : fminf(t1, sw ? ta : tb);
   }
-  if (!missed && t1 < 0.0f) missed = true;
+  missed = missed || t0 > t1 || t1 < 0.0f;
   t0_out = t0;
   return !missed;
 }
@@ -216,116 +217,126 @@ __device__ __forceinline__ bool step_frame(const Ray &r, Walk &w) {
   return ok;
 }
 
+// The 4^3 sub-brick of a leaf-frame cell: the high bits of the intra-brick Morton index.
+__device__ __forceinline__ uint32_t sub_brick(uint32_t cx, uint32_t cy, uint32_t cz) {
+  return (cx >> 2) | ((cy >> 2) << 1) | ((cz >> 2) << 2);
+}
+
+// The scan mask of the active frame: a node's record, or a leaf's sub-brick word.
+__device__ __forceinline__ void load_node(Walk &w, const uint32_t *__restrict__ nodes) {
+  w.mlo = nodes[3ull * w.node];
+  w.mhi = nodes[3ull * w.node + 1];
+  w.cbase = nodes[3ull * w.node + 2];
+}
+__device__ __forceinline__ void load_leaf_word(Walk &w, const uint32_t *__restrict__ leafw) {
+  w.cbase = sub_brick(w.cx, w.cy, w.cz);
+  const uint2 v = *reinterpret_cast<const uint2 *>(leafw + (uint64_t(w.node) * 8 + w.cbase) * 2);
+  w.mlo = v.x;
+  w.mhi = v.y;
+}
+
 // The root frame of a ray that entered the grid at t0 (mirror.rs:278-283).
 __device__ __forceinline__ void begin_walk(const Ray &r, Walk &w, uint32_t k, float t0,
-                                           const uint32_t *__restrict__ nodes) {
+                                           const uint32_t *__restrict__ nodes,
+                                           const uint32_t *__restrict__ leafw) {
   w.root_level = k == 0 ? 1u : k + 1;
   w.level = w.root_level;
   w.node = 0;
   w.ox = w.oy = w.oz = 0;
   w.iter = 0;
-  w.mlo = w.mhi = w.cbase = 0;
   make_frame(r, w, w.level, fmaxf(t0, 0.0f));
-  if (w.level >= 2) {
-    w.mlo = nodes[0];
-    w.mhi = nodes[1];
-    w.cbase = nodes[2];
-  }
+  if (w.level >= 2) load_node(w, nodes);
+  else load_leaf_word(w, leafw);
 }
 
 constexpr uint32_t kWalking = 0, kHit = 1, kLeft = 2, kCapped = 3;
 
-// One visit of the walk (mirror.rs:284-325): test the active cell -- a voxel bit of the
-// leaf (one word load) or a child bit of the node record in registers; on a hit voxel
-// stop; on an occupied child whose leaf the ray reaches descend; else step, popping
-// parents until one steps to a new cell (mirror.rs:155-165). `frames` is this lane's
-// column of the LDS frame rows, `fstride` the row pitch. kStats counts the descents into
-// leaves and into nodes (the mirror's walk statistics).
+// Bit of Walk::iter set while the active frame was just restored by an ascent: its next
+// visit steps it without testing its cell again (ascend, mirror.rs:155-165).
+constexpr uint32_t kPendingStep = 1u << 31;
+
+// One iteration of the walk (mirror.rs:284-325): test the active cell against the frame's
+// scan mask (a node's child mask, or the leaf word of the cell's sub-brick, both in
+// registers); on a hit voxel stop; on an occupied child whose leaf the ray reaches descend;
+// else step, reloading the leaf word when the step crosses into another sub-brick. A frame
+// that cannot step is popped, one level per iteration: the parent is restored and steps at
+// its next iteration, as the mirror's ascend steps the parent. Popping one level per
+// iteration (rather than looping until some ancestor steps) keeps each iteration's code
+// short and lets the other lanes of the wave walk on while one climbs several levels.
+// `frames` is this lane's column of the LDS frame rows, `fstride` the row pitch. kStats
+// counts the descents into leaves and into nodes and the visits (cell tests; the mirror's
+// walk statistics).
 template <bool kStats = false>
 __device__ __forceinline__ uint32_t walk_visit(const Ray &r, Walk &w,
                                                const uint32_t *__restrict__ nodes,
                                                const uint32_t *__restrict__ leafw,
                                                const uint32_t *__restrict__ bounds,
                                                uint32_t *frames, uint32_t fstride,
-                                               uint32_t *leaf_desc = nullptr,
-                                               uint32_t *node_desc = nullptr) {
-  if (++w.iter > kMaxIter) return kCapped;  // bounded: a legitimate walk never comes close
-  uint32_t word, bitpos;
-  const uint32_t mlo = w.mlo, mhi = w.mhi;
-  if (w.level == 1) {
-    const uint32_t m = dev::morton9(w.cx, w.cy, w.cz);
-    word = leafw[uint64_t(w.node) * 16 + (m >> 5)];
-    bitpos = m & 31;
-  } else {
-    const uint32_t bit = dev::child_bit(w.cx, w.cy, w.cz);
-    word = bit < 32 ? mlo : mhi;
-    bitpos = bit & 31;
-  }
-  const bool has = (word >> bitpos) & 1u;
-  if (w.level == 1 && has) return kHit;
-  if (w.level > 1 && has) {
-    const uint32_t bit = dev::child_bit(w.cx, w.cy, w.cz);
-    const uint32_t size = frame_cell_size(w.level);
-    const uint32_t cox = w.ox + w.cx * size, coy = w.oy + w.cy * size, coz = w.oz + w.cz * size;
-    const uint32_t below = bit < 32 ? __popc(mlo & ((1u << bit) - 1u))
-                                    : __popc(mlo) + __popc(mhi & ((1u << (bit - 32)) - 1u));
-    const uint32_t slot = w.cbase + below;
-    const bool descend = w.level != 2 || leaf_reaches(bounds[slot], r, cox, coy, coz, w.te);
-    if (descend) {
-      {
-        uint32_t *f = frames + (w.level - 2) * kFrameWords * fstride;
-        f[0] = w.node;
-        f[fstride] = w.cx | (w.cy << 8) | (w.cz << 16);
-        f[2 * fstride] = __float_as_uint(w.tm0);
-        f[3 * fstride] = __float_as_uint(w.tm1);
-        f[4 * fstride] = __float_as_uint(w.tm2);
+                                               uint32_t *stats = nullptr) {
+  if (!(w.iter & kPendingStep)) {
+    if (kStats) stats[2] += 1;
+    const uint32_t mlo = w.mlo, mhi = w.mhi;
+    const uint32_t bit = dev::child_bit(w.cx & 3u, w.cy & 3u, w.cz & 3u);
+    const bool has = ((bit < 32 ? mlo : mhi) >> (bit & 31u)) & 1u;
+    if (has && w.level == 1) return kHit;
+    if (has) {
+      const uint32_t size = frame_cell_size(w.level);
+      const uint32_t cox = w.ox + w.cx * size, coy = w.oy + w.cy * size, coz = w.oz + w.cz * size;
+      const uint32_t below = bit < 32 ? __popc(mlo & ((1u << bit) - 1u))
+                                      : __popc(mlo) + __popc(mhi & ((1u << (bit - 32)) - 1u));
+      const uint32_t slot = w.cbase + below;
+      const bool descend = w.level != 2 || leaf_reaches(bounds[slot], r, cox, coy, coz, w.te);
+      if (descend) {
+        if (++w.iter > kMaxDescents) return kCapped;
+        {
+          uint32_t *f = frames + (w.level - 2) * kFrameWords * fstride;
+          f[0] = w.node;
+          f[fstride] = w.cx | (w.cy << 8) | (w.cz << 16);
+          f[2 * fstride] = __float_as_uint(w.tm0);
+          f[3 * fstride] = __float_as_uint(w.tm1);
+          f[4 * fstride] = __float_as_uint(w.tm2);
+        }
+        w.level -= 1;
+        if (kStats) stats[w.level == 1 ? 0 : 1] += 1;
+        w.node = slot;
+        w.ox = cox;
+        w.oy = coy;
+        w.oz = coz;
+        make_frame(r, w, w.level, w.te);
+        if (w.level >= 2) load_node(w, nodes);
+        else load_leaf_word(w, leafw);
+        return kWalking;
       }
-      w.level -= 1;
-      if (kStats) {
-        if (w.level == 1) *leaf_desc += 1;
-        else *node_desc += 1;
-      }
-      w.node = slot;
-      w.ox = cox;
-      w.oy = coy;
-      w.oz = coz;
-      make_frame(r, w, w.level, w.te);
-      if (w.level >= 2) {
-        w.mlo = nodes[3ull * w.node];
-        w.mhi = nodes[3ull * w.node + 1];
-        w.cbase = nodes[3ull * w.node + 2];
-      }
-      return kWalking;
     }
   }
-  bool stepped = step_frame(r, w);
-  while (!stepped) {
-    if (w.level == w.root_level) return kLeft;
-    w.level += 1;
-    const uint32_t size = frame_cell_size(w.level);
-    {
-      const uint32_t *f = frames + (w.level - 2) * kFrameWords * fstride;
-      w.node = f[0];
-      const uint32_t cell = f[fstride];
-      w.cx = cell & 0xffu;
-      w.cy = (cell >> 8) & 0xffu;
-      w.cz = cell >> 16;
-      w.tm0 = __uint_as_float(f[2 * fstride]);
-      w.tm1 = __uint_as_float(f[3 * fstride]);
-      w.tm2 = __uint_as_float(f[4 * fstride]);
-      const float cs = float(size);
-      w.td0 = axis_delta(cs, r.d[0], r.rinv[0], r.big[0]);
-      w.td1 = axis_delta(cs, r.d[1], r.rinv[1], r.big[1]);
-      w.td2 = axis_delta(cs, r.d[2], r.rinv[2], r.big[2]);
-    }
-    w.ox -= w.cx * size;
-    w.oy -= w.cy * size;
-    w.oz -= w.cz * size;
-    w.mlo = nodes[3ull * w.node];
-    w.mhi = nodes[3ull * w.node + 1];
-    w.cbase = nodes[3ull * w.node + 2];
-    stepped = step_frame(r, w);
+  w.iter &= ~kPendingStep;
+  if (step_frame(r, w)) {
+    if (w.level == 1 && sub_brick(w.cx, w.cy, w.cz) != w.cbase) load_leaf_word(w, leafw);
+    return kWalking;
   }
+  if (w.level == w.root_level) return kLeft;
+  w.level += 1;
+  const uint32_t size = frame_cell_size(w.level);
+  {
+    const uint32_t *f = frames + (w.level - 2) * kFrameWords * fstride;
+    w.node = f[0];
+    const uint32_t cell = f[fstride];
+    w.cx = cell & 0xffu;
+    w.cy = (cell >> 8) & 0xffu;
+    w.cz = cell >> 16;
+    w.tm0 = __uint_as_float(f[2 * fstride]);
+    w.tm1 = __uint_as_float(f[3 * fstride]);
+    w.tm2 = __uint_as_float(f[4 * fstride]);
+    const float cs = float(size);
+    w.td0 = axis_delta(cs, r.d[0], r.rinv[0], r.big[0]);
+    w.td1 = axis_delta(cs, r.d[1], r.rinv[1], r.big[1]);
+    w.td2 = axis_delta(cs, r.d[2], r.rinv[2], r.big[2]);
+  }
+  w.ox -= w.cx * size;
+  w.oy -= w.cy * size;
+  w.oz -= w.cz * size;
+  load_node(w, nodes);  // a restored frame is a node frame
+  w.iter |= kPendingStep;
   return kWalking;
 }
 
@@ -346,7 +357,7 @@ __device__ __forceinline__ double hit_t(const float o[3], const float d[3], uint
 }
 
 // Writes ray ri's result: (x, y, z, 1) for a hit, (0, 0, 0, 0) for none, (0, 0, 0, ~0) for
-// a walk over the iteration bound; and t when tout is given.
+// a walk over the descent bound; and t when tout is given.
 __device__ __forceinline__ void store_result(uint64_t ri, const Ray &r, const Walk &w,
                                              uint32_t status, uint4 *__restrict__ hits,
                                              double *__restrict__ tout) {
@@ -369,7 +380,7 @@ __device__ __forceinline__ void traverse_ray(
   uint32_t status = kLeft;
   float t0;
   if (clip_grid(r, n, leaf_count, t0)) {
-    begin_walk(r, w, k, t0, nodes);
+    begin_walk(r, w, k, t0, nodes, leafw);
     do {
       status = walk_visit(r, w, nodes, leafw, bounds, frames_lds + threadIdx.x, blockDim.x);
     } while (status == kWalking);
@@ -414,7 +425,7 @@ __global__ __launch_bounds__(64) void k_traverse_stats(
     uint32_t k, uint64_t leaf_count, unsigned long long *__restrict__ sums) {
   extern __shared__ uint32_t frames_lds[];
   const uint64_t ri = uint64_t(blockIdx.x) * 64 + threadIdx.x;
-  uint32_t ld = 0, nd = 0, visits = 0;
+  uint32_t st3[3] = {0, 0, 0};  // leaf descents, node descents, visits
   if (ri < R) {
     Ray r;
     load_ray(rays, ri, r);
@@ -422,15 +433,14 @@ __global__ __launch_bounds__(64) void k_traverse_stats(
     w.ox = w.oy = w.oz = w.cx = w.cy = w.cz = 0;
     float t0;
     if (clip_grid(r, n, leaf_count, t0)) {
-      begin_walk(r, w, k, t0, nodes);
+      begin_walk(r, w, k, t0, nodes, leafw);
       uint32_t st;
       do {
-        visits++;
-        st = walk_visit<true>(r, w, nodes, leafw, bounds, frames_lds + threadIdx.x, 64, &ld, &nd);
+        st = walk_visit<true>(r, w, nodes, leafw, bounds, frames_lds + threadIdx.x, 64, st3);
       } while (st == kWalking);
     }
   }
-  unsigned long long v[3] = {ld, nd, visits};
+  unsigned long long v[3] = {st3[0], st3[1], st3[2]};
 #pragma unroll
   for (int i = 0; i < 3; i++) {
 #pragma unroll
@@ -488,7 +498,7 @@ __global__ __launch_bounds__(64) void k_traverse_pl(
           load_ray(rays, ri, r);
           float t0;
           if (clip_grid(r, n, leaf_count, t0)) {
-            begin_walk(r, w, k, t0, nodes);
+            begin_walk(r, w, k, t0, nodes, leafw);
             active = true;
           } else {
             store_result(ri, r, w, kLeft, hits, nullptr);
@@ -558,14 +568,14 @@ __global__ __launch_bounds__(64) void k_render(vxl_camera cam, const uint32_t *_
       r.d[a] = v[a] / len;
       r.rinv[a] = 1.0f / r.d[a];
       r.big[a] = !(fabsf(r.d[a]) < 0x1p100f);
-    }
+      }
   }
   Walk wk;
   wk.ox = wk.oy = wk.oz = wk.cx = wk.cy = wk.cz = 0;
   uint32_t status = kLeft;
   float t0;
   if (clip_grid(r, n, leaf_count, t0)) {
-    begin_walk(r, wk, k, t0, nodes);
+    begin_walk(r, wk, k, t0, nodes, leafw);
     do {
       status = walk_visit(r, wk, nodes, leafw, bounds, frames_lds + threadIdx.x, 64);
     } while (status == kWalking);
