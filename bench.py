@@ -35,10 +35,17 @@ HBM_PEAK_GBS = 8000.0
 # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB per dispatch, factors from a 1 GiB copy kernel);
 # profiles/k_sat__terrain_1024.md holds the commands and the raw counter values.
 K_SAT_TRAFFIC_BYTES = 347_315_342
-# Algorithmic VALU lane-ops of one ray walk, counted on the reference's f32 walker: a cell
-# visit (bit test + Walker::step) ~22, a descent (Walker::enter on 3 axes + the frame save)
-# ~50, ray setup and clip ~60, the f64 t of a hit ~20.
-DDA_OPS_VISIT, DDA_OPS_DESCENT, DDA_OPS_RAY = 22, 50, 80
+# Algorithmic lane-ops of one ray walk, counted operation by operation on the reference's
+# f32 walker (voxel-core/src/mirror.rs), each arithmetic, compare, select or bit operation one:
+# a visit = the cell test (child_bit / 9-bit Morton index, word and bit extraction: 15) plus
+# Walker::step (:87-116, two t_max compares, the finiteness and bound checks, cell and t_max
+# updates: 8); a node descent = Walker::enter on 3 axes (:48-85, 17 each) + cell size and
+# child origin (8) + child_slot's masked popcount (4); a leaf descent adds leaf_reaches_f32
+# (:202-249: bounds unpack 12, box 18, three slabs 33, compares 2); the ascent after each
+# descent steps the parent (8); a ray = ray_aabb_f32 (:169-192, 36) + the t0 clamps (2) + the
+# root Walker::enter (53); a hit's f64 t = three slabs (18).
+DDA_OPS_VISIT, DDA_OPS_NODE_DESCENT, DDA_OPS_LEAF_DESCENT = 23, 63, 128
+DDA_OPS_ASCENT, DDA_OPS_RAY, DDA_OPS_HIT_T = 8, 91, 18
 
 
 def log(msg):
@@ -103,6 +110,65 @@ def bench_voxelize(ctx, torch, W, steps, warmup):
     mean = lambda v: sum(v) / len(v)
     return dict(T=T, n=n, elapsed=elapsed, steps=steps, sat_ms=mean(sat_ms),
                 setup_ms=mean(setup_ms), build_ms=mean(build_ms), **info), tris
+
+
+def bench_c1(ctx, torch, W, steps, warmup, cpu, parity):
+    """The plain CPU-reference configuration: the seeded 128x64 UV sphere (16,128 triangles)
+    at 256^3 in Resolution(512). GPU: voxelize + tree on triangles resident in HBM, K steps
+    back to back (as the headline); CPU: the oracle's per-stage wall times (serial voxelize,
+    SparseTree::build on every core, from_sparse), and its tree against the GPU's."""
+    import numpy as np
+    tris = W.uv_sphere()
+    T = tris.shape[0]
+    n, vs = W.grid_for(256)
+    d = torch.from_numpy(tris).to("cuda")
+    torch.cuda.synchronize()
+    log(f"c1: sphere T={T} n={n}")
+    for _ in range(warmup):
+        ctx.voxelize_tree_dev(d.data_ptr(), T, n, (0, 0, 0), vs).free()
+    torch.cuda.synchronize()
+    trees = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        trees.append(ctx.voxelize_tree_dev(d.data_ptr(), T, n, (0, 0, 0), vs))
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    gpu_tree = trees[-1].download()
+    out = dict(workload="UV sphere 128x64 (16,128 tris, r 0.4, +-2% radius noise) at 256^3 in "
+                        "Resolution(512)", triangles=T, pairs=trees[0].pairs,
+               leaves=trees[0].leaf_count, nodes=trees[0].node_count, ms_per_step=round(ms, 4),
+               mtris_per_s=round(T / ms / 1e3, 1))
+    for tr in trees:
+        tr.free()
+    ctx.set_timing(True)
+    st = []
+    for _ in range(5):
+        ctx.voxelize_tree_dev(d.data_ptr(), T, n, (0, 0, 0), vs).free()
+        st.append(ctx.last_timing()[:3])
+    ctx.set_timing(False)
+    out["gpu_stages_ms"] = {k: round(float(np.median([x[i] for x in st])), 4)
+                            for i, k in enumerate(("setup", "sat_kernel", "tree_build"))}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        cores = min(os.cpu_count() or 1, 64)
+        runs = []
+        for _ in range(5):
+            a = time.perf_counter()
+            words, _, _ = O.voxelize_surface(tris, n, (0, 0, 0), vs, 1e-4)
+            b = time.perf_counter()
+            otree = O.Tree.build(words, n, threads=cores)
+            build_s, serial_s = otree.stage_seconds()
+            runs.append((b - a, build_s, serial_s))
+        med = [float(np.median([r[i] for r in runs])) for i in range(3)]
+        out["cpu_stages_ms"] = {"voxelize_surface_cpu_1_thread": round(med[0] * 1e3, 3),
+                                f"sparse_tree_build_{cores}_threads": round(med[1] * 1e3, 3),
+                                "from_sparse_1_thread": round(med[2] * 1e3, 3)}
+        out["cpu_mtris_per_s"] = round(T / sum(med) / 1e6, 3)
+        out["cpu_cores"] = cores
+        parity["sphere_256_tree_identical"] = all(
+            np.array_equal(a, b) for a, b in zip(otree.export(), gpu_tree))
+    return out
 
 
 def bench_materials(ctx, torch, W, plain_ms, reps=5):
@@ -417,10 +483,13 @@ def bench_batch(ctx, torch, W, reps=3):
         timing="wall clock, median of %d, inputs resident in HBM" % reps)
 
 
-def bench_serve(ctx, torch, W, duration_s=2.0):
+def bench_serve(ctx, torch, W, duration_s=2.0, sweep=(5000.0, 10000.0, 20000.0),
+                sweep_requests=4000):
     """Poisson stream at 2k req/s (70 % ray queries against 64 resident trees, 30 % voxelize
     requests), served by the continuous-batching request loop and by the plain
-    one-request-per-launch loop on the same schedule; latency = submit -> completion."""
+    one-request-per-launch loop on the same schedule; latency = submit -> completion. Then the
+    same workload at higher rates (about `sweep_requests` requests each), both modes, to show
+    where continuous batching separates from launch-per-request."""
     import numpy as np
     import vxl
     tree_plan, mesh_plan, reqs = W.serve_plan(duration_s=duration_s)
@@ -434,17 +503,14 @@ def bench_serve(ctx, torch, W, duration_s=2.0):
              for m in sorted({p["m"] for p in tree_plan})}
     meshes = [W.batch_mesh(p) for p in mesh_plan]
     d_mesh = [torch.from_numpy(np.ascontiguousarray(m)).to("cuda") for m in meshes]
-    ray_reqs = [r for r in reqs if r[1] == "rays"]
-    R_total = sum(r[3] for r in ray_reqs)
-    out = {}
-    results = {}
-    runs = [("batching", 1.0), ("plain", 1.0), ("batching_saturated", 0.0),
-            ("plain_saturated", 0.0)]
-    for mode, pace in runs:
+
+    def run_stream(requests, batching, pace):
+        """One pass of `requests` through a fresh server; returns (stats row, hits, t, vox)."""
+        R_total = sum(r[3] for r in requests if r[1] == "rays")
         hits = torch.zeros((R_total, 4), dtype=torch.int32, device="cuda")
         tt = torch.zeros(R_total, dtype=torch.float64, device="cuda")
         torch.cuda.synchronize()
-        srv = vxl.Server(0, batching=mode.startswith("batching"))
+        srv = vxl.Server(0, batching=batching)
         srv.hold(resident)
         # warm both lanes (scratch, pools) outside the measured stream
         for i in range(4):
@@ -454,10 +520,9 @@ def bench_serve(ctx, torch, W, duration_s=2.0):
         pool0 = pools[tree_plan[0]["m"]]
         srv.wait(srv.submit_rays(trees[0], pool0.data_ptr(), 1000, hits.data_ptr(), 0))
         srv.reset_stats()
-        log(f"serve: {mode} stream")
         ids, at = [], 0
         t0 = time.perf_counter()
-        for r in reqs:
+        for r in requests:
             while time.perf_counter() - t0 < r[0] * pace:
                 pass
             if r[1] == "rays":
@@ -482,26 +547,35 @@ def bench_serve(ctx, torch, W, duration_s=2.0):
         st = srv.stats()
         srv.close()
         torch.cuda.synchronize()
-        if mode == "batching":
-            results[mode] = (hits, tt, vox)
-        else:  # compare with the first run, then drop the buffers
-            hb, tb, vb = results["batching"]
-            results[mode] = bool(torch.equal(hb, hits)) and bool(torch.equal(tb, tt)) and vb == vox
+        row = dict(p50_ms=round(st["p50_ms"], 3), p99_ms=round(st["p99_ms"], 3),
+                   mean_ms=round(st["mean_ms"], 3), max_ms=round(st["max_ms"], 3),
+                   ray_p50_ms=round(st["ray_p50_ms"], 3), ray_p99_ms=round(st["ray_p99_ms"], 3),
+                   vox_p50_ms=round(st["vox_p50_ms"], 3), vox_p99_ms=round(st["vox_p99_ms"], 3),
+                   throughput_rps=round(st["throughput_rps"], 1),
+                   completed=int(st["completed"]), failed=int(st["failed"]),
+                   launches=int(st["batches"]),
+                   mean_requests_per_launch=round(st["mean_batch_requests"], 2),
+                   peak_gb=round(st["peak_bytes"] / 1e9, 2), submit_span_s=round(submit_s, 3))
+        return row, hits, tt, vox
+
+    out = {}
+    ray_reqs = [r for r in reqs if r[1] == "rays"]
+    R_total = sum(r[3] for r in ray_reqs)
+    base = None
+    same = True
+    for mode, pace in (("batching", 1.0), ("plain", 1.0), ("batching_saturated", 0.0),
+                       ("plain_saturated", 0.0)):
+        log(f"serve: {mode} stream")
+        row, hits, tt, vox = run_stream(reqs, mode.startswith("batching"), pace)
+        out[mode] = row
+        if base is None:
+            base = (hits, tt, vox)
+        else:  # every mode gives the first run's results
+            same = same and bool(torch.equal(base[0], hits)) and bool(torch.equal(base[1], tt)) \
+                and base[2] == vox
             del hits, tt
-        out[mode] = dict(p50_ms=round(st["p50_ms"], 3), p99_ms=round(st["p99_ms"], 3),
-                         mean_ms=round(st["mean_ms"], 3), max_ms=round(st["max_ms"], 3),
-                         ray_p50_ms=round(st["ray_p50_ms"], 3),
-                         ray_p99_ms=round(st["ray_p99_ms"], 3),
-                         vox_p50_ms=round(st["vox_p50_ms"], 3),
-                         vox_p99_ms=round(st["vox_p99_ms"], 3),
-                         throughput_rps=round(st["throughput_rps"], 1),
-                         completed=int(st["completed"]), failed=int(st["failed"]),
-                         launches=int(st["batches"]),
-                         mean_requests_per_launch=round(st["mean_batch_requests"], 2),
-                         peak_gb=round(st["peak_bytes"] / 1e9, 2),
-                         submit_span_s=round(submit_s, 3))
-    out["identical_results"] = all(results[m] for m in ("plain", "batching_saturated",
-                                                         "plain_saturated"))
+    del base
+    out["identical_results"] = same
     out["workload"] = (f"Poisson {len(reqs) / duration_s:.0f} req/s for {duration_s} s: "
                        f"{len(ray_reqs)} ray queries (log-U[1k, 500k] rays, {R_total} in all) "
                        f"against 64 resident trees (m in {{256, 512, 1024}}, "
@@ -514,6 +588,36 @@ def bench_serve(ctx, torch, W, duration_s=2.0):
         max(out["plain_saturated"]["throughput_rps"], 1e-9), 3)
     out["saturated_note"] = ("*_saturated: the same requests submitted at once (no pacing); "
                              "throughput = completed / (last completion - first submit)")
+    # rate sweep: the same request mix at higher Poisson rates, both modes, results compared
+    curve = []
+    for rate in sweep:
+        dur = sweep_requests / rate
+        _, _, rq = W.serve_plan(duration_s=dur, rate=rate, seed=0x5EED_0008 + int(rate))
+        log(f"serve sweep: {rate:.0f} req/s, {len(rq)} requests")
+        rb, hb, tb, vb = run_stream(rq, True, 1.0)
+        rp, hp, tp, vp = run_stream(rq, False, 1.0)
+        ident = bool(torch.equal(hb, hp)) and bool(torch.equal(tb, tp)) and vb == vp
+        del hb, tb, hp, tp
+        curve.append(dict(rate_rps=rate, requests=len(rq),
+                          batching=dict(p50_ms=rb["p50_ms"], p99_ms=rb["p99_ms"],
+                                        throughput_rps=rb["throughput_rps"],
+                                        mean_requests_per_launch=rb["mean_requests_per_launch"]),
+                          plain=dict(p50_ms=rp["p50_ms"], p99_ms=rp["p99_ms"],
+                                     throughput_rps=rp["throughput_rps"]),
+                          p50_speedup=round(rp["p50_ms"] / max(rb["p50_ms"], 1e-9), 3),
+                          p99_speedup=round(rp["p99_ms"] / max(rb["p99_ms"], 1e-9), 3),
+                          identical_results=ident))
+    out["rate_sweep"] = [dict(rate_rps=len(reqs) / duration_s, requests=len(reqs),
+                              batching={k: out["batching"][k] for k in
+                                        ("p50_ms", "p99_ms", "throughput_rps",
+                                         "mean_requests_per_launch")},
+                              plain={k: out["plain"][k] for k in
+                                     ("p50_ms", "p99_ms", "throughput_rps")},
+                              p50_speedup=out["p50_speedup"], p99_speedup=out["p99_speedup"],
+                              identical_results=same)] + curve
+    sep = [c["rate_rps"] for c in out["rate_sweep"] if c["p99_speedup"] >= 1.5]
+    out["p99_separates_at_rps"] = min(sep) if sep else None
+    out["identical_results"] = same and all(c["identical_results"] for c in curve)
     for t in trees:
         t.free()
     return out
@@ -528,7 +632,8 @@ def dda_walk_stats(ctx, tree, d_rays, R):
     d_leaf, d_node, visits = ld / R, nd / R, visits / R
     return dict(bytes=24 + 16 + 8 + 68 * d_leaf + 12 * d_node, leaf_descents=d_leaf,
                 node_descents=d_node, visits=visits,
-                ops=DDA_OPS_RAY + DDA_OPS_VISIT * visits + DDA_OPS_DESCENT * (d_leaf + d_node))
+                ops=(DDA_OPS_RAY + DDA_OPS_VISIT * visits + DDA_OPS_NODE_DESCENT * d_node +
+                     DDA_OPS_LEAF_DESCENT * d_leaf + DDA_OPS_ASCENT * (d_node + d_leaf)))
 
 
 def cpu_baseline(tris, W, gpu_tree):
@@ -679,6 +784,9 @@ def main():
         "roofline": {"kernel": "k_sat", "bound": "valu", "achieved": round(achieved, 3),
                      "peak": VALU_PEAK_TOPS, "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4),
                      "traffic": K_SAT_TRAFFIC_BYTES,
+                     "traffic_source": "profiled constant: FETCH_SIZE x2 + WRITE_SIZE of k_sat on "
+                                       "this workload (profiles/k_sat__terrain_1024.md), not "
+                                       "measured by this run",
                      "traffic_gbs": round(K_SAT_TRAFFIC_BYTES / (vox["sat_ms"] / 1e3) / 1e9, 1),
                      "traffic_frac_of_hbm": round(K_SAT_TRAFFIC_BYTES / (vox["sat_ms"] / 1e3) / 1e9
                                                   / HBM_PEAK_GBS, 4),
@@ -687,6 +795,11 @@ def main():
                                   "every product, so an FMA would change results, and packed FP32 is not "
                                   "used; the 157.3 TFLOP/s FP32 vector figure counts a packed FMA as 4"},
     }
+    probe = valu_probe(torch)
+    line["roofline"]["peak_measured"] = probe["f32_mul_add"]["tops"]
+    line["roofline"]["frac_of_measured"] = round(achieved / probe["f32_mul_add"]["tops"], 4)
+    line["valu_probe"] = {k: probe[k]["tops"] for k in ("f32_mul_add", "u32_add_xor")}
+    line["valu_probe"]["unit"] = "TOP/s (non-packed, non-FMA lane-ops, 8 waves per SIMD)"
     # tree build: HBM-bound by its algorithmic bytes (brick-flag scan at 1 bit per brick
     # code, 64 B leaf + 4 B bounds per leaf, 12 B per node) against the stage's event time
     bricks = (vox["n"] // 8) ** 3
@@ -697,6 +810,7 @@ def main():
         "frac": round(build_gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes": build_bytes,
         "note": "a chain of short kernels and one read-back of the level sizes: launch latency, "
                 "not bandwidth, sets its time"}
+    line["c1"] = bench_c1(ctx, torch, W, args.steps, args.warmup, not args.no_cpu, parity)
     if not args.no_materials:
         line["materials"] = bench_materials(ctx, torch, W, line["ms_per_step"])
         line["edits"] = bench_edits(ctx, torch, W)
@@ -713,19 +827,21 @@ def main():
                     tree, rays, gpu_hits)
             del tree, rays, gpu_hits
             secs = rec["kernel_ms"] / 1e3
+            ops = ws["ops"] + DDA_OPS_HIT_T * rec["hit_fraction"]
             gbs = ws["bytes"] * rec["rays"] / secs / 1e9
-            tops = ws["ops"] * rec["rays"] / secs / 1e12
+            tops = ops * rec["rays"] / secs / 1e12
             rec.pop("_keep")
             rec["walk"] = {k: round(ws[k], 3) for k in ("visits", "leaf_descents", "node_descents")}
-            rec["roofline"] = {"bound": "valu", "ops_per_ray": round(ws["ops"], 1),
+            rec["roofline"] = {"bound": "valu", "ops_per_ray": round(ops, 1),
                                "achieved": round(tops, 3), "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
                                "frac": round(tops / VALU_PEAK_TOPS, 4),
                                "hbm": {"bytes_per_ray": round(ws["bytes"], 1),
                                        "achieved_gbs": round(gbs, 1),
                                        "frac": round(gbs / HBM_PEAK_GBS, 4)},
-                               "note": "walks diverge: 9-14% of lanes active per VALU instruction "
-                                       "(VALUUtilization), so the issued work is several times "
-                                       "the algorithmic work"}
+                               "note": "algorithmic ops (the mirror's operations, counted) over the "
+                                       "VALU roof; the kernel issues VALU instructions near the roof "
+                                       "with 16-21% of lanes active (divergent walks), and fetches "
+                                       "well under the HBM roof: issue-bound, not bandwidth-bound"}
         line["dda"] = dda
     if not args.no_batch:
         line["batched"] = bench_batch(ctx, torch, W)

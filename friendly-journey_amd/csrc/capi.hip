@@ -112,10 +112,14 @@ void validate_grid(const vxl_grid *g, uint32_t *k) {
                 "resolution " + std::to_string(g->n) + " exceeds the supported 8*4^5 = 8192");
   if (!std::isfinite(g->voxel_size) || g->voxel_size <= 0.0f)
     throw Error(VXL_INVALID_ARG, "voxel_size must be finite and > 0");
-  if (!std::isfinite(1.0f / g->voxel_size))
+  if (!g->has_world_to_grid && !std::isfinite(1.0f / g->voxel_size))
     throw Error(VXL_INVALID_ARG, "voxel_size is too small: its reciprocal is not finite");
   for (int a = 0; a < 3; a++)
     if (!std::isfinite(g->origin[a])) throw Error(VXL_INVALID_ARG, "origin_world must be finite");
+  if (g->has_world_to_grid)
+    for (int i = 0; i < 16; i++)
+      if (!std::isfinite(g->world_to_grid[i]))
+        throw Error(VXL_INVALID_ARG, "world_to_grid must be finite");
 }
 
 // Whether any of n binary32 values is not finite (all 8 exponent bits set): branch-free,
@@ -321,6 +325,9 @@ vxl::VoxelizeParams make_params(const vxl_grid *g, float eps) {
     p.tx[a] = t;
   }
   p.eps = eps;
+  p.has_mat = g->has_world_to_grid ? 1u : 0u;
+  for (int c = 0; c < 4; c++)
+    for (int r = 0; r < 3; r++) p.m[3 * c + r] = p.has_mat ? g->world_to_grid[4 * c + r] : 0.0f;
   return p;
 }
 

@@ -92,6 +92,20 @@ __device__ __forceinline__ int32_t sat_i32(float v) {
 
 __device__ __forceinline__ void tri_grid(const float *__restrict__ tri, const VoxelizeParams &p,
                                          float g[9]) {
+  if (p.has_mat) {
+    // Mat4::transform_point3 of the override: x_axis * x, then + y_axis * y, + z_axis * z,
+    // + w_axis, per component (glam's column order; every product and sum rounded)
+#pragma unroll
+    for (int v = 0; v < 3; v++)
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        float s = p.m[a] * tri[3 * v];
+        s = s + p.m[3 + a] * tri[3 * v + 1];
+        s = s + p.m[6 + a] * tri[3 * v + 2];
+        g[3 * v + a] = s + p.m[9 + a];
+      }
+    return;
+  }
 #pragma unroll
   for (int v = 0; v < 3; v++)
 #pragma unroll

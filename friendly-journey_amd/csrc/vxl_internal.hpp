@@ -195,6 +195,8 @@ struct VoxelizeParams {
   float inv;       // 1 / voxel_size
   float tx[3];     // inv * (-origin)
   float eps;
+  uint32_t has_mat;  // world_to_grid override: g_a = ((m[0][a] px + m[1][a] py) + m[2][a] pz) + m[3][a]
+  float m[12];       // columns x, y, z, w of the override, rows 0..2 (m[3 * col + row])
 };
 
 // One mesh of a batch group, as the device sees it. Mesh g owns the leaf-grid words

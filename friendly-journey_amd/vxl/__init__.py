@@ -59,7 +59,8 @@ class VxlError(RuntimeError):
 
 
 class Grid(C.Structure):
-    _fields_ = [("n", C.c_uint32), ("origin", C.c_float * 3), ("voxel_size", C.c_float)]
+    _fields_ = [("n", C.c_uint32), ("origin", C.c_float * 3), ("voxel_size", C.c_float),
+                ("has_world_to_grid", C.c_uint32), ("world_to_grid", C.c_float * 16)]
 
 
 class TreeInfo(C.Structure):
@@ -262,12 +263,18 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(_vp)
 
 
-def make_grid(n: int, origin=(0.0, 0.0, 0.0), voxel_size: float = 1.0) -> Grid:
+def make_grid(n: int, origin=(0.0, 0.0, 0.0), voxel_size: float = 1.0, mat=None) -> Grid:
+    """vxl_grid; mat (optional): the world_to_grid override as 16 floats, column-major."""
     g = Grid()
     g.n = n
     for i in range(3):
         g.origin[i] = origin[i]
     g.voxel_size = voxel_size
+    if mat is not None:
+        m = np.asarray(mat, np.float32).reshape(16)
+        g.has_world_to_grid = 1
+        for i in range(16):
+            g.world_to_grid[i] = float(m[i])
     return g
 
 
@@ -350,11 +357,11 @@ class Context:
             pass
 
     # -- voxelize ------------------------------------------------------------------------
-    def voxelize_dense(self, tris, n, origin=(0.0, 0.0, 0.0), voxel_size=1.0, eps=1e-4):
+    def voxelize_dense(self, tris, n, origin=(0.0, 0.0, 0.0), voxel_size=1.0, eps=1e-4, mat=None):
         t = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
         words = np.zeros((n ** 3 + 31) // 32, np.uint32)
         pairs = C.c_uint64(0)
-        g = make_grid(n, origin, voxel_size)
+        g = make_grid(n, origin, voxel_size, mat)
         _check(load().vxl_voxelize_dense(self.handle, _ptr(t), t.shape[0], C.byref(g), eps,
                                          _ptr(words), C.byref(pairs)))
         return words, int(pairs.value)
@@ -374,11 +381,11 @@ class Context:
             _ptr(own) if owner else None, _ptr(col) if color else None, C.byref(pairs)))
         return words, own, col, int(pairs.value)
 
-    def voxelize_tree(self, tris, n, origin=(0.0, 0.0, 0.0), voxel_size=1.0, eps=1e-4):
+    def voxelize_tree(self, tris, n, origin=(0.0, 0.0, 0.0), voxel_size=1.0, eps=1e-4, mat=None):
         t = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
         out = _vp()
         pairs = C.c_uint64(0)
-        g = make_grid(n, origin, voxel_size)
+        g = make_grid(n, origin, voxel_size, mat)
         _check(load().vxl_voxelize_tree(self.handle, _ptr(t), t.shape[0], C.byref(g), eps,
                                         C.byref(out), C.byref(pairs)))
         tree = Tree(out.value, self)
@@ -561,9 +568,8 @@ class Context:
     @staticmethod
     def _grids(grids):
         arr = (Grid * max(len(grids), 1))()
-        for i, g in enumerate(grids):
-            n, origin, vs = g
-            arr[i] = make_grid(n, origin, vs)
+        for i, g in enumerate(grids):  # (n, origin, voxel_size[, world_to_grid])
+            arr[i] = make_grid(*g)
         return arr
 
     def _batch_trees(self, out, pairs, count):

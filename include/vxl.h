@@ -47,11 +47,19 @@ typedef struct vxl_ctx vxl_ctx;
 typedef struct vxl_tree vxl_tree;
 
 /* A cubic grid: n = 8*4^k voxels per axis, placed by origin + uniform voxel size
- * (VoxelGrid, voxelizer/src/core.rs:25-35). */
+ * (VoxelGrid, voxelizer/src/core.rs:25-35), or by an explicit world-to-grid affine
+ * (VoxelGrid::world_to_grid, core.rs:30-34,101-104): when has_world_to_grid is non-zero,
+ * world_to_grid is the glam Mat4 in its column-major storage (x_axis, y_axis, z_axis,
+ * w_axis, four floats each) and each vertex is mapped as Mat4::transform_point3 does, per
+ * component ((x_axis*p.x + y_axis*p.y) + z_axis*p.z) + w_axis (the fourth row is not used);
+ * the 16 values must be finite and voxel_size, origin still finite (VoxelGrid::validate,
+ * core.rs:116-132). Zero-initialise the struct to leave the override off. */
 typedef struct vxl_grid {
   uint32_t n;
   float origin[3];
   float voxel_size;
+  uint32_t has_world_to_grid;
+  float world_to_grid[16];
 } vxl_grid;
 
 /* School-B node, 12 bytes: voxel-core/src/node.rs:24-31 (GpuNode). */

@@ -56,6 +56,10 @@ def lib():
         L.vo_voxelize_surface.restype = C.c_uint64
         L.vo_voxelize_surface.argtypes = [_f32p, C.c_uint64, C.c_uint32, _f32p, C.c_float,
                                           C.c_float, _u32p, _u32p]
+        L.vo_world_to_grid_mat.argtypes = [_f32p, _f32p, _f32p]
+        L.vo_voxelize_surface_mat.restype = C.c_uint64
+        L.vo_voxelize_surface_mat.argtypes = [_f32p, C.c_uint64, C.c_uint32, _f32p, C.c_float,
+                                              _u32p, _u32p]
         L.vo_voxelize_surface_mt.restype = C.c_uint64
         L.vo_voxelize_surface_mt.argtypes = [_f32p, C.c_uint64, C.c_uint32, _f32p, C.c_float,
                                              C.c_float, _u32p, _u32p, C.c_int]
@@ -94,6 +98,7 @@ def lib():
                                           C.c_float]
         L.vo_tree_build.restype = C.c_void_p
         L.vo_tree_build.argtypes = [_u32p, C.c_uint32, C.c_int]
+        L.vo_tree_stage_seconds.argtypes = [C.c_void_p, _f64p]
         L.vo_tree_build_rows.restype = C.c_void_p
         L.vo_tree_build_rows.argtypes = [_u32p, C.c_uint32, C.c_int]
         L.vo_tree_from_school_b.restype = C.c_void_p
@@ -202,6 +207,27 @@ def voxelize_surface(tris, n, origin=(0.0, 0.0, 0.0), voxel_size=1.0, eps=1e-4, 
     pairs = lib().vo_voxelize_surface(_p(t, _f32p), t.shape[0], n, _p(o, _f32p), voxel_size, eps,
                                       _p(words, _u32p),
                                       _p(own, _u32p) if owner else None)
+    return words, own, int(pairs)
+
+
+def world_to_grid_mat(p, mat):
+    """Mat4::transform_point3 of p by the column-major 16-float matrix mat."""
+    pa = np.ascontiguousarray(p, np.float32)
+    m = np.ascontiguousarray(mat, np.float32).reshape(16)
+    out = np.zeros(3, np.float32)
+    lib().vo_world_to_grid_mat(_p(pa, _f32p), _p(m, _f32p), _p(out, _f32p))
+    return out
+
+
+def voxelize_surface_mat(tris, n, mat, eps=1e-4, owner=False):
+    """voxelize_surface_cpu on a grid with the world_to_grid override mat (column-major)."""
+    t = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    m = np.ascontiguousarray(mat, np.float32).reshape(16)
+    nv = n ** 3
+    words = np.zeros((nv + 31) // 32, np.uint32)
+    own = np.zeros(nv, np.uint32) if owner else None
+    pairs = lib().vo_voxelize_surface_mat(_p(t, _f32p), t.shape[0], n, _p(m, _f32p), eps,
+                                          _p(words, _u32p), _p(own, _u32p) if owner else None)
     return words, own, int(pairs)
 
 
@@ -397,6 +423,12 @@ class Tree:
         h = lib().vo_tree_from_school_b(n, _p(nd, _u32p), nd.shape[0], _p(lv, _u64p),
                                         lv.shape[0], _p(bd, _u32p))
         return cls(h, n)
+
+    def stage_seconds(self):
+        """(SparseTree::build s, from_sparse s) of the build that made this tree."""
+        out = np.zeros(2, np.float64)
+        lib().vo_tree_stage_seconds(self._h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return float(out[0]), float(out[1])
 
     def export(self):
         nodes = np.zeros((max(self.node_count, 0), 3), np.uint32)

@@ -21,6 +21,7 @@
 #include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 /* ------------------------------------------------------------------------- */
 /* Rust numeric-cast semantics                                               */
@@ -222,8 +223,28 @@ static void tri_box(const float g[9], uint32_t n, float eps, int32_t mn[3], int3
   }
 }
 
+/* core.rs:101-104 with the world_to_grid override, through glam's Mat4::transform_point3
+ * (column-major m: m[0..3] = x_axis, m[4..7] = y_axis, m[8..11] = z_axis, m[12..15] = w_axis):
+ * res = x_axis * p.x; res = y_axis * p.y + res; res = z_axis * p.z + res; res = w_axis + res,
+ * lane by lane, each product and sum rounded (glam 0.33 src/f32/{scalar,sse2}/mat4.rs). */
+void vo_world_to_grid_mat(const float p[3], const float m[16], float out[3]) {
+  for (int i = 0; i < 3; i++) {
+    float r = m[i] * p[0];
+    r = m[4 + i] * p[1] + r;
+    r = m[8 + i] * p[2] + r;
+    out[i] = m[12 + i] + r;
+  }
+}
+
+static const float *g_mat = NULL; /* set by vo_voxelize_surface_mat for its call (serial) */
+
 static void tri_to_grid(const float *tri, const float origin[3], float vs, float g[9]) {
-  for (int v = 0; v < 3; v++) vo_world_to_grid(tri + 3 * v, origin, vs, g + 3 * v);
+  for (int v = 0; v < 3; v++) {
+    if (g_mat)
+      vo_world_to_grid_mat(tri + 3 * v, g_mat, g + 3 * v);
+    else
+      vo_world_to_grid(tri + 3 * v, origin, vs, g + 3 * v);
+  }
 }
 
 uint64_t vo_count_candidates(const float *tris, uint64_t T, uint32_t n, const float origin[3],
@@ -365,6 +386,16 @@ uint64_t vo_voxelize_surface(const float *tris, uint64_t T, uint32_t n, const fl
   return pairs;
 }
 
+/* voxelize_surface_cpu on a grid whose world_to_grid override is m (serial). */
+uint64_t vo_voxelize_surface_mat(const float *tris, uint64_t T, uint32_t n, const float m[16],
+                                 float eps, uint32_t *words, uint32_t *owner) {
+  const float zero[3] = {0.0f, 0.0f, 0.0f};
+  g_mat = m;
+  uint64_t pairs = vo_voxelize_surface(tris, T, n, zero, 1.0f, eps, words, owner);
+  g_mat = NULL;
+  return pairs;
+}
+
 /* The same scan split over threads by z (test speed only, not a baseline): thread i takes
  * the z-planes whose 8-plane row index (z >> 3) is i modulo the thread count, for every
  * triangle. Occupancy is an OR and the owner a minimum, so neither depends on the order in
@@ -455,7 +486,14 @@ struct vo_tree {
   uint64_t level_count[16];
   vnode *nodes_b; /* School-B single buffer */
   uint64_t nb;
+  double stage_s[2]; /* wall seconds: SparseTree::build (scan, sort, levels), from_sparse */
 };
+
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
 
 static inline uint64_t vmask(vnode v) { return ((uint64_t)v.hi << 32) | v.lo; }
 
@@ -591,6 +629,8 @@ static void finish_tree(vo_tree *t) {
     nc = m;
   }
   if (codes != t->codes) free(codes);
+  const double t_levels = now_s();
+  t->stage_s[0] = t_levels - t->stage_s[0]; /* started by tree_build */
   /* school_b.rs:113-136 */
   uint64_t total = 0;
   for (int L = 2; L < 16; L++) total += t->level_count[L];
@@ -607,6 +647,12 @@ static void finish_tree(vo_tree *t) {
   t->bounds = (uint32_t *)malloc((t->nleaves ? t->nleaves : 1) * 4);
   for (uint64_t i = 0; i < t->nleaves; i++)
     t->bounds[i] = vo_leaf_occupied_bounds(t->leaves + 8 * i);
+  t->stage_s[1] = now_s() - t_levels;
+}
+
+void vo_tree_stage_seconds(const vo_tree *t, double out[2]) {
+  out[0] = t->stage_s[0];
+  out[1] = t->stage_s[1];
 }
 
 static vo_tree *tree_build(const uint32_t *words, uint32_t n, int threads, int rows);
@@ -625,6 +671,7 @@ static vo_tree *tree_build(const uint32_t *words, uint32_t n, int threads, int r
   uint32_t k = 0;
   if (vo_resolution(n, &k, NULL, NULL) != 0) return NULL;
   vo_tree *t = (vo_tree *)calloc(1, sizeof(vo_tree));
+  t->stage_s[0] = now_s();
   t->n = n;
   t->k = k;
   uint32_t bpa = n / 8;

@@ -40,6 +40,11 @@ void vo_world_to_grid(const float p[3], const float origin[3], float voxel_size,
  * of (triangle, candidate voxel) pairs the epsilon-padded AABB scan enumerates. */
 uint64_t vo_voxelize_surface(const float *tris, uint64_t T, uint32_t n, const float origin[3],
                              float voxel_size, float eps, uint32_t *words, uint32_t *owner);
+/* the world_to_grid override (core.rs:30-34,101-104): glam Mat4::transform_point3 of a point,
+ * column-major m; and voxelize_surface_cpu on a grid carrying that override (serial). */
+void vo_world_to_grid_mat(const float p[3], const float m[16], float out[3]);
+uint64_t vo_voxelize_surface_mat(const float *tris, uint64_t T, uint32_t n, const float m[16],
+                                 float eps, uint32_t *words, uint32_t *owner);
 /* vo_voxelize_surface with the scan split over threads by z-plane rows: the same words,
  * owners and pair count (occupancy is an OR, the owner a minimum); for fast checks only. */
 uint64_t vo_voxelize_surface_mt(const float *tris, uint64_t T, uint32_t n, const float origin[3],
@@ -73,6 +78,9 @@ uint64_t vo_count_candidates(const float *tris, uint64_t T, uint32_t n, const fl
 typedef struct vo_tree vo_tree;
 /* SparseTree::build over an x-major word field, then SchoolBBuffer::from_sparse. */
 vo_tree *vo_tree_build(const uint32_t *words, uint32_t n, int threads);
+/* Wall seconds of the last build's stages: out[0] SparseTree::build (brick scan, Morton sort,
+ * build_levels), out[1] SchoolBBuffer::from_sparse (parent-first emit + leaf bounds). */
+void vo_tree_stage_seconds(const vo_tree *t, double out[2]);
 /* The same tree, reading each 8-voxel brick row of the words as one byte (for checks). */
 vo_tree *vo_tree_build_rows(const uint32_t *words, uint32_t n, int threads);
 /* Wrap School-B arrays produced elsewhere (e.g. by the GPU) for traversal. */

@@ -176,3 +176,47 @@ def test_over_16m_triangles_take_the_device_wide_scan(ctx):
     tree = ctx.voxelize_tree(tris, n, (0, 0, 0), vs, 1e-4)
     assert tree.pairs == reps * pairs
     _tree_equal(tree, O.Tree.build(words, n))
+
+
+def _affine(n):
+    """A world-to-grid override that is not a scale + translation: a rotation of 30 degrees
+    about z then 20 about x, an anisotropic scale and a translation into [0, n)^3, as glam's
+    column-major Mat4 (16 f32)."""
+    cz, sz = np.cos(np.radians(30)), np.sin(np.radians(30))
+    cx, sx = np.cos(np.radians(20)), np.sin(np.radians(20))
+    rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
+    rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+    a = rx @ rz @ np.diag([0.9 * n, 0.75 * n, 0.8 * n])
+    t = np.array([n / 2, n / 2, n / 2]) - a @ np.array([0.5, 0.5, 0.5])
+    m = np.zeros((4, 4))
+    m[:3, :3], m[:3, 3], m[3, 3] = a, t, 1.0
+    return m.T.reshape(16).astype(np.float32)  # column-major
+
+
+def test_world_to_grid_override_bit_exact(ctx):
+    """VoxelGrid::world_to_grid (core.rs:30-34,101-104): vertices go through the override with
+    Mat4::transform_point3; dense words, pair count and tree equal the oracle's, single and
+    batched."""
+    n = 128
+    mat = _affine(n)
+    meshes = [W.uv_sphere(48, 24), W.soup(3000, seed=78, mu=np.log(3 / 128))]
+    for i, tris in enumerate(meshes):
+        _say(f"world_to_grid override mesh {i} T={len(tris)}")
+        want, _, pairs = O.voxelize_surface_mat(tris, n, mat, 1e-4)
+        got, gpairs = ctx.voxelize_dense(tris, n, (0, 0, 0), 1.0, 1e-4, mat=mat)
+        assert gpairs == pairs
+        assert np.array_equal(got, want), f"{int((got != want).sum())} words differ"
+        _tree_equal(ctx.voxelize_tree(tris, n, (0, 0, 0), 1.0, 1e-4, mat=mat),
+                    O.Tree.build(want, n))
+    trees = ctx.voxelize_batch(meshes, [(n, (0, 0, 0), 1.0, mat), (512, (0, 0, 0), 1 / 128),
+                                        ][:1] * 2)
+    for tris, tr in zip(meshes, trees):
+        want, _, _ = O.voxelize_surface_mat(tris, n, mat, 1e-4)
+        _tree_equal(tr, O.Tree.build(want, n))
+    bad = mat.copy()
+    bad[5] = np.inf
+    with pytest.raises(vxl.VxlError) as e:
+        ctx.voxelize_tree(meshes[0], n, (0, 0, 0), 1.0, 1e-4, mat=bad)
+    assert e.value.name == "VXL_INVALID_ARG"
+    # with an override, a voxel size whose reciprocal overflows is not an error (core.rs:124)
+    ctx.voxelize_tree(meshes[0], n, (0, 0, 0), 1e-39, 1e-4, mat=mat).free()

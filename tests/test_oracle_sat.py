@@ -179,3 +179,45 @@ def test_threaded_scan_equals_serial(threads):
         assert p1 == p2
         assert np.array_equal(w1, w2)
         assert np.array_equal(o1, o2)
+
+
+def _glam_default_matrix(origin, voxel_size):
+    """Mat4::from_scale(1/vs) * Mat4::from_translation(-o) as glam evaluates the product
+    (column-major): the diagonal is inv, the translation column inv*(-o) (each entry one
+    product plus exact zero products), the last entry 1."""
+    inv = np.float32(1.0) / np.float32(voxel_size)
+    m = np.zeros(16, np.float32)
+    m[0] = m[5] = m[10] = inv
+    for i in range(3):
+        m[12 + i] = inv * np.float32(-np.float32(origin[i]))
+    m[15] = 1.0
+    return m
+
+
+def test_world_to_grid_override_of_the_default_matrix_equals_the_default_grid():
+    """A grid carrying, as its override, the matrix world_to_grid_matrix would build from its
+    origin and voxel size voxelizes exactly as the grid without the override (core.rs:101-107):
+    the zero entries contribute exact zeros, leaving inv*p + inv*(-o)."""
+    rng = np.random.default_rng(31)
+    from vxl import workloads as W
+    for origin, vs, n in (((0.0, 0.0, 0.0), 1 / 64, 64), ((-0.25, 0.125, 0.3), 1 / 24.0, 32),
+                          ((0.37, -0.61, 0.05), 0.0371, 32)):
+        tris = (rng.uniform(-0.5, 1.5, (300, 9)) * n * vs).astype(np.float32)
+        tris = np.concatenate([tris, W.uv_sphere(24, 12) * np.float32(n * vs * 0.9)])
+        w1, _, p1 = O.voxelize_surface(tris, n, origin, vs, 1e-4)
+        w2, _, p2 = O.voxelize_surface_mat(tris, n, _glam_default_matrix(origin, vs), 1e-4)
+        assert p1 == p2 and np.array_equal(w1, w2)
+
+
+def test_world_to_grid_override_association_kat():
+    """transform_point3 adds x_axis*p.x, then y_axis*p.y, then z_axis*p.z, then w_axis (glam
+    0.33 Mat4). With x*px = 1e8, y*py = -1e8, z*pz = 1 and w = 0.5 that order gives
+    ((1e8 - 1e8) + 1) + 0.5 = 1.5 exactly, while x + (y + z) would give 0.5 (-1e8 + 1 rounds to
+    -1e8 in binary32) and w first 1.0."""
+    m = np.zeros(16, np.float32)
+    m[0:3] = 1e8
+    m[4:7] = -1e8
+    m[8:11] = 1.0
+    m[12:15] = 0.5
+    m[15] = 1.0
+    assert O.world_to_grid_mat([1.0, 1.0, 1.0], m).tolist() == [1.5, 1.5, 1.5]
