@@ -544,9 +544,10 @@ vxl_status vxl_ctx_create(int device, vxl_ctx **out) {
   try {
     VXL_HIP(hipSetDevice(device));
     VXL_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-    VXL_HIP(hipHostMalloc(&ctx->host_pinned, 4096, hipHostMallocDefault));
-    // pinned memory can come back from an earlier owner (another context) with its contents:
-    // the tree build's size read-back waits for a sequence number here, so start from zero
+    // fine-grained (coherent) and mapped: kernels write sizes and flags here that the host
+    // polls while they run
+    VXL_HIP(hipHostMalloc(&ctx->host_pinned, 4096, hipHostMallocMapped | hipHostMallocCoherent));
+    // pinned memory can come back from an earlier owner (another context) with its contents
     std::memset(ctx->host_pinned, 0, 4096);
     VXL_HIP(hipHostGetDevicePointer(&ctx->host_pinned_dev, ctx->host_pinned, 0));
     hipDeviceProp_t prop;

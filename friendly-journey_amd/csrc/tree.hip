@@ -401,12 +401,6 @@ __device__ __forceinline__ void small_levels(const LevelPtrs &lp, int first, int
   }
 }
 
-__global__ __launch_bounds__(kSmallThreads) void k_small_levels(
-    LevelPtrs lp, int first, int top, const unsigned long long *__restrict__ bin_first,
-    uint32_t *__restrict__ tot) {
-  small_levels(lp, first, top, bin_first, tot);
-}
-
 // One workgroup per mesh of a batch group; its upper bitmap words start at b1_begin / 64.
 __global__ __launch_bounds__(kSmallThreads) void k_small_levels_batch(
     const BatchMesh *__restrict__ mesh, const LevelPtrs *__restrict__ lps,
@@ -481,73 +475,6 @@ __device__ __forceinline__ void upper_place(const LevelPtrs &lp, int k,
     }
     __syncthreads();
   }
-}
-
-// The positions upper_place computes (pos, bs, desc of levels 3..k+1), for one tree with
-// 2 <= k <= 4 (at most kSmallWords level-3 nodes), with fewer dependent memory round trips:
-// the level sizes are read once, each node's subtree size stays in a register between the
-// scan and the store, and the root's children (level k, in code order) take positions
-// 1 + c without a parent lookup. Node entries are written later (k_place_tree).
-__device__ __forceinline__ void upper_positions(const LevelPtrs &lp, int k,
-                                                const uint32_t *__restrict__ tot) {
-  __shared__ uint32_t sh[kSmallThreads / 64];
-  constexpr int kIpt = int(kSmallWords / kSmallThreads);
-  const uint32_t tid = threadIdx.x;
-  const uint32_t n3 = tot[2 * 3 + 1], n4 = tot[2 * 4 + 1], n5 = tot[2 * 5 + 1];
-  for (int L = 3; L <= k + 1; L++) {
-    const uint32_t nl = L == 3 ? n3 : (L == 4 ? n4 : n5);
-    uint32_t d[kIpt];
-    uint32_t local = 0;
-#pragma unroll
-    for (int i = 0; i < kIpt; i++) {
-      const uint32_t c = tid * kIpt + i;
-      d[i] = 0;
-      if (c < nl) {
-        const uint32_t m = __popcll(lp.a_mask[L][c]);
-        const uint32_t b = lp.a_base[L][c];
-        d[i] = m + (L > 3 ? lp.desc[L - 1][b + m] - lp.desc[L - 1][b] : 0u);
-      }
-      local += d[i];
-    }
-    uint32_t excl;
-    const uint32_t total = block_scan_u32(local, &excl, sh);
-#pragma unroll
-    for (int i = 0; i < kIpt; i++) {
-      const uint32_t c = tid * kIpt + i;
-      if (c < nl) lp.desc[L][c] = excl;  // E_L: exclusive scan of desc
-      excl += d[i];
-    }
-    if (tid == 0) lp.desc[L][nl] = total;
-    __syncthreads();
-  }
-  // the root (level k + 1) at position 0 with its block at 1; its children are all of
-  // level k's nodes in order, so child c sits at 1 + c and its block follows the root's
-  // block and the subtrees of the children before it (E_k[0] = 0)
-  if (tid == 0) {
-    lp.pos[k + 1][0] = 0;
-    lp.bs[k + 1][0] = 1;
-  }
-  if (k >= 3) {
-    const uint32_t nl = k == 3 ? n3 : n4;  // level-k nodes
-    const uint32_t mp = __popcll(lp.a_mask[k + 1][0]);
-    for (uint32_t c = tid; c < nl; c += kSmallThreads) {
-      lp.pos[k][c] = 1 + c;
-      lp.bs[k][c] = 1 + mp + lp.desc[k][c];
-    }
-  }
-  __syncthreads();
-  if (k == 4) {  // level 3 under its level-4 parents
-    for (uint32_t c = tid; c < n3; c += kSmallThreads) {
-      const uint32_t q = lp.a_code[3][c];
-      const uint32_t p = lp.ni[4][q >> 6];
-      const uint32_t pb = lp.a_base[4][p];
-      const uint32_t mp = __popcll(lp.a_mask[4][p]);
-      const uint32_t pbs = lp.bs[4][p];
-      lp.pos[3][c] = pbs + (c - pb);
-      lp.bs[3][c] = pbs + mp + (lp.desc[3][c] - lp.desc[3][pb]);
-    }
-  }
-  __syncthreads();
 }
 
 // ---- batch groups -------------------------------------------------------------------
@@ -801,20 +728,27 @@ static vxl_tree *alloc_tree(vxl_ctx *ctx, uint32_t n, uint32_t k, uint64_t nodes
 }
 
 // Sizes a build needs before the tree is allocated, written by the device straight into
-// mapped pinned memory (no copy call): the packed level-2 total, the per-level totals of
-// levels >= 3, and the SAT pair count.
+// mapped fine-grained host memory (no copy call): leaves, level-2 nodes, the SAT pair count
+// (two 32-bit halves) and the node counts of levels 3..5. Each is one 64-bit word, the value
+// in the low 48 bits and the build's tag (1..65535) in the high 16. The host zeroes the words
+// before it queues the size kernel; the device stores each word with one relaxed
+// system-scope atomic store, with no release fence (a system-scope fence writes the L2 back
+// first, about 4 us here); the host takes the set once every word carries the build's tag.
+// A word of an earlier build, or a set only partly written, is never taken.
+enum : int { kSzLeaves, kSzLevel2, kSzPairsLo, kSzPairsHi, kSzLevel3, kSzWords = kSzLevel3 + 3 };
 struct BuildSizes {
-  uint64_t pn2_total;
-  uint64_t pairs;
-  uint32_t tot[32];
-  uint32_t seq;  // written last: the call's sequence number once the fields above are in
-  uint32_t pad;
+  uint64_t w[kSzWords];
 };
+constexpr uint64_t kSzValue = (uint64_t(1) << 48) - 1;
+
+__device__ __forceinline__ void put_size(BuildSizes *out, int i, uint64_t v, uint64_t tag) {
+  __hip_atomic_store(&out->w[i], v | (tag << 48), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
 // ---- single-tree level 2 without a device-wide scan ------------------------------------
 // k_level2_sums leaves each block's sum of the packed (popcount | nonzero << 32) values of
 // its B1 words (a block's popcounts sum to at most 256 * 64, so the halves never carry);
-// k_small_levels_sizes scans those sums in its one workgroup before the upper levels and
+// k_upper_levels_sizes scans those sums in its one workgroup before the upper levels and
 // then writes the sizes the host reads; k_emit_a2_local rebuilds each word's packed offset
 // from a block-local scan plus the block's offset.
 __device__ __forceinline__ uint64_t packed_word(unsigned long long w) {
@@ -859,48 +793,186 @@ __global__ __launch_bounds__(kTB) void k_level2_sums(const unsigned long long *_
   if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
 }
 
-// One workgroup: the level-2 block sums (at most kSmallThreads of them) to their exclusive
-// prefix sums in place, the upper levels 3..top with their School-B positions (k = top - 1
-// >= 2), and the sizes (packed level-2 total, SAT pairs, per-level totals) into the
-// host-mapped BuildSizes. The positions need no output array, so they are done here, before
-// the host reads the sizes and allocates the tree.
-__global__ __launch_bounds__(kSmallThreads) void k_small_levels_sizes(
-    LevelPtrs lp, int top, const unsigned long long *__restrict__ b2, uint32_t *__restrict__ tot,
+// Exclusive scan of one u64 per lane over one wave; returns the wave's total.
+__device__ __forceinline__ uint64_t wave_scan_u64(uint64_t v, uint64_t *excl) {
+  const int lane = threadIdx.x & 63;
+  uint64_t inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint64_t o = uint64_t(__shfl_up((long long)inc, d, 64));
+    if (lane >= d) inc += o;
+  }
+  *excl = inc - v;
+  return uint64_t(__shfl((long long)inc, 63, 64));
+}
+
+// One workgroup, for one tree with k = top - 1 <= 4: the level-2 block sums (at most
+// kSmallThreads of them) to their exclusive prefix sums in place; level 3 (at most
+// kSmallWords input words, kW per thread) with one packed scan; levels 4 and 5 (at most 64
+// input words) in wave 0; the sizes into the host-mapped BuildSizes; then the School-B
+// positions of levels 3..top. Everything the positions read stays in LDS (level 3's child
+// bases are its exclusive subtree-size scan E_3, since a level-3 node's subtree size is its
+// child count), so the chain has no dependent global loads after the first; the positions
+// are computed after the sizes are out, while the host allocates the tree.
+__global__ __launch_bounds__(kSmallThreads) void k_upper_levels_sizes(
+    LevelPtrs lp, int top, const unsigned long long *__restrict__ b2,
     uint64_t *__restrict__ block_sums, uint32_t nb2, const uint64_t *__restrict__ pairs,
-    BuildSizes *__restrict__ out, uint32_t seq) {
+    BuildSizes *__restrict__ out, uint32_t tag) {
+  constexpr int kW = int(kSmallWords / kSmallThreads);
+  static_assert(64 % kW == 0, "a thread's level-3 words share one level-4 input word");
+  __shared__ uint32_t s_base3[kSmallWords + 1];  // a_base of level 3 (= E_3); [n3] = n2
+  __shared__ uint16_t s_code3[kSmallWords];
+  __shared__ unsigned long long s_w4[64], s_mask4[64];
+  __shared__ uint32_t s_base4[64], s_ni4[64], s_bs4[64], s_n[2];
+  __shared__ unsigned long long s_mask5;
+  __shared__ uint64_t s_pairs;
   const uint32_t tid = threadIdx.x;
+  const uint64_t win3 = top >= 3 ? lp.win[3] : 0;
+  // every global read of the kernel, issued together
+  if (tid == 0) s_pairs = pairs ? *pairs : 0;
   const uint64_t v = tid < nb2 ? block_sums[tid] : 0;
+  unsigned long long w[kW];
+#pragma unroll
+  for (int i = 0; i < kW; i++) {
+    const uint64_t j = uint64_t(tid) * kW + i;
+    w[i] = j < win3 ? b2[j] : 0ull;
+  }
+  if (tid < 64) s_w4[tid] = 0ull;
   uint64_t excl;
   const uint64_t pn2_total = block_scan_u64<kSmallThreads>(v, &excl);
   if (tid < nb2) block_sums[tid] = excl;
-  if (top >= 3) {
-    small_levels(lp, 3, top, b2, tot);
-    __syncthreads();  // tot[] of every level is written
-    upper_positions(lp, top - 1, tot);
+  uint64_t packed = 0;
+#pragma unroll
+  for (int i = 0; i < kW; i++) packed += packed_word(w[i]);
+  uint64_t e3;
+  const uint64_t t3 = block_scan_u64<kSmallThreads>(packed, &e3);  // zero when top < 3
+  const uint32_t n2 = uint32_t(t3), n3 = uint32_t(t3 >> 32);
+  // level 3: per input word its node index (read by the level-2 placement), per node its
+  // mask, child base and code; the level-4 input bits
+  uint32_t epop = uint32_t(e3), enz = uint32_t(e3 >> 32);
+  unsigned long long bits = 0;
+#pragma unroll
+  for (int i = 0; i < kW; i++) {
+    const uint32_t j = tid * kW + i;
+    if (j < win3) lp.ni[3][j] = enz;
+    if (w[i]) {
+      lp.a_mask[3][enz] = w[i];
+      lp.a_base[3][enz] = epop;
+      s_base3[enz] = epop;
+      s_code3[enz] = uint16_t(j);
+      bits |= 1ull << (j & 63);
+    }
+    epop += __popcll(w[i]);
+    enz += w[i] != 0ull;
   }
-  // one thread writes every size (tot[] holds its own earlier writes), makes them visible
-  // to the host (system-scope fence), then writes the sequence number the host waits for
+  if (bits) atomicOr(&s_w4[(tid * kW) >> 6], bits);
+  if (tid == 0) s_base3[n3] = n2;
+  __syncthreads();
+  // levels 4 and 5 in wave 0: lane j holds level-4 input word j
+  if (top >= 4 && tid < 64) {
+    const uint64_t win4 = lp.win[4];
+    const unsigned long long x = tid < win4 ? s_w4[tid] : 0ull;
+    uint64_t e4;
+    const uint64_t t4 = wave_scan_u64(packed_word(x), &e4);
+    const uint32_t i4 = uint32_t(e4 >> 32);
+    s_ni4[tid] = i4;
+    if (x) {
+      s_mask4[i4] = x;
+      s_base4[i4] = uint32_t(e4);
+      lp.a_mask[4][i4] = x;
+      lp.a_base[4][i4] = uint32_t(e4);
+      lp.a_code[4][i4] = tid;
+    }
+    const unsigned long long x5 = __ballot(x != 0ull);  // the one level-5 input word
+    if (tid == 0) {
+      s_n[0] = uint32_t(t4 >> 32);
+      s_n[1] = (top >= 5 && x5) ? 1u : 0u;
+      s_mask5 = x5;
+      if (top >= 5 && x5) {
+        lp.a_mask[5][0] = x5;
+        lp.a_base[5][0] = 0;
+        lp.a_code[5][0] = 0;
+      }
+    }
+  }
+  __syncthreads();
+  const uint32_t n4 = top >= 4 ? s_n[0] : 0u, n5 = top >= 5 ? s_n[1] : 0u;
   if (tid == 0) {
-    out->pn2_total = pn2_total;
-    out->pairs = pairs ? *pairs : 0;
-    for (int i = 0; i < 32; i++) out->tot[i] = (top >= 3 && i >= 6 && i < 2 * top + 2) ? tot[i] : 0u;
-    __threadfence_system();
-    *reinterpret_cast<volatile uint32_t *>(&out->seq) = seq;
+    put_size(out, kSzLeaves, pn2_total & 0xffffffffull, tag);
+    put_size(out, kSzLevel2, pn2_total >> 32, tag);
+    put_size(out, kSzPairsLo, s_pairs & 0xffffffffull, tag);
+    put_size(out, kSzPairsHi, s_pairs >> 32, tag);
+    put_size(out, kSzLevel3, n3, tag);
+    put_size(out, kSzLevel3 + 1, n4, tag);
+    put_size(out, kSzLevel3 + 2, n5, tag);
+  }
+  if (top < 3) return;
+  // positions. k = 4: the root (level 5) at 0 with its block at 1; level-4 node c, the
+  // root's child c, at 1 + c with its block after the root's block and the subtrees of the
+  // children before it; subtree size of a level-4 node = its child count + the level-3
+  // subtree sizes under it, E_3[b + m] - E_3[b]
+  if (top == 5 && tid < 64) {
+    uint32_t d = 0;
+    if (tid < n4) {
+      const uint32_t m = __popcll(s_mask4[tid]), b = s_base4[tid];
+      d = m + s_base3[b + m] - s_base3[b];
+    }
+    uint64_t e4;
+    wave_scan_u64(d, &e4);
+    if (tid < n4) {
+      const uint32_t bs = 1u + uint32_t(__popcll(s_mask5)) + uint32_t(e4);
+      s_bs4[tid] = bs;
+      lp.pos[4][tid] = 1u + tid;
+      lp.bs[4][tid] = bs;
+    }
+    if (tid == 0) {
+      lp.pos[5][0] = 0;
+      lp.bs[5][0] = 1;
+    }
+  }
+  if (top == 4 && tid == 0) {
+    lp.pos[4][0] = 0;
+    lp.bs[4][0] = 1;
+  }
+  __syncthreads();
+  // level 3: under its level-4 parent (k = 4), as the root's children (k = 3), or the root
+  const uint32_t mp4 = top == 4 ? uint32_t(__popcll(s_w4[0])) : 0u;  // root mask of k = 3
+  for (uint32_t c = tid; c < n3; c += kSmallThreads) {
+    uint32_t pos, bs;
+    if (top == 5) {
+      const uint32_t p = s_ni4[s_code3[c] >> 6];
+      const uint32_t pb = s_base4[p], pbs = s_bs4[p];
+      pos = pbs + (c - pb);
+      bs = pbs + uint32_t(__popcll(s_mask4[p])) + (s_base3[c] - s_base3[pb]);
+    } else if (top == 4) {
+      pos = 1u + c;
+      bs = 1u + mp4 + s_base3[c];
+    } else {
+      pos = 0;
+      bs = 1;
+    }
+    lp.pos[3][c] = pos;
+    lp.bs[3][c] = bs;
   }
 }
 
-// Waits until the size kernel has written `seq` to the mapped BuildSizes, so the host goes
-// on while the kernel queued after it runs. Every 256 polls the event recorded after those
-// kernels is checked too: a failed launch ends the wait with its error, and finished
-// kernels whose number is still missing are an error rather than an endless wait.
-static void wait_sizes(const BuildSizes *hs, uint32_t seq, hipEvent_t ev) {
-  const volatile uint32_t *flag = &hs->seq;
+// Waits until every size word carries `tag`, so the host goes on while the size kernel and
+// the kernel queued after it run. Every 256 polls the event recorded after those kernels is
+// checked too: a failed launch ends the wait with its error, and finished kernels whose
+// sizes are still missing are an error rather than an endless wait.
+static void wait_sizes(const BuildSizes *hs, uint64_t tag, hipEvent_t ev, uint64_t *v) {
+  const volatile uint64_t *w = hs->w;
+  auto all_in = [&] {
+    for (int j = 0; j < kSzWords; j++)
+      if ((w[j] >> 48) != tag) return false;
+    return true;
+  };
   for (uint32_t i = 1;; i++) {
-    if (*flag == seq) break;
+    if (all_in()) break;
     if ((i & 255u) == 0) {
       const hipError_t e = hipEventQuery(ev);
       if (e == hipSuccess) {
-        if (*flag == seq) break;
+        if (all_in()) break;
         throw Error(VXL_DEVICE_ERROR, "tree build: the size kernel finished without its sizes");
       }
       if (e != hipErrorNotReady) {
@@ -909,7 +981,7 @@ static void wait_sizes(const BuildSizes *hs, uint32_t seq, hipEvent_t ev) {
       }
     }
   }
-  std::atomic_thread_fence(std::memory_order_acquire);
+  for (int j = 0; j < kSzWords; j++) v[j] = w[j] & kSzValue;
 }
 
 __global__ __launch_bounds__(kTB) void k_emit_a2_local(
@@ -945,7 +1017,6 @@ static vxl_tree *build_tree_fast(vxl_ctx *ctx, uint32_t n, uint32_t k, const uin
   for (int L = 2; L <= top; L++) scratch += 10 * (((lp.win[L] + 1) * 8 + 255) / 256) * 256;
   ctx->levelscratch.reserve(scratch);
   char *p = static_cast<char *>(ctx->levelscratch.ptr);
-  uint32_t *tot = carve<uint32_t>(p, 64);
   const uint32_t nb2 = uint32_t(div_up(w1, kTB));  // level-2 blocks
   if (nb2 > uint32_t(kSmallThreads))  // n <= 2048: (256^3 / 64) / 256 = 1024 blocks at most
     throw Error(VXL_INVALID_ARG, "tree build: level-2 block count exceeds one workgroup");
@@ -967,13 +1038,13 @@ static vxl_tree *build_tree_fast(vxl_ctx *ctx, uint32_t n, uint32_t k, const uin
   hipLaunchKernelGGL(k_level2_sums, dim3(nb2), dim3(kTB), 0, s, b1, w1, l2sums, b2);
   // 2. the level-2 block offsets, the upper levels and the sizes, in one workgroup
   BuildSizes *hs = static_cast<BuildSizes *>(ctx->host_pinned);
-  const uint32_t seq = ++ctx->size_seq;
-  // the number this build waits for is not in the mapped area before its size kernel writes
-  // it: earlier builds of this context wrote smaller numbers, and the area starts zeroed
-  *reinterpret_cast<volatile uint32_t *>(&hs->seq) = 0u;
-  hipLaunchKernelGGL(k_small_levels_sizes, dim3(1), dim3(kSmallThreads), 0, s, lp, top, b2, tot,
+  const uint32_t tag = ctx->size_seq++ % 65535u + 1u;
+  // no size word holds this build's tag before its size kernel writes it (other read-backs
+  // share the area, so it is cleared here, before the kernel is queued)
+  for (int j = 0; j < kSzWords; j++) reinterpret_cast<volatile uint64_t *>(hs->w)[j] = 0ull;
+  hipLaunchKernelGGL(k_upper_levels_sizes, dim3(1), dim3(kSmallThreads), 0, s, lp, top, b2,
                      l2sums, nb2, pairs_dev, static_cast<BuildSizes *>(ctx->host_pinned_dev),
-                     seq);
+                     tag);
   // 3. level-2 School-A arrays (no output array needed: queued before the host waits)
   hipLaunchKernelGGL(k_emit_a2_local, dim3(nb2), dim3(kTB), 0, s, b1, w1, l2sums, lp.a_mask[2],
                      lp.a_base[2], lp.a_code[2]);
@@ -981,14 +1052,15 @@ static vxl_tree *build_tree_fast(vxl_ctx *ctx, uint32_t n, uint32_t k, const uin
   if (!ctx->ev_wait) VXL_HIP(hipEventCreateWithFlags(&ctx->ev_wait, hipEventDisableTiming));
   VXL_HIP(hipEventRecord(ctx->ev_wait, s));  // after both kernels: no marker between them
   // 4. the sizes (the one host round trip of the call)
-  wait_sizes(hs, seq, ctx->ev_wait);
-  const uint64_t leaves = uint32_t(hs->pn2_total);
-  const uint64_t n2 = hs->pn2_total >> 32;
-  if (pairs) *pairs = hs->pairs;
+  uint64_t sz[kSzWords];
+  wait_sizes(hs, tag, ctx->ev_wait, sz);
+  const uint64_t leaves = sz[kSzLeaves];
+  const uint64_t n2 = sz[kSzLevel2];
+  if (pairs) *pairs = sz[kSzPairsLo] | (sz[kSzPairsHi] << 32);
   uint64_t nodes = 0;
   uint64_t level_nodes[16] = {0};
   level_nodes[2] = n2;
-  for (int L = 3; L <= top; L++) level_nodes[L] = hs->tot[2 * L + 1];
+  for (int L = 3; L <= top; L++) level_nodes[L] = sz[kSzLevel3 + L - 3];
   // the kernels that follow index per-level scratch and the new tree by these sizes: refuse
   // any that the grid cannot hold rather than launch them out of bounds
   const uint64_t bricks = uint64_t(n / 8) * (n / 8) * (n / 8);

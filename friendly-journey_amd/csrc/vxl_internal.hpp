@@ -132,7 +132,7 @@ struct vxl_ctx {
   hipEvent_t staging_free = nullptr;  // recorded after the last upload from batch_host
   hipEvent_t trav_done = nullptr;     // recorded after the last batched traversal kernel
   hipEvent_t ev_wait = nullptr;       // wait_stream's completion marker
-  uint32_t size_seq = 0;              // sequence number of the last single-tree build's sizes
+  uint32_t size_seq = 0;              // single-tree builds so far (their size read-back tag)
   // host triangle uploads: two pinned chunk buffers, each free once its event has completed
   void *up_host[2] = {nullptr, nullptr};
   hipEvent_t up_ev[2] = {nullptr, nullptr};
