@@ -21,7 +21,7 @@
 //  * The leaf gather copies each occupied brick's 8 words out of the leaf grid into the
 //    Morton-sorted leaf array, computes the packed LeafBounds (leaf.rs:185-213), and
 //    writes zeros back, so the grid and B1 are all-zero again when the build ends.
-#include <atomic>
+#include <algorithm>
 
 #include "vxl_internal.hpp"
 
@@ -102,13 +102,13 @@ __device__ __forceinline__ void word_extent(unsigned long long w, int axis, uint
 
 // One wave per level-2 node: 8 children at a time, 8 lanes (one per u64 word) per child.
 // In a batch group (n2_beg non-null) the stored brick code is relative to the mesh's region.
-__global__ __launch_bounds__(kTB) void k_leaf_gather(
-    uint64_t n2, const unsigned long long *__restrict__ a_mask, const uint32_t *__restrict__ a_base,
-    const uint32_t *__restrict__ a_code, unsigned long long *__restrict__ leafgrid,
-    unsigned long long *__restrict__ brickbits, unsigned long long *__restrict__ leaves,
-    uint32_t *__restrict__ bounds, unsigned long long *__restrict__ codes,
-    const uint64_t *__restrict__ n2_beg, uint32_t G, const BatchMesh *__restrict__ mesh) {
-  const uint64_t node = (uint64_t(blockIdx.x) * kTB + threadIdx.x) >> 6;
+__device__ __forceinline__ void gather_node(
+    uint64_t node, uint64_t n2, const unsigned long long *__restrict__ a_mask,
+    const uint32_t *__restrict__ a_base, const uint32_t *__restrict__ a_code,
+    unsigned long long *__restrict__ leafgrid, unsigned long long *__restrict__ brickbits,
+    unsigned long long *__restrict__ leaves, uint32_t *__restrict__ bounds,
+    unsigned long long *__restrict__ codes, const uint64_t *__restrict__ n2_beg, uint32_t G,
+    const BatchMesh *__restrict__ mesh) {
   if (node >= n2) return;
   const int lane = threadIdx.x & 63;
   const unsigned long long mask = a_mask[node];
@@ -167,6 +167,16 @@ __global__ __launch_bounds__(kTB) void k_leaf_gather(
   if (lane == 0) brickbits[j] = 0ull;  // B1 back to all-zero
 }
 
+__global__ __launch_bounds__(kTB) void k_leaf_gather(
+    uint64_t n2, const unsigned long long *__restrict__ a_mask, const uint32_t *__restrict__ a_base,
+    const uint32_t *__restrict__ a_code, unsigned long long *__restrict__ leafgrid,
+    unsigned long long *__restrict__ brickbits, unsigned long long *__restrict__ leaves,
+    uint32_t *__restrict__ bounds, unsigned long long *__restrict__ codes,
+    const uint64_t *__restrict__ n2_beg, uint32_t G, const BatchMesh *__restrict__ mesh) {
+  gather_node((uint64_t(blockIdx.x) * kTB + threadIdx.x) >> 6, n2, a_mask, a_base, a_code,
+              leafgrid, brickbits, leaves, bounds, codes, n2_beg, G, mesh);
+}
+
 // desc_L[i] = m_i + sum of desc over its children (level L-1), from the exclusive scan E.
 __global__ void k_desc(uint64_t nl, const unsigned long long *__restrict__ a_mask,
                        const uint32_t *__restrict__ a_base, const uint32_t *__restrict__ e_child,
@@ -217,37 +227,60 @@ __global__ void k_place(uint64_t nl, int is_level2, const unsigned long long *__
   nodes_b[3ull * mypos + 2] = is_level2 ? a_base[c] : mybs;
 }
 
-// Node entries of one tree once its node array exists: threads [0, n2) place the level-2
-// nodes under their level-3 parents (k_place's rule, child_base = the School-A leaf base);
-// the rest write the entries of levels 3..k+1, whose positions upper_place already
-// computed. Upper node u belongs to level L when ubeg[L] <= u < ubeg[L + 1].
-struct UpperNodes {
-  uint32_t ubeg[kLevels + 1];
-};
-
-__global__ void k_place_tree(uint64_t n2, uint64_t total, LevelPtrs lp, int k, UpperNodes un,
-                             uint32_t *__restrict__ nodes_b) {
-  const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (c >= total) return;
+// School-B entry c of one tree once its node array exists: c < n2 places level-2 node c
+// under its level-3 parent (k_place's rule, child_base = the School-A leaf base); the rest
+// are the nodes of levels 3..k+1 (n3, n4, n5 of them, in that order), whose positions the
+// upper-levels kernel computed. Entries past the node array's capacity are never written.
+__device__ __forceinline__ void place_entry(uint64_t c, uint64_t n2, const LevelPtrs &lp,
+                                            const uint32_t *__restrict__ upper, uint64_t cap,
+                                            uint32_t *__restrict__ nodes_b) {
+  const uint32_t n3 = upper[0], n4 = upper[1], n5 = upper[2];
+  if (c >= n2 + n3 + n4 + n5) return;
+  uint32_t mypos, child_base;
+  unsigned long long m;
   if (c < n2) {
     const uint32_t q = lp.a_code[2][c];
     const uint32_t p = lp.ni[3][q >> 6];
-    const uint32_t mypos = lp.bs[3][p] + (uint32_t(c) - lp.a_base[3][p]);
-    const unsigned long long m = lp.a_mask[2][c];
-    nodes_b[3ull * mypos + 0] = uint32_t(m);
-    nodes_b[3ull * mypos + 1] = uint32_t(m >> 32);
-    nodes_b[3ull * mypos + 2] = lp.a_base[2][c];
-    return;
+    mypos = lp.bs[3][p] + (uint32_t(c) - lp.a_base[3][p]);
+    m = lp.a_mask[2][c];
+    child_base = lp.a_base[2][c];
+  } else {
+    uint32_t i = uint32_t(c - n2);
+    int L = 3;
+    if (i >= n3) {
+      i -= n3;
+      L = 4;
+      if (i >= n4) {
+        i -= n4;
+        L = 5;
+      }
+    }
+    mypos = lp.pos[L][i];
+    m = lp.a_mask[L][i];
+    child_base = lp.bs[L][i];
   }
-  const uint32_t u = uint32_t(c - n2);
-  int L = 3;
-  while (L < k + 1 && u >= un.ubeg[L + 1]) L++;
-  const uint32_t i = u - un.ubeg[L];
-  const uint32_t mypos = lp.pos[L][i];
-  const unsigned long long m = lp.a_mask[L][i];
+  if (mypos >= cap) return;
   nodes_b[3ull * mypos + 0] = uint32_t(m);
   nodes_b[3ull * mypos + 1] = uint32_t(m >> 32);
-  nodes_b[3ull * mypos + 2] = lp.bs[L][i];
+  nodes_b[3ull * mypos + 2] = child_base;
+}
+
+// The leaf gather (blocks [0, gather_blocks), one wave per level-2 node) and the School-B
+// node entries (the blocks after them, one thread per entry) of one tree with k >= 2, in
+// one launch: both read only what the kernels before them wrote.
+__global__ __launch_bounds__(kTB) void k_gather_place(
+    uint64_t n2, uint32_t gather_blocks, LevelPtrs lp, const uint32_t *__restrict__ upper,
+    uint64_t cap, unsigned long long *__restrict__ leafgrid,
+    unsigned long long *__restrict__ brickbits, unsigned long long *__restrict__ leaves,
+    uint32_t *__restrict__ bounds, unsigned long long *__restrict__ codes,
+    uint32_t *__restrict__ nodes_b) {
+  if (blockIdx.x < gather_blocks) {
+    gather_node((uint64_t(blockIdx.x) * kTB + threadIdx.x) >> 6, n2, lp.a_mask[2], lp.a_base[2],
+                lp.a_code[2], leafgrid, brickbits, leaves, bounds, codes, nullptr, 0u, nullptr);
+    return;
+  }
+  place_entry(uint64_t(blockIdx.x - gather_blocks) * kTB + threadIdx.x, n2, lp, upper, cap,
+              nodes_b);
 }
 
 // x-major dense words from the leaf grid: word i covers x0..x0+31 of row (y, z).
@@ -412,7 +445,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_levels_batch(
 
 // Subtree sizes (levels 3..k+1), the root, and School-B placement of levels k..3, in one
 // workgroup; the level-2 placement follows as a multi-block pass. With nodes_b null only the
-// positions (pos, bs) are computed; the node entries are written later (k_place_tree).
+// positions (pos, bs) are computed.
 __device__ __forceinline__ void upper_place(const LevelPtrs &lp, int k,
                                             const uint32_t *__restrict__ tot,
                                             uint32_t *__restrict__ nodes_b) {
@@ -701,29 +734,31 @@ void tree_free_device(vxl_tree *t) {
   t->materials = nullptr;
 }
 
-static vxl_tree *alloc_tree(vxl_ctx *ctx, uint32_t n, uint32_t k, uint64_t nodes,
+// One arena block for a tree's four arrays: leaves, bounds, codes, then room for node_cap
+// School-B entries, so a build can allocate before it knows its exact node count (it sets
+// node_count and bytes once the count is in).
+static vxl_tree *alloc_tree(vxl_ctx *ctx, uint32_t n, uint32_t k, uint64_t node_cap,
                             uint64_t leaves) {
   vxl_tree *t = new vxl_tree();
   t->ctx = ctx;
   t->n = n;
   t->k = k;
   t->leaf_count = leaves;
-  t->node_count = nodes;
-  // one arena block for the four arrays
+  t->node_count = node_cap;
   auto up = [](uint64_t b) { return ((b ? b : 16) + 255) / 256 * 256; };
-  const uint64_t bn = up(nodes * 12), bl = up(leaves * 64), bb = up(leaves * 4), bc = up(leaves * 8);
+  const uint64_t bl = up(leaves * 64), bb = up(leaves * 4), bc = up(leaves * 8), bn = up(node_cap * 12);
   char *c = nullptr;
   try {
-    c = static_cast<char *>(arena_alloc(ctx, bn + bl + bb + bc, &t->slab));
+    c = static_cast<char *>(arena_alloc(ctx, bl + bb + bc + bn, &t->slab));
   } catch (...) {
     delete t;
     throw;
   }
-  t->nodes = reinterpret_cast<uint32_t *>(c);
-  t->leaves = reinterpret_cast<uint64_t *>(c + bn);
-  t->bounds = reinterpret_cast<uint32_t *>(c + bn + bl);
-  t->codes = reinterpret_cast<uint64_t *>(c + bn + bl + bb);
-  t->bytes = nodes * 12 + leaves * 76;
+  t->leaves = reinterpret_cast<uint64_t *>(c);
+  t->bounds = reinterpret_cast<uint32_t *>(c + bl);
+  t->codes = reinterpret_cast<uint64_t *>(c + bl + bb);
+  t->nodes = reinterpret_cast<uint32_t *>(c + bl + bb + bc);
+  t->bytes = node_cap * 12 + leaves * 76;
   return t;
 }
 
@@ -748,9 +783,9 @@ __device__ __forceinline__ void put_size(BuildSizes *out, int i, uint64_t v, uin
 // ---- single-tree level 2 without a device-wide scan ------------------------------------
 // k_level2_sums leaves each block's sum of the packed (popcount | nonzero << 32) values of
 // its B1 words (a block's popcounts sum to at most 256 * 64, so the halves never carry);
-// k_upper_levels_sizes scans those sums in its one workgroup before the upper levels and
-// then writes the sizes the host reads; k_emit_a2_local rebuilds each word's packed offset
-// from a block-local scan plus the block's offset.
+// k_upper_emit's first block reduces those sums to the sizes the host reads before the
+// upper levels; its other blocks rebuild each word's packed offset from a block-local scan
+// plus the sum of the block sums before the block.
 __device__ __forceinline__ uint64_t packed_word(unsigned long long w) {
   return uint64_t(__popcll(w)) | (uint64_t(w != 0ull) << 32);
 }
@@ -806,18 +841,21 @@ __device__ __forceinline__ uint64_t wave_scan_u64(uint64_t v, uint64_t *excl) {
   return uint64_t(__shfl((long long)inc, 63, 64));
 }
 
-// One workgroup, for one tree with k = top - 1 <= 4: the level-2 block sums (at most
-// kSmallThreads of them) to their exclusive prefix sums in place; level 3 (at most
-// kSmallWords input words, kW per thread) with one packed scan; levels 4 and 5 (at most 64
-// input words) in wave 0; the sizes into the host-mapped BuildSizes; then the School-B
-// positions of levels 3..top. Everything the positions read stays in LDS (level 3's child
-// bases are its exclusive subtree-size scan E_3, since a level-3 node's subtree size is its
-// child count), so the chain has no dependent global loads after the first; the positions
-// are computed after the sizes are out, while the host allocates the tree.
-__global__ __launch_bounds__(kSmallThreads) void k_upper_levels_sizes(
-    LevelPtrs lp, int top, const unsigned long long *__restrict__ b2,
-    uint64_t *__restrict__ block_sums, uint32_t nb2, const uint64_t *__restrict__ pairs,
-    BuildSizes *__restrict__ out, uint32_t tag) {
+// The upper levels of one tree with k = top - 1 <= 4, in one workgroup: the total of the
+// level-2 block sums (at most kSmallThreads of them) gives the leaf and level-2 node counts,
+// which go to the host-mapped BuildSizes at once (the host allocates the tree with them);
+// then level 3 (at most kSmallWords input words, kW per thread) with one packed scan,
+// levels 4 and 5 (at most 64 input words) in wave 0, their node counts (to the host and to
+// `upper` for the node-entry pass), and the School-B positions of levels 3..top.
+// Everything the positions read stays in LDS (level 3's child bases are its exclusive
+// subtree-size scan E_3, since a level-3 node's subtree size is its child count), so the
+// chain has no dependent global loads after the first.
+__device__ __forceinline__ void upper_block(const LevelPtrs &lp, int top,
+                                            const unsigned long long *__restrict__ b2,
+                                            const uint64_t *__restrict__ block_sums, uint32_t nb2,
+                                            const uint64_t *__restrict__ pairs,
+                                            BuildSizes *__restrict__ out, uint32_t tag,
+                                            uint32_t *__restrict__ upper) {
   constexpr int kW = int(kSmallWords / kSmallThreads);
   static_assert(64 % kW == 0, "a thread's level-3 words share one level-4 input word");
   __shared__ uint32_t s_base3[kSmallWords + 1];  // a_base of level 3 (= E_3); [n3] = n2
@@ -840,7 +878,12 @@ __global__ __launch_bounds__(kSmallThreads) void k_upper_levels_sizes(
   if (tid < 64) s_w4[tid] = 0ull;
   uint64_t excl;
   const uint64_t pn2_total = block_scan_u64<kSmallThreads>(v, &excl);
-  if (tid < nb2) block_sums[tid] = excl;
+  if (tid == 0) {  // s_pairs was written before the scan's barriers
+    put_size(out, kSzLeaves, pn2_total & 0xffffffffull, tag);
+    put_size(out, kSzLevel2, pn2_total >> 32, tag);
+    put_size(out, kSzPairsLo, s_pairs & 0xffffffffull, tag);
+    put_size(out, kSzPairsHi, s_pairs >> 32, tag);
+  }
   uint64_t packed = 0;
 #pragma unroll
   for (int i = 0; i < kW; i++) packed += packed_word(w[i]);
@@ -898,13 +941,12 @@ __global__ __launch_bounds__(kSmallThreads) void k_upper_levels_sizes(
   __syncthreads();
   const uint32_t n4 = top >= 4 ? s_n[0] : 0u, n5 = top >= 5 ? s_n[1] : 0u;
   if (tid == 0) {
-    put_size(out, kSzLeaves, pn2_total & 0xffffffffull, tag);
-    put_size(out, kSzLevel2, pn2_total >> 32, tag);
-    put_size(out, kSzPairsLo, s_pairs & 0xffffffffull, tag);
-    put_size(out, kSzPairsHi, s_pairs >> 32, tag);
     put_size(out, kSzLevel3, n3, tag);
     put_size(out, kSzLevel3 + 1, n4, tag);
     put_size(out, kSzLevel3 + 2, n5, tag);
+    upper[0] = n3;
+    upper[1] = n4;
+    upper[2] = n5;
   }
   if (top < 3) return;
   // positions. k = 4: the root (level 5) at 0 with its block at 1; level-4 node c, the
@@ -956,14 +998,53 @@ __global__ __launch_bounds__(kSmallThreads) void k_upper_levels_sizes(
   }
 }
 
-// Waits until every size word carries `tag`, so the host goes on while the size kernel and
-// the kernel queued after it run. Every 256 polls the event recorded after those kernels is
+// Level-2 School-A arrays of B1 words [e * kSmallThreads, (e + 1) * kSmallThreads): the
+// block's offset is the sum of the level-2 block sums (kTB words each) before its first
+// word, then a block-local scan of the packed words.
+__device__ __forceinline__ void emit_block(uint32_t e, const unsigned long long *__restrict__ b1,
+                                           uint64_t w1, const uint64_t *__restrict__ block_sums,
+                                           unsigned long long *__restrict__ a_mask,
+                                           uint32_t *__restrict__ a_base,
+                                           uint32_t *__restrict__ a_code) {
+  static_assert(kSmallThreads % kTB == 0, "an emit block covers whole level-2 blocks");
+  const uint32_t tid = threadIdx.x;
+  const uint64_t j = uint64_t(e) * kSmallThreads + tid;
+  const unsigned long long w = j < w1 ? b1[j] : 0ull;
+  const uint32_t before = e * uint32_t(kSmallThreads / kTB);  // at most nb2 <= kSmallThreads
+  uint64_t unused, excl;
+  const uint64_t off = block_scan_u64<kSmallThreads>(tid < before ? block_sums[tid] : 0ull, &unused);
+  block_scan_u64<kSmallThreads>(packed_word(w), &excl);
+  if (!w) return;
+  const uint64_t v = off + excl;
+  const uint32_t i = uint32_t(v >> 32);
+  a_mask[i] = w;
+  a_base[i] = uint32_t(v);
+  a_code[i] = uint32_t(j);
+}
+
+// Block 0: the upper levels and the sizes (upper_block); blocks 1..: the level-2 School-A
+// arrays (emit_block). Both read only what k_level2_sums wrote.
+__global__ __launch_bounds__(kSmallThreads) void k_upper_emit(
+    LevelPtrs lp, int top, const unsigned long long *__restrict__ b1,
+    const unsigned long long *__restrict__ b2, const uint64_t *__restrict__ block_sums,
+    uint32_t nb2, const uint64_t *__restrict__ pairs, BuildSizes *__restrict__ out, uint32_t tag,
+    uint32_t *__restrict__ upper) {
+  if (blockIdx.x == 0)
+    upper_block(lp, top, b2, block_sums, nb2, pairs, out, tag, upper);
+  else
+    emit_block(blockIdx.x - 1, b1, lp.win[2], block_sums, lp.a_mask[2], lp.a_base[2],
+               lp.a_code[2]);
+}
+
+// Waits until size words [first, first + count) carry `tag`, so the host goes on while the
+// kernel that writes them runs. Every 256 polls the event recorded after those kernels is
 // checked too: a failed launch ends the wait with its error, and finished kernels whose
 // sizes are still missing are an error rather than an endless wait.
-static void wait_sizes(const BuildSizes *hs, uint64_t tag, hipEvent_t ev, uint64_t *v) {
+static void wait_sizes(const BuildSizes *hs, uint64_t tag, hipEvent_t ev, int first, int count,
+                       uint64_t *v) {
   const volatile uint64_t *w = hs->w;
   auto all_in = [&] {
-    for (int j = 0; j < kSzWords; j++)
+    for (int j = first; j < first + count; j++)
       if ((w[j] >> 48) != tag) return false;
     return true;
   };
@@ -981,23 +1062,7 @@ static void wait_sizes(const BuildSizes *hs, uint64_t tag, hipEvent_t ev, uint64
       }
     }
   }
-  for (int j = 0; j < kSzWords; j++) v[j] = w[j] & kSzValue;
-}
-
-__global__ __launch_bounds__(kTB) void k_emit_a2_local(
-    const unsigned long long *__restrict__ bin, uint64_t win,
-    const uint64_t *__restrict__ block_offsets, unsigned long long *__restrict__ a_mask,
-    uint32_t *__restrict__ a_base, uint32_t *__restrict__ a_code) {
-  const uint64_t j = uint64_t(blockIdx.x) * kTB + threadIdx.x;
-  const unsigned long long w = j < win ? bin[j] : 0ull;
-  uint64_t excl;
-  block_scan_u64<kTB>(packed_word(w), &excl);
-  if (!w) return;
-  const uint64_t v = block_offsets[blockIdx.x] + excl;
-  const uint32_t i = uint32_t(v >> 32);
-  a_mask[i] = w;
-  a_base[i] = uint32_t(v);
-  a_code[i] = uint32_t(j);
+  for (int j = first; j < first + count; j++) v[j] = w[j] & kSzValue;
 }
 
 // n <= 2048 (k <= 4): every level above 2 has at most 4096 input words.
@@ -1033,69 +1098,83 @@ static vxl_tree *build_tree_fast(vxl_ctx *ctx, uint32_t n, uint32_t k, const uin
     lp.pos[L] = carve<uint32_t>(p, e);
     lp.bs[L] = carve<uint32_t>(p, e);
   }
+  uint32_t *upper = carve<uint32_t>(p, 4);  // node counts of levels 3..5, on the device
   unsigned long long *b1 = ctx->bitmap.as<unsigned long long>();
   // 1. level 2: per-block sums of the packed popcount/flag values of B1, B2 by ballot
   hipLaunchKernelGGL(k_level2_sums, dim3(nb2), dim3(kTB), 0, s, b1, w1, l2sums, b2);
-  // 2. the level-2 block offsets, the upper levels and the sizes, in one workgroup
+  // 2. the sizes, upper levels and positions (block 0) beside the level-2 School-A arrays
   BuildSizes *hs = static_cast<BuildSizes *>(ctx->host_pinned);
   const uint32_t tag = ctx->size_seq++ % 65535u + 1u;
   // no size word holds this build's tag before its size kernel writes it (other read-backs
   // share the area, so it is cleared here, before the kernel is queued)
   for (int j = 0; j < kSzWords; j++) reinterpret_cast<volatile uint64_t *>(hs->w)[j] = 0ull;
-  hipLaunchKernelGGL(k_upper_levels_sizes, dim3(1), dim3(kSmallThreads), 0, s, lp, top, b2,
-                     l2sums, nb2, pairs_dev, static_cast<BuildSizes *>(ctx->host_pinned_dev),
-                     tag);
-  // 3. level-2 School-A arrays (no output array needed: queued before the host waits)
-  hipLaunchKernelGGL(k_emit_a2_local, dim3(nb2), dim3(kTB), 0, s, b1, w1, l2sums, lp.a_mask[2],
-                     lp.a_base[2], lp.a_code[2]);
+  const uint32_t emit_blocks = uint32_t(div_up(w1, kSmallThreads));
+  hipLaunchKernelGGL(k_upper_emit, dim3(1 + emit_blocks), dim3(kSmallThreads), 0, s, lp, top, b1,
+                     b2, l2sums, nb2, pairs_dev, static_cast<BuildSizes *>(ctx->host_pinned_dev),
+                     tag, upper);
   VXL_HIP(hipGetLastError());
   if (!ctx->ev_wait) VXL_HIP(hipEventCreateWithFlags(&ctx->ev_wait, hipEventDisableTiming));
-  VXL_HIP(hipEventRecord(ctx->ev_wait, s));  // after both kernels: no marker between them
-  // 4. the sizes (the one host round trip of the call)
+  VXL_HIP(hipEventRecord(ctx->ev_wait, s));
+  // 3. leaves, level-2 nodes and pairs: the first words the kernel writes, and all the
+  //    allocation needs (the node array gets room for the most nodes n2 level-2 nodes can
+  //    have above them, at most one per input word of each level)
   uint64_t sz[kSzWords];
-  wait_sizes(hs, tag, ctx->ev_wait, sz);
+  wait_sizes(hs, tag, ctx->ev_wait, kSzLeaves, kSzLevel3 - kSzLeaves, sz);
   const uint64_t leaves = sz[kSzLeaves];
   const uint64_t n2 = sz[kSzLevel2];
   if (pairs) *pairs = sz[kSzPairsLo] | (sz[kSzPairsHi] << 32);
-  uint64_t nodes = 0;
-  uint64_t level_nodes[16] = {0};
-  level_nodes[2] = n2;
-  for (int L = 3; L <= top; L++) level_nodes[L] = sz[kSzLevel3 + L - 3];
   // the kernels that follow index per-level scratch and the new tree by these sizes: refuse
   // any that the grid cannot hold rather than launch them out of bounds
   const uint64_t bricks = uint64_t(n / 8) * (n / 8) * (n / 8);
-  bool sane = leaves <= bricks && n2 <= w1 && n2 <= leaves && (leaves == 0) == (n2 == 0);
-  for (int L = 3; L <= top; L++) sane = sane && level_nodes[L] <= lp.win[L] && level_nodes[L] <= n2;
-  if (!sane) throw Error(VXL_DEVICE_ERROR, "tree build: the sizes read back are inconsistent");
-  if (k >= 1 && leaves)
-    for (int L = 2; L <= int(k) + 1; L++) nodes += level_nodes[L];
-  vxl_tree *t = alloc_tree(ctx, n, k, leaves ? nodes : 0, leaves);
-  if (k >= 1 && leaves)
-    for (int L = 2; L <= int(k) + 1; L++) t->level_nodes[L] = level_nodes[L];
-  if (leaves == 0) return t;
-  // 5. leaf gather (clears the grid and B1)
-  hipLaunchKernelGGL(k_leaf_gather, dim3(div_up(n2 * 64, kTB)), dim3(kTB), 0, s, n2,
-                     lp.a_mask[2], lp.a_base[2], lp.a_code[2],
-                     ctx->leafgrid.as<unsigned long long>(), b1,
-                     reinterpret_cast<unsigned long long *>(t->leaves), t->bounds,
-                     reinterpret_cast<unsigned long long *>(t->codes), nullptr, 0u, nullptr);
-  // 6. School-B node entries (upper positions came with the sizes)
-  if (k == 1) {
-    hipLaunchKernelGGL(k_root, dim3(1), dim3(1), 0, s, lp.a_mask[2], lp.a_base[2], 1, t->nodes,
-                       lp.pos[2], lp.bs[2]);
-  } else if (k >= 2) {
-    UpperNodes un;
-    std::memset(&un, 0, sizeof(un));
-    uint64_t u = 0;
-    for (int L = 3; L <= int(k) + 1; L++) {
-      un.ubeg[L] = uint32_t(u);
-      u += level_nodes[L];
-    }
-    un.ubeg[k + 2] = uint32_t(u);
-    hipLaunchKernelGGL(k_place_tree, dim3(div_up(n2 + u, kTB)), dim3(kTB), 0, s, n2, n2 + u, lp,
-                       int(k), un, t->nodes);
+  if (!(leaves <= bricks && n2 <= w1 && n2 <= leaves && (leaves == 0) == (n2 == 0)))
+    throw Error(VXL_DEVICE_ERROR, "tree build: the sizes read back are inconsistent");
+  uint64_t cap = 0;
+  if (k >= 1 && leaves) {
+    cap = n2;
+    for (int L = 3; L <= int(k) + 1; L++) cap += std::min<uint64_t>(n2, lp.win[L]);
   }
-  VXL_HIP(hipGetLastError());
+  vxl_tree *t = alloc_tree(ctx, n, k, cap, leaves);
+  if (leaves == 0) return t;
+  try {
+    // 4. leaf gather (clears the grid and B1) and the School-B node entries
+    const uint32_t gather_blocks = uint32_t(div_up(n2 * 64, kTB));
+    if (k >= 2) {
+      hipLaunchKernelGGL(k_gather_place, dim3(gather_blocks + div_up(cap, kTB)), dim3(kTB), 0, s,
+                         n2, gather_blocks, lp, upper, cap,
+                         ctx->leafgrid.as<unsigned long long>(), b1,
+                         reinterpret_cast<unsigned long long *>(t->leaves), t->bounds,
+                         reinterpret_cast<unsigned long long *>(t->codes), t->nodes);
+    } else {
+      hipLaunchKernelGGL(k_leaf_gather, dim3(gather_blocks), dim3(kTB), 0, s, n2, lp.a_mask[2],
+                         lp.a_base[2], lp.a_code[2], ctx->leafgrid.as<unsigned long long>(), b1,
+                         reinterpret_cast<unsigned long long *>(t->leaves), t->bounds,
+                         reinterpret_cast<unsigned long long *>(t->codes), nullptr, 0u, nullptr);
+      if (k == 1)
+        hipLaunchKernelGGL(k_root, dim3(1), dim3(1), 0, s, lp.a_mask[2], lp.a_base[2], 1,
+                           t->nodes, lp.pos[2], lp.bs[2]);
+    }
+    VXL_HIP(hipGetLastError());
+    // 5. the node counts of levels 3..5, written by the size kernel after the first words
+    wait_sizes(hs, tag, ctx->ev_wait, kSzLevel3, kSzWords - kSzLevel3, sz);
+    uint64_t level_nodes[16] = {0};
+    level_nodes[2] = n2;
+    for (int L = 3; L <= top; L++) level_nodes[L] = sz[kSzLevel3 + L - 3];
+    uint64_t nodes = 0;
+    bool sane = true;
+    for (int L = 3; L <= top; L++) sane = sane && level_nodes[L] <= lp.win[L] && level_nodes[L] <= n2;
+    if (k >= 1)
+      for (int L = 2; L <= int(k) + 1; L++) nodes += level_nodes[L];
+    if (!sane || nodes > cap)
+      throw Error(VXL_DEVICE_ERROR, "tree build: the node counts read back are inconsistent");
+    t->node_count = nodes;
+    t->bytes = nodes * 12 + leaves * 76;
+    if (k >= 1)
+      for (int L = 2; L <= int(k) + 1; L++) t->level_nodes[L] = level_nodes[L];
+  } catch (...) {
+    tree_free_device(t);
+    delete t;
+    throw;
+  }
   return t;
 }
 
