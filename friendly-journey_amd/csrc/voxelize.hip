@@ -797,6 +797,10 @@ __device__ __forceinline__ void table_flush(unsigned long long *tk, unsigned lon
   __builtin_amdgcn_wave_barrier();
 }
 
+// Chunks of a unit walked with 32-bit pair offsets from one base: every offset in a span,
+// and the 64 positions after it, stay below 0xffffffff, the clamp for starts past the span.
+constexpr uint64_t kSpanChunks = (uint64_t(1) << 26) - 2;
+
 // Each wave walks chunks [c0, c1) of 64 consecutive pairs. A lane's triangle is the
 // last triangle whose first pair is at or before the lane's pair: the starts of the 64
 // triangles after the chunk's first triangle are loaded one per lane and searched
@@ -851,17 +855,30 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))
   const uint64_t c0 = uint64_t(unit) * chunks_per_unit;
   const uint64_t c1 = min(c0 + chunks_per_unit, nchunks);
   uint32_t t_first = ustart[unit];
-  uint64_t start_first = coff[t_first];
-  uint64_t offn;  // start offset of triangle t_first + 1 + lane (P past the last triangle)
+  uint64_t offn_raw;  // start offset of triangle t_first + 1 + lane (P past the last triangle)
   {
     const uint64_t idx = uint64_t(t_first) + 1 + uint64_t(lane);
-    offn = idx <= T ? coff[idx] : ~0ull;
+    offn_raw = idx <= T ? coff[idx] : ~0ull;
   }
-  for (uint64_t c = c0; c < c1; c++) {
-    const uint64_t base = c * 64;
-    const uint64_t p = base + uint64_t(lane);
-    const bool valid = p < P;
-    const uint64_t rpos = offn - base;  // >= 1
+  // The unit's chunks in spans of at most kSpanChunks: positions inside a span are 32-bit
+  // offsets from its first pair; a triangle start is kept as its offset from there, clamped
+  // to 0xffffffff past the span, or taken modulo 2^32 for the start of the span's first
+  // triangle, which may lie before it (p - start is then taken modulo 2^32 below).
+  for (uint64_t sb = c0; sb < c1; sb += kSpanChunks) {
+  const uint64_t ubase = sb * 64;
+  const uint32_t pend = uint32_t(min(P - ubase, uint64_t(0xffffffffu)));
+  const uint32_t nloc = uint32_t(min(c1 - sb, kSpanChunks));
+  auto rel = [ubase](uint64_t v) -> uint32_t {
+    const uint64_t r = v - ubase;
+    return r > 0xffffffffull ? 0xffffffffu : uint32_t(r);
+  };
+  uint32_t start_first = uint32_t(coff[t_first] - ubase);
+  for (uint32_t cl = 0; cl < nloc; cl++) {
+    const uint32_t base = cl * 64;
+    const uint32_t p = base + uint32_t(lane);
+    const bool valid = p < pend;
+    const uint32_t offn = rel(offn_raw);
+    const uint32_t rpos = offn - base;  // >= 1
     // Chunk positions where a triangle starts: every lane whose start falls in the chunk
     // sends a flag to the lane of that position by a forward permute (the others send to
     // lane 0, which no start occupies), and a ballot of the received flags is the mark
@@ -872,58 +889,60 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))
     const unsigned long long marks = __ballot(flag != 0) & ~1ull;
     const uint32_t starts_in = __popcll(__ballot(in_chunk));
     uint32_t t;
-    uint64_t start;
+    uint32_t start;
     if (__popcll(marks) == starts_in) {
       // one start per marked position: lane's triangle = t_first + marks at positions 1..lane,
       // and its first pair is the highest of those positions
       const unsigned long long m = marks & (below << 1);
       const uint32_t k = __popcll(m);
       t = t_first + k;
-      start = k ? base + uint64_t(63 - __clzll(m)) : start_first;
+      start = k ? base + uint32_t(63 - __clzll(m)) : start_first;
     } else {
       // several triangles start at one position (triangles without candidates)
       uint32_t k = 0;
 #pragma unroll
       for (int step = 32; step >= 1; step >>= 1) {
-        const uint64_t v = shfl_u64(offn, int(k + step) - 1);
+        const uint32_t v = uint32_t(__shfl(int(offn), int(k + step) - 1, 64));
         if (v <= p) k += step;
       }
-      if (shfl_u64(offn, 63) <= p) k = 64;  // the search above tops out at 63
-      const uint64_t kstart = shfl_u64(offn, k ? int(k) - 1 : 0);
+      if (uint32_t(__shfl(int(offn), 63, 64)) <= p) k = 64;  // the search above tops out at 63
+      const uint32_t kstart = uint32_t(__shfl(int(offn), k ? int(k) - 1 : 0, 64));
       t = t_first + k;
       start = k ? kstart : start_first;
       if (k == 64 && valid) {  // 64 starts inside one chunk
+        const uint64_t pg = ubase + p;
         uint64_t lo = t, hi = T - 1;  // last triangle with start <= p
         while (lo < hi) {
           const uint64_t mid = (lo + hi + 1) >> 1;
-          if (coff[mid] <= p) lo = mid; else hi = mid - 1;
+          if (coff[mid] <= pg) lo = mid; else hi = mid - 1;
         }
         t = uint32_t(lo);
-        start = coff[lo];
+        start = uint32_t(coff[lo] - ubase);
       }
     }
     // next chunk's first triangle (the last one starting at or before base + 64) and the
     // prefetch of its window of starts
     uint32_t t_next = t_first;
-    uint64_t start_next = start_first;
+    uint32_t start_next = start_first;
     {
-      const uint64_t nb = base + 64;
+      const uint32_t nb = base + 64;
       const uint32_t adv = __popcll(__ballot(offn <= nb));
-      if (adv == 64 && nb < P) {  // saturated window: search (wave-uniform)
+      if (adv == 64 && ubase + nb < P) {  // saturated window: search (wave-uniform)
+        const uint64_t nbg = ubase + nb;
         uint64_t lo = uint64_t(t_first) + 64, hi = T - 1;
         while (lo < hi) {
           const uint64_t mid = (lo + hi + 1) >> 1;
-          if (coff[mid] <= nb) lo = mid; else hi = mid - 1;
+          if (coff[mid] <= nbg) lo = mid; else hi = mid - 1;
         }
         t_next = uint32_t(lo);
-        start_next = coff[lo];
+        start_next = uint32_t(coff[lo] - ubase);
       } else if (adv) {
-        start_next = readlane_u64(offn, int(adv) - 1);
+        start_next = uint32_t(__builtin_amdgcn_readlane(int(offn), int(adv) - 1));
         t_next = t_first + adv;
       }
     }
     uint64_t offn_next = ~0ull;
-    if (c + 1 < c1) {
+    if (sb + cl + 1 < c1) {
       const uint64_t idx = uint64_t(t_next) + 1 + uint64_t(lane);
       offn_next = idx <= T ? coff[idx] : ~0ull;
     }
@@ -932,18 +951,19 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))
     uint32_t x = 0, y = 0, z = 0;
     if (valid) {
       const TriRec r = rec[t];  // one copy: the record's loads issue together
-      const uint64_t li = p - start;
       const uint32_t ex = r.mnz_ex >> 16;
       {
-        // the narrow decode runs on every lane (li < 2^22 unless kWide), so the record's
-        // reciprocals are loaded with the rest of it; wide triangles decode again below
-        const uint32_t l32 = uint32_t(li) & 0x3fffffu;
+        // the narrow decode runs on every lane (p - start < 2^22 unless kWide), so the
+        // record's reciprocals are loaded with the rest of it; wide triangles decode again
+        // below
+        const uint32_t l32 = (p - start) & 0x3fffffu;
         z = udiv_narrow(l32, r.exy, r.rcp_exy);
         const uint32_t rem = l32 - mul24(z, r.exy);
         y = udiv_narrow(rem, ex, r.rcp_ex);
         x = rem - mul24(y, ex);
       }
       if (__builtin_expect((r.flags & kWide) != 0, 0)) {
+        const uint64_t li = ubase + p - coff[t];  // the triangle's start, which may lie far back
         const uint64_t zz = li / r.exy, rem = li - zz * r.exy;
         z = uint32_t(zz);
         y = uint32_t(rem / ex);
@@ -1010,9 +1030,10 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))
         }
       }
     }
-    offn = offn_next;
+    offn_raw = offn_next;
     t_first = t_next;
     start_first = start_next;
+  }
   }
   }
   __builtin_amdgcn_wave_barrier();
