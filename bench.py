@@ -75,10 +75,13 @@ def bench_voxelize(ctx, torch, W, steps, warmup):
     n, vs = W.grid_for(1024)
     d = torch.from_numpy(tris).to("cuda")
     torch.cuda.synchronize()
+    # warm-up as the timed steps run (trees kept until the end, so the tree arena holds the
+    # slabs the timed steps reuse), long enough for the clocks to settle
     log(f"voxelize warmup x{warmup}: T={T} n={n}")
-    for _ in range(warmup):
-        ctx.voxelize_tree_dev(d.data_ptr(), T, n, (0, 0, 0), vs).free()
+    warm = [ctx.voxelize_tree_dev(d.data_ptr(), T, n, (0, 0, 0), vs) for _ in range(warmup)]
     torch.cuda.synchronize()
+    for tr in warm:
+        tr.free()
     # the timed steps run without stage events: reading them waits for each call to finish,
     # which a caller streaming meshes through the library does not do
     trees = []
@@ -753,7 +756,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dda", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
