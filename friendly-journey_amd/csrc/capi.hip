@@ -1097,6 +1097,98 @@ void vxl_graph_destroy(vxl_graph *graph) {
   delete graph;
 }
 
+vxl_status vxl_batch_graph_create(vxl_ctx *ctx, const float *d_tris, const uint64_t *tri_offsets,
+                                  uint32_t count, const vxl_grid *grids, float epsilon,
+                                  const float *d_rays, const uint64_t *ray_offsets,
+                                  uint32_t *d_hits, double *d_t, double headroom,
+                                  vxl_batch_graph **out) {
+  return guarded(ctx, [&] {
+    if (!out) throw Error(VXL_INVALID_ARG, "out is null");
+    *out = nullptr;
+    if (count == 0) throw Error(VXL_INVALID_ARG, "batch graph: no meshes");
+    if (!tri_offsets || !grids) throw Error(VXL_INVALID_ARG, "null offsets or grids");
+    if (headroom == 0.0) headroom = 1.5;
+    if (!(headroom >= 1.0 && headroom <= 64.0))
+      throw Error(VXL_INVALID_ARG, "batch graph: headroom must lie in [1, 64] (0: 1.5)");
+    validate_eps(epsilon);
+    std::vector<vxl::BatchInput> in;
+    for (uint32_t g = 0; g < count; g++) {
+      uint32_t k = 0;
+      validate_grid(&grids[g], &k);
+      if (k < 2 || k > 4)
+        throw Error(VXL_INVALID_ARG, "batch graph: every mesh needs n in 128..2048");
+      if (tri_offsets[g + 1] < tri_offsets[g])
+        throw Error(VXL_INVALID_ARG, "tri_offsets must be non-decreasing");
+      const uint64_t T = tri_offsets[g + 1] - tri_offsets[g];
+      validate_count(T);
+      in.push_back(vxl::BatchInput{g, d_tris + 9 * tri_offsets[g], T, grids[g].n, k,
+                                   make_params(&grids[g], epsilon)});
+    }
+    if (tri_offsets[count] > tri_offsets[0] && !d_tris)
+      throw Error(VXL_INVALID_ARG, "triangles pointer is null");
+    std::vector<uint64_t> roff(count + 1, 0);
+    if (ray_offsets) {
+      for (uint32_t g = 0; g < count; g++)
+        if (ray_offsets[g + 1] < ray_offsets[g])
+          throw Error(VXL_INVALID_ARG, "ray_offsets must be non-decreasing");
+      for (uint32_t g = 0; g <= count; g++) roff[g] = ray_offsets[g];
+      if (roff[count] > roff[0] && (!d_rays || !d_hits))
+        throw Error(VXL_INVALID_ARG, "null ray or hit buffer");
+    }
+    auto *bg = new vxl_batch_graph();
+    bg->ctx = ctx;
+    try {
+      vxl::batch_graph_build(bg, in, d_rays, roff.data(), d_hits, d_t, headroom);
+    } catch (...) {
+      vxl::batch_graph_release(bg);
+      delete bg;
+      throw;
+    }
+    *out = bg;
+  });
+}
+
+vxl_status vxl_batch_graph_launch(vxl_batch_graph *graph, void *stream) {
+  if (!graph) {
+    vxl::set_error("null batch graph");
+    return VXL_INVALID_ARG;
+  }
+  return guarded(graph->ctx, [&] {
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : graph->ctx->stream;
+    vxl::batch_graph_launch(graph, s);
+  });
+}
+
+vxl_status vxl_batch_graph_trees(vxl_batch_graph *graph, vxl_tree **trees, uint64_t *pairs) {
+  if (!graph) {
+    vxl::set_error("null batch graph");
+    return VXL_INVALID_ARG;
+  }
+  return guarded(graph->ctx, [&] { vxl::batch_graph_result(graph, trees, pairs); });
+}
+
+vxl_status vxl_batch_graph_info(const vxl_batch_graph *graph, uint64_t *arena_bytes,
+                                uint32_t *groups, uint64_t *graph_nodes) {
+  if (!graph) {
+    vxl::set_error("null batch graph");
+    return VXL_INVALID_ARG;
+  }
+  if (arena_bytes) *arena_bytes = graph->arena_bytes;
+  if (groups) *groups = uint32_t(graph->groups ? graph->groups->size() : 0);
+  if (graph_nodes) *graph_nodes = graph->kernel_nodes;
+  return VXL_OK;
+}
+
+void vxl_batch_graph_destroy(vxl_batch_graph *graph) {
+  if (!graph) return;
+  {
+    std::lock_guard<std::mutex> lock(graph->ctx->mu);
+    (void)hipSetDevice(graph->ctx->device);
+    vxl::batch_graph_release(graph);
+  }
+  delete graph;
+}
+
 vxl_status vxl_traverse_stats(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, uint64_t R,
                               uint64_t out[3]) {
   return guarded(ctx, [&] {

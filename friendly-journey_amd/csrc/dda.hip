@@ -598,16 +598,6 @@ __global__ __launch_bounds__(64) void k_render(vxl_camera cam, const uint32_t *_
   out[uint64_t(py) * cam.width + px] = color;
 }
 
-// One segment of a batched traversal, as the kernel sees it.
-struct TraverseSeg {
-  const uint32_t *nodes, *leafw, *bounds;
-  const float *rays;
-  uint4 *hits;
-  double *t;
-  uint64_t leaf_count, R;
-  uint32_t n, k;
-};
-
 // Segment g owns blocks [blk_begin[g], blk_begin[g+1]), so every block reads one tree.
 __global__ __launch_bounds__(256) void k_traverse_batch(const TraverseSeg *__restrict__ seg,
                                                         const uint32_t *__restrict__ blk_begin,
@@ -768,6 +758,17 @@ void launch_traverse_batch(vxl_ctx *ctx, const RaySegment *segs, uint32_t S, hip
                      reinterpret_cast<const uint32_t *>(d + seg_bytes), S);
   VXL_HIP(hipGetLastError());
   VXL_HIP(hipEventRecord(ctx->trav_done, s));
+}
+
+void launch_traverse_segs(vxl_ctx *ctx, const TraverseSeg *d_segs, const uint32_t *d_blk,
+                          uint32_t S, uint64_t blocks, uint32_t kmax, hipStream_t s) {
+  if (blocks == 0) return;
+  if (blocks > 0xffffffffull)
+    throw Error(VXL_WORKGROUPS_EXCEEDED, "traverse batch exceeds the grid limit");
+  const uint32_t tb = uint32_t(ctx->dda_block);
+  hipLaunchKernelGGL(k_traverse_batch, dim3(uint32_t(blocks)), dim3(tb), frame_lds_bytes(kmax, tb),
+                     s, d_segs, d_blk, S);
+  VXL_HIP(hipGetLastError());
 }
 
 namespace {

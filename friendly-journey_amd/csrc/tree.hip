@@ -102,13 +102,14 @@ __device__ __forceinline__ void word_extent(unsigned long long w, int axis, uint
 
 // One wave per level-2 node: 8 children at a time, 8 lanes (one per u64 word) per child.
 // In a batch group (n2_beg non-null) the stored brick code is relative to the mesh's region.
+// With store false the node's grid words are read and cleared but nothing is written.
 __device__ __forceinline__ void gather_node(
     uint64_t node, uint64_t n2, const unsigned long long *__restrict__ a_mask,
     const uint32_t *__restrict__ a_base, const uint32_t *__restrict__ a_code,
     unsigned long long *__restrict__ leafgrid, unsigned long long *__restrict__ brickbits,
     unsigned long long *__restrict__ leaves, uint32_t *__restrict__ bounds,
     unsigned long long *__restrict__ codes, const uint64_t *__restrict__ n2_beg, uint32_t G,
-    const BatchMesh *__restrict__ mesh) {
+    const BatchMesh *__restrict__ mesh, bool store = true) {
   if (node >= n2) return;
   const int lane = threadIdx.x & 63;
   const unsigned long long mask = a_mask[node];
@@ -133,7 +134,7 @@ __device__ __forceinline__ void gather_node(
     if (act) {
       v = leafgrid[code * 8 + w];
       leafgrid[code * 8 + w] = 0ull;
-      leaves[(uint64_t(base) + s) * 8 + w] = v;
+      if (store) leaves[(uint64_t(base) + s) * 8 + w] = v;
     }
     // LeafBounds over the 8 words of this leaf (8-lane groups): each lane marks the
     // coordinates its 4^3 sub-brick word occupies along x, y, z (one byte each, shifted to
@@ -155,7 +156,7 @@ __device__ __forceinline__ void gather_node(
       mn[a] = m ? uint32_t(__ffs(m) - 1) : 7u;
       mx[a] = m ? 31u - uint32_t(__clz(m)) : 0u;
     }
-    if (act && w == 0) {
+    if (store && act && w == 0) {
       uint32_t b = (cnt > 64 || cnt == 0)
                        ? kFull
                        : (mn[0] | (mn[1] << 3) | (mn[2] << 6) | (mx[0] << 9) | (mx[1] << 12) |
@@ -550,7 +551,32 @@ __global__ __launch_bounds__(kSmallThreads) void k_upper_place_batch(
   upper_place(lps[g], int(mesh[g].k), lps[g].tot, nodes_ptr[g]);
 }
 
-// Level-2 placement under level-3 parents, for every level-2 node of the group.
+// Level-2 placement under level-3 parents: level-2 node c of the group (nothing for a mesh
+// whose node array is null: no tree placed).
+__device__ __forceinline__ void place2_one(uint64_t c, uint32_t G, const uint64_t *__restrict__ n2_beg,
+                                           const BatchMesh *__restrict__ mesh,
+                                           const LevelPtrs *__restrict__ lps,
+                                           const BatchTotals *__restrict__ tot,
+                                           uint32_t *const *__restrict__ nodes_ptr,
+                                           const unsigned long long *__restrict__ a_mask,
+                                           const uint32_t *__restrict__ a_base,
+                                           const uint32_t *__restrict__ a_code) {
+  const uint32_t g = dev::upper_index(n2_beg, G, c);
+  uint32_t *nodes_b = nodes_ptr[g];
+  if (!nodes_b) return;
+  const LevelPtrs &lp = lps[g];
+  const uint32_t cl = uint32_t(c - n2_beg[g]);
+  const uint32_t q = uint32_t(a_code[c] - mesh[g].b1_begin);
+  const uint32_t p = lp.ni[3][q >> 6];
+  const uint32_t pb = lp.a_base[3][p];
+  const uint32_t mypos = lp.bs[3][p] + (cl - pb);
+  const unsigned long long m = a_mask[c];
+  nodes_b[3ull * mypos + 0] = uint32_t(m);
+  nodes_b[3ull * mypos + 1] = uint32_t(m >> 32);
+  nodes_b[3ull * mypos + 2] = uint32_t(a_base[c] - tot[g].leaf_begin);
+}
+
+// For every level-2 node of the group.
 __global__ void k_place2_batch(uint64_t N2, uint32_t G, const uint64_t *__restrict__ n2_beg,
                                const BatchMesh *__restrict__ mesh,
                                const LevelPtrs *__restrict__ lps,
@@ -560,19 +586,7 @@ __global__ void k_place2_batch(uint64_t N2, uint32_t G, const uint64_t *__restri
                                const uint32_t *__restrict__ a_base,
                                const uint32_t *__restrict__ a_code) {
   const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (c >= N2) return;
-  const uint32_t g = dev::upper_index(n2_beg, G, c);
-  const LevelPtrs &lp = lps[g];
-  const uint32_t cl = uint32_t(c - n2_beg[g]);
-  const uint32_t q = uint32_t(a_code[c] - mesh[g].b1_begin);
-  const uint32_t p = lp.ni[3][q >> 6];
-  const uint32_t pb = lp.a_base[3][p];
-  const uint32_t mypos = lp.bs[3][p] + (cl - pb);
-  uint32_t *nodes_b = nodes_ptr[g];
-  const unsigned long long m = a_mask[c];
-  nodes_b[3ull * mypos + 0] = uint32_t(m);
-  nodes_b[3ull * mypos + 1] = uint32_t(m >> 32);
-  nodes_b[3ull * mypos + 2] = uint32_t(a_base[c] - tot[g].leaf_begin);
+  if (c < N2) place2_one(c, G, n2_beg, mesh, lps, tot, nodes_ptr, a_mask, a_base, a_code);
 }
 
 struct Level {
@@ -1178,6 +1192,26 @@ static vxl_tree *build_tree_fast(vxl_ctx *ctx, uint32_t n, uint32_t k, const uin
   return t;
 }
 
+vxl_tree *tree_copy_arrays(vxl_ctx *ctx, uint32_t n, uint32_t k, uint64_t leaves,
+                           uint64_t nodes, const uint64_t *level_nodes, const void *src_nodes,
+                           const void *src_leaves, const void *src_bounds,
+                           const void *src_codes, hipStream_t s) {
+  if (!leaves) return empty_tree_device(ctx, n, k);
+  vxl_tree *t = alloc_tree(ctx, n, k, nodes, leaves);
+  try {
+    VXL_HIP(hipMemcpyAsync(t->nodes, src_nodes, nodes * 12, hipMemcpyDeviceToDevice, s));
+    VXL_HIP(hipMemcpyAsync(t->leaves, src_leaves, leaves * 64, hipMemcpyDeviceToDevice, s));
+    VXL_HIP(hipMemcpyAsync(t->bounds, src_bounds, leaves * 4, hipMemcpyDeviceToDevice, s));
+    VXL_HIP(hipMemcpyAsync(t->codes, src_codes, leaves * 8, hipMemcpyDeviceToDevice, s));
+  } catch (...) {
+    tree_free_device(t);
+    delete t;
+    throw;
+  }
+  for (int L = 0; L < 16; L++) t->level_nodes[L] = level_nodes[L];
+  return t;
+}
+
 vxl_tree *empty_tree_device(vxl_ctx *ctx, uint32_t n, uint32_t k) {
   return alloc_tree(ctx, n, k, 0, 0);
 }
@@ -1322,6 +1356,168 @@ void build_trees_batch(vxl_ctx *ctx, const std::vector<BatchMesh> &meshes,
                      d_tot, nodes_ptr);
   hipLaunchKernelGGL(k_place2_batch, dim3(div_up(N2, kTB)), dim3(kTB), 0, s, N2, G, n2_beg,
                      d_mesh, d_lps, d_tot, nodes_ptr, a_mask2, a_base2, a_code2);
+  VXL_HIP(hipGetLastError());
+}
+
+// ---- batch groups of a graph: the sizes stay on the device -------------------------
+namespace {
+
+// Places a group's trees in its arena region: mesh node arrays one after another (each
+// 256-byte aligned, as the host-sized path lays them out), leaves / bounds / codes at each
+// mesh's group-wide leaf_begin. When the group's trees do not fit the region, st->ok = 0
+// and nothing is placed: every node pointer and segment tree is null, so the gather only
+// clears the pooled grid and the traversal reports misses. One wave.
+__global__ __launch_bounds__(64) void k_graph_place(
+    uint32_t G, const BatchMesh *__restrict__ mesh, const BatchTotals *__restrict__ tot,
+    char *node_region, uint64_t node_cap, char *leaves_base, char *bounds_base,
+    char *codes_base, uint64_t leaf_cap, uint32_t **__restrict__ nodes_ptr,
+    uint64_t *__restrict__ node_off, TraverseSeg *__restrict__ segs,
+    GraphState *__restrict__ st) {
+  const uint32_t lane = threadIdx.x;
+  uint64_t carry = 0;
+  for (uint32_t g0 = 0; g0 < G; g0 += 64) {
+    const uint32_t g = g0 + lane;
+    uint64_t bytes = 0;
+    if (g < G && tot[g].leaves) {
+      uint64_t nn = tot[g].n2;
+      for (int L = 3; L <= int(mesh[g].k) + 1; L++) nn += tot[g].nodes[L - 3];
+      bytes = (nn * 12 + 255) / 256 * 256;
+    }
+    uint64_t excl;
+    const uint64_t sum = wave_scan_u64(bytes, &excl);
+    if (g < G) node_off[g] = carry + excl;
+    carry += sum;
+  }
+  const uint64_t leaves = G ? tot[G - 1].leaf_begin + tot[G - 1].leaves : 0;
+  const bool ok = carry <= node_cap && leaves <= leaf_cap;
+  for (uint32_t g = lane; g < G; g += 64) {
+    const BatchTotals &t = tot[g];
+    const bool live = ok && t.leaves != 0;
+    uint32_t *np = live ? reinterpret_cast<uint32_t *>(node_region + node_off[g]) : nullptr;
+    nodes_ptr[g] = np;
+    TraverseSeg &sg = segs[g];
+    sg.nodes = np;
+    sg.leafw = live ? reinterpret_cast<const uint32_t *>(leaves_base + t.leaf_begin * 64) : nullptr;
+    sg.bounds = live ? reinterpret_cast<const uint32_t *>(bounds_base) + t.leaf_begin : nullptr;
+    sg.leaf_count = live ? t.leaves : 0;
+  }
+  if (lane == 0) {
+    st->ok = ok ? 1u : 0u;
+    st->node_bytes = carry;
+    st->leaves = leaves;
+  }
+}
+
+// k_leaf_gather over a level-2 node count that is on the device: a fixed grid whose waves
+// stride over the nodes. The grid words are cleared whether or not the trees fit the arena,
+// so the pooled grid is all-zero for the next replay either way.
+__global__ __launch_bounds__(kTB) void k_leaf_gather_graph(
+    const uint64_t *__restrict__ n2_beg, uint32_t G, const GraphState *__restrict__ st,
+    const unsigned long long *__restrict__ a_mask, const uint32_t *__restrict__ a_base,
+    const uint32_t *__restrict__ a_code, unsigned long long *__restrict__ leafgrid,
+    unsigned long long *__restrict__ brickbits, unsigned long long *__restrict__ leaves,
+    uint32_t *__restrict__ bounds, unsigned long long *__restrict__ codes,
+    const BatchMesh *__restrict__ mesh) {
+  const uint64_t n2 = n2_beg[G];
+  const bool store = st->ok != 0;
+  const uint64_t waves = uint64_t(gridDim.x) * (kTB / 64);
+  for (uint64_t node = (uint64_t(blockIdx.x) * kTB + threadIdx.x) >> 6; node < n2; node += waves)
+    gather_node(node, n2, a_mask, a_base, a_code, leafgrid, brickbits, leaves, bounds, codes,
+                n2_beg, G, mesh, store);
+}
+
+// k_place2_batch over a level-2 node count that is on the device (a fixed, striding grid).
+__global__ __launch_bounds__(kTB) void k_place2_graph(
+    uint32_t G, const uint64_t *__restrict__ n2_beg, const BatchMesh *__restrict__ mesh,
+    const LevelPtrs *__restrict__ lps, const BatchTotals *__restrict__ tot,
+    uint32_t *const *__restrict__ nodes_ptr, const unsigned long long *__restrict__ a_mask,
+    const uint32_t *__restrict__ a_base, const uint32_t *__restrict__ a_code) {
+  const uint64_t N2 = n2_beg[G];
+  const uint64_t stride = uint64_t(gridDim.x) * kTB;
+  for (uint64_t c = uint64_t(blockIdx.x) * kTB + threadIdx.x; c < N2; c += stride)
+    place2_one(c, G, n2_beg, mesh, lps, tot, nodes_ptr, a_mask, a_base, a_code);
+}
+
+uint64_t up256(uint64_t b) { return (b + 255) / 256 * 256; }
+
+// The upper-level arrays of one mesh (levels 3..k+1; level 2 is group-wide), as
+// build_trees_batch carves them.
+uint64_t mesh_levels(const BatchMesh &m, char **p, LevelPtrs *lp) {
+  uint64_t bytes = 0, w = m.w1;
+  if (lp) std::memset(lp, 0, sizeof(LevelPtrs));
+  for (int L = 2; L <= int(m.k) + 1; L++) {
+    const uint64_t win = w;
+    w = div_up(w, 64);
+    if (lp) lp->win[L] = win;
+    if (L < 3) continue;
+    const uint64_t e = win + 1;
+    bytes += 8 * up256(e * 8);
+    if (lp) {
+      lp->cb[L] = carve<uint32_t>(*p, e * 2);
+      lp->ni[L] = carve<uint32_t>(*p, e * 2);
+      lp->a_mask[L] = carve<unsigned long long>(*p, e);
+      lp->a_base[L] = carve<uint32_t>(*p, e * 2);
+      lp->a_code[L] = carve<uint32_t>(*p, e * 2);
+      lp->desc[L] = carve<uint32_t>(*p, e * 2);
+      lp->pos[L] = carve<uint32_t>(*p, e * 2);
+      lp->bs[L] = carve<uint32_t>(*p, e * 2);
+    }
+  }
+  if (lp) lp->tot = carve<uint32_t>(*p, 64);
+  return bytes + up256(64 * 4);
+}
+
+}  // namespace
+
+size_t group_scratch_bytes(const std::vector<BatchMesh> &meshes, uint64_t W1) {
+  size_t need = up256((W1 + 1) * 8) + up256((W1 / 64 + 1) * 8) + up256((W1 + 1) * 8) +
+                2 * up256((W1 + 1) * 4);
+  for (const BatchMesh &m : meshes) need += mesh_levels(m, nullptr, nullptr);
+  return need;
+}
+
+GroupScratch carve_group_scratch(const std::vector<BatchMesh> &meshes, uint64_t W1, char *p,
+                                 LevelPtrs *lps) {
+  GroupScratch gs;
+  gs.pn2 = carve<uint64_t>(p, W1 + 1);
+  gs.b2 = carve<unsigned long long>(p, W1 / 64 + 1);
+  gs.a_mask2 = carve<unsigned long long>(p, W1 + 1);
+  gs.a_base2 = carve<uint32_t>(p, W1 + 1);
+  gs.a_code2 = carve<uint32_t>(p, W1 + 1);
+  for (size_t g = 0; g < meshes.size(); g++) mesh_levels(meshes[g], &p, &lps[g]);
+  return gs;
+}
+
+void enqueue_trees_graph(vxl_ctx *ctx, const GraphGroup &gg, const GroupScratch &gs,
+                         TraverseSeg *segs, hipStream_t s) {
+  const uint32_t G = gg.G;
+  const uint64_t W1 = gg.W1;
+  unsigned long long *b1 = ctx->pool_b1.as<unsigned long long>();
+  const uint32_t cus = uint32_t(ctx->cu_count > 0 ? ctx->cu_count : 256);
+  // 1. level 2 over the whole group, the per-mesh upper levels, the per-mesh sizes
+  hipLaunchKernelGGL(k_level2, dim3(div_up(W1, kTB)), dim3(kTB), 0, s, b1, W1, gs.pn2, gs.b2);
+  scan_exclusive_u64(gs.pn2, gs.pn2, W1, gs.pn2 + W1, ctx->scan_tmp, s);
+  hipLaunchKernelGGL(k_small_levels_batch, dim3(G), dim3(kSmallThreads), 0, s, gg.d_mesh,
+                     gg.d_lps, gs.b2);
+  hipLaunchKernelGGL(k_batch_totals, dim3(div_up(G, 64)), dim3(64), 0, s, gg.d_mesh, gg.d_lps, G,
+                     gs.pn2, ctx->coff.as<uint64_t>(), gg.d_ref, gg.d_tot, gg.n2_beg);
+  // 2. placement in the arena region, decided on the device
+  hipLaunchKernelGGL(k_graph_place, dim3(1), dim3(64), 0, s, G, gg.d_mesh, gg.d_tot,
+                     gg.node_region, gg.node_cap, gg.leaves_base, gg.bounds_base, gg.codes_base,
+                     gg.leaf_cap, gg.nodes_ptr, gg.node_off, segs, gg.state);
+  // 3. level-2 School-A arrays, leaf gather (clears the pooled grid and B1), placement
+  hipLaunchKernelGGL(k_emit_a2, dim3(div_up(W1, kTB)), dim3(kTB), 0, s, b1, W1, gs.pn2,
+                     gs.a_mask2, gs.a_base2, gs.a_code2);
+  hipLaunchKernelGGL(k_leaf_gather_graph, dim3(cus * 8), dim3(kTB), 0, s, gg.n2_beg, G,
+                     gg.state, gs.a_mask2, gs.a_base2, gs.a_code2,
+                     ctx->pool_grid.as<unsigned long long>(), b1,
+                     reinterpret_cast<unsigned long long *>(gg.leaves_base),
+                     reinterpret_cast<uint32_t *>(gg.bounds_base),
+                     reinterpret_cast<unsigned long long *>(gg.codes_base), gg.d_mesh);
+  hipLaunchKernelGGL(k_upper_place_batch, dim3(G), dim3(kSmallThreads), 0, s, gg.d_mesh,
+                     gg.d_lps, gg.d_tot, gg.nodes_ptr);
+  hipLaunchKernelGGL(k_place2_graph, dim3(cus * 4), dim3(kTB), 0, s, G, gg.n2_beg, gg.d_mesh,
+                     gg.d_lps, gg.d_tot, gg.nodes_ptr, gs.a_mask2, gs.a_base2, gs.a_code2);
   VXL_HIP(hipGetLastError());
 }
 

@@ -1209,6 +1209,23 @@ __global__ void k_owner_color(const uint32_t *__restrict__ owner, uint32_t *__re
 }
 }  // namespace
 
+namespace {
+__global__ void k_fill_u64(unsigned long long *__restrict__ p, uint64_t count,
+                           unsigned long long v) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+       i += uint64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+}  // namespace
+
+void fill_u64(unsigned long long *p, uint64_t count, unsigned long long v, hipStream_t s) {
+  if (count == 0) return;
+  uint64_t blocks = (count + 255) / 256;
+  if (blocks > 1024) blocks = 1024;
+  hipLaunchKernelGGL(k_fill_u64, dim3(uint32_t(blocks)), dim3(256), 0, s, p, count, v);
+  VXL_HIP(hipGetLastError());
+}
+
 void owner_colors(vxl_ctx *ctx, const uint32_t *owner, uint32_t *color, uint64_t N) {
   if (N == 0) return;
   uint64_t blocks = (N + 255) / 256;  // grid-stride kernel: at most 2^22 blocks

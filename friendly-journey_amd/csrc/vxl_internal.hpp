@@ -22,6 +22,7 @@
 #include <mutex>
 #include <set>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/vxl.h"
@@ -153,6 +154,31 @@ struct vxl_tree {
   size_t bytes = 0;
   int slab = -1;  // arena slab holding the four arrays (-1: separate device allocations)
   vxl::SharedBlock *shared = nullptr;  // set for trees of a batch: arrays live in `shared`
+};
+
+namespace vxl {
+struct GraphGroup;
+struct TraverseSeg;
+}  // namespace vxl
+
+// A whole batch (setup, SAT, tree build, traversal of every mesh) captured in one HIP graph.
+// The graph runs on a private context whose scratch nothing else touches, so replays see
+// the buffers the capture recorded; the trees go to an arena sized at creation.
+struct vxl_batch_graph {
+  vxl_ctx *ctx = nullptr;  // the caller's context: default launch stream, result trees
+  vxl_ctx *own = nullptr;  // private context (its stream, scratch and pooled grid)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint32_t count = 0;                    // meshes
+  std::vector<vxl::GraphGroup> *groups = nullptr;
+  std::vector<uint32_t> n, k;            // per mesh grid
+  void *desc = nullptr;                  // device descriptors of every group + segments
+  void *arena = nullptr;                 // the trees
+  size_t arena_bytes = 0, desc_bytes = 0;
+  vxl::TraverseSeg *d_segs = nullptr;
+  hipEvent_t done = nullptr;             // recorded after each launch
+  bool launched = false;
+  size_t kernel_nodes = 0;
 };
 
 namespace vxl {
@@ -326,6 +352,20 @@ struct RaySegment {
 };
 // All segments in one kernel launch on stream s.
 void launch_traverse_batch(vxl_ctx *ctx, const RaySegment *segs, uint32_t S, hipStream_t s);
+// One segment of a batched traversal, as the kernel sees it.
+struct TraverseSeg {
+  const uint32_t *nodes, *leafw, *bounds;
+  const float *rays;
+  uint4 *hits;
+  double *t;
+  uint64_t leaf_count, R;
+  uint32_t n, k;
+};
+// The batched traversal kernel over descriptors already in device memory: segment g owns
+// blocks [d_blk[g], d_blk[g+1]) of `blocks` workgroups of ctx->dda_block lanes; kmax is the
+// largest tree level count k of any segment. Nothing but the launch (graph capture uses it).
+void launch_traverse_segs(vxl_ctx *ctx, const TraverseSeg *d_segs, const uint32_t *d_blk,
+                          uint32_t S, uint64_t blocks, uint32_t kmax, hipStream_t s);
 // tree.hip: SparseTree::set_voxel over distinct in-bounds edits on the device; patches
 // leaves in place or, when a brick appears or empties, rebuilds the tree's arrays (the
 // handle stays the same).
@@ -352,6 +392,12 @@ uint32_t check_voxelize_args(const vxl_grid *grid, uint64_t T, float eps);
 VoxelizeParams voxelize_params(const vxl_grid *grid, float eps);
 // tree.hip: an empty tree (one padded entry per array, like upload_structure's padding)
 vxl_tree *empty_tree_device(vxl_ctx *ctx, uint32_t n, uint32_t k);
+// tree.hip: a tree of ctx holding copies (enqueued on s) of device arrays of another
+// (leaves = 0: an empty tree); level_nodes: 16 per-level node counts.
+vxl_tree *tree_copy_arrays(vxl_ctx *ctx, uint32_t n, uint32_t k, uint64_t leaves,
+                           uint64_t nodes, const uint64_t *level_nodes, const void *src_nodes,
+                           const void *src_leaves, const void *src_bounds,
+                           const void *src_codes, hipStream_t s);
 
 // batch.hip
 // Pinned host staging for descriptor uploads: staging_begin waits until earlier uploads
@@ -371,6 +417,71 @@ struct BatchInput {
 };
 void voxelize_batch(vxl_ctx *ctx, const std::vector<BatchInput> &in, vxl_tree **out,
                     uint64_t *pairs);
+// The groups voxelize_batch splits `in` into: [first, end) runs whose pooled leaf grids fit
+// the context's batch budget.
+std::vector<std::pair<size_t, size_t>> batch_groups(const vxl_ctx *ctx,
+                                                    const std::vector<BatchInput> &in);
+// The group descriptors (BatchMesh, setup-block starts) of in[first, end), as run_group
+// uploads them; returns the group's setup blocks and brick-bitmap words.
+void batch_group_meshes(const BatchInput *in, uint32_t G, std::vector<BatchMesh> &meshes,
+                        std::vector<uint32_t> &sb_begin, uint64_t *blocks, uint64_t *W1);
+
+// A batch group of a graph (batch.hip builds it, tree.hip enqueues its tree build). Device
+// descriptors are the graph's own and stay put between replays; the level arrays and the
+// group-wide scratch come from the graph's private context.
+struct GraphState {  // written on the device by each replay, per group
+  uint32_t ok;       // 1: the group's trees fit its arena region
+  uint32_t pad;
+  uint64_t node_bytes, leaves;  // what the group's trees needed
+};
+struct GraphGroup {
+  uint32_t first = 0, G = 0;     // meshes [first, first + G) of the batch
+  uint32_t setup_blocks = 0;
+  uint64_t W1 = 0;               // brick-bitmap words of the group
+  std::vector<BatchMesh> meshes;
+  // device, graph-owned
+  BatchMesh *d_mesh = nullptr;
+  uint32_t *d_sb = nullptr;
+  unsigned long long *d_ref = nullptr;
+  LevelPtrs *d_lps = nullptr;
+  BatchTotals *d_tot = nullptr;
+  uint64_t *n2_beg = nullptr;    // G + 1
+  uint32_t **nodes_ptr = nullptr;
+  uint64_t *node_off = nullptr;  // byte offset of mesh g's nodes in node_region
+  GraphState *state = nullptr;
+  // the group's arena region
+  char *node_region = nullptr;
+  uint64_t node_cap = 0;         // bytes
+  char *leaves_base = nullptr, *bounds_base = nullptr, *codes_base = nullptr;
+  uint64_t leaf_cap = 0;         // leaves
+};
+// batch.hip: the graph of a batch (vxl_batch_graph_create): every mesh of `in` (in[g].index
+// = g, 2 <= k <= 4); rays [roff[g], roff[g+1]) of d_rays walk mesh g's tree. Launch enqueues
+// a replay on s; result copies the last replay's trees (trees and pairs nullable) or throws
+// VXL_STORAGE_EXCEEDED; release frees what build made (a partly built graph too).
+void batch_graph_build(vxl_batch_graph *bg, const std::vector<BatchInput> &in,
+                       const float *d_rays, const uint64_t *roff, uint32_t *d_hits,
+                       double *d_t, double headroom);
+void batch_graph_launch(vxl_batch_graph *bg, hipStream_t s);
+void batch_graph_result(vxl_batch_graph *bg, vxl_tree **trees, uint64_t *pairs);
+void batch_graph_release(vxl_batch_graph *bg);
+// tree.hip: the group-wide scratch of a graph group's tree build (level-2 scan and
+// bitmap, School-A level-2 arrays) and every mesh's upper-level arrays, carved from one
+// buffer of group_scratch_bytes(meshes, W1) bytes; lps receives one LevelPtrs per mesh.
+struct GroupScratch {
+  uint64_t *pn2;
+  unsigned long long *b2, *a_mask2;
+  uint32_t *a_base2, *a_code2;
+};
+size_t group_scratch_bytes(const std::vector<BatchMesh> &meshes, uint64_t W1);
+GroupScratch carve_group_scratch(const std::vector<BatchMesh> &meshes, uint64_t W1, char *p,
+                                 LevelPtrs *lps);
+// tree.hip: the group's tree build with the sizes left on the device: kernels only, trees
+// placed in the group's arena region by the device, segs[first + g] given mesh g's tree.
+void enqueue_trees_graph(vxl_ctx *ctx, const GraphGroup &gg, const GroupScratch &gs,
+                         TraverseSeg *segs, hipStream_t s);
+// voxelize.hip: a u64 fill kernel (graphs clear counters with a kernel, not a memset node).
+void fill_u64(unsigned long long *p, uint64_t count, unsigned long long v, hipStream_t s);
 
 void tree_free_device(vxl_tree *t);
 

@@ -38,6 +38,8 @@ EXPORTS = [
     "vxl_tree_from_compact", "vxl_tree_materials_download", "vxl_render", "vxl_render_dev",
     "vxl_tree_set_voxels", "vxl_generate_noise_tree", "vxl_traverse_stats",
     "vxl_traverse_graph_create", "vxl_graph_launch", "vxl_graph_destroy",
+    "vxl_batch_graph_create", "vxl_batch_graph_launch", "vxl_batch_graph_trees",
+    "vxl_batch_graph_info", "vxl_batch_graph_destroy",
     "vxl_tree_from_words", "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload",
     "vxl_tree_free", "vxl_traverse", "vxl_traverse_dev", "vxl_traverse_dev_noalloc",
     "vxl_set_timing", "vxl_last_timing",
@@ -128,6 +130,13 @@ def load(path: str = LIB_PATH):
     L.vxl_graph_launch.argtypes = [_vp, _vp]
     L.vxl_graph_destroy.argtypes = [_vp]
     L.vxl_graph_destroy.restype = None
+    L.vxl_batch_graph_create.argtypes = [_vp, _vp, _pu64, C.c_uint32, _vp, C.c_float, _vp,
+                                         _pu64, _vp, _vp, C.c_double, C.POINTER(_vp)]
+    L.vxl_batch_graph_launch.argtypes = [_vp, _vp]
+    L.vxl_batch_graph_trees.argtypes = [_vp, _vp, _pu64]
+    L.vxl_batch_graph_info.argtypes = [_vp, _pu64, C.POINTER(C.c_uint32), _pu64]
+    L.vxl_batch_graph_destroy.argtypes = [_vp]
+    L.vxl_batch_graph_destroy.restype = None
     L.vxl_generate_noise_tree.argtypes = [_vp, C.c_uint32, C.POINTER(Noise), C.POINTER(_vp)]
     L.vxl_tree_set_voxels.argtypes = [_vp, _vp, _vp, _u64, _pu64, C.POINTER(C.c_int)]
     L.vxl_render.argtypes = [_vp, _vp, C.POINTER(Camera), _vp, C.c_uint32, _vp]
@@ -184,6 +193,8 @@ def load(path: str = LIB_PATH):
                  "vxl_render", "vxl_render_dev", "vxl_tree_set_voxels",
                  "vxl_generate_noise_tree", "vxl_traverse_stats",
                  "vxl_traverse_graph_create", "vxl_graph_launch",
+                 "vxl_batch_graph_create", "vxl_batch_graph_launch", "vxl_batch_graph_trees",
+                 "vxl_batch_graph_info",
                  "vxl_voxelize_tree", "vxl_voxelize_tree_dev", "vxl_tree_from_words",
                  "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload", "vxl_traverse",
                  "vxl_traverse_dev", "vxl_traverse_dev_noalloc", "vxl_set_tuning", "vxl_set_dda_refill", "vxl_voxelize_batch",
@@ -226,6 +237,50 @@ class TraverseGraph:
     def free(self):
         if self._h:
             load().vxl_graph_destroy(_vp(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class BatchGraph:
+    """A whole batch (voxelize every mesh, build its tree, walk its rays) as one HIP graph
+    with no host round trip; launch() replays it, trees() copies the last replay's trees."""
+
+    def __init__(self, handle: int, ctx, count: int):
+        self._h = handle
+        self._ctx = ctx
+        self.count = count
+
+    def launch(self, stream: int = 0):
+        _check(load().vxl_batch_graph_launch(_vp(self._h), _vp(stream)))
+
+    def trees(self, want_trees=True):
+        """Waits for the last launch; returns (trees or None, pairs). Raises VxlError
+        (VXL_STORAGE_EXCEEDED) when a group's trees did not fit the arena."""
+        out = (_vp * self.count)() if want_trees else None
+        pairs = np.zeros(self.count, np.uint64)
+        _check(load().vxl_batch_graph_trees(_vp(self._h), out, pairs.ctypes.data_as(_pu64)))
+        if not want_trees:
+            return None, pairs
+        trees = []
+        for i in range(self.count):
+            t = Tree(out[i], self._ctx)
+            t.pairs = int(pairs[i])
+            trees.append(t)
+        return trees, pairs
+
+    def info(self):
+        a, g, n = C.c_uint64(0), C.c_uint32(0), C.c_uint64(0)
+        _check(load().vxl_batch_graph_info(_vp(self._h), C.byref(a), C.byref(g), C.byref(n)))
+        return dict(arena_bytes=int(a.value), groups=int(g.value), graph_nodes=int(n.value))
+
+    def free(self):
+        if self._h:
+            load().vxl_batch_graph_destroy(_vp(self._h))
             self._h = None
 
     def __del__(self):
@@ -629,6 +684,22 @@ class Context:
         _check(load().vxl_traverse_batch_dev(self.handle, handles, count, _vp(d_rays), _ptr(off),
                                              _vp(d_hits), _vp(d_t) if d_t else None,
                                              _vp(stream)))
+
+    def batch_graph(self, d_tris: int, tri_offsets, grids, d_rays: int = 0, ray_offsets=None,
+                    d_hits: int = 0, d_t: int = 0, eps=1e-4, headroom=0.0):
+        """vxl_batch_graph_create: the batch (meshes at tri_offsets of d_tris, rays at
+        ray_offsets of d_rays against them) captured in one HIP graph."""
+        off = np.ascontiguousarray(tri_offsets, np.uint64)
+        count = len(off) - 1
+        roff = np.ascontiguousarray(ray_offsets, np.uint64) if ray_offsets is not None else None
+        out = _vp()
+        _check(load().vxl_batch_graph_create(
+            self.handle, _vp(d_tris) if d_tris else None, off.ctypes.data_as(_pu64), count,
+            self._grids(grids), eps, _vp(d_rays) if d_rays else None,
+            roff.ctypes.data_as(_pu64) if roff is not None else None,
+            _vp(d_hits) if d_hits else None, _vp(d_t) if d_t else None, float(headroom),
+            C.byref(out)))
+        return BatchGraph(out.value, self, count)
 
     def set_batch_budget(self, nbytes: int):
         _check(load().vxl_set_batch_budget(self.handle, int(nbytes)))

@@ -262,6 +262,36 @@ vxl_status vxl_traverse_graph_create(vxl_ctx *ctx, const vxl_tree *const *trees,
 vxl_status vxl_graph_launch(vxl_graph *graph, void *stream);
 void vxl_graph_destroy(vxl_graph *graph);
 
+/* ---- a whole batch as one HIP graph: every mesh voxelized, its tree built and its rays
+ *      walked, with no host round trip (C4's hundreds of meshes and uneven ray batches as
+ *      one replayable submission; no reference counterpart: the reference voxelizes mesh by
+ *      mesh, crates/voxel-cli/src/main.rs:1288-1300, and traverses each tree on its own,
+ *      main.rs:785-792). Creation runs the batch once on the current triangles to size an
+ *      arena for the trees (headroom x their size; 0 means 1.5), then captures, for every
+ *      batch group, setup, SAT and tree build -- the tree sizes stay on the device, which
+ *      places each tree in the group's arena region -- and one traversal kernel over all
+ *      segments: rays [ray_offsets[g], ray_offsets[g+1]) of d_rays against mesh g's tree,
+ *      hits to d_hits (4 u32 per ray) and t to d_t (nullable); ray_offsets null: no rays.
+ *      Every mesh needs n in 128..2048 (k = 2..4). Each replay reads d_tris and d_rays and
+ *      writes d_hits and d_t: their contents may change between replays, their sizes and
+ *      the grids may not. Replays use the graph's own scratch: they must not overlap one
+ *      another (launch them on one stream, or wait). vxl_batch_graph_trees, after a launch:
+ *      waits for it and returns copies of its trees (owned by ctx; trees and pairs
+ *      nullable), or VXL_STORAGE_EXCEEDED when a group's trees did not fit its region (the
+ *      hits of that group's meshes are then misses; a later replay whose trees fit is whole
+ *      again). vxl_batch_graph_info: the arena's bytes, the batch groups, the graph's nodes. */
+typedef struct vxl_batch_graph vxl_batch_graph;
+vxl_status vxl_batch_graph_create(vxl_ctx *ctx, const float *d_tris, const uint64_t *tri_offsets,
+                                  uint32_t count, const vxl_grid *grids, float epsilon,
+                                  const float *d_rays, const uint64_t *ray_offsets,
+                                  uint32_t *d_hits, double *d_t, double headroom,
+                                  vxl_batch_graph **out);
+vxl_status vxl_batch_graph_launch(vxl_batch_graph *graph, void *stream);
+vxl_status vxl_batch_graph_trees(vxl_batch_graph *graph, vxl_tree **trees, uint64_t *pairs);
+vxl_status vxl_batch_graph_info(const vxl_batch_graph *graph, uint64_t *arena_bytes,
+                                uint32_t *groups, uint64_t *graph_nodes);
+void vxl_batch_graph_destroy(vxl_batch_graph *graph);
+
 /* ---- walk statistics of a ray batch (the reference's measurement counters,
  *      voxel-core/src/measure.rs:189-235, over the mirror's walk mirror.rs:263-326): sums
  *      over the rays of descents into leaves, descents into nodes and cell visits, in
