@@ -461,16 +461,43 @@ def bench_batch(ctx, torch, W, reps=3):
         torch.cuda.synchronize()
         graph_ms.append((time.perf_counter() - t0) * 1e3)
     graph.free()
+    # the whole batch -- voxelize, tree build and traversal of every mesh -- captured once
+    # in a HIP graph (tree sizes stay on the device) and replayed
+    log("batch: whole-batch graph")
+    hg = torch.empty_like(hp)
+    tg = torch.empty_like(tp)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bgraph = ctx.batch_graph(tp0, toff, grids, rp0, roff, hg.data_ptr(), tg.data_ptr())
+    create_ms = (time.perf_counter() - t0) * 1e3
+    ginfo = bgraph.info()
+    bgraph.launch()
+    torch.cuda.synchronize()
+    bg_ms = []
+    for _ in range(reps):
+        hg.fill_(-1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        bgraph.launch()
+        torch.cuda.synchronize()
+        bg_ms.append((time.perf_counter() - t0) * 1e3)
+    gtrees, _ = bgraph.trees()
+    bgraph.free()
     same_hits = bool(torch.equal(hp, hb)) and bool(torch.equal(tp, tb))
-    same_trees = True
-    for i, (a, b) in enumerate(zip(keep_p, keep_b)):
+    graph_hits = bool(torch.equal(hg, hb)) and bool(torch.equal(tg, tb))
+    same_trees = graph_trees = True
+    for i, (a, b, c) in enumerate(zip(keep_p, keep_b, gtrees)):
         if (a.leaf_count, a.node_count, a.pairs) != (b.leaf_count, b.node_count, b.pairs):
             same_trees = False
+        if (c.leaf_count, c.node_count, c.pairs) != (b.leaf_count, b.node_count, b.pairs):
+            graph_trees = False
         elif i % 8 == 0:
-            for x, y in zip(a.download(), b.download()):
+            for x, y, z in zip(a.download(), b.download(), c.download()):
                 same_trees = same_trees and np.array_equal(x, y)
-    for t in keep_p + keep_b:
+                graph_trees = graph_trees and np.array_equal(z, y)
+    for t in keep_p + keep_b + gtrees:
         t.free()
+    del hg, tg
     med = {k: sorted(v)[len(v) // 2] for k, v in res.items()}
     return dict(
         workload=(f"{len(plan)} meshes (T log-U[100, 2e5], sphere/soup coin, m in "
@@ -487,6 +514,17 @@ def bench_batch(ctx, torch, W, reps=3):
                   "batched_ms": round(med["batched_trav"], 2),
                   "speedup": round(med["plain_trav"] / med["batched_trav"], 3),
                   "batched_mrays_per_s": round(R / med["batched_trav"] / 1e3, 1)},
+        whole_batch_graph={
+            "ms": round(sorted(bg_ms)[len(bg_ms) // 2], 2),
+            "speedup_vs_plain": round(med["plain"] / sorted(bg_ms)[len(bg_ms) // 2], 3),
+            "speedup_vs_batched": round(med["batched"] / sorted(bg_ms)[len(bg_ms) // 2], 3),
+            "groups": ginfo["groups"], "graph_nodes": ginfo["graph_nodes"],
+            "arena_mb": round(ginfo["arena_bytes"] / 2**20, 1),
+            "create_ms": round(create_ms, 1),
+            "identical_trees": graph_trees, "identical_hits": graph_hits,
+            "note": ("setup, SAT and tree build of every group and one traversal kernel over "
+                     "all segments, replayed with no host round trip; trees compared with "
+                     "the batched calls' (counts for all, arrays for every 8th mesh)")},
         tuning=tune, identical_trees=same_trees, identical_hits=same_hits,
         timing="wall clock, median of %d, inputs resident in HBM" % reps)
 
