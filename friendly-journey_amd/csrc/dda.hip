@@ -134,6 +134,8 @@ struct Walk {
   float tm0, tm1, tm2, td0, td1, td2, te;
   uint32_t mlo, mhi, cbase;
   uint32_t root_level, iter;
+  uint32_t lbad;  // leaf frame: per axis (4 bits each, x low) 1 + the first cell past the
+                  // leaf's occupied bounds in the ray's direction (15: the axis never steps)
 };
 
 __device__ __forceinline__ void load_ray(const float *__restrict__ rays, uint64_t ri, Ray &r) {
@@ -190,7 +192,8 @@ __device__ __forceinline__ void make_frame(const Ray &r, Walk &w, uint32_t level
 // stepping along different axes do not split the wave.
 // The fields are read into locals first: a conditional expression over struct fields would
 // become a load from a selected address, which keeps the walk state out of registers.
-__device__ __forceinline__ bool step_frame(const Ray &r, Walk &w) {
+template <bool kSkip>
+__device__ __forceinline__ bool step_frame(const Ray &r, Walk &w, bool &past) {
   const uint32_t dim = w.level == 1 ? 8u : 4u;
   const float tm0 = w.tm0, tm1 = w.tm1, tm2 = w.tm2;
   const float d0 = r.d[0], d1 = r.d[1], d2 = r.d[2];
@@ -213,8 +216,32 @@ __device__ __forceinline__ bool step_frame(const Ray &r, Walk &w) {
     w.tm1 = on1 ? tm1 + td1 : tm1;
     w.tm2 = a2 ? tm2 + td2 : tm2;
     w.te = best;
+    // the stepped axis's new cell against the leaf's bound on it: past it, every later cell
+    // of the frame is too (the walk is monotone along each axis)
+    if (kSkip) {
+      const uint32_t sh = a2 ? 8u : (a1 ? 4u : 0u);
+      past = w.level == 1 && ((w.lbad >> sh) & 15u) == ca + inc + 1u;
+    }
   }
   return ok;
+}
+
+// The leaf frame just entered: lbad from the leaf's packed bounds (LeafBounds), and whether
+// its entry cell already lies past them on some axis (then no cell of the frame is occupied).
+__device__ __forceinline__ bool leaf_limits(const Ray &r, Walk &w, uint32_t packed) {
+  const uint32_t c[3] = {w.cx, w.cy, w.cz};
+  uint32_t bad = 0;
+  bool past = false;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const uint32_t mn = (packed >> (3 * a)) & 7u, mx = (packed >> (9 + 3 * a)) & 7u;
+    const float d = r.d[a];
+    const uint32_t f = d > 0.0f ? mx + 2u : (d < 0.0f ? mn : 15u);
+    bad |= f << (4 * a);
+    past = past || (d > 0.0f ? c[a] > mx : (d < 0.0f ? c[a] < mn : (c[a] < mn || c[a] > mx)));
+  }
+  w.lbad = bad;
+  return past;
 }
 
 // The 4^3 sub-brick of a leaf-frame cell: the high bits of the intra-brick Morton index.
@@ -245,6 +272,7 @@ __device__ __forceinline__ void begin_walk(const Ray &r, Walk &w, uint32_t k, fl
   w.ox = w.oy = w.oz = 0;
   w.iter = 0;
   make_frame(r, w, w.level, fmaxf(t0, 0.0f));
+  w.lbad = 0xFFFu;
   if (w.level >= 2) load_node(w, nodes);
   else load_leaf_word(w, leafw);
 }
@@ -254,6 +282,9 @@ constexpr uint32_t kWalking = 0, kHit = 1, kLeft = 2, kCapped = 3;
 // Bit of Walk::iter set while the active frame was just restored by an ascent: its next
 // visit steps it without testing its cell again (ascend, mirror.rs:155-165).
 constexpr uint32_t kPendingStep = 1u << 31;
+// Bit of Walk::iter set while the active leaf frame was entered at a cell past the leaf's
+// occupied bounds: its next visit pops it without testing or stepping.
+constexpr uint32_t kLeafPast = 1u << 30;
 
 // One iteration of the walk (mirror.rs:284-325): test the active cell against the frame's
 // scan mask (a node's child mask, or the leaf word of the cell's sub-brick, both in
@@ -263,17 +294,23 @@ constexpr uint32_t kPendingStep = 1u << 31;
 // its next iteration, as the mirror's ascend steps the parent. Popping one level per
 // iteration (rather than looping until some ancestor steps) keeps each iteration's code
 // short and lets the other lanes of the wave walk on while one climbs several levels.
+// With kSkip a leaf frame is also popped as soon as the walk leaves the leaf's occupied
+// bounds (LeafBounds) along the axis it steps on, or enters past them: every cell it would
+// still visit is empty, so the mirror would only step through them and pop the frame at its
+// end; the hit is the same, with fewer visits and leaf-word loads. It pays where lanes wait
+// on leaf loads from HBM (the persistent-lane walk of trees beyond the Infinity Cache); on
+// cache-resident trees the extra instructions of every step cost more than the visits saved.
 // `frames` is this lane's column of the LDS frame rows, `fstride` the row pitch. kStats
 // counts the descents into leaves and into nodes and the visits (cell tests; the mirror's
-// walk statistics).
-template <bool kStats = false>
+// walk statistics), so it walks every cell as the mirror does.
+template <bool kStats = false, bool kSkip = false>
 __device__ __forceinline__ uint32_t walk_visit(const Ray &r, Walk &w,
                                                const uint32_t *__restrict__ nodes,
                                                const uint32_t *__restrict__ leafw,
                                                const uint32_t *__restrict__ bounds,
                                                uint32_t *frames, uint32_t fstride,
                                                uint32_t *stats = nullptr) {
-  if (!(w.iter & kPendingStep)) {
+  if (!(w.iter & (kSkip ? kPendingStep | kLeafPast : kPendingStep))) {
     if (kStats) stats[2] += 1;
     const uint32_t mlo = w.mlo, mhi = w.mhi;
     const uint32_t bit = dev::child_bit(w.cx & 3u, w.cy & 3u, w.cz & 3u);
@@ -285,7 +322,8 @@ __device__ __forceinline__ uint32_t walk_visit(const Ray &r, Walk &w,
       const uint32_t below = bit < 32 ? __popc(mlo & ((1u << bit) - 1u))
                                       : __popc(mlo) + __popc(mhi & ((1u << (bit - 32)) - 1u));
       const uint32_t slot = w.cbase + below;
-      const bool descend = w.level != 2 || leaf_reaches(bounds[slot], r, cox, coy, coz, w.te);
+      const uint32_t lb = w.level == 2 ? bounds[slot] : kFull;
+      const bool descend = w.level != 2 || leaf_reaches(lb, r, cox, coy, coz, w.te);
       if (descend) {
         if (++w.iter > kMaxDescents) return kCapped;
         {
@@ -303,17 +341,27 @@ __device__ __forceinline__ uint32_t walk_visit(const Ray &r, Walk &w,
         w.oy = coy;
         w.oz = coz;
         make_frame(r, w, w.level, w.te);
-        if (w.level >= 2) load_node(w, nodes);
-        else load_leaf_word(w, leafw);
+        if (w.level >= 2) {
+          load_node(w, nodes);
+        } else if (kStats || !kSkip) {
+          w.lbad = 0xFFFu;
+          load_leaf_word(w, leafw);
+        } else if (leaf_limits(r, w, lb)) {
+          w.iter |= kLeafPast;
+        } else {
+          load_leaf_word(w, leafw);
+        }
         return kWalking;
       }
     }
   }
   w.iter &= ~kPendingStep;
-  if (step_frame(r, w)) {
+  bool past = false;
+  if (!(kSkip && (w.iter & kLeafPast)) && step_frame<kSkip>(r, w, past) && !past) {
     if (w.level == 1 && sub_brick(w.cx, w.cy, w.cz) != w.cbase) load_leaf_word(w, leafw);
     return kWalking;
   }
+  if (kSkip) w.iter &= ~kLeafPast;
   if (w.level == w.root_level) return kLeft;
   w.level += 1;
   const uint32_t size = frame_cell_size(w.level);
@@ -514,7 +562,7 @@ __global__ __launch_bounds__(64) void k_traverse_pl(
     const uint32_t until = exhausted ? 64u : refill_min;
     for (;;) {
       if (active) {
-        const uint32_t status = walk_visit(r, w, nodes, leafw, bounds, frames, 64);
+        const uint32_t status = walk_visit<false, true>(r, w, nodes, leafw, bounds, frames, 64);
         if (status != kWalking) {
           store_result(ri, r, w, status, hits, nullptr);
           active = false;
@@ -543,6 +591,8 @@ __device__ __forceinline__ uint32_t rgba8(float r, float g, float b, float a) {
 // render.wgsl:74-89: a hit takes its voxel's material (0 without materials), shaded by
 // position (voxel / n) for material 0 or by colors[material] (RGBA8, R in the low byte;
 // an id past the table reads 0); a miss takes the sky gradient. Output RGBA8 per pixel.
+// kSkip: the leaf-bounds pop of walk_visit (trees beyond the Infinity Cache).
+template <bool kSkip>
 __global__ __launch_bounds__(64) void k_render(vxl_camera cam, const uint32_t *__restrict__ nodes,
                                                const uint32_t *__restrict__ leafw,
                                                const uint32_t *__restrict__ bounds, uint32_t n,
@@ -577,7 +627,7 @@ __global__ __launch_bounds__(64) void k_render(vxl_camera cam, const uint32_t *_
   if (clip_grid(r, n, leaf_count, t0)) {
     begin_walk(r, wk, k, t0, nodes, leafw);
     do {
-      status = walk_visit(r, wk, nodes, leafw, bounds, frames_lds + threadIdx.x, 64);
+      status = walk_visit<false, kSkip>(r, wk, nodes, leafw, bounds, frames_lds + threadIdx.x, 64);
     } while (status == kWalking);
   }
   uint32_t color;
@@ -714,7 +764,9 @@ void launch_render(vxl_ctx *ctx, const vxl_tree *tree, const vxl_camera &cam,
                    const uint32_t *d_colors, uint32_t ncolors, uint32_t *d_rgba, hipStream_t s) {
   if (cam.width == 0 || cam.height == 0) return;
   const dim3 grid((cam.width + 7) / 8, (cam.height + 7) / 8, 1);
-  hipLaunchKernelGGL(k_render, grid, dim3(64), frame_lds_bytes(tree->k, 64), s, cam, tree->nodes,
+  const bool skip = tree->leaf_count * 68 + tree->node_count * 12 > kCacheBytes;
+  hipLaunchKernelGGL(skip ? k_render<true> : k_render<false>, grid, dim3(64),
+                     frame_lds_bytes(tree->k, 64), s, cam, tree->nodes,
                      reinterpret_cast<const uint32_t *>(tree->leaves), tree->bounds, tree->n,
                      tree->k, tree->leaf_count, tree->materials, d_colors, ncolors, d_rgba);
   VXL_HIP(hipGetLastError());

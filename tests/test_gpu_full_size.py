@@ -153,6 +153,18 @@ def test_soup_2048_one_lane_per_ray(ctx, soup_tree):
         ctx.set_dda_refill(-1)
 
 
+def test_soup_2048_render(ctx, soup_tree):
+    """The render path on the 0.97 GB tree (the walk pops leaf frames at their occupied
+    bounds there): the oracle's image byte for byte (render.wgsl:54-91)."""
+    gt, _, ot = soup_tree
+    cam = W.render_camera(2048, 2048)
+    _say("soup_2048 render 320x180")
+    got = ctx.render(gt, cam, 320, 180)
+    want = O.render(ot, cam, 320, 180, None, None)
+    bad = got != want
+    assert not bad.any(), f"{int(bad.sum())} pixels differ, first {np.argwhere(bad)[:3]}"
+
+
 def _many_mesh_subset():
     """Every resolution of the many-mesh workload x {sphere, soup}, triangle counts drawn
     log-uniform in [100, 200,000] as there, plus two more of each kind at 1024."""
