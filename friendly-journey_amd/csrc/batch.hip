@@ -186,6 +186,7 @@ void batch_graph_build(vxl_batch_graph *bg, const std::vector<BatchInput> &in,
   vxl_ctx *own = bg->own;
   own->sat_waves_per_cu = ctx->sat_waves_per_cu;
   own->sat_units_per_wave = ctx->sat_units_per_wave;
+  own->sat_merge_words = ctx->sat_merge_words;
   own->dda_block = ctx->dda_block;
   own->pool_budget = ctx->pool_budget;
   for (const BatchInput &b : in) {

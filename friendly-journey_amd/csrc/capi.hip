@@ -635,6 +635,13 @@ vxl_status vxl_set_tuning(vxl_ctx *ctx, int sat_waves_per_cu, int sat_units_per_
   });
 }
 
+vxl_status vxl_set_merge_words(vxl_ctx *ctx, int words) {
+  return guarded(ctx, [&] {
+    if (words < 1 || words > 64) throw Error(VXL_INVALID_ARG, "merge words must be in [1, 64]");
+    ctx->sat_merge_words = words;
+  });
+}
+
 vxl_status vxl_set_dda_refill(vxl_ctx *ctx, int refill_min) {
   return guarded(ctx, [&] {
     if (refill_min < -1 || refill_min > 64)
@@ -1508,7 +1515,9 @@ vxl_status vxl_autotune_batch(vxl_ctx *ctx, const float *d_tris, const uint64_t 
     const int waves[] = {16, 20, 24, 28, 32};
     const int units[] = {1, 2, 4};
     const int blocks[] = {64, 128, 256};
+    const int merge[] = {16, 32, 48, 64};
     const int keep_w = ctx->sat_waves_per_cu, keep_u = ctx->sat_units_per_wave;
+    const int keep_m = ctx->sat_merge_words;
     try {
       time_voxelize();  // warm: scratch and pools reach their size
       float best = 1e30f;
@@ -1526,8 +1535,19 @@ vxl_status vxl_autotune_batch(vxl_ctx *ctx, const float *d_tris, const uint64_t 
         }
       ctx->sat_waves_per_cu = bw;
       ctx->sat_units_per_wave = bu;
+      int bm = keep_m;
+      for (int m : merge) {
+        ctx->sat_merge_words = m;
+        const float ms = time_voxelize();
+        if (ms < best) {
+          best = ms;
+          bm = m;
+        }
+      }
+      ctx->sat_merge_words = bm;
       result->sat_waves_per_cu = bw;
       result->sat_units_per_wave = bu;
+      result->sat_merge_words = bm;
       result->voxelize_ms = time_voxelize();
       result->dda_block = ctx->dda_block;
       result->traverse_ms = 0;
@@ -1560,6 +1580,7 @@ vxl_status vxl_autotune_batch(vxl_ctx *ctx, const float *d_tris, const uint64_t 
     } catch (...) {
       ctx->sat_waves_per_cu = keep_w;
       ctx->sat_units_per_wave = keep_u;
+      ctx->sat_merge_words = keep_m;
       (void)hipStreamSynchronize(ctx->stream);
       for (auto &t : trees)
         if (t) {

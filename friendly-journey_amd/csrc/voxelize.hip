@@ -679,9 +679,13 @@ constexpr int kWaves = 4;    // waves per block
 constexpr int kQueue = 128;  // survivor queue entries per wave
 constexpr int kTable = 128;  // leaf-word merge table slots per wave
 constexpr int kTableBits = 7;
-constexpr int kFlushAt = 64;  // flush the table once this many slots are in use
+// The table is flushed once `flush_at` slots are in use (a launch argument, 1..kFlushMax):
+// an insert adds at most 64 keys, so at most kFlushMax - 1 + 64 < kTable slots are ever in
+// use and a probe always finds a free or matching slot.
+constexpr int kFlushMax = 64;
 constexpr unsigned long long kEmpty = ~0ull;
 static_assert(kTable == 1 << kTableBits, "the slot hash yields kTableBits bits");
+static_assert(kFlushMax - 1 + 64 < kTable, "a flush threshold of kFlushMax keeps a free slot");
 
 // Merge-table key of a hit voxel: its mesh's grid base and its 4^3 sub-brick (one leaf-grid
 // word) as packed coordinates, sx | sy << 11 | sz << 22 | grid_base << 33. Sub-brick
@@ -822,7 +826,7 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))
     const uint32_t *__restrict__ ustart, uint64_t T, SatPlan *__restrict__ plan,
     unsigned long long *__restrict__ leafgrid, unsigned long long *__restrict__ brickbits,
     uint32_t *__restrict__ owner, const uint32_t *__restrict__ leaf_map,
-    uint32_t own_n) {
+    uint32_t own_n, uint32_t flush_at) {
   __shared__ uint32_t q_t[kWaves][kQueue], q_xy[kWaves][kQueue], q_z[kWaves][kQueue];
   __shared__ unsigned long long t_key[kWaves][kTable], t_val[kWaves][kTable];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1024,7 +1028,7 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))
       if (kOwn != kOwnNone && hit) owner_min<kOwn>(owner, leaf_map, own_n, qt[e], qxy[e], qz[e]);
       if (kOwn != kOwnLeaf) {
         used += table_insert(tk, tv, hit, key, bit);
-        if (used >= kFlushAt) {
+        if (used >= flush_at) {
           table_flush(tk, tv, lane, leafgrid, brickbits);
           used = 0;
         }
@@ -1058,6 +1062,11 @@ namespace {
 // The Tp window records follow the Tp triangle records in ctx->rec.
 WinRec *win_records(vxl_ctx *ctx, uint64_t Tp) {
   return reinterpret_cast<WinRec *>(ctx->rec.as<char>() + Tp * sizeof(TriRec));
+}
+
+uint32_t merge_words(const vxl_ctx *ctx) {
+  const int w = ctx->sat_merge_words;
+  return uint32_t(w < 1 ? 1 : (w > kFlushMax ? kFlushMax : w));
 }
 
 uint64_t plan_want(vxl_ctx *ctx, uint64_t *max_waves) {
@@ -1112,13 +1121,13 @@ void sat_after_setup(vxl_ctx *ctx, uint64_t Tp, unsigned long long *leafgrid,
   const dim3 g(static_cast<uint32_t>(blocks), 1, 1), b(kWaves * 64, 1, 1);
   if (owner && leaf_map)
     hipLaunchKernelGGL((k_sat<kOwnLeaf>), g, b, 0, s, rec, wrec, coff, ustart, Tp, plan, leafgrid,
-                       brickbits, owner, leaf_map, own_n);
+                       brickbits, owner, leaf_map, own_n, merge_words(ctx));
   else if (owner)
     hipLaunchKernelGGL((k_sat<kOwnDense>), g, b, 0, s, rec, wrec, coff, ustart, Tp, plan, leafgrid,
-                       brickbits, owner, nullptr, own_n);
+                       brickbits, owner, nullptr, own_n, merge_words(ctx));
   else
     hipLaunchKernelGGL((k_sat<kOwnNone>), g, b, 0, s, rec, wrec, coff, ustart, Tp, plan, leafgrid,
-                       brickbits, nullptr, nullptr, 0u);
+                       brickbits, nullptr, nullptr, 0u, merge_words(ctx));
   mark(ctx, 2);
   VXL_HIP(hipGetLastError());
 }
@@ -1163,7 +1172,7 @@ void sat_owner_pass(vxl_ctx *ctx, uint64_t T, uint32_t *owner, const uint32_t *l
   hipLaunchKernelGGL((k_sat<kOwnLeaf>), g, b, 0, s, ctx->rec.as<TriRec>(), win_records(ctx, T),
                      ctx->coff.as<uint64_t>(), ustart, T, plan,
                      ctx->leafgrid.as<unsigned long long>(), ctx->bitmap.as<unsigned long long>(),
-                     owner, leaf_map, own_n);
+                     owner, leaf_map, own_n, merge_words(ctx));
   VXL_HIP(hipGetLastError());
 }
 

@@ -109,6 +109,8 @@ struct vxl_ctx {
   uint32_t sat_epoch = 0;  // setup + SAT calls enqueued (numbers the plan's parity slots)
   int sat_waves_per_cu = 32;  // resident SAT waves per CU (8 per SIMD; persistent, work-queue fed)
   int sat_units_per_wave = 1;  // work units per resident wave
+  int sat_merge_words = 64;  // leaf-grid words a SAT wave merges in LDS before it writes them
+                             // out (1..64: the 128-slot table takes 64 more per insert)
   int dda_refill = -1;  // > 0: persistent traversal lanes, refilled when that many are
                         // idle; 0: one lane per ray; -1: chosen per tree by its size
   int dda_block = 64;  // one-wave workgroups: a finished wave frees its slot at once

@@ -43,7 +43,7 @@ EXPORTS = [
     "vxl_tree_from_words", "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload",
     "vxl_tree_free", "vxl_traverse", "vxl_traverse_dev", "vxl_traverse_dev_noalloc",
     "vxl_set_timing", "vxl_last_timing",
-    "vxl_set_tuning", "vxl_set_dda_refill", "vxl_voxelize_batch", "vxl_voxelize_batch_dev", "vxl_traverse_batch",
+    "vxl_set_tuning", "vxl_set_merge_words", "vxl_set_dda_refill", "vxl_voxelize_batch", "vxl_voxelize_batch_dev", "vxl_traverse_batch",
     "vxl_traverse_batch_dev", "vxl_set_batch_budget", "vxl_autotune_batch",
     "vxl_server_create", "vxl_server_destroy", "vxl_server_hold", "vxl_server_submit_rays",
     "vxl_server_submit_voxelize", "vxl_server_poll", "vxl_server_wait", "vxl_server_stats_get",
@@ -73,7 +73,8 @@ class TreeInfo(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("sat_waves_per_cu", C.c_int), ("sat_units_per_wave", C.c_int),
-                ("dda_block", C.c_int), ("voxelize_ms", C.c_float), ("traverse_ms", C.c_float)]
+                ("dda_block", C.c_int), ("voxelize_ms", C.c_float), ("traverse_ms", C.c_float),
+                ("sat_merge_words", C.c_int)]
 
 
 class ServerConfig(C.Structure):
@@ -164,6 +165,7 @@ def load(path: str = LIB_PATH):
     L.vxl_last_timing.restype = None
     L.vxl_set_tuning.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
     L.vxl_set_dda_refill.argtypes = [_vp, C.c_int]
+    L.vxl_set_merge_words.argtypes = [_vp, C.c_int]
     L.vxl_voxelize_batch.argtypes = [_vp, _vp, _vp, C.c_uint32, _vp, C.c_float, _vp, _vp]
     L.vxl_voxelize_batch_dev.argtypes = [_vp, _vp, _vp, C.c_uint32, _vp, C.c_float, _vp, _vp,
                                          _vp]
@@ -197,7 +199,8 @@ def load(path: str = LIB_PATH):
                  "vxl_batch_graph_info",
                  "vxl_voxelize_tree", "vxl_voxelize_tree_dev", "vxl_tree_from_words",
                  "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload", "vxl_traverse",
-                 "vxl_traverse_dev", "vxl_traverse_dev_noalloc", "vxl_set_tuning", "vxl_set_dda_refill", "vxl_voxelize_batch",
+                 "vxl_traverse_dev", "vxl_traverse_dev_noalloc", "vxl_set_tuning", "vxl_set_merge_words",
+                 "vxl_set_dda_refill", "vxl_voxelize_batch",
                  "vxl_voxelize_batch_dev", "vxl_traverse_batch", "vxl_traverse_batch_dev",
                  "vxl_set_batch_budget", "vxl_autotune_batch"):
         getattr(L, name).restype = C.c_int
@@ -613,6 +616,11 @@ class Context:
     def set_tuning(self, sat_waves_per_cu=0, sat_units_per_wave=0, dda_block=0):
         _check(load().vxl_set_tuning(self.handle, sat_waves_per_cu, sat_units_per_wave, dda_block))
 
+    def set_merge_words(self, words: int):
+        """The SAT's merge tile: leaf-grid words a wave merges in LDS before writing them
+        out, in [1, 64]."""
+        _check(load().vxl_set_merge_words(self.handle, int(words)))
+
     def set_dda_refill(self, refill_min: int):
         """Lane schedule of traverse / traverse_dev: refill_min in [1, 64] runs persistent
         waves whose idle lanes take new rays once refill_min are idle; 0 is one lane per
@@ -715,7 +723,8 @@ class Context:
                                          _vp(d_hits) if d_hits else None,
                                          _vp(d_t) if d_t else None, C.byref(res)))
         return dict(sat_waves_per_cu=res.sat_waves_per_cu,
-                    sat_units_per_wave=res.sat_units_per_wave, dda_block=res.dda_block,
+                    sat_units_per_wave=res.sat_units_per_wave,
+                    sat_merge_words=res.sat_merge_words, dda_block=res.dda_block,
                     voxelize_ms=res.voxelize_ms, traverse_ms=res.traverse_ms)
 
 

@@ -356,12 +356,15 @@ vxl_status vxl_set_batch_budget(vxl_ctx *ctx, uint64_t bytes);
 typedef struct vxl_tuning {
   int sat_waves_per_cu, sat_units_per_wave, dda_block;
   float voxelize_ms, traverse_ms; /* batch times with the chosen values */
+  int sat_merge_words;
 } vxl_tuning;
 /* Autotuner: times vxl_voxelize_batch_dev on the given meshes for each SAT launch shape
- * (resident waves per CU x work units per wave), keeps the fastest, then times
- * vxl_traverse_batch_dev on the given rays (nullable) for each DDA workgroup size; the
- * chosen values stay set on ctx and are returned in *result. Trees made while tuning are
- * freed; d_hits / d_t are overwritten. */
+ * (resident waves per CU x work units per wave: the work-queue chunk a wave takes is the
+ * pairs / (CUs x waves x units) candidate pairs of one unit), keeps the fastest, then for
+ * it each merge-tile size (leaf-grid words a wave merges in LDS before writing them out:
+ * 16, 32, 48, 64), then times vxl_traverse_batch_dev on the given rays (nullable) for each
+ * DDA workgroup size; the chosen values stay set on ctx and are returned in *result. Trees
+ * made while tuning are freed; d_hits / d_t are overwritten. */
 vxl_status vxl_autotune_batch(vxl_ctx *ctx, const float *d_tris, const uint64_t *tri_offsets,
                               uint32_t count, const vxl_grid *grids, float epsilon,
                               const float *d_rays, const uint64_t *ray_offsets, uint32_t *d_hits,
@@ -427,6 +430,9 @@ void vxl_last_timing(vxl_ctx *ctx, float out_ms[8]);
  * size. 0 keeps the current value. */
 vxl_status vxl_set_tuning(vxl_ctx *ctx, int sat_waves_per_cu, int sat_units_per_wave,
                           int dda_block);
+/* The SAT's merge tile: leaf-grid words (4^3 sub-bricks) a wave merges in its LDS table
+ * before it ORs them into the grid, in [1, 64] (default 64). Results do not depend on it. */
+vxl_status vxl_set_merge_words(vxl_ctx *ctx, int words);
 /* DDA lane schedule of vxl_traverse / vxl_traverse_dev: refill_min in [1, 64] runs a
  * persistent grid whose waves hand a lane the next ray once refill_min of their lanes are
  * idle; 0 launches one lane per ray; -1 (the default) picks per tree: persistent lanes

@@ -161,6 +161,34 @@ def test_batch_traverse_dev_block_sizes(ctx):
         t.free()
 
 
+def test_merge_words_keep_results(ctx):
+    """The SAT's merge tile (words merged in LDS per wave before the flush) changes no tree,
+    from a flush after every insert to the largest tile; out-of-range values are refused."""
+    mix = [m for m in _mix(seed=23) if 128 <= m[2] <= 2048][:4]
+    grids = [(n, (0, 0, 0), vs) for _, _, n, vs in mix]
+    tris = [t for _, t, _, _ in mix]
+    ref = ctx.voxelize_batch(tris, grids)
+    try:
+        for words in (1, 16, 63):
+            _say(f"merge words {words}")
+            ctx.set_merge_words(words)
+            got = ctx.voxelize_batch(tris, grids)
+            for (name, *_), a, b in zip(mix, ref, got):
+                assert a.pairs == b.pairs, name
+                _assert_same_tree(a, b, f"{name} merge {words}")
+            single = ctx.voxelize_tree(tris[0], mix[0][2], (0, 0, 0), mix[0][3])
+            _assert_same_tree(single, ref[0], f"{mix[0][0]} single, merge {words}")
+            for t in got + [single]:
+                t.free()
+        for bad in (0, 65, -1):
+            with pytest.raises(vxl.VxlError):
+                ctx.set_merge_words(bad)
+    finally:
+        ctx.set_merge_words(64)
+    for t in ref:
+        t.free()
+
+
 def test_autotune_keeps_results(ctx):
     import torch
     mix = _mix(seed=19)
@@ -184,6 +212,7 @@ def test_autotune_keeps_results(ctx):
         assert res["sat_waves_per_cu"] in (16, 20, 24, 28, 32)
         assert res["sat_units_per_wave"] in (1, 2, 4)
         assert res["dda_block"] in (64, 128, 256)
+        assert res["sat_merge_words"] in (16, 32, 48, 64)
         assert res["voxelize_ms"] > 0 and res["traverse_ms"] > 0
         after = ctx.voxelize_batch_dev(dtri.data_ptr(), off, grids)
         torch.cuda.synchronize()
@@ -191,6 +220,7 @@ def test_autotune_keeps_results(ctx):
             _assert_same_tree(a, b, name)
     finally:
         ctx.set_tuning(sat_waves_per_cu=28, sat_units_per_wave=1, dda_block=64)
+        ctx.set_merge_words(64)
     for t in before + after:
         t.free()
 
