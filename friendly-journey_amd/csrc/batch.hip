@@ -188,6 +188,7 @@ void batch_graph_build(vxl_batch_graph *bg, const std::vector<BatchInput> &in,
   own->sat_units_per_wave = ctx->sat_units_per_wave;
   own->sat_merge_words = ctx->sat_merge_words;
   own->dda_block = ctx->dda_block;
+  own->dda_batch_refill = ctx->dda_batch_refill;
   own->pool_budget = ctx->pool_budget;
   for (const BatchInput &b : in) {
     bg->n.push_back(b.n);
@@ -242,7 +243,8 @@ void batch_graph_build(vxl_batch_graph *bg, const std::vector<BatchInput> &in,
   }
   const size_t seg_bytes = up256(count * sizeof(TraverseSeg));
   const size_t blk_bytes = up256((count + 1) * 4);
-  desc_bytes += seg_bytes + blk_bytes;
+  const size_t run_bytes = up256((count + 1) * 8);
+  desc_bytes += seg_bytes + blk_bytes + run_bytes + 256;
   own->batch_scratch.reserve(scratch);
   VXL_HIP(hipMalloc(&bg->desc, desc_bytes));
   bg->desc_bytes = desc_bytes;
@@ -290,25 +292,34 @@ void batch_graph_build(vxl_batch_graph *bg, const std::vector<BatchInput> &in,
     gg.codes_base = abase + aoff;
     aoff += up256(gg.leaf_cap * 8);
   }
-  const size_t o_seg = take(count * sizeof(TraverseSeg)), o_blk = take((count + 1) * 4);
+  const size_t o_seg = take(count * sizeof(TraverseSeg)), o_blk = take((count + 1) * 4),
+               o_run = take((count + 1) * 8), o_ctr = take(8);
   bg->d_segs = reinterpret_cast<TraverseSeg *>(dbase + o_seg);
   const uint32_t *d_blk = reinterpret_cast<const uint32_t *>(dbase + o_blk);
+  const uint64_t *d_run = reinterpret_cast<const uint64_t *>(dbase + o_run);
+  unsigned long long *d_ctr = reinterpret_cast<unsigned long long *>(dbase + o_ctr);
   TraverseSeg *hs = reinterpret_cast<TraverseSeg *>(image.data() + o_seg);
   uint32_t *hb = reinterpret_cast<uint32_t *>(image.data() + o_blk);
+  uint64_t *hr = reinterpret_cast<uint64_t *>(image.data() + o_run);
   const uint32_t tb = uint32_t(own->dda_block);
-  uint64_t blocks = 0;
+  uint64_t blocks = 0, runs = 0;
   uint32_t kmax = 0;
+  bool want_t = false;
   for (uint32_t g = 0; g < count; g++) {
     const uint64_t a = roff[g], R = roff[g + 1] - roff[g];
     hs[g] = TraverseSeg{nullptr, nullptr, nullptr, d_rays + 6 * a,
                         reinterpret_cast<uint4 *>(d_hits + 4 * a), d_t ? d_t + a : nullptr, 0, R,
                         in[g].n, in[g].k};
     hb[g] = uint32_t(blocks);
+    hr[g] = runs;
     blocks += (R + tb - 1) / tb;
+    runs += (R + kBatchRayRun - 1) / kBatchRayRun;
     kmax = std::max(kmax, in[g].k);
+    want_t = want_t || (d_t && R);
   }
   hb[count] = uint32_t(blocks);
-  if (blocks > 0xffffffffull)
+  hr[count] = runs;
+  if (blocks > 0xffffffffull || runs > 0xffffffffull)
     throw Error(VXL_WORKGROUPS_EXCEEDED, "batch graph: rays exceed the grid limit");
   VXL_HIP(hipMemcpy(bg->desc, image.data(), desc_bytes, hipMemcpyHostToDevice));
 
@@ -324,7 +335,8 @@ void batch_graph_build(vxl_batch_graph *bg, const std::vector<BatchInput> &in,
                     gg.d_ref);
       enqueue_trees_graph(own, gg, gss[r], bg->d_segs + gg.first, cs);
     }
-    launch_traverse_segs(own, bg->d_segs, d_blk, count, blocks, kmax, cs);
+    launch_traverse_segs(own, bg->d_segs, d_blk, d_run, count, blocks, runs, kmax, want_t,
+                         d_ctr, cs);
   } catch (...) {
     hipGraph_t dead = nullptr;
     (void)hipStreamEndCapture(cs, &dead);

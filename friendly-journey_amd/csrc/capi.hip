@@ -642,6 +642,14 @@ vxl_status vxl_set_merge_words(vxl_ctx *ctx, int words) {
   });
 }
 
+vxl_status vxl_set_batch_refill(vxl_ctx *ctx, int refill_min) {
+  return guarded(ctx, [&] {
+    if (refill_min < 0 || refill_min > 64)
+      throw Error(VXL_INVALID_ARG, "batch refill_min must be 0 (one lane per ray) or in [1, 64]");
+    ctx->dda_batch_refill = refill_min;
+  });
+}
+
 vxl_status vxl_set_dda_refill(vxl_ctx *ctx, int refill_min) {
   return guarded(ctx, [&] {
     if (refill_min < -1 || refill_min > 64)
@@ -1516,6 +1524,8 @@ vxl_status vxl_autotune_batch(vxl_ctx *ctx, const float *d_tris, const uint64_t 
     const int units[] = {1, 2, 4};
     const int blocks[] = {64, 128, 256};
     const int merge[] = {16, 32, 48, 64};
+    const int refills[] = {8, 16, 32};
+    const int keep_r = ctx->dda_batch_refill;
     const int keep_w = ctx->sat_waves_per_cu, keep_u = ctx->sat_units_per_wave;
     const int keep_m = ctx->sat_merge_words;
     try {
@@ -1550,6 +1560,7 @@ vxl_status vxl_autotune_batch(vxl_ctx *ctx, const float *d_tris, const uint64_t 
       result->sat_merge_words = bm;
       result->voxelize_ms = time_voxelize();
       result->dda_block = ctx->dda_block;
+      result->dda_batch_refill = ctx->dda_batch_refill;
       result->traverse_ms = 0;
       const bool rays = count && ray_offsets && ray_offsets[count] > ray_offsets[0] && d_rays &&
                         d_hits;
@@ -1558,6 +1569,7 @@ vxl_status vxl_autotune_batch(vxl_ctx *ctx, const float *d_tris, const uint64_t 
         float bt = 1e30f;
         int bb = ctx->dda_block;
         const auto segs = segments(trees.data(), count, ray_offsets, d_rays, d_hits, d_t);
+        ctx->dda_batch_refill = 0;
         for (int b : blocks) {
           ctx->dda_block = b;
           vxl::launch_traverse_batch(ctx, segs.data(), count, ctx->stream);  // warm
@@ -1573,7 +1585,25 @@ vxl_status vxl_autotune_batch(vxl_ctx *ctx, const float *d_tris, const uint64_t 
           }
         }
         ctx->dda_block = bb;
+        // persistent lanes over the batch's run table, refilled at 8, 16 or 32 idle lanes
+        int br = 0;
+        for (int refill : refills) {
+          ctx->dda_batch_refill = refill;
+          vxl::launch_traverse_batch(ctx, segs.data(), count, ctx->stream);  // warm
+          VXL_HIP(hipEventRecord(e0, ctx->stream));
+          vxl::launch_traverse_batch(ctx, segs.data(), count, ctx->stream);
+          VXL_HIP(hipEventRecord(e1, ctx->stream));
+          VXL_HIP(hipEventSynchronize(e1));
+          float ms = 0;
+          VXL_HIP(hipEventElapsedTime(&ms, e0, e1));
+          if (ms < bt) {
+            bt = ms;
+            br = refill;
+          }
+        }
+        ctx->dda_batch_refill = br;
         result->dda_block = bb;
+        result->dda_batch_refill = br;
         result->traverse_ms = bt;
       }
       free_trees();
@@ -1581,6 +1611,7 @@ vxl_status vxl_autotune_batch(vxl_ctx *ctx, const float *d_tris, const uint64_t 
       ctx->sat_waves_per_cu = keep_w;
       ctx->sat_units_per_wave = keep_u;
       ctx->sat_merge_words = keep_m;
+      ctx->dda_batch_refill = keep_r;
       (void)hipStreamSynchronize(ctx->stream);
       for (auto &t : trees)
         if (t) {

@@ -448,11 +448,9 @@ __global__ __launch_bounds__(256) void k_traverse(
 // t of every ray from its hit record, as store_result computes it: the persistent walk
 // leaves t to this pass, where all 64 lanes of a wave work, instead of computing it for the
 // few lanes that finish in one visit.
-__global__ __launch_bounds__(256) void k_hit_t(const float *__restrict__ rays, uint64_t R,
-                                               const uint4 *__restrict__ hits,
-                                               double *__restrict__ tout) {
-  const uint64_t ri = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (ri >= R) return;
+__device__ __forceinline__ void hit_t_one(const float *__restrict__ rays, uint64_t ri,
+                                          const uint4 *__restrict__ hits,
+                                          double *__restrict__ tout) {
   const uint4 h = hits[ri];
   double t = 0.0;
   if (h.w == 1u) {
@@ -463,6 +461,13 @@ __global__ __launch_bounds__(256) void k_hit_t(const float *__restrict__ rays, u
     t = hit_t(o, d, h);
   }
   tout[ri] = t;
+}
+
+__global__ __launch_bounds__(256) void k_hit_t(const float *__restrict__ rays, uint64_t R,
+                                               const uint4 *__restrict__ hits,
+                                               double *__restrict__ tout) {
+  const uint64_t ri = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (ri < R) hit_t_one(rays, ri, hits, tout);
 }
 
 // The mirror's walk statistics over a ray batch (the reference's measure.rs counters):
@@ -507,6 +512,7 @@ __global__ __launch_bounds__(64) void k_traverse_stats(
 // wave is not held by its longest walk. Every ray's walk is the same as in k_traverse;
 // only the lane that runs it differs. t is left to k_hit_t.
 constexpr uint32_t kRayRun = 256;
+static_assert(kRayRun == kBatchRayRun, "the batch run table counts runs of kRayRun rays");
 
 __global__ __launch_bounds__(64) void k_traverse_pl(
     const float *__restrict__ rays, uint64_t R, const uint32_t *__restrict__ nodes,
@@ -571,6 +577,95 @@ __global__ __launch_bounds__(64) void k_traverse_pl(
       if (uint32_t(__popcll(__ballot(!active))) >= until) break;  // wave-uniform
     }
   }
+}
+
+// Persistent lanes over the segments of a batched traversal. Segment g's rays form runs of
+// kRayRun ([run_begin[g], run_begin[g+1]) in the run table), so a run never spans two trees;
+// waves take runs with one atomic and hand idle lanes the next rays of their run, as
+// k_traverse_pl does for one tree. A lane keeps the tree of the ray it walks (its pointers
+// in vector registers), so lanes of one wave may walk different segments' trees. t is left
+// to k_hit_t_batch.
+__global__ __launch_bounds__(64) void k_traverse_batch_pl(
+    const TraverseSeg *__restrict__ seg, const uint64_t *__restrict__ run_begin, uint32_t S,
+    unsigned long long *__restrict__ counter, uint32_t refill_min) {
+  extern __shared__ uint32_t frames_lds[];
+  uint32_t *const frames = frames_lds + threadIdx.x;
+  const uint32_t lane = threadIdx.x;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const uint64_t nruns = run_begin[S];
+  Ray r;
+  Walk w;
+  w.ox = w.oy = w.oz = w.cx = w.cy = w.cz = 0;
+  const uint32_t *nodes = nullptr, *leafw = nullptr, *bounds = nullptr;  // the lane's tree
+  uint4 *hit = nullptr;                                                   // the lane's record
+  uint32_t cur = 0;               // segment of the wave's run (wave-uniform)
+  uint64_t run = 0, run_end = 0;  // its rays not yet handed out (wave-uniform)
+  bool active = false, exhausted = false;
+  for (;;) {
+    const unsigned long long idle = __ballot(!active);
+    const uint32_t ni = __popcll(idle);
+    if (!exhausted) {  // wave-uniform
+      if (run == run_end) {
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(counter, 1ull);
+        base = __shfl(base, 0, 64);
+        if (base >= nruns) {
+          exhausted = true;
+          run = run_end = 0;
+        } else {
+          cur = dev::upper_index(run_begin, S, uint64_t(base));
+          run = (base - run_begin[cur]) * kRayRun;
+          const uint64_t R = seg[cur].R;
+          run_end = run + kRayRun < R ? run + kRayRun : R;
+        }
+      }
+      const uint64_t take = run_end - run < ni ? run_end - run : ni;
+      if (!active) {
+        const uint32_t rank = __popcll(idle & below);
+        if (rank < take) {
+          const TraverseSeg &sg = seg[cur];
+          const uint64_t ri = run + rank;
+          load_ray(sg.rays, ri, r);
+          hit = sg.hits + ri;
+          float t0;
+          if (clip_grid(r, sg.n, sg.leaf_count, t0)) {
+            nodes = sg.nodes;
+            leafw = sg.leafw;
+            bounds = sg.bounds;
+            begin_walk(r, w, sg.k, t0, nodes, leafw);
+            active = true;
+          } else {
+            store_result(0, r, w, kLeft, hit, nullptr);
+          }
+        }
+      }
+      run += take;
+    } else if (ni == 64) {
+      break;  // no runs left and every lane idle (wave-uniform)
+    }
+    const uint32_t until = exhausted ? 64u : refill_min;
+    for (;;) {
+      if (active) {
+        const uint32_t status = walk_visit(r, w, nodes, leafw, bounds, frames, 64);
+        if (status != kWalking) {
+          store_result(0, r, w, status, hit, nullptr);
+          active = false;
+        }
+      }
+      if (uint32_t(__popcll(__ballot(!active))) >= until) break;  // wave-uniform
+    }
+  }
+}
+
+// t of every ray of a batched traversal from its hit record: block b takes run b.
+__global__ __launch_bounds__(kRayRun) void k_hit_t_batch(const TraverseSeg *__restrict__ seg,
+                                                         const uint64_t *__restrict__ run_begin,
+                                                         uint32_t S) {
+  const uint64_t b = blockIdx.x;
+  const uint32_t g = dev::upper_index(run_begin, S, b);
+  const TraverseSeg &sg = seg[g];
+  const uint64_t ri = (b - run_begin[g]) * kRayRun + threadIdx.x;
+  if (ri < sg.R && sg.t) hit_t_one(sg.rays, ri, sg.hits, sg.t);
 }
 
 // ---- device ray generation + shading (render.wgsl:54-91) ----------------------------------
@@ -774,49 +869,83 @@ void launch_render(vxl_ctx *ctx, const vxl_tree *tree, const vxl_camera &cam,
 
 void launch_traverse_batch(vxl_ctx *ctx, const RaySegment *segs, uint32_t S, hipStream_t s) {
   const uint32_t tb = uint32_t(ctx->dda_block);
-  uint64_t blocks = 0;
-  for (uint32_t g = 0; g < S; g++) blocks += (segs[g].R + tb - 1) / tb;
+  uint64_t blocks = 0, runs = 0;
+  uint32_t kmax = 0;
+  bool want_t = false;
+  for (uint32_t g = 0; g < S; g++) {
+    blocks += (segs[g].R + tb - 1) / tb;
+    runs += (segs[g].R + kRayRun - 1) / kRayRun;
+    kmax = segs[g].tree->k > kmax ? segs[g].tree->k : kmax;
+    want_t = want_t || (segs[g].t && segs[g].R);
+  }
   if (blocks == 0) return;
-  if (blocks > 0xffffffffull)
+  if (blocks > 0xffffffffull || runs > 0xffffffffull)
     throw Error(VXL_WORKGROUPS_EXCEEDED, "traverse batch exceeds the grid limit");
   const size_t seg_bytes = (S * sizeof(TraverseSeg) + 255) / 256 * 256;
   const size_t blk_bytes = ((S + 1) * 4 + 255) / 256 * 256;
-  staging_begin(ctx, seg_bytes + blk_bytes);
-  char *h = static_cast<char *>(staging_take(ctx, seg_bytes + blk_bytes));
+  const size_t run_bytes = ((S + 1) * 8 + 255) / 256 * 256;
+  const size_t bytes = seg_bytes + blk_bytes + run_bytes;
+  staging_begin(ctx, bytes);
+  char *h = static_cast<char *>(staging_take(ctx, bytes));
   TraverseSeg *hs = reinterpret_cast<TraverseSeg *>(h);
   uint32_t *hb = reinterpret_cast<uint32_t *>(h + seg_bytes);
-  blocks = 0;
+  uint64_t *hr = reinterpret_cast<uint64_t *>(h + seg_bytes + blk_bytes);
+  blocks = runs = 0;
   for (uint32_t g = 0; g < S; g++) {
     const vxl_tree *t = segs[g].tree;
     hs[g] = TraverseSeg{t->nodes, reinterpret_cast<const uint32_t *>(t->leaves), t->bounds,
                         segs[g].rays, reinterpret_cast<uint4 *>(segs[g].hits), segs[g].t,
                         t->leaf_count, segs[g].R, t->n, t->k};
     hb[g] = uint32_t(blocks);
+    hr[g] = runs;
     blocks += (segs[g].R + tb - 1) / tb;
+    runs += (segs[g].R + kRayRun - 1) / kRayRun;
   }
   hb[S] = uint32_t(blocks);
-  ctx->trav_desc.reserve(seg_bytes + blk_bytes);
+  hr[S] = runs;
+  ctx->trav_desc.reserve(bytes);
   char *d = static_cast<char *>(ctx->trav_desc.ptr);
   // the previous batch's kernel (possibly on another stream) reads the descriptors
   if (ctx->trav_done) VXL_HIP(hipStreamWaitEvent(s, ctx->trav_done, 0));
   else VXL_HIP(hipEventCreateWithFlags(&ctx->trav_done, hipEventDisableTiming));
-  VXL_HIP(hipMemcpyAsync(d, h, seg_bytes + blk_bytes, hipMemcpyHostToDevice, s));
+  VXL_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
   staging_end(ctx, s);
-  uint32_t kmax = 0;
-  for (uint32_t g = 0; g < S; g++) kmax = segs[g].tree->k > kmax ? segs[g].tree->k : kmax;
-  hipLaunchKernelGGL(k_traverse_batch, dim3(uint32_t(blocks)), dim3(tb),
-                     frame_lds_bytes(kmax, tb), s,
-                     reinterpret_cast<const TraverseSeg *>(d),
-                     reinterpret_cast<const uint32_t *>(d + seg_bytes), S);
-  VXL_HIP(hipGetLastError());
+  unsigned long long *ctr = nullptr;
+  if (ctx->dda_batch_refill > 0) {
+    ctx->trav_ctr.reserve(kCounterSlots * 8 + 64);
+    ctr = ctx->trav_ctr.as<unsigned long long>() + (ctx->trav_ctr_next++ % kCounterSlots);
+  }
+  launch_traverse_segs(ctx, reinterpret_cast<const TraverseSeg *>(d),
+                       reinterpret_cast<const uint32_t *>(d + seg_bytes),
+                       reinterpret_cast<const uint64_t *>(d + seg_bytes + blk_bytes), S, blocks,
+                       runs, kmax, want_t, ctr, s);
   VXL_HIP(hipEventRecord(ctx->trav_done, s));
 }
 
 void launch_traverse_segs(vxl_ctx *ctx, const TraverseSeg *d_segs, const uint32_t *d_blk,
-                          uint32_t S, uint64_t blocks, uint32_t kmax, hipStream_t s) {
+                          const uint64_t *d_runs, uint32_t S, uint64_t blocks, uint64_t runs,
+                          uint32_t kmax, bool want_t, unsigned long long *counter,
+                          hipStream_t s) {
   if (blocks == 0) return;
-  if (blocks > 0xffffffffull)
+  if (blocks > 0xffffffffull || runs > 0xffffffffull)
     throw Error(VXL_WORKGROUPS_EXCEEDED, "traverse batch exceeds the grid limit");
+  if (ctx->dda_batch_refill > 0) {
+    if (!counter) throw Error(VXL_DEVICE_ERROR, "internal: persistent batch without a counter");
+    const size_t lds64 = frame_lds_bytes(kmax, 64);
+    int per_cu = 0;
+    VXL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_traverse_batch_pl, 64, lds64));
+    if (per_cu < 1) per_cu = 1;
+    uint64_t grid = uint64_t(ctx->cu_count > 0 ? ctx->cu_count : 256) * uint64_t(per_cu);
+    if (grid > runs * (kRayRun / 64)) grid = runs * (kRayRun / 64);
+    hipLaunchKernelGGL(k_clear_counter, dim3(1), dim3(1), 0, s, counter);
+    hipLaunchKernelGGL(k_traverse_batch_pl, dim3(uint32_t(grid)), dim3(64), lds64, s, d_segs,
+                       d_runs, S, counter, uint32_t(ctx->dda_batch_refill));
+    if (want_t)
+      hipLaunchKernelGGL(k_hit_t_batch, dim3(uint32_t(runs)), dim3(kRayRun), 0, s, d_segs,
+                         d_runs, S);
+    VXL_HIP(hipGetLastError());
+    return;
+  }
   const uint32_t tb = uint32_t(ctx->dda_block);
   hipLaunchKernelGGL(k_traverse_batch, dim3(uint32_t(blocks)), dim3(tb), frame_lds_bytes(kmax, tb),
                      s, d_segs, d_blk, S);

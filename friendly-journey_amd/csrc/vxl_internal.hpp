@@ -114,6 +114,8 @@ struct vxl_ctx {
   int dda_refill = -1;  // > 0: persistent traversal lanes, refilled when that many are
                         // idle; 0: one lane per ray; -1: chosen per tree by its size
   int dda_block = 64;  // one-wave workgroups: a finished wave frees its slot at once
+  int dda_batch_refill = 0;  // batched traversals: 0 one lane per ray, > 0 persistent lanes
+                             // refilled once that many of a wave's lanes are idle
   // last-call timing (ms), filled when timing is enabled
   bool timing = false;
   hipEvent_t ev[8] = {};  // ev[0] call start, ev[1]/ev[2] around the SAT kernel, ev[3] end
@@ -363,11 +365,18 @@ struct TraverseSeg {
   uint64_t leaf_count, R;
   uint32_t n, k;
 };
-// The batched traversal kernel over descriptors already in device memory: segment g owns
-// blocks [d_blk[g], d_blk[g+1]) of `blocks` workgroups of ctx->dda_block lanes; kmax is the
-// largest tree level count k of any segment. Nothing but the launch (graph capture uses it).
+// The batched traversal over descriptors already in device memory, in the context's batch
+// lane schedule: one lane per ray (segment g owns blocks [d_blk[g], d_blk[g+1]) of `blocks`
+// workgroups of ctx->dda_block lanes), or persistent lanes over the run table d_runs
+// (segment g owns runs [d_runs[g], d_runs[g+1]) of 256 rays; `counter` is cleared by a
+// kernel first, and t comes from a second pass when want_t). kmax is the largest tree level
+// count k of any segment. Kernel launches only (graph capture uses it).
 void launch_traverse_segs(vxl_ctx *ctx, const TraverseSeg *d_segs, const uint32_t *d_blk,
-                          uint32_t S, uint64_t blocks, uint32_t kmax, hipStream_t s);
+                          const uint64_t *d_runs, uint32_t S, uint64_t blocks, uint64_t runs,
+                          uint32_t kmax, bool want_t, unsigned long long *counter,
+                          hipStream_t s);
+// Rays per run of the persistent batched traversal's run table.
+constexpr uint64_t kBatchRayRun = 256;
 // tree.hip: SparseTree::set_voxel over distinct in-bounds edits on the device; patches
 // leaves in place or, when a brick appears or empties, rebuilds the tree's arrays (the
 // handle stays the same).

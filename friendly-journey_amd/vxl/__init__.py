@@ -43,7 +43,7 @@ EXPORTS = [
     "vxl_tree_from_words", "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload",
     "vxl_tree_free", "vxl_traverse", "vxl_traverse_dev", "vxl_traverse_dev_noalloc",
     "vxl_set_timing", "vxl_last_timing",
-    "vxl_set_tuning", "vxl_set_merge_words", "vxl_set_dda_refill", "vxl_voxelize_batch", "vxl_voxelize_batch_dev", "vxl_traverse_batch",
+    "vxl_set_tuning", "vxl_set_merge_words", "vxl_set_batch_refill", "vxl_set_dda_refill", "vxl_voxelize_batch", "vxl_voxelize_batch_dev", "vxl_traverse_batch",
     "vxl_traverse_batch_dev", "vxl_set_batch_budget", "vxl_autotune_batch",
     "vxl_server_create", "vxl_server_destroy", "vxl_server_hold", "vxl_server_submit_rays",
     "vxl_server_submit_voxelize", "vxl_server_poll", "vxl_server_wait", "vxl_server_stats_get",
@@ -74,7 +74,7 @@ class TreeInfo(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("sat_waves_per_cu", C.c_int), ("sat_units_per_wave", C.c_int),
                 ("dda_block", C.c_int), ("voxelize_ms", C.c_float), ("traverse_ms", C.c_float),
-                ("sat_merge_words", C.c_int)]
+                ("sat_merge_words", C.c_int), ("dda_batch_refill", C.c_int)]
 
 
 class ServerConfig(C.Structure):
@@ -166,6 +166,7 @@ def load(path: str = LIB_PATH):
     L.vxl_set_tuning.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
     L.vxl_set_dda_refill.argtypes = [_vp, C.c_int]
     L.vxl_set_merge_words.argtypes = [_vp, C.c_int]
+    L.vxl_set_batch_refill.argtypes = [_vp, C.c_int]
     L.vxl_voxelize_batch.argtypes = [_vp, _vp, _vp, C.c_uint32, _vp, C.c_float, _vp, _vp]
     L.vxl_voxelize_batch_dev.argtypes = [_vp, _vp, _vp, C.c_uint32, _vp, C.c_float, _vp, _vp,
                                          _vp]
@@ -200,7 +201,7 @@ def load(path: str = LIB_PATH):
                  "vxl_voxelize_tree", "vxl_voxelize_tree_dev", "vxl_tree_from_words",
                  "vxl_tree_info_get", "vxl_tree_download", "vxl_tree_upload", "vxl_traverse",
                  "vxl_traverse_dev", "vxl_traverse_dev_noalloc", "vxl_set_tuning", "vxl_set_merge_words",
-                 "vxl_set_dda_refill", "vxl_voxelize_batch",
+                 "vxl_set_batch_refill", "vxl_set_dda_refill", "vxl_voxelize_batch",
                  "vxl_voxelize_batch_dev", "vxl_traverse_batch", "vxl_traverse_batch_dev",
                  "vxl_set_batch_budget", "vxl_autotune_batch"):
         getattr(L, name).restype = C.c_int
@@ -621,6 +622,10 @@ class Context:
         out, in [1, 64]."""
         _check(load().vxl_set_merge_words(self.handle, int(words)))
 
+    def set_batch_refill(self, refill_min: int):
+        """Lane schedule of batched traversals: 0 one lane per ray, 1..64 persistent lanes."""
+        _check(load().vxl_set_batch_refill(self.handle, int(refill_min)))
+
     def set_dda_refill(self, refill_min: int):
         """Lane schedule of traverse / traverse_dev: refill_min in [1, 64] runs persistent
         waves whose idle lanes take new rays once refill_min are idle; 0 is one lane per
@@ -725,6 +730,7 @@ class Context:
         return dict(sat_waves_per_cu=res.sat_waves_per_cu,
                     sat_units_per_wave=res.sat_units_per_wave,
                     sat_merge_words=res.sat_merge_words, dda_block=res.dda_block,
+                    dda_batch_refill=res.dda_batch_refill,
                     voxelize_ms=res.voxelize_ms, traverse_ms=res.traverse_ms)
 
 

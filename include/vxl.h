@@ -356,15 +356,16 @@ vxl_status vxl_set_batch_budget(vxl_ctx *ctx, uint64_t bytes);
 typedef struct vxl_tuning {
   int sat_waves_per_cu, sat_units_per_wave, dda_block;
   float voxelize_ms, traverse_ms; /* batch times with the chosen values */
-  int sat_merge_words;
+  int sat_merge_words, dda_batch_refill;
 } vxl_tuning;
 /* Autotuner: times vxl_voxelize_batch_dev on the given meshes for each SAT launch shape
  * (resident waves per CU x work units per wave: the work-queue chunk a wave takes is the
  * pairs / (CUs x waves x units) candidate pairs of one unit), keeps the fastest, then for
  * it each merge-tile size (leaf-grid words a wave merges in LDS before writing them out:
  * 16, 32, 48, 64), then times vxl_traverse_batch_dev on the given rays (nullable) for each
- * DDA workgroup size; the chosen values stay set on ctx and are returned in *result. Trees
- * made while tuning are freed; d_hits / d_t are overwritten. */
+ * DDA workgroup size with one lane per ray and for persistent lanes refilled at 8, 16 or 32
+ * idle lanes (vxl_set_batch_refill); the chosen values stay set on ctx and are returned in
+ * *result. Trees made while tuning are freed; d_hits / d_t are overwritten. */
 vxl_status vxl_autotune_batch(vxl_ctx *ctx, const float *d_tris, const uint64_t *tri_offsets,
                               uint32_t count, const vxl_grid *grids, float epsilon,
                               const float *d_rays, const uint64_t *ray_offsets, uint32_t *d_hits,
@@ -430,6 +431,11 @@ void vxl_last_timing(vxl_ctx *ctx, float out_ms[8]);
  * size. 0 keeps the current value. */
 vxl_status vxl_set_tuning(vxl_ctx *ctx, int sat_waves_per_cu, int sat_units_per_wave,
                           int dda_block);
+/* Lane schedule of the batched traversals (vxl_traverse_batch[_dev], the batch graph):
+ * 0 (default) one lane per ray; refill_min in [1, 64] persistent lanes that take runs of 256
+ * rays of one segment and hand idle lanes the next ray once refill_min of a wave's lanes are
+ * idle. Results do not depend on it. */
+vxl_status vxl_set_batch_refill(vxl_ctx *ctx, int refill_min);
 /* The SAT's merge tile: leaf-grid words (4^3 sub-bricks) a wave merges in its LDS table
  * before it ORs them into the grid, in [1, 64] (default 64). Results do not depend on it. */
 vxl_status vxl_set_merge_words(vxl_ctx *ctx, int words);

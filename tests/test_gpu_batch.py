@@ -118,14 +118,25 @@ def test_batch_traverse_equals_single(ctx):
         m = max(int(round(1 / vs)), 1)
         R = [1000, 0, 4097, 333, 1, 2500, 64, 256, 700, 50, 900, 1200, 3000][i % 13]
         rays.append(W.random_rays(R, m, seed=500 + i) if R else np.zeros((0, 6), np.float32))
-    _say(f"traverse batch of {len(trees)} segments, {sum(len(r) for r in rays)} rays")
-    got = ctx.traverse_batch(trees, rays)
-    for (name, *_), tr, r, (h, t) in zip(mix, trees, rays, got):
-        if len(r) == 0:
-            continue
-        sh, stt = ctx.traverse(tr, r)
-        assert np.array_equal(h, sh), name
-        assert np.array_equal(t, stt), name
+    want = [ctx.traverse(tr, r) if len(r) else None for tr, r in zip(trees, rays)]
+    try:
+        # one lane per ray, then persistent lanes over the run table (runs of 256 rays of one
+        # segment; a wave's lanes walk several trees at once), refilled at 1, 8 and 64 idle lanes
+        for refill in (0, 1, 8, 64):
+            _say(f"traverse batch of {len(trees)} segments, "
+                 f"{sum(len(r) for r in rays)} rays, batch refill {refill}")
+            ctx.set_batch_refill(refill)
+            got = ctx.traverse_batch(trees, rays)
+            for (name, *_), r, w, (h, t) in zip(mix, rays, want, got):
+                if len(r) == 0:
+                    continue
+                assert np.array_equal(h, w[0]), f"{name} refill {refill}"
+                assert np.array_equal(t, w[1]), f"{name} refill {refill}"
+        for bad in (-1, 65):
+            with pytest.raises(vxl.VxlError):
+                ctx.set_batch_refill(bad)
+    finally:
+        ctx.set_batch_refill(0)
     for t in trees:
         t.free()
 
@@ -213,6 +224,7 @@ def test_autotune_keeps_results(ctx):
         assert res["sat_units_per_wave"] in (1, 2, 4)
         assert res["dda_block"] in (64, 128, 256)
         assert res["sat_merge_words"] in (16, 32, 48, 64)
+        assert res["dda_batch_refill"] in (0, 8, 16, 32)
         assert res["voxelize_ms"] > 0 and res["traverse_ms"] > 0
         after = ctx.voxelize_batch_dev(dtri.data_ptr(), off, grids)
         torch.cuda.synchronize()
@@ -221,6 +233,7 @@ def test_autotune_keeps_results(ctx):
     finally:
         ctx.set_tuning(sat_waves_per_cu=28, sat_units_per_wave=1, dda_block=64)
         ctx.set_merge_words(64)
+        ctx.set_batch_refill(0)
     for t in before + after:
         t.free()
 

@@ -172,6 +172,33 @@ def test_graph_replays_equal_eager_batch(ctx):
             tr.free()
 
 
+def test_graph_persistent_traversal_equals_eager(ctx):
+    """The graph captured with the persistent batched traversal (runs of 256 rays of one
+    segment, lanes refilled at 8 idle) gives the eager batch calls' trees and hits."""
+    batch = _batch(0x5EED_0C00)
+    buf = _Buffers(batch)
+    ctx.set_batch_refill(8)
+    try:
+        ref = buf.eager(ctx)
+        g = buf.graph(ctx)
+        try:
+            for i in range(2):
+                _say(f"persistent-lane graph, replay {i + 1}")
+                g.launch()
+                _check_replay(g, buf, ref, f"persistent replay {i + 1}")
+        finally:
+            g.free()
+        ctx.set_batch_refill(0)
+        plain = buf.eager(ctx)
+        assert np.array_equal(plain[1], ref[1]) and np.array_equal(plain[2], ref[2])
+        for tr in plain[0]:
+            tr.free()
+    finally:
+        ctx.set_batch_refill(0)
+        for tr in ref[0]:
+            tr.free()
+
+
 def test_graph_overflow_reports_and_recovers(ctx):
     batch = [b for b in _batch(0x5EED_0B00) if b[0] in ("soup_256", "sphere_64", "soup_64")]
     buf = _Buffers(batch)
