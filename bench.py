@@ -522,9 +522,10 @@ def bench_batch(ctx, torch, W, reps=3):
             "arena_mb": round(ginfo["arena_bytes"] / 2**20, 1),
             "create_ms": round(create_ms, 1),
             "identical_trees": graph_trees, "identical_hits": graph_hits,
-            "note": ("setup, SAT and tree build of every group and one traversal kernel over "
-                     "all segments, replayed with no host round trip; trees compared with "
-                     "the batched calls' (counts for all, arrays for every 8th mesh)")},
+            "note": ("setup, SAT and tree build of every group and one batched traversal "
+                     "over all segments (the tuned lane schedule), replayed with no host round "
+                     "trip; trees compared with the batched calls' (counts for all, arrays for "
+                     "every 8th mesh)")},
         tuning=tune, identical_trees=same_trees, identical_hits=same_hits,
         timing="wall clock, median of %d, inputs resident in HBM" % reps)
 
