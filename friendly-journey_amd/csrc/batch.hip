@@ -350,6 +350,8 @@ void batch_graph_build(vxl_batch_graph *bg, const std::vector<BatchInput> &in,
 }
 
 void batch_graph_launch(vxl_batch_graph *bg, hipStream_t s) {
+  // replays share the graph's scratch: one launched on another stream waits for the last
+  if (bg->launched) VXL_HIP(hipStreamWaitEvent(s, bg->done, 0));
   VXL_HIP(hipGraphLaunch(bg->exec, s));
   VXL_HIP(hipEventRecord(bg->done, s));
   bg->launched = true;

@@ -274,8 +274,8 @@ void vxl_graph_destroy(vxl_graph *graph);
  *      hits to d_hits (4 u32 per ray) and t to d_t (nullable); ray_offsets null: no rays.
  *      Every mesh needs n in 128..2048 (k = 2..4). Each replay reads d_tris and d_rays and
  *      writes d_hits and d_t: their contents may change between replays, their sizes and
- *      the grids may not. Replays use the graph's own scratch: they must not overlap one
- *      another (launch them on one stream, or wait). vxl_batch_graph_trees, after a launch:
+ *      the grids may not. Replays use the graph's own scratch, so a replay launched on
+ *      another stream first waits for the previous one. vxl_batch_graph_trees, after a launch:
  *      waits for it and returns copies of its trees (owned by ctx; trees and pairs
  *      nullable), or VXL_STORAGE_EXCEEDED when a group's trees did not fit its region (the
  *      hits of that group's meshes are then misses; a later replay whose trees fit is whole
