@@ -531,7 +531,7 @@ def bench_batch(ctx, torch, W, reps=3):
 
 
 def bench_serve(ctx, torch, W, duration_s=2.0, sweep=(5000.0, 10000.0, 20000.0),
-                sweep_requests=4000):
+                sweep_requests=4000, batch_refill=0):
     """Poisson stream at 2k req/s (70 % ray queries against 64 resident trees, 30 % voxelize
     requests), served by the continuous-batching request loop and by the plain
     one-request-per-launch loop on the same schedule; latency = submit -> completion. Then the
@@ -557,7 +557,7 @@ def bench_serve(ctx, torch, W, duration_s=2.0, sweep=(5000.0, 10000.0, 20000.0),
         hits = torch.zeros((R_total, 4), dtype=torch.int32, device="cuda")
         tt = torch.zeros(R_total, dtype=torch.float64, device="cuda")
         torch.cuda.synchronize()
-        srv = vxl.Server(0, batching=batching)
+        srv = vxl.Server(0, batching=batching, batch_refill=batch_refill if batching else 0)
         srv.hold(resident)
         # warm both lanes (scratch, pools) outside the measured stream
         for i in range(4):
@@ -926,7 +926,10 @@ def main():
     if not args.no_batch:
         line["batched"] = bench_batch(ctx, torch, W)
     if not args.no_serve:
-        line["serving"] = bench_serve(ctx, torch, W)
+        # the batching server's ray launches take the lane schedule the batch autotuner chose
+        refill = line.get("batched", {}).get("tuning", {}).get("dda_batch_refill", 0)
+        line["serving"] = bench_serve(ctx, torch, W, batch_refill=refill)
+        line["serving"]["batch_refill"] = refill
     if not args.no_cpu:
         log("cpu baseline (oracle) on the full workload")
         model, _ = host_cpu()

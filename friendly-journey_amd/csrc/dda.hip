@@ -922,6 +922,18 @@ void launch_traverse_batch(vxl_ctx *ctx, const RaySegment *segs, uint32_t S, hip
   VXL_HIP(hipEventRecord(ctx->trav_done, s));
 }
 
+uint64_t persistent_batch_min_rays(vxl_ctx *ctx) {
+  if (!ctx->batch_pl_min_rays) {
+    int per_cu = 0;
+    VXL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_traverse_batch_pl, 64,
+                                                         frame_lds_bytes(4, 64)));
+    if (per_cu < 1) per_cu = 1;
+    ctx->batch_pl_min_rays =
+        4ull * uint64_t(ctx->cu_count > 0 ? ctx->cu_count : 256) * uint64_t(per_cu) * 64ull;
+  }
+  return ctx->batch_pl_min_rays;
+}
+
 void launch_traverse_segs(vxl_ctx *ctx, const TraverseSeg *d_segs, const uint32_t *d_blk,
                           const uint64_t *d_runs, uint32_t S, uint64_t blocks, uint64_t runs,
                           uint32_t kmax, bool want_t, unsigned long long *counter,

@@ -154,7 +154,11 @@ vxl_tree *voxelize_one(vxl_ctx *ctx, const Request *r, uint64_t *pairs) {
 
 // Enqueues one batch on the lane's stream; returns its completion event. The plain mode
 // serves its single request through the per-request entry points' own kernels.
-hipEvent_t launch(vxl_ctx *ctx, Kind kind, std::vector<Request *> &batch, bool plain) {
+// A batch of rays takes the configured persistent-lane schedule only when it fills the
+// device's resident waves several times over: a smaller one runs one lane per ray, where
+// every ray starts at once (persistent waves would walk their runs' rays in turn).
+hipEvent_t launch(vxl_ctx *ctx, Kind kind, std::vector<Request *> &batch, bool plain,
+                  uint32_t refill) {
   std::lock_guard<std::mutex> lock(ctx->mu);
   if (plain) {
     Request *r = batch[0];
@@ -167,7 +171,12 @@ hipEvent_t launch(vxl_ctx *ctx, Kind kind, std::vector<Request *> &batch, bool p
     }
   } else if (kind == kRays) {
     std::vector<RaySegment> segs(batch.size());
-    for (size_t i = 0; i < batch.size(); i++) segs[i] = batch[i]->seg;
+    uint64_t rays = 0;
+    for (size_t i = 0; i < batch.size(); i++) {
+      segs[i] = batch[i]->seg;
+      rays += segs[i].R;
+    }
+    ctx->dda_batch_refill = refill && rays >= persistent_batch_min_rays(ctx) ? int(refill) : 0;
     launch_traverse_batch(ctx, segs.data(), uint32_t(segs.size()), ctx->stream);
   } else {
     std::vector<BatchInput> in;
@@ -224,7 +233,7 @@ void worker(vxl_server *S, int lane_id) {
         vxl_status st = VXL_OK;
         std::string msg;
         try {
-          f.ev = launch(ctx, batch[0]->kind, f.reqs, plain);
+          f.ev = launch(ctx, batch[0]->kind, f.reqs, plain, S->cfg.batch_refill);
         } catch (const Error &e) {
           st = e.code;
           msg = e.msg;
@@ -330,6 +339,10 @@ vxl_status vxl_server_create(int device, const vxl_server_config *cfg, vxl_serve
     return VXL_INVALID_ARG;
   }
   *out = nullptr;
+  if (cfg->batch_refill > 64) {
+    vxl::set_error("batch_refill must be 0 (one lane per ray) or in [1, 64]");
+    return VXL_INVALID_ARG;
+  }
   vxl_server *S = new vxl_server();
   S->device = device;
   S->cfg = *cfg;

@@ -116,6 +116,7 @@ struct vxl_ctx {
   int dda_block = 64;  // one-wave workgroups: a finished wave frees its slot at once
   int dda_batch_refill = 0;  // batched traversals: 0 one lane per ray, > 0 persistent lanes
                              // refilled once that many of a wave's lanes are idle
+  uint64_t batch_pl_min_rays = 0;  // 4 x the rays the persistent batch kernel holds at once
   // last-call timing (ms), filled when timing is enabled
   bool timing = false;
   hipEvent_t ev[8] = {};  // ev[0] call start, ev[1]/ev[2] around the SAT kernel, ev[3] end
@@ -375,6 +376,9 @@ void launch_traverse_segs(vxl_ctx *ctx, const TraverseSeg *d_segs, const uint32_
                           const uint64_t *d_runs, uint32_t S, uint64_t blocks, uint64_t runs,
                           uint32_t kmax, bool want_t, unsigned long long *counter,
                           hipStream_t s);
+// Rays that fill the device's resident persistent batch waves four times over (the
+// server's threshold for its persistent-lane schedule).
+uint64_t persistent_batch_min_rays(vxl_ctx *ctx);
 // Rays per run of the persistent batched traversal's run table.
 constexpr uint64_t kBatchRayRun = 256;
 // tree.hip: SparseTree::set_voxel over distinct in-bounds edits on the device; patches

@@ -80,7 +80,7 @@ class Tuning(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("budget_bytes", C.c_uint64), ("batching", C.c_uint32),
                 ("max_batch_requests", C.c_uint32), ("max_batch_rays", C.c_uint64),
-                ("max_batch_tris", C.c_uint64)]
+                ("max_batch_tris", C.c_uint64), ("batch_refill", C.c_uint32)]
 
 
 class ServerStats(C.Structure):
@@ -738,9 +738,10 @@ class Server:
     """The request loop (continuous batching of ray queries and voxelize requests)."""
 
     def __init__(self, device: int = 0, budget_bytes: int = 0, batching: bool = True,
-                 max_batch_requests: int = 0, max_batch_rays: int = 0, max_batch_tris: int = 0):
+                 max_batch_requests: int = 0, max_batch_rays: int = 0, max_batch_tris: int = 0,
+                 batch_refill: int = 0):
         cfg = ServerConfig(budget_bytes, 1 if batching else 0, max_batch_requests,
-                           max_batch_rays, max_batch_tris)
+                           max_batch_rays, max_batch_tris, batch_refill)
         h = _vp()
         _check(load().vxl_server_create(device, C.byref(cfg), C.byref(h)))
         self._h = h.value

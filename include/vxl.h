@@ -387,6 +387,10 @@ typedef struct vxl_server_config {
   uint32_t max_batch_requests; /* requests per launch (0 = 1024) */
   uint64_t max_batch_rays;     /* rays per traversal launch (0 = 64 Mi) */
   uint64_t max_batch_tris;     /* triangles per voxelize batch (0 = 16 Mi) */
+  uint32_t batch_refill;       /* lane schedule of the batched ray launches: 0 one lane per
+                                  ray; 1..64 persistent lanes (vxl_set_batch_refill) for
+                                  launches of at least four times the rays the device's
+                                  resident waves hold (smaller ones: one lane per ray) */
 } vxl_server_config;
 typedef struct vxl_server_stats {
   uint64_t submitted, completed, failed, batches;

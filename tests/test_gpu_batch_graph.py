@@ -166,6 +166,14 @@ def test_graph_replays_equal_eager_batch(ctx):
         g.launch()
         g.launch()
         _check_replay(g, buf, ref0, "replay 4")
+
+        _say("replay 5 on the context's stream, 6 on another stream right after it")
+        side = torch.cuda.Stream()
+        buf.d_hits.zero_()
+        torch.cuda.synchronize()
+        g.launch()
+        g.launch(side.cuda_stream)
+        _check_replay(g, buf, ref0, "replay 6 (other stream)")
     finally:
         g.free()
         for tr in ref0[0]:

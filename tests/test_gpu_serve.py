@@ -95,10 +95,11 @@ def fx(ctx):
     return Fixture(ctx)
 
 
-@pytest.mark.parametrize("batching", [True, False], ids=["batching", "plain"])
-def test_served_results_equal_alone(fx, batching):
-    _say(f"server batching={batching}: {len(fx.reqs)} requests")
-    srv = vxl.Server(0, batching=batching)
+@pytest.mark.parametrize("batching,refill", [(True, 0), (True, 16), (False, 0)],
+                         ids=["batching", "batching_persistent_lanes", "plain"])
+def test_served_results_equal_alone(fx, batching, refill):
+    _say(f"server batching={batching} batch_refill={refill}: {len(fx.reqs)} requests")
+    srv = vxl.Server(0, batching=batching, batch_refill=refill)
     out, hits, t = fx.run(srv)
     fx.check(out, hits, t)
     st = srv.stats()
@@ -109,6 +110,11 @@ def test_served_results_equal_alone(fx, batching):
         assert st["batches"] == len(fx.reqs)
     assert st["p99_ms"] >= st["p50_ms"] > 0
     srv.close()
+
+
+def test_server_rejects_bad_refill():
+    with pytest.raises(vxl.VxlError):
+        vxl.Server(0, batch_refill=65)
 
 
 def test_admission_under_tight_budget(fx):
