@@ -152,15 +152,16 @@ vxl_tree *voxelize_one(vxl_ctx *ctx, const Request *r, uint64_t *pairs) {
   return build_tree_from_grid(ctx, r->n, r->k, sat_pairs_dev(ctx), pairs);
 }
 
-// Enqueues one batch on the lane's stream; returns its completion event. The plain mode
-// serves its single request through the per-request entry points' own kernels.
-// A batch of rays takes the configured persistent-lane schedule only when it fills the
-// device's resident waves several times over: a smaller one runs one lane per ray, where
-// every ray starts at once (persistent waves would walk their runs' rays in turn).
+// Enqueues one batch on the lane's stream; returns its completion event. A batch of one
+// request (every plain-mode batch, and a batching lane's when requests arrive one at a
+// time) runs through the per-request entry points' own kernels, with no segment table to
+// upload. A batch of rays takes the configured persistent-lane schedule only when it fills
+// the device's resident waves several times over: a smaller one runs one lane per ray,
+// where every ray starts at once (persistent waves would walk their runs' rays in turn).
 hipEvent_t launch(vxl_ctx *ctx, Kind kind, std::vector<Request *> &batch, bool plain,
                   uint32_t refill) {
   std::lock_guard<std::mutex> lock(ctx->mu);
-  if (plain) {
+  if (plain || batch.size() == 1) {
     Request *r = batch[0];
     if (kind == kRays) {
       launch_traverse(ctx, r->seg.tree, r->seg.rays, r->seg.R, r->seg.hits, r->seg.t,
