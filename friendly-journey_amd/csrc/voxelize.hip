@@ -9,14 +9,12 @@
 // Stage 3 (k_offsets_units): per setup block, the pair offsets and the triangle holding the
 //   first pair of each work unit. (Calls with over 16.7M triangles take a device-wide scan,
 //   k_sat_plan and k_bstart instead.)
-// Stage 4 (k_sat): a persistent grid of waves takes work units from an atomic counter and
-//   walks them 64 pairs at a time, one pair per lane. The lane's triangle comes from the
-//   starts of the next 64 triangles, held one per lane: a forward lane permute and a
-//   ballot mark the chunk positions where triangles start, and the lane's triangle is the
-//   number of marks at or below it. The pair decodes to its voxel in the reference's scan
-//   order (x fastest, reference_cpu.rs:152-154), the plane test runs first, survivors
-//   queue in LDS, and
-//   every 64 queued survivors run the remaining 12 axes as a full wave. Each hit merges
+// Stage 4 (k_sat): a persistent grid of waves takes work units (ranges of consecutive
+//   pairs) from an atomic counter. A unit's triangles split into segments of up to 12
+//   consecutive pairs of one triangle; a lane walks one segment, decoding its first pair to
+//   a voxel once and stepping through the candidate box in the reference's scan order
+//   (x fastest, reference_cpu.rs:152-154). The plane test runs first, survivors queue in
+//   LDS, and every 64 queued survivors run the remaining 12 axes as a full wave. Each hit merges
 //   into a per-wave LDS table keyed by leaf-grid word; the table is flushed with 64-bit
 //   atomic ORs into the leaf grid and the brick bitmap (bit per 8^3 brick).
 // Many meshes share one launch: each record carries its mesh's leaf-grid base, so the
@@ -62,6 +60,10 @@ struct __align__(16) TriRec {
   uint32_t grid_base;  // mesh's first leaf-grid word / 512 (0 for a single mesh)
 };
 static_assert(sizeof(TriRec) == 80, "TriRec is one 80-byte record");
+static_assert(offsetof(TriRec, nrm) == 36 && offsetof(TriRec, rp) == 48 &&
+                  offsetof(TriRec, mnxy) == 52 && offsetof(TriRec, mnz_ex) == 56 &&
+                  offsetof(TriRec, exy) == 60,
+              "the SAT walk reads g, nrm and rp..exy as 16-byte blocks 0, 2 and 3");
 
 // The window of a kWindow triangle, in a side array beside the records (written for
 // windowed triangles only, so the others cost no bytes).
@@ -659,6 +661,21 @@ __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
   return (uint64_t(hi) << 32) | lo;
 }
 
+// Inclusive sum of v over lanes 0..lane of the wave, by row shifts within each row of 16
+// lanes and the two row broadcasts (lane 15 to row 1, lane 47 to row 3; then lane 31 to
+// rows 2 and 3). Lanes a shift or broadcast does not reach add 0. No lane addresses are
+// formed, so nothing is held in registers between calls.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  int x = int(v);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return uint32_t(x);
+}
+
 // Low 32 bits of the product of the low 24 bits of a and b (one full-rate multiply), as
 // an unsigned value: HIP's __umul24 returns int, which would make shifts of it arithmetic.
 __device__ __forceinline__ uint32_t mul24(uint32_t a, uint32_t b) {
@@ -801,21 +818,29 @@ __device__ __forceinline__ void table_flush(unsigned long long *tk, unsigned lon
   __builtin_amdgcn_wave_barrier();
 }
 
-// Chunks of a unit walked with 32-bit pair offsets from one base: every offset in a span,
-// and the 64 positions after it, stay below 0xffffffff, the clamp for starts past the span.
-constexpr uint64_t kSpanChunks = (uint64_t(1) << 26) - 2;
+// At most kSegPairs pairs of one triangle per segment (a triangle's pairs in a unit split
+// into that many segments of nearly equal size: 8 to 16 measured, 12 best on the terrain,
+// whose boxes hold 9, 18 or 27 cells), and the largest pair range whose per-lane counts
+// and segment numbers are taken in 32 bits.
+constexpr uint32_t kSegPairs = 12;
+constexpr uint64_t kSpanPairs = uint64_t(1) << 30;
 
-// Each wave walks chunks [c0, c1) of 64 consecutive pairs. A lane's triangle is the
-// last triangle whose first pair is at or before the lane's pair: the starts of the 64
-// triangles after the chunk's first triangle are loaded one per lane and searched
-// across lanes. Phase A evaluates the plane test and queues survivors in LDS; every 64
-// queued survivors run phase B as a full wave. Hits are merged per leaf word in an LDS
-// table and flushed to HBM in bulk.
+// Each wave walks units of consecutive pairs [u0, u1), taken from an atomic counter. The
+// unit's triangles are loaded 64 at a time, one per lane, with their pair ranges clipped
+// to the unit; each range splits into segments of at most kSegPairs pairs, and the wave
+// walks the segments 64 at a time, one per lane: the lane finds its segment's triangle
+// by a binary search over the lanes' running segment counts, loads the triangle's record
+// once, decodes the segment's first pair to its cell (two divisions) and then steps
+// through the box cell by cell (first axis fastest, the reference's scan order) with
+// adds. Phase A evaluates the plane test and queues survivors in LDS; every 64 queued
+// survivors run phase B as a full wave. Hits are merged per leaf word in an LDS table and
+// flushed to HBM in bulk. So the cost of finding a pair's triangle and cell is paid once
+// per segment, not once per pair.
 //
 // The kernel is held to 64 VGPRs (8 waves per SIMD, the most a SIMD holds; the LDS of 8
-// blocks, 8 x 14 KiB, fits a CU). Unit-level values (unit, chunk range, first triangle)
-// stay wave-uniform in scalar registers, which is what keeps the per-lane state within 64
-// VGPRs. The window decode is a branch that calls without windowed triangles never take.
+// blocks, 8 x 14 KiB, fits a CU). Unit-level values (unit, pair range, first triangle)
+// stay wave-uniform in scalar registers. The window decode is a branch that calls without
+// windowed triangles never take.
 //
 // kOwn: also lower each hit voxel's owner to the triangle index (single-mesh calls, where
 // the record index is the triangle index); kOwnLeaf writes owners only.
@@ -831,7 +856,6 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))
   __shared__ unsigned long long t_key[kWaves][kTable], t_val[kWaves][kTable];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint64_t P = plan->P, chunks_per_unit = plan->cpu, nunits = plan->nunits;
-  const uint64_t nchunks = (P + 63) / 64;
   unsigned int *next_unit = &plan->counter;
   uint32_t *qt = q_t[wave], *qxy = q_xy[wave], *qz = q_z[wave];
   unsigned long long *tk = t_key[wave], *tv = t_val[wave];
@@ -856,188 +880,173 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))
     unit = __builtin_amdgcn_readfirstlane(unit);
     if (uint64_t(unit) >= nunits) break;  // wave-uniform exit: every wave ends here
   }
-  const uint64_t c0 = uint64_t(unit) * chunks_per_unit;
-  const uint64_t c1 = min(c0 + chunks_per_unit, nchunks);
-  uint32_t t_first = ustart[unit];
-  uint64_t offn_raw;  // start offset of triangle t_first + 1 + lane (P past the last triangle)
-  {
-    const uint64_t idx = uint64_t(t_first) + 1 + uint64_t(lane);
-    offn_raw = idx <= T ? coff[idx] : ~0ull;
-  }
-  // The unit's chunks in spans of at most kSpanChunks: positions inside a span are 32-bit
-  // offsets from its first pair; a triangle start is kept as its offset from there, clamped
-  // to 0xffffffff past the span, or taken modulo 2^32 for the start of the span's first
-  // triangle, which may lie before it (p - start is then taken modulo 2^32 below).
-  for (uint64_t sb = c0; sb < c1; sb += kSpanChunks) {
-  const uint64_t ubase = sb * 64;
-  const uint32_t pend = uint32_t(min(P - ubase, uint64_t(0xffffffffu)));
-  const uint32_t nloc = uint32_t(min(c1 - sb, kSpanChunks));
-  auto rel = [ubase](uint64_t v) -> uint32_t {
-    const uint64_t r = v - ubase;
-    return r > 0xffffffffull ? 0xffffffffu : uint32_t(r);
-  };
-  uint32_t start_first = uint32_t(coff[t_first] - ubase);
-  for (uint32_t cl = 0; cl < nloc; cl++) {
-    const uint32_t base = cl * 64;
-    const uint32_t p = base + uint32_t(lane);
-    const bool valid = p < pend;
-    const uint32_t offn = rel(offn_raw);
-    const uint32_t rpos = offn - base;  // >= 1
-    // Chunk positions where a triangle starts: every lane whose start falls in the chunk
-    // sends a flag to the lane of that position by a forward permute (the others send to
-    // lane 0, which no start occupies), and a ballot of the received flags is the mark
-    // mask. Lanes no one sends to receive 0. Two starts at one position (a triangle with
-    // no candidates between them) leave fewer marks than starts: the search below.
-    const bool in_chunk = rpos < 64;
-    const int flag = __builtin_amdgcn_ds_permute(in_chunk ? int(rpos) * 4 : 0, 1);
-    const unsigned long long marks = __ballot(flag != 0) & ~1ull;
-    const uint32_t starts_in = __popcll(__ballot(in_chunk));
-    uint32_t t;
-    uint32_t start;
-    if (__popcll(marks) == starts_in) {
-      // one start per marked position: lane's triangle = t_first + marks at positions 1..lane,
-      // and its first pair is the highest of those positions
-      const unsigned long long m = marks & (below << 1);
-      const uint32_t k = __popcll(m);
-      t = t_first + k;
-      start = k ? base + uint32_t(63 - __clzll(m)) : start_first;
-    } else {
-      // several triangles start at one position (triangles without candidates)
+  const uint64_t u0 = uint64_t(unit) * chunks_per_unit * 64;
+  const uint64_t u1 = min(u0 + chunks_per_unit * 64, P);
+  uint32_t tb = ustart[unit];  // the triangle holding pair u0 (wave-uniform)
+  // The unit's pairs [u0, u1) in batches of the 64 triangles tb .. tb + 63, clipped to spans
+  // of at most kSpanPairs pairs so that per-lane counts stay 32-bit.
+  for (uint64_t pos = u0; pos < u1;) {
+    const uint64_t span_end = min(u1, pos + kSpanPairs);
+    const uint64_t idx = uint64_t(tb) + uint64_t(lane);
+    const uint64_t s = idx < T ? coff[idx] : P;  // coff[T] == P
+    const uint64_t e = idx < T ? coff[idx + 1] : P;
+    const uint64_t a = max(s, pos), b = min(e, span_end);
+    const uint32_t len = b > a ? uint32_t(b - a) : 0u;  // this lane's triangle's pairs here
+    // the first of them as the triangle's local pair index: 0 but for lane 0, whose
+    // triangle holds pos (every later triangle starts at or after the end of lane 0's)
+    const uint64_t skip0 = readlane_u64(a - s, 0);
+    // ceil(len / kSegPairs) segments of nearly equal size: the first len % nseg of them one
+    // pair longer than len / nseg
+    const uint32_t nseg = (len + kSegPairs - 1) / kSegPairs;
+    const uint32_t sq = nseg ? len / nseg : 0u;
+    const uint32_t sr = len - sq * nseg;
+    const uint32_t incl = wave_incl_scan(nseg);  // segment counts summed over lanes <= lane
+    const uint32_t total = uint32_t(__builtin_amdgcn_readlane(int(incl), 63));
+    for (uint32_t gb = 0; gb < total; gb += 64) {  // wave-uniform: 64 segments per pass
+      const uint32_t g = gb + uint32_t(lane);
+      // the lane holding segment g: the number of lanes whose inclusive count is <= g
       uint32_t k = 0;
 #pragma unroll
       for (int step = 32; step >= 1; step >>= 1) {
-        const uint32_t v = uint32_t(__shfl(int(offn), int(k + step) - 1, 64));
-        if (v <= p) k += step;
+        const uint32_t v = uint32_t(__shfl(int(incl), int(k) + step - 1, 64));
+        if (v <= g) k += uint32_t(step);
       }
-      if (uint32_t(__shfl(int(offn), 63, 64)) <= p) k = 64;  // the search above tops out at 63
-      const uint32_t kstart = uint32_t(__shfl(int(offn), k ? int(k) - 1 : 0, 64));
-      t = t_first + k;
-      start = k ? kstart : start_first;
-      if (k == 64 && valid) {  // 64 starts inside one chunk
-        const uint64_t pg = ubase + p;
-        uint64_t lo = t, hi = T - 1;  // last triangle with start <= p
-        while (lo < hi) {
-          const uint64_t mid = (lo + hi + 1) >> 1;
-          if (coff[mid] <= pg) lo = mid; else hi = mid - 1;
+      k = k > 63 ? 63 : k;  // only lanes past the last segment (g >= total) reach 64
+      const uint32_t first = uint32_t(__shfl(int(incl - nseg), int(k), 64));
+      const uint32_t sqk = uint32_t(__shfl(int(sq), int(k), 64));
+      const uint32_t srk = uint32_t(__shfl(int(sr), int(k), 64));
+      const uint64_t skipk = k == 0 ? skip0 : 0;
+      const uint32_t si = g - first;  // the segment's index in lane k's pairs
+      const uint32_t soff = si * sqk + min(si, srk);
+      uint32_t cnt = g < total ? sqk + (si < srk ? 1u : 0u) : 0u;
+      const uint32_t t = tb + k;
+      // the segment's first pair decoded in the triangle's box (first axis fastest)
+      uint32_t dx = 0, dy = 0, dz = 0, yrow = 0;
+      if (cnt) {
+        const TriRec &r = rec[t];
+        const uint32_t ex = r.mnz_ex >> 16, exy = r.exy;
+        const uint64_t l = skipk + soff;
+        if (__builtin_expect((r.flags & kWide) == 0, 1)) {
+          const uint32_t l32 = uint32_t(l);  // < 2^22 without kWide
+          dz = udiv_narrow(l32, exy, r.rcp_exy);
+          const uint32_t rem = l32 - mul24(dz, exy);
+          dy = udiv_narrow(rem, ex, r.rcp_ex);
+          dx = rem - mul24(dy, ex);
+        } else {
+          const uint64_t zz = l / exy, rem = l - zz * exy;
+          dz = uint32_t(zz);
+          dy = uint32_t(rem / ex);
+          dx = uint32_t(rem - uint64_t(dy) * ex);
         }
-        t = uint32_t(lo);
-        start = uint32_t(coff[lo] - ubase);
+        yrow = dy * ex;
       }
-    }
-    // next chunk's first triangle (the last one starting at or before base + 64) and the
-    // prefetch of its window of starts
-    uint32_t t_next = t_first;
-    uint32_t start_next = start_first;
-    {
-      const uint32_t nb = base + 64;
-      const uint32_t adv = __popcll(__ballot(offn <= nb));
-      if (adv == 64 && ubase + nb < P) {  // saturated window: search (wave-uniform)
-        const uint64_t nbg = ubase + nb;
-        uint64_t lo = uint64_t(t_first) + 64, hi = T - 1;
-        while (lo < hi) {
-          const uint64_t mid = (lo + hi + 1) >> 1;
-          if (coff[mid] <= nbg) lo = mid; else hi = mid - 1;
+      // walk the segment: one pair per lane per step, survivors of the plane test queued.
+      // The record's fields are read again at each step (cache hits) rather than held in
+      // registers across the survivor pass below, which needs them.
+      uint32_t tr = t;
+      while (__ballot(cnt != 0)) {  // wave-uniform
+        bool surv = false;
+        uint32_t x = 0, y = 0, z = 0;
+        if (cnt) {
+          asm volatile("" : "+v"(tr));  // the loads below stay in the loop
+          // the record's 16-byte blocks 0, 2 and 3 and its flags, loaded together
+          const TriRec &r = rec[tr];
+          const float4 q0 = reinterpret_cast<const float4 *>(&r)[0];  // g[0..3]
+          const float4 q2 = reinterpret_cast<const float4 *>(&r)[2];  // g[8], nrm[0..2]
+          const uint4 q3 = reinterpret_cast<const uint4 *>(&r)[3];    // rp, mnxy, mnz_ex, exy
+          const uint32_t flags = r.flags;
+          const uint32_t mnxy = q3.y, mnz_ex = q3.z, exy = q3.w;
+          const uint32_t ex = mnz_ex >> 16;
+          bool inside = true;
+          if (__builtin_expect((flags & kWindow) != 0, 0)) {
+            // (w, la, lb) -> the cell w of the window of column (a, b) along c
+            const WinRec wf = wrec[tr];
+            const uint32_t cinfo = wf.cinfo;
+            const uint32_t ax = cinfo & 3u, ec = cinfo >> 6;
+            const uint32_t m0 = mnxy & 0xffffu, m1 = mnxy >> 16, m2 = mnz_ex & 0xffffu;
+            const uint32_t ma = ax == 0 ? m1 : m0, mb = ax == 2 ? m1 : m2;
+            const uint32_t mc = ax == 0 ? m0 : (ax == 1 ? m1 : m2);
+            const uint32_t ca = ma + dy, cb = mb + dz;
+            const int32_t cc =
+                int32_t(floorf(wf.k0 + wf.ka * float(ca) + wf.kb * float(cb))) + int32_t(dx);
+            inside = cc >= int32_t(mc) && cc < int32_t(mc + ec);
+            const uint32_t ucc = uint32_t(cc);
+            x = ax == 0 ? ucc : ca;
+            y = ax == 1 ? ucc : (ax == 0 ? ca : cb);
+            z = ax == 2 ? ucc : cb;
+          } else {
+            x = dx + (mnxy & 0xffffu);
+            y = dy + (mnxy >> 16);
+            z = dz + (mnz_ex & 0xffffu);
+          }
+          const float cx = float(x) + 0.5f, cy = float(y) + 0.5f, cz = float(z) + 0.5f;
+          if (!inside) {
+            surv = false;
+          } else if (flags & kPlaneHoist) {
+            // the plane test of reference_cpu.rs:76-82 with per-triangle n and r:
+            // |normal.dot(ZERO) + d| = |n . v0| for a finite normal
+            const float v0x = q0.x - cx, v0y = q0.y - cy, v0z = q0.z - cz;
+            float dd = q2.y * v0x;
+            dd = dd + q2.z * v0y;
+            dd = dd + q2.w * v0z;
+            surv = !(fabsf(dd) > __uint_as_float(q3.x));
+          } else {
+            const Rel v = rel_to(r.g, cx, cy, cz);
+            surv = (flags & kLiteral) ? plane_passes<true>(v) : plane_passes<false>(v);
+          }
+          // next cell of the box, first axis fastest
+          cnt -= 1;
+          dx += 1;
+          if (dx == ex) {
+            dx = 0;
+            dy += 1;
+            yrow += ex;
+            if (yrow == exy) {
+              dy = 0;
+              yrow = 0;
+              dz += 1;
+            }
+          }
         }
-        t_next = uint32_t(lo);
-        start_next = uint32_t(coff[lo] - ubase);
-      } else if (adv) {
-        start_next = uint32_t(__builtin_amdgcn_readlane(int(offn), int(adv) - 1));
-        t_next = t_first + adv;
-      }
-    }
-    uint64_t offn_next = ~0ull;
-    if (sb + cl + 1 < c1) {
-      const uint64_t idx = uint64_t(t_next) + 1 + uint64_t(lane);
-      offn_next = idx <= T ? coff[idx] : ~0ull;
-    }
-
-    bool surv = false;
-    uint32_t x = 0, y = 0, z = 0;
-    if (valid) {
-      const TriRec r = rec[t];  // one copy: the record's loads issue together
-      const uint32_t ex = r.mnz_ex >> 16;
-      {
-        // the narrow decode runs on every lane (p - start < 2^22 unless kWide), so the
-        // record's reciprocals are loaded with the rest of it; wide triangles decode again
-        // below
-        const uint32_t l32 = (p - start) & 0x3fffffu;
-        z = udiv_narrow(l32, r.exy, r.rcp_exy);
-        const uint32_t rem = l32 - mul24(z, r.exy);
-        y = udiv_narrow(rem, ex, r.rcp_ex);
-        x = rem - mul24(y, ex);
-      }
-      if (__builtin_expect((r.flags & kWide) != 0, 0)) {
-        const uint64_t li = ubase + p - coff[t];  // the triangle's start, which may lie far back
-        const uint64_t zz = li / r.exy, rem = li - zz * r.exy;
-        z = uint32_t(zz);
-        y = uint32_t(rem / ex);
-        x = uint32_t(rem - uint64_t(y) * ex);
-      }
-      bool inside = true;
-      if (__builtin_expect((r.flags & kWindow) != 0, 0)) {
-        // (w, la, lb) -> the cell w of the window of column (a, b) along c
-        const WinRec wf = wrec[t];
-        const uint32_t cinfo = wf.cinfo;
-        const uint32_t ax = cinfo & 3u, ec = cinfo >> 6;
-        const uint32_t m0 = r.mnxy & 0xffffu, m1 = r.mnxy >> 16, m2 = r.mnz_ex & 0xffffu;
-        const uint32_t ma = ax == 0 ? m1 : m0, mb = ax == 2 ? m1 : m2;
-        const uint32_t mc = ax == 0 ? m0 : (ax == 1 ? m1 : m2);
-        const uint32_t ca = ma + y, cb = mb + z;
-        const int32_t cc = int32_t(floorf(wf.k0 + wf.ka * float(ca) + wf.kb * float(cb))) + int32_t(x);
-        inside = cc >= int32_t(mc) && cc < int32_t(mc + ec);
-        const uint32_t ucc = uint32_t(cc);
-        x = ax == 0 ? ucc : ca;
-        y = ax == 1 ? ucc : (ax == 0 ? ca : cb);
-        z = ax == 2 ? ucc : cb;
-      } else {
-        x += r.mnxy & 0xffffu;
-        y += r.mnxy >> 16;
-        z += r.mnz_ex & 0xffffu;
-      }
-      const float cx = float(x) + 0.5f, cy = float(y) + 0.5f, cz = float(z) + 0.5f;
-      if (!inside) {
-        surv = false;
-      } else if (r.flags & kPlaneHoist) {
-        // the plane test of reference_cpu.rs:76-82 with per-triangle n and r:
-        // |normal.dot(ZERO) + d| = |n . v0| for a finite normal
-        const float v0x = r.g[0] - cx, v0y = r.g[1] - cy, v0z = r.g[2] - cz;
-        float dd = r.nrm[0] * v0x;
-        dd = dd + r.nrm[1] * v0y;
-        dd = dd + r.nrm[2] * v0z;
-        surv = !(fabsf(dd) > r.rp);
-      } else {
-        const Rel v = rel_to(r.g, cx, cy, cz);
-        surv = (r.flags & kLiteral) ? plane_passes<true>(v) : plane_passes<false>(v);
-      }
-    }
-    // queue survivors
-    const unsigned long long sm = __ballot(surv);
-    if (surv) {
-      const uint32_t pos = qn + __popcll(sm & below);
-      qt[pos] = t;
-      qxy[pos] = x | (y << 16);
-      qz[pos] = z;
-    }
-    qn += __popcll(sm);
-    if (qn >= 64) {
-      __builtin_amdgcn_wave_barrier();
-      qn -= 64;
-      const uint32_t e = qn + lane;
-      unsigned long long key, bit;
-      const bool hit = finish_pair(rec, qt[e], qxy[e], qz[e], key, bit);
-      if (kOwn != kOwnNone && hit) owner_min<kOwn>(owner, leaf_map, own_n, qt[e], qxy[e], qz[e]);
-      if (kOwn != kOwnLeaf) {
-        used += table_insert(tk, tv, hit, key, bit);
-        if (used >= flush_at) {
-          table_flush(tk, tv, lane, leafgrid, brickbits);
-          used = 0;
+        // queue survivors
+        const unsigned long long sm = __ballot(surv);
+        if (surv) {
+          const uint32_t qp = qn + __popcll(sm & below);
+          qt[qp] = tr;
+          qxy[qp] = x | (y << 16);
+          qz[qp] = z;
+        }
+        qn += __popcll(sm);
+        if (qn >= 64) {
+          __builtin_amdgcn_wave_barrier();
+          qn -= 64;
+          const uint32_t qe = qn + lane;
+          unsigned long long key, bit;
+          const bool hit = finish_pair(rec, qt[qe], qxy[qe], qz[qe], key, bit);
+          if (kOwn != kOwnNone && hit)
+            owner_min<kOwn>(owner, leaf_map, own_n, qt[qe], qxy[qe], qz[qe]);
+          if (kOwn != kOwnLeaf) {
+            used += table_insert(tk, tv, hit, key, bit);
+            if (used >= flush_at) {
+              table_flush(tk, tv, lane, leafgrid, brickbits);
+              used = 0;
+            }
+          }
         }
       }
     }
-    offn_raw = offn_next;
-    t_first = t_next;
-    start_first = start_next;
-  }
+    // the next batch starts at the first triangle past this one's pairs
+    uint64_t ib = idx;
+    asm volatile("" : "+v"(ib));  // a second load, not the first one's value kept live
+    const uint64_t eb = ib < T ? coff[ib + 1] : P;
+    const uint64_t e63 = readlane_u64(eb, 63);
+    if (e63 < span_end) {
+      pos = e63;
+      tb += 64;
+    } else {
+      tb += uint32_t(__popcll(__ballot(eb <= span_end)));
+      pos = span_end;
+    }
   }
   }
   __builtin_amdgcn_wave_barrier();
