@@ -552,15 +552,21 @@ __global__ __launch_bounds__(64) void k_traverse_pl(
   w.ox = w.oy = w.oz = w.cx = w.cy = w.cz = 0;
   uint64_t ri = 0;
   uint64_t run = 0, run_end = 0;  // the wave's rays not yet handed out (wave-uniform)
-  bool active = false, exhausted = false;
+  bool active = false, exhausted = false, first = true;
   for (;;) {
     // refill: the idle lanes take the next rays of the wave's run
     const unsigned long long idle = __ballot(!active);
     const uint32_t ni = __popcll(idle);
     if (!exhausted) {  // wave-uniform; ni >= refill_min here (or the wave just started)
       if (run == run_end) {
-        unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(counter, (unsigned long long)kRayRun);
+        // the wave's first run is its own (run = its block index); later runs come from
+        // the counter, past the grid's first runs (one counter address serializes its
+        // atomics, so the first runs do not wait on it)
+        unsigned long long base = uint64_t(blockIdx.x) * kRayRun;
+        if (!first && lane == 0)
+          base = atomicAdd(counter, (unsigned long long)kRayRun) +
+                 uint64_t(gridDim.x) * kRayRun;
+        first = false;
         run = __shfl(base, 0, 64);
         run_end = run + kRayRun < R ? run + kRayRun : R;
         if (run >= R) {
@@ -624,14 +630,16 @@ __global__ __launch_bounds__(64) void k_traverse_batch_pl(
   uint4 *hit = nullptr;                                                   // the lane's record
   uint32_t cur = 0;               // segment of the wave's run (wave-uniform)
   uint64_t run = 0, run_end = 0;  // its rays not yet handed out (wave-uniform)
-  bool active = false, exhausted = false;
+  bool active = false, exhausted = false, first = true;
   for (;;) {
     const unsigned long long idle = __ballot(!active);
     const uint32_t ni = __popcll(idle);
     if (!exhausted) {  // wave-uniform
       if (run == run_end) {
-        unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(counter, 1ull);
+        // the wave's first run is its own, later ones from the counter (as k_traverse_pl)
+        unsigned long long base = blockIdx.x;
+        if (!first && lane == 0) base = atomicAdd(counter, 1ull) + gridDim.x;
+        first = false;
         base = __shfl(base, 0, 64);
         if (base >= nruns) {
           exhausted = true;
