@@ -361,7 +361,7 @@ __global__ void k_sat_plan(const uint64_t *__restrict__ total, uint64_t want,
   plan->P_ref = P + skip;
   plan->cpu = cpu;
   plan->nunits = (nchunks + cpu - 1) / cpu;
-  plan->counter = 0;
+  for (int c = 0; c < kUnitCounters; c++) plan->counter[c][0] = 0;
 }
 
 // ustart[u] = triangle holding the first pair of unit u (triangles with no candidate never
@@ -459,7 +459,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_setup_plan(uint64_t *__restric
       plan->P_ref = P + skip;
       plan->cpu = cpu;
       plan->nunits = (nchunks + cpu - 1) / cpu;
-      plan->counter = 0;
+      for (int c = 0; c < kUnitCounters; c++) plan->counter[c][0] = 0;
     }
   }
 }
@@ -856,14 +856,18 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))
   __shared__ unsigned long long t_key[kWaves][kTable], t_val[kWaves][kTable];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint64_t P = plan->P, chunks_per_unit = plan->cpu, nunits = plan->nunits;
-  unsigned int *next_unit = &plan->counter;
+  // the counter of this workgroup's class (every class has workgroups: grids hold far more
+  // than kUnitCounters of them, and a smaller grid uses as many classes as it has)
+  const uint32_t nctr = gridDim.x < uint32_t(kUnitCounters) ? gridDim.x : uint32_t(kUnitCounters);
+  const uint32_t cls = blockIdx.x % nctr;
+  unsigned int *next_unit = &plan->counter[cls][0];
   uint32_t *qt = q_t[wave], *qxy = q_xy[wave], *qz = q_z[wave];
   unsigned long long *tk = t_key[wave], *tv = t_val[wave];
   // Every wave's first unit is its own (unit = the wave's index in the grid); units past
-  // the grid's waves are handed out by an atomic counter, and only when there are any. One
-  // counter address serves the whole device and its atomics serialize: when all 8192 waves
-  // took their first unit there, the last ones started tens of microseconds late (terrain
-  // SAT 0.419 -> 0.347 ms without it, the 16k-triangle sphere 0.136 -> 0.047 ms).
+  // the grid's waves are handed out by atomic counters, and only when there are any. The
+  // atomics on one counter address serialize: when all 8192 waves took their first unit
+  // from one counter, the last ones started tens of microseconds late (terrain SAT 0.419 ->
+  // 0.347 ms without it, the 16k-triangle sphere 0.136 -> 0.047 ms).
   // A wave without a unit leaves before touching LDS (the grid is sized for the device).
   const uint32_t nwaves = gridDim.x * kWaves;
   uint32_t unit = blockIdx.x * kWaves + uint32_t(wave);
@@ -880,9 +884,12 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))
   for (bool first = true;; first = false) {
   if (!first) {
     if (uint64_t(nwaves) >= nunits) break;  // no units past the waves' own
-    if (lane == 0) unit = atomicAdd(next_unit, 1u) + nwaves;
-    unit = __builtin_amdgcn_readfirstlane(unit);  // lane 0's value, in a scalar register
-    if (uint64_t(unit) >= nunits) break;  // wave-uniform exit: every wave ends here
+    uint32_t k = 0;
+    if (lane == 0) k = atomicAdd(next_unit, 1u);
+    k = __builtin_amdgcn_readfirstlane(k);  // lane 0's value, in a scalar register
+    const uint64_t u = uint64_t(nwaves) + cls + uint64_t(nctr) * k;
+    if (u >= nunits) break;  // wave-uniform exit: every wave ends here
+    unit = uint32_t(u);
   }
   const uint64_t u0 = uint64_t(unit) * chunks_per_unit * 64;
   const uint64_t u1 = min(u0 + chunks_per_unit * 64, P);
@@ -1170,7 +1177,9 @@ void run_sat(vxl_ctx *ctx, const float *d_tris, uint64_t T, const VoxelizeParams
 const uint64_t *sat_pairs_dev(vxl_ctx *ctx) { return &ctx->bstart.as<SatPlan>()->P_ref; }
 
 namespace {
-__global__ void k_plan_rearm(SatPlan *__restrict__ plan) { plan->counter = 0; }
+__global__ void k_plan_rearm(SatPlan *__restrict__ plan) {
+  for (int c = 0; c < kUnitCounters; c++) plan->counter[c][0] = 0;
+}
 }  // namespace
 
 void sat_owner_pass(vxl_ctx *ctx, uint64_t T, uint32_t *owner, const uint32_t *leaf_map,

@@ -268,6 +268,7 @@ inline size_t batch_staging_bytes(uint32_t G) {
 }
 // The SAT launch plan the device derives from the pair total (voxelize.hip).
 constexpr int kSkipSlots = 64;
+constexpr int kUnitCounters = 8;
 constexpr size_t kPlanBytes = 2048;  // SatPlan, then the unit start table
 struct SatPlan {
   uint64_t P;       // decoded pairs (what the SAT kernel walks)
@@ -278,7 +279,10 @@ struct SatPlan {
   // kSkipSlots slots by setup block, so the atomics do not meet on one address); the plan
   // of call e sums them and clears skipped[(e + 1) & 1] for the next call on the stream
   unsigned long long skipped[2][kSkipSlots];
-  unsigned int counter;    // next unit to hand out
+  // units past the SAT grid's waves' own first units, handed out by kUnitCounters counters
+  // 64 bytes apart (counter c serves the workgroups with index c mod kUnitCounters and
+  // hands out every kUnitCounters-th unit), so the atomics do not queue on one address
+  unsigned int counter[kUnitCounters][16];
 };
 // Enqueues setup + SAT into ctx->leafgrid / ctx->bitmap (level-1); no host round trip.
 // The reference's pair count stays on the device at sat_pairs_dev(ctx) until the next call.
