@@ -220,3 +220,24 @@ def test_world_to_grid_override_bit_exact(ctx):
     assert e.value.name == "VXL_INVALID_ARG"
     # with an override, a voxel size whose reciprocal overflows is not an error (core.rs:124)
     ctx.voxelize_tree(meshes[0], n, (0, 0, 0), 1e-39, 1e-4, mat=mat).free()
+
+
+@pytest.mark.parametrize("waves,units", [(32, 1), (8, 4), (16, 8), (4, 64), (1, 1024)])
+def test_work_unit_schedules_bit_exact(waves, units):
+    """Every SAT work-distribution shape gives the oracle's words: each wave's own first unit
+    alone (1 unit per wave), and units past the grid's waves handed out by the 8 interleaved
+    counters (up to 1024 units per wave, so most come from the counters), on a sphere whose
+    triangles split into several segments and a soup with many one-cell boxes."""
+    c = vxl.Context(0)
+    try:
+        c.set_tuning(sat_waves_per_cu=waves, sat_units_per_wave=units)
+        for name, tris, n, vs in (("sphere_256", W.uv_sphere(64, 32), 512, 1 / 256),
+                                  ("soup_128", W.soup(20000, seed=91, mu=np.log(3 / 128)), 128,
+                                   1 / 128)):
+            _say(f"units {name} waves/cu={waves} units/wave={units}")
+            want, _, pairs = O.voxelize_surface(tris, n, (0, 0, 0), vs, 1e-4)
+            got, gpairs = c.voxelize_dense(tris, n, (0, 0, 0), vs, 1e-4)
+            assert gpairs == pairs
+            assert np.array_equal(got, want), f"{int((got != want).sum())} words differ"
+    finally:
+        c.close()
