@@ -10,8 +10,9 @@
 //   first pair of each work unit. (Calls with over 16.7M triangles take a device-wide scan,
 //   k_sat_plan and k_bstart instead.)
 // Stage 4 (k_sat): a persistent grid of waves takes work units (ranges of consecutive
-//   pairs) from an atomic counter. A unit's triangles split into segments of up to 12
-//   consecutive pairs of one triangle; a lane walks one segment, decoding its first pair to
+//   pairs): its own first unit, then more from atomic counters. A unit's triangles split
+//   into segments of up to 12 consecutive pairs of one triangle; a lane walks one segment,
+//   decoding its first pair to
 //   a voxel once and stepping through the candidate box in the reference's scan order
 //   (x fastest, reference_cpu.rs:152-154). The plane test runs first, survivors queue in
 //   LDS, and every 64 queued survivors run the remaining 12 axes as a full wave. Each hit merges
@@ -825,8 +826,9 @@ __device__ __forceinline__ void table_flush(unsigned long long *tk, unsigned lon
 constexpr uint32_t kSegPairs = 12;
 constexpr uint64_t kSpanPairs = uint64_t(1) << 30;
 
-// Each wave walks units of consecutive pairs [u0, u1), taken from an atomic counter. The
-// unit's triangles are loaded 64 at a time, one per lane, with their pair ranges clipped
+// Each wave walks units of consecutive pairs [u0, u1): its own, then any left over from the
+// counters (below). The unit's triangles are loaded 64 at a time, one per lane, with their
+// pair ranges clipped
 // to the unit; each range splits into segments of at most kSegPairs pairs, and the wave
 // walks the segments 64 at a time, one per lane: the lane finds its segment's triangle
 // by a binary search over the lanes' running segment counts, loads the triangle's record
