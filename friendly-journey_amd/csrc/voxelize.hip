@@ -12,11 +12,11 @@
 // Stage 4 (k_sat): a persistent grid of waves takes work units (ranges of consecutive
 //   pairs): its own first unit, then more from atomic counters. A unit's triangles split
 //   into segments of up to 12 consecutive pairs of one triangle; a lane walks one segment,
-//   decoding its first pair to
-//   a voxel once and stepping through the candidate box in the reference's scan order
-//   (x fastest, reference_cpu.rs:152-154). The plane test runs first, survivors queue in
-//   LDS, and every 64 queued survivors run the remaining 12 axes as a full wave. Each hit merges
-//   into a per-wave LDS table keyed by leaf-grid word; the table is flushed with 64-bit
+//   decoding its first pair to a voxel once and stepping through the candidate box in the
+//   reference's scan order (x fastest, reference_cpu.rs:152-154). The plane test runs
+//   first, survivors queue in LDS, and every 64 queued survivors run the remaining 12 axes
+//   as a full wave. Each hit merges into a per-wave LDS table keyed by leaf-grid word; the
+//   table is flushed with 64-bit
 //   atomic ORs into the leaf grid and the brick bitmap (bit per 8^3 brick).
 // Many meshes share one launch: each record carries its mesh's leaf-grid base, so the
 //   pair loop is the same for one mesh or hundreds.
