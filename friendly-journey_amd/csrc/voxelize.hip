@@ -16,8 +16,8 @@
 //   reference's scan order (x fastest, reference_cpu.rs:152-154). The plane test runs
 //   first, survivors queue in LDS, and every 64 queued survivors run the remaining 12 axes
 //   as a full wave. Each hit merges into a per-wave LDS table keyed by leaf-grid word; the
-//   table is flushed with 64-bit
-//   atomic ORs into the leaf grid and the brick bitmap (bit per 8^3 brick).
+//   table is flushed with 64-bit atomic ORs into the leaf grid and the brick bitmap (bit per
+//   8^3 brick).
 // Many meshes share one launch: each record carries its mesh's leaf-grid base, so the
 //   pair loop is the same for one mesh or hundreds.
 // Owners (VoxelizeOpts::store_owner, reference_cpu.rs:163-169): with an owner array, each
