@@ -319,6 +319,9 @@ def bench_dda(ctx, torch, W, reps=5):
     n1, vs1 = W.grid_for(256)
     cases.append(("sphere_256", W.uv_sphere(), n1, vs1, 256))
     cases.append(("soup_2048", W.soup(8_388_608, seed=0x5EED_0006), 2048, 1.0 / 2048, 2048))
+    # the headline's terrain tree at 1024^3 (Resolution(2048)) under the same camera + random rays
+    n2, vs2 = W.grid_for(1024)
+    cases.append(("terrain_1024", W.terrain_1024(), n2, vs2, 1024))
     for name, tris, n, vs, m in cases:
         log(f"dda {name}: voxelize T={len(tris)} n={n}")
         tree = ctx.voxelize_tree(tris, n, (0, 0, 0), vs, 1e-4)

@@ -73,6 +73,8 @@ __device__ __forceinline__ uint32_t floor_to_cell(float v, uint32_t dim) {
 struct Ray {
   float o[3], d[3], rinv[3];
   bool big[3];
+  uint32_t rsel;  // per axis (a byte each, x low): the row of the ray's direction in the
+                  // range-mask table (below)
 };
 
 // Walker::enter (mirror.rs:48-85) for one axis. The d > 0 and d < 0 cases differ only in
@@ -138,17 +140,55 @@ struct Walk {
                   // leaf's occupied bounds in the ray's direction (15: the axis never steps)
 };
 
+// Cells of a 4^3 node frame (64 bits in child_bit order) whose coordinate along axis a is
+// >= c (s = 0) or <= c (s = 1); table row a * 8 + s * 4 + c.
+constexpr unsigned long long range_mask(int a, int s, int c) {
+  unsigned long long m = 0;
+  for (int b = 0; b < 64; b++) {
+    const int coord = a == 0 ? ((b & 1) | ((b >> 2) & 2))
+                             : (a == 1 ? (((b >> 1) & 1) | ((b >> 3) & 2))
+                                       : (((b >> 2) & 1) | ((b >> 4) & 2)));
+    if (s ? coord <= c : coord >= c) m |= 1ull << b;
+  }
+  return m;
+}
+#define VXL_RANGE4(a, s) range_mask(a, s, 0), range_mask(a, s, 1), range_mask(a, s, 2), \
+                         range_mask(a, s, 3)
+__constant__ unsigned long long kRangeMasks[24] = {VXL_RANGE4(0, 0), VXL_RANGE4(0, 1),
+                                                   VXL_RANGE4(1, 0), VXL_RANGE4(1, 1),
+                                                   VXL_RANGE4(2, 0), VXL_RANGE4(2, 1)};
+#undef VXL_RANGE4
+static_assert(range_mask(0, 0, 1) == 0xFFAAFFAAFFAAFFAAull, "x >= 1: x bit 0 or x bit 1");
+static_assert(range_mask(2, 1, 1) == 0x00000000FFFFFFFFull, "z <= 1: z bit 1 clear");
+
+// Every block of a walking kernel copies the table into LDS before any thread leaves.
+__device__ __forceinline__ void load_range_masks(unsigned long long *rmask) {
+  if (threadIdx.x < 24) rmask[threadIdx.x] = kRangeMasks[threadIdx.x];
+  __syncthreads();
+}
+
+// 1/d, the |d| >= 2^100 flags and the range-mask rows of a ray whose o and d are set. The
+// row follows the sign bit: a ray that moves towards +axis only reaches cells >= its own
+// along it, one towards -axis cells <= its own; d = +-0 or NaN never steps along the axis,
+// so either row holds the cells it reaches.
+__device__ __forceinline__ void finish_ray(Ray &r) {
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    r.rinv[a] = 1.0f / r.d[a];
+    r.big[a] = !(fabsf(r.d[a]) < 0x1p100f);
+  }
+  r.rsel = ((__float_as_uint(r.d[0]) >> 31) * 4) |
+           ((8 + (__float_as_uint(r.d[1]) >> 31) * 4) << 8) |
+           ((16 + (__float_as_uint(r.d[2]) >> 31) * 4) << 16);
+}
+
 __device__ __forceinline__ void load_ray(const float *__restrict__ rays, uint64_t ri, Ray &r) {
   const float2 a = *reinterpret_cast<const float2 *>(rays + 6 * ri);
   const float2 b = *reinterpret_cast<const float2 *>(rays + 6 * ri + 2);
   const float2 c = *reinterpret_cast<const float2 *>(rays + 6 * ri + 4);
   r.o[0] = a.x; r.o[1] = a.y; r.o[2] = b.x;
   r.d[0] = b.y; r.d[1] = c.x; r.d[2] = c.y;
-#pragma unroll
-  for (int a = 0; a < 3; a++) {
-    r.rinv[a] = 1.0f / r.d[a];
-    r.big[a] = !(fabsf(r.d[a]) < 0x1p100f);
-  }
+  finish_ray(r);
 }
 
 // Grid clip, ray_aabb_f32 (mirror.rs:169-192) + the t1 < 0 check (:273-277); false when
@@ -304,6 +344,12 @@ constexpr uint32_t kLeafPast = 1u << 30;
 // steps once more, so a lane crossing empty cells needs fewer iterations (soup_2048: -2.5%;
 // a third step per iteration, or the second on the cache-resident sphere, costs more than
 // it saves).
+// After a step, a node frame whose child mask holds no cell in the box still ahead of the ray
+// pops at once (the cells on the ray's side of the new cell along each axis, from the
+// range-mask table `rmask` in LDS: the walk is monotone along each axis, so every cell the
+// mirror would still visit in the frame is in that box and empty). The walk's hit is the
+// mirror's; the empty visits are skipped (terrain_1024's tree: 4.40 -> 3.96 ms with one lane
+// per ray, 3.88 -> 3.49 ms on persistent lanes; sphere_256 -2%, soup_2048 -2%).
 // `frames` is this lane's column of the LDS frame rows, `fstride` the row pitch. kStats
 // counts the descents into leaves and into nodes and the visits (cell tests; the mirror's
 // walk statistics), so it walks every cell as the mirror does.
@@ -313,6 +359,7 @@ __device__ __forceinline__ uint32_t walk_visit(const Ray &r, Walk &w,
                                                const uint32_t *__restrict__ leafw,
                                                const uint32_t *__restrict__ bounds,
                                                uint32_t *frames, uint32_t fstride,
+                                               const unsigned long long *rmask,
                                                uint32_t *stats = nullptr) {
   if (!(w.iter & (kSkip ? kPendingStep | kLeafPast : kPendingStep))) {
     if (kStats) stats[2] += 1;
@@ -366,6 +413,17 @@ __device__ __forceinline__ uint32_t walk_visit(const Ray &r, Walk &w,
       load_leaf_word(w, leafw);
       return kWalking;
     }
+    // a node frame whose child mask holds nothing among the cells still ahead of the ray
+    // (those on the ray's side of the new cell along each axis, the cell included) pops now:
+    // the mirror would step through them, all empty, and pop at the frame's end
+    bool none_ahead = false;
+    if (!kStats && w.level >= 2) {
+      const uint32_t sel = r.rsel;
+      const unsigned long long m = rmask[(sel & 0xffu) + w.cx] &
+                                   rmask[((sel >> 8) & 0xffu) + w.cy] & rmask[(sel >> 16) + w.cz];
+      none_ahead = ((w.mlo & uint32_t(m)) | (w.mhi & uint32_t(m >> 32))) == 0;
+    }
+    if (!none_ahead) {
     if (kSkip && !kStats) {
       // a second visit in the same iteration when the new cell is empty: step again. What
       // this second step cannot finish is left to the next visit through the flags: an
@@ -384,6 +442,7 @@ __device__ __forceinline__ uint32_t walk_visit(const Ray &r, Walk &w,
       }
     }
     return kWalking;
+    }
   }
   if (kSkip) w.iter &= ~kLeafPast;
   if (w.level == w.root_level) return kLeft;
@@ -443,7 +502,8 @@ __device__ __forceinline__ void store_result(uint64_t ri, const Ray &r, const Wa
 __device__ __forceinline__ void traverse_ray(
     uint64_t ri, const float *__restrict__ rays, const uint32_t *__restrict__ nodes,
     const uint32_t *__restrict__ leafw, const uint32_t *__restrict__ bounds, uint32_t n,
-    uint32_t k, uint64_t leaf_count, uint4 *__restrict__ hits, double *__restrict__ tout) {
+    uint32_t k, uint64_t leaf_count, uint4 *__restrict__ hits, double *__restrict__ tout,
+    const unsigned long long *rmask) {
   extern __shared__ uint32_t frames_lds[];
   Ray r;
   load_ray(rays, ri, r);
@@ -454,7 +514,8 @@ __device__ __forceinline__ void traverse_ray(
   if (clip_grid(r, n, leaf_count, t0)) {
     begin_walk(r, w, k, t0, nodes, leafw);
     do {
-      status = walk_visit(r, w, nodes, leafw, bounds, frames_lds + threadIdx.x, blockDim.x);
+      status = walk_visit(r, w, nodes, leafw, bounds, frames_lds + threadIdx.x, blockDim.x,
+                          rmask);
     } while (status == kWalking);
   }
   store_result(ri, r, w, status, hits, tout);
@@ -464,9 +525,11 @@ __global__ __launch_bounds__(256) void k_traverse(
     const float *__restrict__ rays, uint64_t R, const uint32_t *__restrict__ nodes,
     const uint32_t *__restrict__ leafw, const uint32_t *__restrict__ bounds, uint32_t n,
     uint32_t k, uint64_t leaf_count, uint4 *__restrict__ hits, double *__restrict__ tout) {
+  __shared__ unsigned long long rmask[24];
+  load_range_masks(rmask);
   const uint64_t ri = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (ri >= R) return;
-  traverse_ray(ri, rays, nodes, leafw, bounds, n, k, leaf_count, hits, tout);
+  traverse_ray(ri, rays, nodes, leafw, bounds, n, k, leaf_count, hits, tout, rmask);
 }
 
 // t of every ray from its hit record, as store_result computes it: the persistent walk
@@ -513,7 +576,8 @@ __global__ __launch_bounds__(64) void k_traverse_stats(
       begin_walk(r, w, k, t0, nodes, leafw);
       uint32_t st;
       do {
-        st = walk_visit<true>(r, w, nodes, leafw, bounds, frames_lds + threadIdx.x, 64, st3);
+        st = walk_visit<true>(r, w, nodes, leafw, bounds, frames_lds + threadIdx.x, 64, nullptr,
+                              st3);
       } while (st == kWalking);
     }
   }
@@ -544,6 +608,8 @@ __global__ __launch_bounds__(64) void k_traverse_pl(
     uint32_t k, uint64_t leaf_count, uint4 *__restrict__ hits,
     unsigned long long *__restrict__ counter, uint32_t refill_min) {
   extern __shared__ uint32_t frames_lds[];
+  __shared__ unsigned long long rmask[24];
+  load_range_masks(rmask);
   uint32_t *const frames = frames_lds + threadIdx.x;
   const uint32_t lane = threadIdx.x;
   const unsigned long long below = (1ull << lane) - 1ull;
@@ -598,7 +664,8 @@ __global__ __launch_bounds__(64) void k_traverse_pl(
     const uint32_t until = exhausted ? 64u : refill_min;
     for (;;) {
       if (active) {
-        const uint32_t status = walk_visit<false, true>(r, w, nodes, leafw, bounds, frames, 64);
+        const uint32_t status =
+            walk_visit<false, true>(r, w, nodes, leafw, bounds, frames, 64, rmask);
         if (status != kWalking) {
           store_result(ri, r, w, status, hits, nullptr);
           active = false;
@@ -619,6 +686,8 @@ __global__ __launch_bounds__(64) void k_traverse_batch_pl(
     const TraverseSeg *__restrict__ seg, const uint64_t *__restrict__ run_begin, uint32_t S,
     unsigned long long *__restrict__ counter, uint32_t refill_min) {
   extern __shared__ uint32_t frames_lds[];
+  __shared__ unsigned long long rmask[24];
+  load_range_masks(rmask);
   uint32_t *const frames = frames_lds + threadIdx.x;
   const uint32_t lane = threadIdx.x;
   const unsigned long long below = (1ull << lane) - 1ull;
@@ -678,7 +747,7 @@ __global__ __launch_bounds__(64) void k_traverse_batch_pl(
     const uint32_t until = exhausted ? 64u : refill_min;
     for (;;) {
       if (active) {
-        const uint32_t status = walk_visit(r, w, nodes, leafw, bounds, frames, 64);
+        const uint32_t status = walk_visit(r, w, nodes, leafw, bounds, frames, 64, rmask);
         if (status != kWalking) {
           store_result(0, r, w, status, hit, nullptr);
           active = false;
@@ -728,6 +797,8 @@ __global__ __launch_bounds__(64) void k_render(vxl_camera cam, const uint32_t *_
                                                const uint32_t *__restrict__ colors,
                                                uint32_t ncolors, uint32_t *__restrict__ out) {
   extern __shared__ uint32_t frames_lds[];
+  __shared__ unsigned long long rmask[24];
+  load_range_masks(rmask);
   const uint32_t px = blockIdx.x * 8 + (threadIdx.x & 7), py = blockIdx.y * 8 + (threadIdx.x >> 3);
   if (px >= cam.width || py >= cam.height) return;
   const float w = float(cam.width), h = float(cam.height);
@@ -743,9 +814,8 @@ __global__ __launch_bounds__(64) void k_render(vxl_camera cam, const uint32_t *_
     for (int a = 0; a < 3; a++) {
       r.o[a] = cam.eye[a];
       r.d[a] = v[a] / len;
-      r.rinv[a] = 1.0f / r.d[a];
-      r.big[a] = !(fabsf(r.d[a]) < 0x1p100f);
-      }
+    }
+    finish_ray(r);
   }
   Walk wk;
   wk.ox = wk.oy = wk.oz = wk.cx = wk.cy = wk.cz = 0;
@@ -754,7 +824,8 @@ __global__ __launch_bounds__(64) void k_render(vxl_camera cam, const uint32_t *_
   if (clip_grid(r, n, leaf_count, t0)) {
     begin_walk(r, wk, k, t0, nodes, leafw);
     do {
-      status = walk_visit<false, kSkip>(r, wk, nodes, leafw, bounds, frames_lds + threadIdx.x, 64);
+      status = walk_visit<false, kSkip>(r, wk, nodes, leafw, bounds, frames_lds + threadIdx.x, 64,
+                                        rmask);
     } while (status == kWalking);
   }
   uint32_t color;
@@ -782,9 +853,11 @@ __global__ __launch_bounds__(256) void k_traverse_batch(const TraverseSeg *__res
   const uint32_t g = dev::upper_index(blk_begin, S, blockIdx.x);
   const TraverseSeg &sg = seg[g];
   const uint64_t ri = uint64_t(blockIdx.x - blk_begin[g]) * blockDim.x + threadIdx.x;
+  __shared__ unsigned long long rmask[24];
+  load_range_masks(rmask);
   if (ri >= sg.R) return;
   traverse_ray(ri, sg.rays, sg.nodes, sg.leafw, sg.bounds, sg.n, sg.k, sg.leaf_count, sg.hits,
-               sg.t);
+               sg.t, rmask);
 }
 
 // LDS for the suspended frames: levels 2..k+1 (the walk saves a frame when it descends
@@ -800,18 +873,21 @@ constexpr uint32_t kCounterSlots = 1024;
 // a memset, so the pair stays correct when a caller captures the call in a HIP graph.
 __global__ void k_clear_counter(unsigned long long *ctr) { *ctr = 0ull; }
 
-// The automatic lane schedule: a tree whose walked arrays (64-byte leaves, 4-byte bounds,
-// 12-byte nodes) exceed the 256 MiB Infinity Cache is walked from HBM, where a wave's
-// lanes wait on leaf loads and refilling idle lanes pays (soup_2048: 3.18 -> 2.33 ms); a
-// cache-resident tree is bound by instruction issue, where the refills' partial-wave ray
-// setups cost more than the idle lanes (sphere_256: 0.83 vs 0.95 ms).
+// The automatic lane schedule: persistent lanes refilled at 8 idle for a tree whose walked
+// arrays (64-byte leaves, 4-byte bounds, 12-byte nodes) exceed the 256 MiB Infinity Cache,
+// walked from HBM where a wave's lanes wait on leaf loads (soup_2048: 3.18 -> 2.33 ms when
+// this schedule was introduced), and for a tree of k >= 4 levels (n = 2048 and up), whose
+// walks are long and uneven (terrain_1024's tree, 15 MB: 67.8 visits and 14.9 node
+// descents per ray; 3.96 -> 3.49 ms); one lane per ray for the shallower cache-resident
+// trees, bound by instruction issue, where the refills' partial-wave ray setups cost more
+// than the idle lanes (sphere_256, k = 3: 0.77 vs 0.81 ms).
 constexpr uint64_t kCacheBytes = 256ull << 20;
 constexpr int kAutoRefill = 8;
 
 int refill_for(const vxl_ctx *ctx, const vxl_tree *tree) {
   if (ctx->dda_refill >= 0) return ctx->dda_refill;
   const uint64_t walked = tree->leaf_count * 68 + tree->node_count * 12;
-  return walked > kCacheBytes ? kAutoRefill : 0;
+  return walked > kCacheBytes || tree->k >= 4 ? kAutoRefill : 0;
 }
 
 }  // namespace
