@@ -322,6 +322,9 @@ def bench_dda(ctx, torch, W, reps=5):
     # the headline's terrain tree at 1024^3 (Resolution(2048)) under the same camera + random rays
     n2, vs2 = W.grid_for(1024)
     cases.append(("terrain_1024", W.terrain_1024(), n2, vs2, 1024))
+    # the library's threshold for persistent lanes on a deep tree: 4 x the rays its resident
+    # one-wave blocks walk at once (k_traverse_pl holds 6 waves per SIMD, 24 per CU)
+    pl_min_rays = 4 * torch.cuda.get_device_properties(0).multi_processor_count * 24 * 64
     for name, tris, n, vs, m in cases:
         log(f"dda {name}: voxelize T={len(tris)} n={n}")
         tree = ctx.voxelize_tree(tris, n, (0, 0, 0), vs, 1e-4)
@@ -344,9 +347,12 @@ def bench_dda(ctx, torch, W, reps=5):
         out[name] = dict(rays=R, kernel_ms=round(k_ms, 4), mrays_per_s=round(R / k_ms / 1e3, 1),
                          hit_fraction=round(hits / R, 4), leaves=tree.leaf_count,
                          nodes=tree.node_count, tree_mb=round(tree.device_bytes / 2 ** 20, 1),
-                         lane_schedule=("persistent lanes, refill at 8 idle (tree above the "
-                                        "256 MiB Infinity Cache)" if walked > 256 * 2 ** 20
-                                        else "one lane per ray (cache-resident tree)"))
+                         lane_schedule=(
+                             "persistent lanes, refill at 8 idle (tree above the 256 MiB "
+                             "Infinity Cache)" if walked > 256 * 2 ** 20 else
+                             "persistent lanes, refill at 8 idle (deep tree, k >= 4, and over "
+                             "4 x the resident waves' rays)" if tree.k >= 4 and R >= pl_min_rays
+                             else "one lane per ray (cache-resident tree)"))
         # render path: the same camera, rays made on the device, one RGBA8 per pixel
         cam = W.render_camera(m, n)
         img = torch.empty(1920 * 1080, dtype=torch.int32, device="cuda")

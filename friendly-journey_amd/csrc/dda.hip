@@ -878,16 +878,29 @@ __global__ void k_clear_counter(unsigned long long *ctr) { *ctr = 0ull; }
 // walked from HBM where a wave's lanes wait on leaf loads (soup_2048: 3.18 -> 2.33 ms when
 // this schedule was introduced), and for a tree of k >= 4 levels (n = 2048 and up), whose
 // walks are long and uneven (terrain_1024's tree, 15 MB: 67.8 visits and 14.9 node
-// descents per ray; 3.96 -> 3.49 ms); one lane per ray for the shallower cache-resident
-// trees, bound by instruction issue, where the refills' partial-wave ray setups cost more
-// than the idle lanes (sphere_256, k = 3: 0.77 vs 0.81 ms).
+// descents per ray; 3.96 -> 3.49 ms) when the launch holds at least 4 x the rays the
+// device's resident waves walk at once (smaller launches fill fewer lanes on persistent
+// lanes than one lane per ray does: 512 plain traversals of 1k-1M rays, 97 vs 119 ms);
+// one lane per ray for the shallower cache-resident trees, bound by instruction issue,
+// where the refills' partial-wave ray setups cost more than the idle lanes (sphere_256,
+// k = 3: 0.77 vs 0.81 ms).
 constexpr uint64_t kCacheBytes = 256ull << 20;
 constexpr int kAutoRefill = 8;
 
-int refill_for(const vxl_ctx *ctx, const vxl_tree *tree) {
+int refill_for(vxl_ctx *ctx, const vxl_tree *tree, uint64_t R) {
   if (ctx->dda_refill >= 0) return ctx->dda_refill;
   const uint64_t walked = tree->leaf_count * 68 + tree->node_count * 12;
-  return walked > kCacheBytes || tree->k >= 4 ? kAutoRefill : 0;
+  if (walked > kCacheBytes) return kAutoRefill;
+  if (tree->k < 4) return 0;
+  if (!ctx->pl_min_rays) {
+    int per_cu = 0;
+    VXL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_traverse_pl, 64,
+                                                         frame_lds_bytes(4, 64)));
+    if (per_cu < 1) per_cu = 1;
+    ctx->pl_min_rays =
+        4ull * uint64_t(ctx->cu_count > 0 ? ctx->cu_count : 256) * uint64_t(per_cu) * 64ull;
+  }
+  return R >= ctx->pl_min_rays ? kAutoRefill : 0;
 }
 
 }  // namespace
@@ -896,7 +909,7 @@ void launch_traverse(vxl_ctx *ctx, const vxl_tree *tree, const float *d_rays, ui
                      uint32_t *d_hits, double *d_t, hipStream_t s, unsigned long long *counter) {
   if (R == 0) return;
   const size_t lds64 = frame_lds_bytes(tree->k, 64);
-  const int refill = refill_for(ctx, tree);
+  const int refill = refill_for(ctx, tree, R);
   if (refill > 0) {
     // persistent lanes: as many one-wave blocks as the device holds at once, each launch
     // with a counter slot of its own (a ring, so launches on other streams keep theirs)

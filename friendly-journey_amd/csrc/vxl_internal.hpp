@@ -117,6 +117,7 @@ struct vxl_ctx {
   int dda_batch_refill = 0;  // batched traversals: 0 one lane per ray, > 0 persistent lanes
                              // refilled once that many of a wave's lanes are idle
   uint64_t batch_pl_min_rays = 0;  // 4 x the rays the persistent batch kernel holds at once
+  uint64_t pl_min_rays = 0;        // 4 x the rays the persistent ray kernel holds at once
   // last-call timing (ms), filled when timing is enabled
   bool timing = false;
   hipEvent_t ev[8] = {};  // ev[0] call start, ev[1]/ev[2] around the SAT kernel, ev[3] end
