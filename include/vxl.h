@@ -445,9 +445,10 @@ vxl_status vxl_set_batch_refill(vxl_ctx *ctx, int refill_min);
 vxl_status vxl_set_merge_words(vxl_ctx *ctx, int words);
 /* DDA lane schedule of vxl_traverse / vxl_traverse_dev: refill_min in [1, 64] runs a
  * persistent grid whose waves hand a lane the next ray once refill_min of their lanes are
- * idle; 0 launches one lane per ray; -1 (the default) picks per tree: persistent lanes
- * (refill 8) for trees whose leaves, bounds and nodes exceed 256 MiB, else one lane per
- * ray. Results are the same either way. */
+ * idle; 0 launches one lane per ray; -1 (the default) picks per call: persistent lanes
+ * (refill 8) for trees whose leaves, bounds and nodes exceed 256 MiB, and for trees of
+ * k >= 4 levels when the call holds at least 4 x the rays the device's resident waves walk
+ * at once; else one lane per ray. Results are the same either way. */
 vxl_status vxl_set_dda_refill(vxl_ctx *ctx, int refill_min);
 
 #ifdef __cplusplus
