@@ -854,7 +854,8 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))
     unsigned long long *__restrict__ leafgrid, unsigned long long *__restrict__ brickbits,
     uint32_t *__restrict__ owner, const uint32_t *__restrict__ leaf_map,
     uint32_t own_n, uint32_t flush_at) {
-  __shared__ uint32_t q_t[kWaves][kQueue], q_xy[kWaves][kQueue], q_z[kWaves][kQueue];
+  __shared__ uint32_t q_t[kWaves][kQueue];
+  __shared__ uint2 q_p[kWaves][kQueue];  // (x | y << 16, z)
   __shared__ unsigned long long t_key[kWaves][kTable], t_val[kWaves][kTable];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint64_t P = plan->P, chunks_per_unit = plan->cpu, nunits = plan->nunits;
@@ -863,7 +864,8 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))
   const uint32_t nctr = gridDim.x < uint32_t(kUnitCounters) ? gridDim.x : uint32_t(kUnitCounters);
   const uint32_t cls = blockIdx.x % nctr;
   unsigned int *next_unit = &plan->counter[cls][0];
-  uint32_t *qt = q_t[wave], *qxy = q_xy[wave], *qz = q_z[wave];
+  uint32_t *qt = q_t[wave];
+  uint2 *qp2 = q_p[wave];
   unsigned long long *tk = t_key[wave], *tv = t_val[wave];
   // Every wave's first unit is its own (unit = the wave's index in the grid); units past
   // the grid's waves are handed out by atomic counters, and only when there are any. The
@@ -1026,8 +1028,7 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))
         if (surv) {
           const uint32_t qp = qn + __popcll(sm & below);
           qt[qp] = tr;
-          qxy[qp] = x | (y << 16);
-          qz[qp] = z;
+          qp2[qp] = make_uint2(x | (y << 16), z);
         }
         qn += __popcll(sm);
         if (qn >= 64) {
@@ -1035,9 +1036,10 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))
           qn -= 64;
           const uint32_t qe = qn + lane;
           unsigned long long key, bit;
-          const bool hit = finish_pair(rec, qt[qe], qxy[qe], qz[qe], key, bit);
+          const uint2 pe = qp2[qe];
+          const bool hit = finish_pair(rec, qt[qe], pe.x, pe.y, key, bit);
           if (kOwn != kOwnNone && hit)
-            owner_min<kOwn>(owner, leaf_map, own_n, qt[qe], qxy[qe], qz[qe]);
+            owner_min<kOwn>(owner, leaf_map, own_n, qt[qe], pe.x, pe.y);
           if (kOwn != kOwnLeaf) {
             used += table_insert(tk, tv, hit, key, bit);
             if (used >= flush_at) {
@@ -1066,9 +1068,10 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))
   {
     unsigned long long key = 0, bit = 0;
     bool hit = false;
-    if (uint32_t(lane) < qn) hit = finish_pair(rec, qt[lane], qxy[lane], qz[lane], key, bit);
+    const uint2 pe = qp2[lane];
+    if (uint32_t(lane) < qn) hit = finish_pair(rec, qt[lane], pe.x, pe.y, key, bit);
     if (kOwn != kOwnNone && hit)
-      owner_min<kOwn>(owner, leaf_map, own_n, qt[lane], qxy[lane], qz[lane]);
+      owner_min<kOwn>(owner, leaf_map, own_n, qt[lane], pe.x, pe.y);
     if (kOwn != kOwnLeaf) table_insert(tk, tv, hit, key, bit);
   }
   if (kOwn != kOwnLeaf) table_flush(tk, tv, lane, leafgrid, brickbits);
