@@ -840,7 +840,7 @@ constexpr uint64_t kSpanPairs = uint64_t(1) << 30;
 // per segment, not once per pair.
 //
 // The kernel is held to 64 VGPRs (8 waves per SIMD, the most a SIMD holds; the LDS of 8
-// blocks, 8 x 14 KiB, fits a CU). Unit-level values (unit, pair range, first triangle)
+// blocks, 8 x 16 KiB, fits a CU). Unit-level values (unit, pair range, first triangle)
 // stay wave-uniform in scalar registers. The window decode is a branch that calls without
 // windowed triangles never take.
 //
@@ -854,8 +854,7 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))
     unsigned long long *__restrict__ leafgrid, unsigned long long *__restrict__ brickbits,
     uint32_t *__restrict__ owner, const uint32_t *__restrict__ leaf_map,
     uint32_t own_n, uint32_t flush_at) {
-  __shared__ uint32_t q_t[kWaves][kQueue];
-  __shared__ uint2 q_p[kWaves][kQueue];  // (x | y << 16, z)
+  __shared__ uint4 q_p[kWaves][kQueue];  // (t, x | y << 16, z, -)
   __shared__ unsigned long long t_key[kWaves][kTable], t_val[kWaves][kTable];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint64_t P = plan->P, chunks_per_unit = plan->cpu, nunits = plan->nunits;
@@ -864,8 +863,7 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))
   const uint32_t nctr = gridDim.x < uint32_t(kUnitCounters) ? gridDim.x : uint32_t(kUnitCounters);
   const uint32_t cls = blockIdx.x % nctr;
   unsigned int *next_unit = &plan->counter[cls][0];
-  uint32_t *qt = q_t[wave];
-  uint2 *qp2 = q_p[wave];
+  uint4 *qp2 = q_p[wave];
   unsigned long long *tk = t_key[wave], *tv = t_val[wave];
   // Every wave's first unit is its own (unit = the wave's index in the grid); units past
   // the grid's waves are handed out by atomic counters, and only when there are any. The
@@ -1027,8 +1025,7 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))
         const unsigned long long sm = __ballot(surv);
         if (surv) {
           const uint32_t qp = qn + __popcll(sm & below);
-          qt[qp] = tr;
-          qp2[qp] = make_uint2(x | (y << 16), z);
+          qp2[qp] = make_uint4(tr, x | (y << 16), z, 0u);
         }
         qn += __popcll(sm);
         if (qn >= 64) {
@@ -1036,10 +1033,10 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))
           qn -= 64;
           const uint32_t qe = qn + lane;
           unsigned long long key, bit;
-          const uint2 pe = qp2[qe];
-          const bool hit = finish_pair(rec, qt[qe], pe.x, pe.y, key, bit);
+          const uint4 e = qp2[qe];
+          const bool hit = finish_pair(rec, e.x, e.y, e.z, key, bit);
           if (kOwn != kOwnNone && hit)
-            owner_min<kOwn>(owner, leaf_map, own_n, qt[qe], pe.x, pe.y);
+            owner_min<kOwn>(owner, leaf_map, own_n, e.x, e.y, e.z);
           if (kOwn != kOwnLeaf) {
             used += table_insert(tk, tv, hit, key, bit);
             if (used >= flush_at) {
@@ -1068,10 +1065,10 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))
   {
     unsigned long long key = 0, bit = 0;
     bool hit = false;
-    const uint2 pe = qp2[lane];
-    if (uint32_t(lane) < qn) hit = finish_pair(rec, qt[lane], pe.x, pe.y, key, bit);
+    const uint4 e = qp2[lane];
+    if (uint32_t(lane) < qn) hit = finish_pair(rec, e.x, e.y, e.z, key, bit);
     if (kOwn != kOwnNone && hit)
-      owner_min<kOwn>(owner, leaf_map, own_n, qt[lane], pe.x, pe.y);
+      owner_min<kOwn>(owner, leaf_map, own_n, e.x, e.y, e.z);
     if (kOwn != kOwnLeaf) table_insert(tk, tv, hit, key, bit);
   }
   if (kOwn != kOwnLeaf) table_flush(tk, tv, lane, leafgrid, brickbits);
