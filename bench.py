@@ -39,7 +39,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes one k_sat launch moves on terrain_1024, from two rocprofv3 --pmc passes
 # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB per dispatch, factors from a 1 GiB copy kernel);
 # profiles/k_sat__terrain_1024.md holds the commands and the raw counter values.
-K_SAT_TRAFFIC_BYTES = 678_404_526
+K_SAT_TRAFFIC_BYTES = 663_501_721
 # Algorithmic lane-ops of one ray walk, counted operation by operation on the reference's
 # f32 walker (voxel-core/src/mirror.rs), each arithmetic, compare, select or bit operation one:
 # a visit = the cell test (child_bit / 9-bit Morton index, word and bit extraction: 15) plus
